@@ -1,0 +1,134 @@
+/*
+ * oracle.h -- CPU FP64 restatement of the reference ALE-BGK time step.
+ *
+ * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so, and only as the checker. The
+ * product path (alebgk4paper_amd) never links or calls it.
+ *
+ * Semantics follow the reference's GPU path, Solver/CudaBackend.cu
+ * (cudaStep :1505-1870), which is the component the HIP library replaces:
+ * Cholesky with a relative pivot tolerance and a Shepard fallback in 3-D,
+ * pivoted Gauss in 2-D, positivity by fallback to the pre-transport value,
+ * the NAD density clamp, and two global mass rescales per step. Setup
+ * (lattice, neighbour lists, wall state, initial condition) follows the
+ * reference CPU sources it shares with the GPU path.
+ *
+ * Compiled with -ffp-contract=off so a*b+c is never fused: the expressions
+ * evaluate in the order the reference writes them.
+ *
+ * Layout: structure of arrays; g is particle-major, g[i*gstride + lin] with
+ * lin = iv + Nv*(jv + Nv*kv) in 3-D; in 2-D g1 at lin = iv + Nv*jv and g2 at
+ * lin + Nv*Nv.
+ */
+#ifndef ALEBGK_ORACLE_H
+#define ALEBGK_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_NEIGH 128
+
+/* Voxel and lattice geometry (the CalcParameters fields generateParticles
+ * fills, Initialization/ParticleGeneration.cpp:47-114). */
+typedef struct {
+    int dim;
+    int Nx, Ny, Nz;
+    double xleft, xright, ybottom, ytop, zfront, zback;
+    double dx, dy, dz;
+    double radius, minDist;
+    double hBox, vBox, wBox;
+    int nbhBox, nbvBox, nbwBox, nvox;
+} orc_geom;
+
+/* Read-only view of a cloud. */
+typedef struct {
+    int dim, N, Nv, gstride;
+    double vlo, dv;
+    const double *x, *y, *z;
+    const unsigned char *boundary;
+    const double *wnx, *wny, *wnz, *wUx, *wUy, *wUz, *wT;
+    const double *mvx, *mvy, *mvz;        /* ALE mesh velocity, may be NULL */
+    const int *neigh;                     /* N * ORC_MAX_NEIGH */
+    const int *nneigh;
+} orc_cloud;
+
+/* Mutable per-step state. */
+typedef struct {
+    double *g, *gt;                       /* N * gstride each */
+    double *rho, *ux, *uy, *uz, *T;       /* N each */
+    double *limflag;                      /* N: +1 positivity, +2 NAD */
+    int *mlsflag;                         /* N: |1 degenerate, |2 extrapolation */
+    double *mlsovr;                       /* N: overshoot L1, density units */
+} orc_state;
+
+/* ---- setup --------------------------------------------------------------- */
+int  orc_lattice(orc_geom *geo, double *x, double *y, double *z,
+                 unsigned char *boundary, int *voxel);
+int  orc_neighbours(const orc_geom *geo, int N, const double *x,
+                    const double *y, const double *z, int *voxel,
+                    int *neigh, int *nneigh);
+void orc_wall_normal(const orc_geom *geo, double x, double y, double z,
+                     double Tw, double TwTop, const double *Uface /* 6x3:
+                     left,right,top,bottom,front,back */, double *out7);
+void orc_wall_state(const orc_geom *geo, int N, const double *x,
+                    const double *y, const double *z,
+                    const unsigned char *boundary, const int *bodyid,
+                    double Tw, const double *Uface,
+                    double *wnx, double *wny, double *wnz, double *wUx,
+                    double *wUy, double *wUz, double *wT);
+void orc_lid_parabolic(const orc_geom *geo, int N, const double *x,
+                       const double *y, const unsigned char *boundary,
+                       const int *bodyid, double peak, double *wUx);
+void orc_initial_state(int dim, int N, int Nv, double vlo, double dv,
+                       double rho0, double T0, double R, double *g);
+
+/* ---- kernels (one reference kernel each) --------------------------------- */
+double orc_maxwellian3d(double vx, double vy, double vz, double ux,
+                        double uy, double uz, double T, double rho, double R);
+double orc_maxwellian_g1(double vx, double vy, double ux, double uy,
+                         double T, double rho, double R);
+
+void orc_transport3d(const orc_cloud *c, const double *g, double *gt,
+                     double dt, double h, int mlsOrder, double tol,
+                     int clamp, int *mlsflag, double *mlsovr);
+void orc_transport2d(const orc_cloud *c, const double *g, double *gt,
+                     double dt, double h, int mlsOrder);
+void orc_positivity3d(const orc_cloud *c, const double *g, double *gt,
+                      double *limflag);
+void orc_positivity2d(const orc_cloud *c, double *gt, double *limflag);
+void orc_nad3d(const orc_cloud *c, double *gt, const double *rho,
+               double *limflag);
+double orc_sum_mass(const orc_cloud *c, const double *g);
+void orc_scale(const orc_cloud *c, double *g, double s);
+void orc_wallbc3d(const orc_cloud *c, double *g, double R, double *rho,
+                  double *ux, double *uy, double *uz, double *T);
+void orc_wallbc2d(const orc_cloud *c, double *g, double R, double *rho,
+                  double *ux, double *uy, double *T);
+void orc_moments3d(const orc_cloud *c, const double *g, double R,
+                   double *rho, double *ux, double *uy, double *uz,
+                   double *T);
+void orc_moments2d(const orc_cloud *c, const double *g, double R,
+                   double *rho, double *ux, double *uy, double *T);
+void orc_collision3d(const orc_cloud *c, double *g, double dt, double tau,
+                     double R, const double *rho, const double *ux,
+                     const double *uy, const double *uz, const double *T);
+void orc_collision2d(const orc_cloud *c, double *g, double dt, double tau,
+                     double R, const double *rho, const double *ux,
+                     const double *uy, const double *T);
+
+/* The whole cudaStep sequence. Returns massOld. nofix disables the second
+ * mass restore (ALEBGK_NO_MASS_FIX). */
+double orc_step(const orc_cloud *c, orc_state *s, double dt, double tau,
+                double R, double h, int mlsOrder, double tol, int nofix);
+
+/* Surface-stress loads of one body, computeBodyLoads
+ * (RigidBody/BodyDynamics.cpp:17-97). out6 = Fx,Fy,Fz,Tx,Ty,Tz. */
+void orc_body_loads(const orc_cloud *c, const double *g, const int *bodyid,
+                    int body, double cx, double cy, double cz, double area,
+                    double *out6);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,425 @@
+// ref_harness.cpp -- drives the reference CPU functions in place, to write
+// golden fixtures and to time the reference CPU step.
+//
+// This file is test infrastructure. It is compiled by oracle/Makefile against
+// the reference sources where they lie (REF=/root/reference by default) and
+// the binary goes to oracle/_ref/. Nothing in the product loads it.
+//
+// Modes
+//   ref_harness case3d  <outdir> <Nx> <Nv>         cavity3d fixtures
+//   ref_harness case2d  <outdir> <Nx> <Nv>         cavity2d fixtures
+//   ref_harness cube    <outdir> <Nx> <Nv> <scale> cavity3d-cube body loads
+//   ref_harness cpustep <Nx> <Nv> <order> <nsteps> time the reference CPU step
+//
+// Functions called (reference file:line):
+//   configureCase            TestCases/TestCases.cpp:113
+//   makeBodies               TestCases/TestCases.cpp:251
+//   generateParticles        Initialization/ParticleGeneration.cpp:30
+//   embedRigidBodies         Initialization/BodyEmbedding.cpp:18
+//   updateVoxelsAndNeighbours Initialization/NeighbourSearch.cpp:17
+//   initWallData             BoundaryConditions/WallState.cpp:16
+//   applyLidProfile          TestCases/TestCases.cpp:230
+//   applyIC                  Initialization/InitialConditions.cpp:18
+//   mlsTransportCompute      MLSTransport/MLSTransport.cpp:88
+//   computeMoments           MomentCalculations/Moments.cpp:17
+//   bgkCollision             MomentCalculations/Collision.cpp:70
+//   applyBoundaryBC          BoundaryConditions/DiffuseWall.cpp:19
+//   sumMass / scaleG         MomentCalculations/Conservation.cpp:48,92
+//   computeBodyLoads (static) RigidBody/BodyDynamics.cpp:17 (TU included below)
+// =============================================================================
+#include "Config.hpp"
+#include "Types.hpp"
+#include "MLSBasis.hpp"
+#include "Geometry.hpp"
+#include "Initialization.hpp"
+#include "BoundaryConditions.hpp"
+#include "MLSTransport.hpp"
+#include "Moments.hpp"
+#include "TestCases.hpp"
+
+#include <chrono>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+// computeBodyLoads is file-static in the reference; compile that translation
+// unit here (in place, unmodified) so the harness can call it.
+#include "BodyDynamics.cpp"
+
+namespace {
+
+// ---- minimal .npy writer (little-endian, C order) -------------------------
+void writeNpy(const std::string &path, const char *descr, const void *data,
+              size_t elemSize, const std::vector<size_t> &shape)
+{
+    std::string dict = "{'descr': '";
+    dict += descr;
+    dict += "', 'fortran_order': False, 'shape': (";
+    size_t count = 1;
+    for (size_t k = 0; k < shape.size(); ++k)
+    {
+        dict += std::to_string(shape[k]);
+        dict += (shape.size() == 1) ? "," : (k + 1 < shape.size() ? ", " : "");
+        count *= shape[k];
+    }
+    dict += "), }";
+    size_t hdr = 10 + dict.size() + 1;
+    size_t pad = (64 - hdr % 64) % 64;
+    dict.append(pad, ' ');
+    dict += '\n';
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) { std::fprintf(stderr, "cannot write %s\n", path.c_str()); std::exit(2); }
+    const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    std::fwrite(magic, 1, 8, f);
+    uint16_t hl = static_cast<uint16_t>(dict.size());
+    std::fwrite(&hl, 2, 1, f);
+    std::fwrite(dict.data(), 1, dict.size(), f);
+    std::fwrite(data, elemSize, count, f);
+    std::fclose(f);
+}
+void npyF8(const std::string &p, const std::vector<double> &v,
+           std::vector<size_t> shape) { writeNpy(p, "<f8", v.data(), 8, shape); }
+void npyI4(const std::string &p, const std::vector<int32_t> &v,
+           std::vector<size_t> shape) { writeNpy(p, "<i4", v.data(), 4, shape); }
+void npyU1(const std::string &p, const std::vector<uint8_t> &v,
+           std::vector<size_t> shape) { writeNpy(p, "|u1", v.data(), 1, shape); }
+
+struct Setup
+{
+    TestCase tc;
+    SimParameters sp;
+    DomainBoundary dom;
+    BoundaryConditions BC;
+    GasConstants gc;
+    CalcParameters cp{};
+    std::vector<std::unique_ptr<RigidBody>> bodies;
+    std::vector<Particle> P;
+};
+
+// The setup order of main() (TestCases/main.cpp:193-345).
+void build(Setup &S, TestCase tc, int Nx, int Nv, double bodyScale)
+{
+    S.tc = tc;
+    configureCase(tc, BodyMotion::Free, S.sp, S.dom, S.BC, S.gc);
+    S.sp.problem = ProblemID::DrivenCavity;
+    const CaseInfo &info = caseInfo(tc);
+    S.sp.Nx = S.sp.Ny = Nx;
+    if (info.dim == 3) S.sp.Nz = Nx;
+    S.sp.Nv = S.sp.NvMin = S.sp.NvMax = Nv;
+    S.bodies = makeBodies(tc, BodyMotion::Free, S.BC.rho, bodyScale);
+    generateParticles(S.P, S.sp, S.dom, S.cp);
+    embedRigidBodies(S.P, S.bodies, S.sp, S.cp);
+    updateVoxelsAndNeighbours(S.P, S.cp, S.dom);
+    initWallData(S.P, S.dom, S.BC);
+    applyLidProfile(tc, S.P, S.dom);
+    double Lx = S.dom.xright - S.dom.xleft, Ly = S.dom.ytop - S.dom.ybottom;
+    double Lz = (info.dim == 3) ? (S.dom.zback - S.dom.zfront) : 1.0;
+    applyIC(S.P, S.BC.rho, S.gc.R, Lx, Ly, Lz);
+}
+
+// A smooth, positive, non-equilibrium distribution: a local Maxwellian whose
+// density, velocity and temperature vary across the box. Deterministic.
+void perturb(Setup &S)
+{
+    const double L = 1e-6, R = S.gc.R, twoPi = 2.0 * M_PI;
+    for (auto &p : S.P)
+    {
+        double sx = p.x / L, sy = p.y / L, sz = p.z / L;
+        double rho = S.BC.rho * (1.0 + 0.2 * std::sin(twoPi * sx) * std::cos(M_PI * sy));
+        double ux = 60.0 * std::sin(M_PI * sy);
+        double uy = -40.0 * std::sin(M_PI * sx);
+        double uz = 30.0 * std::cos(M_PI * sz);
+        double T = 270.0 * (1.0 + 0.1 * std::sin(M_PI * sz + 0.3));
+        int Nv = p.Nv_local;
+        for (int kv = 0; kv < (p.dim == 3 ? Nv : 1); ++kv)
+            for (int jv = 0; jv < Nv; ++jv)
+                for (int iv = 0; iv < Nv; ++iv)
+                {
+                    double vx = p.vlo + iv * p.dv_local;
+                    double vy = p.vlo + jv * p.dv_local;
+                    double vz = p.vlo + kv * p.dv_local;
+                    if (p.dim == 3)
+                        p.g[iv + Nv * (jv + Nv * kv)] =
+                            maxwellian3D(vx, vy, vz, ux, uy, uz, T, rho, R);
+                    else
+                    {
+                        double g1 = maxwellianG1(vx, vy, ux, uy, T, rho, R);
+                        p.g[iv + Nv * jv] = g1;
+                        p.g[iv + Nv * jv + Nv * Nv] = 0.5 * R * T * g1 * (1.0 + 0.05 * sx);
+                    }
+                }
+    }
+}
+
+std::vector<double> gatherG(const std::vector<Particle> &P, bool gt)
+{
+    size_t gs = P[0].g.size();
+    std::vector<double> out(P.size() * gs);
+    for (size_t i = 0; i < P.size(); ++i)
+        std::memcpy(&out[i * gs], (gt ? P[i].gt : P[i].g).data(), gs * 8);
+    return out;
+}
+void scatterG(std::vector<Particle> &P, const std::vector<double> &in)
+{
+    size_t gs = P[0].g.size();
+    for (size_t i = 0; i < P.size(); ++i)
+        std::memcpy(P[i].g.data(), &in[i * gs], gs * 8);
+}
+std::vector<double> moments(const std::vector<Particle> &P)
+{
+    std::vector<double> m(P.size() * 5);
+    for (size_t i = 0; i < P.size(); ++i)
+    {
+        m[5 * i + 0] = P[i].rho; m[5 * i + 1] = P[i].ux; m[5 * i + 2] = P[i].uy;
+        m[5 * i + 3] = P[i].uz;  m[5 * i + 4] = P[i].T;
+    }
+    return m;
+}
+void setMoments(std::vector<Particle> &P, const std::vector<double> &m)
+{
+    for (size_t i = 0; i < P.size(); ++i)
+    {
+        P[i].rho = m[5 * i + 0]; P[i].ux = m[5 * i + 1]; P[i].uy = m[5 * i + 2];
+        P[i].uz = m[5 * i + 3];  P[i].T = m[5 * i + 4];
+    }
+}
+
+void writeMeta(const std::string &dir, const Setup &S, double dtT)
+{
+    const CalcParameters &cp = S.cp;
+    fs::create_directories(dir);
+    FILE *f = std::fopen((dir + "/meta.json").c_str(), "w");
+    if (!f) { std::fprintf(stderr, "cannot write %s/meta.json\n", dir.c_str()); std::exit(2); }
+    std::fprintf(f, "{\n");
+    std::fprintf(f, "  \"case\": \"%s\",\n", caseInfo(S.tc).name);
+    std::fprintf(f, "  \"dim\": %d, \"N\": %zu, \"Nx\": %d, \"Ny\": %d, \"Nz\": %d, \"Nv\": %d,\n",
+                 cp.dim, S.P.size(), S.sp.Nx, S.sp.Ny, S.sp.Nz, S.sp.Nv);
+    std::fprintf(f, "  \"gStride\": %zu,\n", S.P[0].g.size());
+    std::fprintf(f, "  \"vmin\": %.17g, \"vmax\": %.17g, \"dv\": %.17g,\n",
+                 S.sp.VMin, S.sp.VMax, cp.dv);
+    std::fprintf(f, "  \"dx\": %.17g, \"dy\": %.17g, \"dz\": %.17g,\n", cp.dx, cp.dy, cp.dz);
+    std::fprintf(f, "  \"radius\": %.17g, \"minDist\": %.17g,\n", cp.radius, cp.minDist);
+    std::fprintf(f, "  \"hBox\": %.17g, \"vBox\": %.17g, \"wBox\": %.17g,\n", cp.hBox, cp.vBox, cp.wBox);
+    std::fprintf(f, "  \"nbhBox\": %d, \"nbvBox\": %d, \"nbwBox\": %d, \"nvox\": %d,\n",
+                 cp.nbhBox, cp.nbvBox, cp.nbwBox, cp.nvox);
+    std::fprintf(f, "  \"xleft\": %.17g, \"xright\": %.17g, \"ybottom\": %.17g, \"ytop\": %.17g, "
+                 "\"zfront\": %.17g, \"zback\": %.17g,\n",
+                 S.dom.xleft, S.dom.xright, S.dom.ybottom, S.dom.ytop, S.dom.zfront, S.dom.zback);
+    std::fprintf(f, "  \"R\": %.17g, \"tau\": %.17g, \"dt_case\": %.17g, \"dt_transport\": %.17g,\n",
+                 S.gc.R, S.sp.tao, S.sp.dt, dtT);
+    std::fprintf(f, "  \"rho0\": %.17g, \"Tw\": %.17g, \"mass_in\": 0\n", S.BC.rho, S.BC.Tw);
+    std::fprintf(f, "}\n");
+    std::fclose(f);
+}
+
+void writeCloud(const std::string &dir, const Setup &S)
+{
+    const auto &P = S.P;
+    size_t N = P.size();
+    std::vector<double> x(N), y(N), z(N), wall(N * 7);
+    std::vector<uint8_t> bnd(N);
+    std::vector<int32_t> vox(N), body(N), nn(N), flat;
+    for (size_t i = 0; i < N; ++i)
+    {
+        x[i] = P[i].x; y[i] = P[i].y; z[i] = P[i].z;
+        bnd[i] = P[i].boundary ? 1 : 0; vox[i] = P[i].voxel; body[i] = P[i].bodyId;
+        double w[7] = {P[i].wnx, P[i].wny, P[i].wnz, P[i].wUx, P[i].wUy, P[i].wUz, P[i].wT};
+        std::memcpy(&wall[7 * i], w, sizeof w);
+        nn[i] = static_cast<int32_t>(P[i].neighindex.size());
+        for (int j : P[i].neighindex) flat.push_back(j);
+    }
+    npyF8(dir + "/x.npy", x, {N});
+    npyF8(dir + "/y.npy", y, {N});
+    npyF8(dir + "/z.npy", z, {N});
+    npyU1(dir + "/boundary.npy", bnd, {N});
+    npyI4(dir + "/voxel.npy", vox, {N});
+    npyI4(dir + "/bodyid.npy", body, {N});
+    npyF8(dir + "/wall.npy", wall, {N, 7});
+    npyI4(dir + "/nneigh.npy", nn, {N});
+    npyI4(dir + "/neigh_flat.npy", flat, {flat.size()});
+}
+
+// Particles whose CPU positivity repair fired during mlsTransportCompute: the
+// repair zeroes the negatives, so an exact zero in gt marks them
+// (MLSTransport/MLSTransport.cpp:189-200, :283-309).
+std::vector<uint8_t> positivityMask(const std::vector<Particle> &P)
+{
+    std::vector<uint8_t> m(P.size(), 0);
+    for (size_t i = 0; i < P.size(); ++i)
+        for (double v : P[i].gt)
+            if (v == 0.0) { m[i] = 1; break; }
+    return m;
+}
+
+int modeCase(const std::string &dir, TestCase tc, int Nx, int Nv)
+{
+    Setup S;
+    build(S, tc, Nx, Nv, 1.0);
+    size_t N = S.P.size(), gs = S.P[0].g.size();
+    double dxmin = std::fabs(S.cp.dx);
+    if (std::fabs(S.cp.dy) < dxmin) dxmin = std::fabs(S.cp.dy);
+    if (S.cp.dim == 3 && std::fabs(S.cp.dz) < dxmin) dxmin = std::fabs(S.cp.dz);
+    double dtT = 0.5 * dxmin / S.sp.VMax;        // foot-point CFL 0.5
+    writeMeta(dir, S, dtT);
+    writeCloud(dir, S);
+    npyF8(dir + "/g_ic.npy", gatherG(S.P, false), {N, gs});
+
+    perturb(S);
+    std::vector<double> gin = gatherG(S.P, false);
+    npyF8(dir + "/g_in.npy", gin, {N, gs});
+    std::vector<double> mass{sumMass(S.P)};
+    npyF8(dir + "/mass_in.npy", mass, {1});
+
+    for (int order = 1; order <= 2; ++order)
+    {
+        scatterG(S.P, gin);
+        mlsTransportCompute(S.P, dtT, S.cp, S.dom, S.BC, order);
+        std::string sfx = "_o" + std::to_string(order);
+        npyF8(dir + "/gt" + sfx + ".npy", gatherG(S.P, true), {N, gs});
+        npyU1(dir + "/posfired" + sfx + ".npy", positivityMask(S.P), {N});
+    }
+
+    // Moments of the perturbed field (interior particles; the boundary keeps
+    // the state applyIC gave it).
+    scatterG(S.P, gin);
+    std::vector<double> m0 = moments(S.P);
+    computeMoments(S.P, S.gc.R);
+    std::vector<double> m1 = moments(S.P);
+    npyF8(dir + "/mom_in.npy", m1, {N, 5});
+
+    // BGK collision with those moments.
+    bgkCollision(S.P, S.sp.dt, S.sp.tao, S.gc.R, false, S.sp.d);
+    npyF8(dir + "/g_coll.npy", gatherG(S.P, false), {N, gs});
+
+    // Diffuse wall on the perturbed field.
+    scatterG(S.P, gin);
+    setMoments(S.P, m0);
+    applyBoundaryBC(S.P, S.gc.R, S.dom, S.BC);
+    npyF8(dir + "/g_wall.npy", gatherG(S.P, false), {N, gs});
+    npyF8(dir + "/mom_wall.npy", moments(S.P), {N, 5});
+    std::printf("wrote %s: N=%zu gStride=%zu\n", dir.c_str(), N, gs);
+    return 0;
+}
+
+int modeCube(const std::string &dir, int Nx, int Nv, double scale)
+{
+    Setup S;
+    build(S, TestCase::Cavity3D_Cube, Nx, Nv, scale);
+    size_t N = S.P.size(), gs = S.P[0].g.size();
+    writeMeta(dir, S, 0.0);
+    writeCloud(dir, S);
+    perturb(S);
+    // Body loads are a pure function of the surface particles' g and wall
+    // state; store only those particles' g.
+    std::vector<int32_t> surf;
+    for (size_t i = 0; i < N; ++i)
+        if (S.P[i].boundary && S.P[i].bodyId >= 0) surf.push_back(static_cast<int32_t>(i));
+    std::vector<double> gs_surf(surf.size() * gs);
+    for (size_t k = 0; k < surf.size(); ++k)
+        std::memcpy(&gs_surf[k * gs], S.P[surf[k]].g.data(), gs * 8);
+    npyI4(dir + "/surf_idx.npy", surf, {surf.size()});
+    npyF8(dir + "/g_surf.npy", gs_surf, {surf.size(), gs});
+    // Give the surface a non-zero wall velocity so the load's dependence on
+    // wU is exercised: a rigid rotation plus translation.
+    std::vector<double> wall(surf.size() * 3);
+    for (size_t k = 0; k < surf.size(); ++k)
+    {
+        Particle &p = S.P[surf[k]];
+        const RigidBody &b = *S.bodies[p.bodyId];
+        double rx = p.x - b.cx, ry = p.y - b.cy, rz = p.z - b.cz;
+        double vx = 0.7, vy = -0.4, vz = 0.2, ox = 1e6, oy = -2e6, oz = 3e6;
+        p.wUx = vx + oy * rz - oz * ry;
+        p.wUy = vy + oz * rx - ox * rz;
+        p.wUz = vz + ox * ry - oy * rx;
+        wall[3 * k] = p.wUx; wall[3 * k + 1] = p.wUy; wall[3 * k + 2] = p.wUz;
+    }
+    npyF8(dir + "/surf_wU.npy", wall, {surf.size(), 3});
+    size_t nb = S.bodies.size();
+    std::vector<double> Fx(nb), Fy(nb), Fz(nb), Tx(nb), Ty(nb), Tz(nb);
+    computeBodyLoads(S.P, S.bodies, Fx, Fy, Fz, Tx, Ty, Tz);
+    std::vector<double> loads{Fx[0], Fy[0], Fz[0], Tx[0], Ty[0], Tz[0]};
+    npyF8(dir + "/loads.npy", loads, {6});
+    const RigidBody &b = *S.bodies[0];
+    std::vector<double> bstate{b.cx, b.cy, b.cz, b.getSurfaceArea(), b.mass,
+                               b.getMomentOfInertia()};
+    npyF8(dir + "/body.npy", bstate, {6});
+    std::printf("wrote %s: N=%zu surface=%zu\n", dir.c_str(), N, surf.size());
+    return 0;
+}
+
+// The CPU time loop body of runSolver (Solver/Solver.cpp:632-705), run for
+// nsteps fixed steps; prints one JSON line with the per-step wall times.
+int modeCpuStep(int Nx, int Nv, int order, int nsteps)
+{
+    Setup S;
+    build(S, TestCase::Cavity3D, Nx, Nv, 1.0);
+    S.sp.mlsOrder = order;
+    auto &P = S.P;
+    const SimParameters &sp = S.sp;
+    double dt = sp.dt, R = S.gc.R;
+    std::vector<double> sec;
+    for (int s = 0; s < nsteps; ++s)
+    {
+        auto t0 = std::chrono::steady_clock::now();
+        double massOld = sumMass(P);
+        mlsTransportCompute(P, dt, S.cp, S.dom, S.BC, sp.mlsOrder);
+        if (sp.limiter == LimiterMode::MOOD)
+        {
+            int nf = checkTransportPAD(P);
+            if (nf > 0) mlsTransportCompute(P, dt, S.cp, S.dom, S.BC, 1, true);
+        }
+        mlsTransportCommit(P);
+        double massNew = sumMass(P);
+        if (massNew > 1.0e-30) scaleG(P, massOld / massNew);
+        applyGravity(P, sp.gravityY, dt);
+        applyBoundaryBC(P, R, S.dom, S.BC);
+        computeMoments(P, R);
+        bgkCollision(P, dt, sp.tao, R, sp.tauVariable, sp.d);
+        computeMoments(P, R);
+        if (sp.limiter == LimiterMode::MOOD && moodCheck(P))
+        {
+            bgkCollision(P, dt, sp.tao, R, sp.tauVariable, sp.d);
+            computeMoments(P, R);
+        }
+        sec.push_back(std::chrono::duration<double>(
+            std::chrono::steady_clock::now() - t0).count());
+        std::fprintf(stderr, "[cpustep] step %d %.3f s\n", s + 1, sec.back());
+    }
+    int threads = 1;
+#ifdef _OPENMP
+    threads = omp_get_max_threads();
+#endif
+    double rho = 0, T = 0; int n = 0;
+    for (auto &p : P) if (!p.boundary) { rho += p.rho; T += p.T; ++n; }
+    std::printf("{\"N\": %zu, \"Nv\": %d, \"order\": %d, \"threads\": %d, \"step_seconds\": [",
+                P.size(), Nv, order, threads);
+    for (size_t k = 0; k < sec.size(); ++k)
+        std::printf("%s%.6f", k ? ", " : "", sec[k]);
+    std::printf("], \"rho_avg\": %.10f, \"T_avg\": %.10f}\n", rho / n, T / n);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 1; }
+    std::string mode = argv[1];
+    if (mode == "case3d" && argc == 5)
+        return modeCase(argv[2], TestCase::Cavity3D, std::atoi(argv[3]), std::atoi(argv[4]));
+    if (mode == "case2d" && argc == 5)
+        return modeCase(argv[2], TestCase::Cavity2D, std::atoi(argv[3]), std::atoi(argv[4]));
+    if (mode == "cube" && argc == 6)
+        return modeCube(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atof(argv[5]));
+    if (mode == "cpustep" && argc == 6)
+        return modeCpuStep(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
+                           std::atoi(argv[5]));
+    std::fprintf(stderr, "bad arguments\n");
+    return 1;
+}

@@ -1,0 +1,18 @@
+#!/usr/bin/env bash
+# Writes the reference-CPU golden fixtures under tests/golden/.
+#
+# Runs in the build container only (it needs the reference sources at
+# $REF, default /root/reference): builds oracle/_ref/ref_harness from those
+# sources in place (oracle/Makefile) and lets it call the reference functions
+# on small cases. The fixtures are data: inputs and the reference's outputs.
+set -euo pipefail
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+ROOT="$(cd "$HERE/../.." && pwd)"
+make -s -C "$ROOT/oracle" ref
+H="$ROOT/oracle/_ref/ref_harness"
+"$H" case3d "$HERE/cavity3d_n6_v5" 6 5
+"$H" case3d "$HERE/cavity3d_n9_v3" 9 3
+"$H" case2d "$HERE/cavity2d_n10_v8" 10 8
+"$H" cube   "$HERE/cube_n12_v5" 12 5 1.0
+( cd "$HERE" && find . -name '*.npy' -o -name 'meta.json' | sort | xargs sha256sum > MANIFEST.sha256 )
+echo "fixtures written under $HERE"
