@@ -1,0 +1,350 @@
+"""alebgk4paper_amd -- the ALE-BGK time step on MI355X (gfx950).
+
+Python mirror of the C ABI in include/alebgk_dev.h (libalebgk_hip.so) and of
+the host case builder in include/alebgk_host.h (libalebgk_host.so). The
+compute path is the HIP library; there is no CPU fallback: importing works
+anywhere, but creating a Solver fails loudly when the library is missing or
+no GPU is present.
+
+    cs = configure("cavity3d", nx=40, nv=10)
+    cloud = build_lattice(cs, precision=PREC_MIXED, mls_order=2)
+    sol = Solver(cloud)
+    sol.step(cs.dt)
+    rho, ux, uy, uz, T = sol.moments()
+
+Reference boundary replaced: Solver/Solver.cpp:67-89 (cudaUploadCloud,
+cudaStep, cudaDownload*), Solver/CudaBackend.cu.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+HIP_LIB = os.path.join(PKG, "libalebgk_hip.so")
+HOST_LIB = os.path.join(PKG, "libalebgk_host.so")
+
+MAX_NEIGH = 128
+PREC_FP64, PREC_MIXED, PREC_FP64_TABLE = 0, 1, 2
+PREC_NAMES = {PREC_FP64: "fp64", PREC_MIXED: "mixed", PREC_FP64_TABLE: "fp64-table"}
+ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "call out of order",
+          4: "unsupported configuration", 5: "device allocation failed"}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_up = C.POINTER(C.c_ubyte)
+
+
+class ABConfig(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n_particles", C.c_int), ("nv", C.c_int),
+                ("vmin", C.c_double), ("vmax", C.c_double), ("R", C.c_double),
+                ("tau", C.c_double), ("radius", C.c_double), ("mls_order", C.c_int),
+                ("mls_tol", C.c_double), ("mls_clamp", C.c_int), ("no_mass_fix", C.c_int),
+                ("precision", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("xleft", C.c_double), ("ybottom", C.c_double), ("zfront", C.c_double),
+                ("hbox", C.c_double), ("vbox", C.c_double), ("wbox", C.c_double),
+                ("nbhbox", C.c_int), ("nbvbox", C.c_int), ("nbwbox", C.c_int)]
+
+
+class ABCloudView(C.Structure):
+    _fields_ = [("x", _dp), ("y", _dp), ("z", _dp), ("boundary", _up),
+                ("wnx", _dp), ("wny", _dp), ("wnz", _dp), ("wux", _dp), ("wuy", _dp),
+                ("wuz", _dp), ("wt", _dp), ("mvx", _dp), ("mvy", _dp), ("mvz", _dp),
+                ("rho", _dp), ("ux", _dp), ("uy", _dp), ("uz", _dp), ("t", _dp),
+                ("g", _dp), ("neigh", _ip), ("nneigh", _ip)]
+
+
+class ABStepStats(C.Structure):
+    _fields_ = [("mass_old", C.c_double), ("mass_after_transport", C.c_double),
+                ("mass_after_collision", C.c_double), ("n_positivity", C.c_int),
+                ("n_nad", C.c_int), ("n_mls_degenerate", C.c_int),
+                ("n_mls_extrapolate", C.c_int), ("mls_ovr_sum", C.c_double),
+                ("mls_ovr_max", C.c_double)]
+
+
+class ALHCase(C.Structure):
+    _fields_ = [("id", C.c_int), ("name", C.c_char * 32), ("dim", C.c_int),
+                ("has_body", C.c_int), ("lid", C.c_int), ("lid_speed", C.c_double),
+                ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nv", C.c_int),
+                ("vmin", C.c_double), ("vmax", C.c_double), ("dt", C.c_double),
+                ("tfinal", C.c_double), ("tau", C.c_double), ("rho0", C.c_double),
+                ("Tw", C.c_double), ("R", C.c_double), ("d_hs", C.c_double),
+                ("save_every", C.c_int), ("mls_order", C.c_int), ("dom", C.c_double * 6)]
+
+
+class ALEBGKError(RuntimeError):
+    pass
+
+
+def _ptr(a, t):
+    if a is None:
+        return C.cast(None, t)
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("arrays passed to the library must be C-contiguous")
+    return a.ctypes.data_as(t)
+
+
+_hip = None
+_host = None
+
+HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebuild_neighbours",
+               "ab_download_neighbours", "ab_step", "ab_steps", "ab_transport_only",
+               "ab_download_g", "ab_download_gt", "ab_download_moments", "ab_total_mass",
+               "ab_body_loads", "ab_set_wall_state", "ab_synchronize", "ab_profile",
+               "ab_profile_read", "ab_stage_name"]
+HOST_EXPORTS = ["alh_configure", "alh_lattice_size", "alh_build_lattice"]
+
+
+def hip_lib():
+    """libalebgk_hip.so; raises if it has not been built."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise ALEBGKError("libalebgk_hip.so is not built: run __graft_entry__.build() "
+                              "or python -m alebgk4paper_amd.build")
+        L = C.CDLL(HIP_LIB)
+        L.ab_last_error.restype = C.c_char_p
+        L.ab_stage_name.restype = C.c_char_p
+        L.ab_step.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
+        L.ab_steps.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.ab_transport_only.argtypes = [C.c_void_p, C.c_double]
+        _hip = L
+    return _hip
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB):
+            raise ALEBGKError("libalebgk_host.so is not built")
+        _host = C.CDLL(HOST_LIB)
+    return _host
+
+
+@dataclass
+class Case:
+    """One reference case after overrides (TestCases.cpp:113-228)."""
+    raw: ALHCase
+
+    def __getattr__(self, k):
+        v = getattr(self.raw, k)
+        return v.decode() if isinstance(v, bytes) else v
+
+    @property
+    def dom(self):
+        return tuple(self.raw.dom)
+
+
+def configure(token, nx=0, nv=0, mls_order=None):
+    cs = ALHCase()
+    rc = host_lib().alh_configure(str(token).encode(), C.c_int(nx), C.c_int(nv), C.byref(cs))
+    if rc:
+        raise ALEBGKError(f"unknown case {token!r}")
+    if mls_order is not None:
+        cs.mls_order = int(mls_order)
+    return Case(cs)
+
+
+@dataclass
+class Cloud:
+    """Host arrays of one cloud plus its ab_config."""
+    cfg: ABConfig
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    boundary: np.ndarray
+    wall: np.ndarray          # N x 7: wn(3), wU(3), wT
+    mom: np.ndarray           # N x 5: rho, ux, uy, uz, T
+    g: np.ndarray             # N x gstride
+    neigh: np.ndarray | None = None
+    nneigh: np.ndarray | None = None
+    mesh: np.ndarray | None = None   # N x 3 ALE mesh velocity
+
+    @property
+    def N(self):
+        return len(self.x)
+
+    @property
+    def gstride(self):
+        return self.g.shape[1]
+
+
+def build_lattice(case, precision=PREC_FP64, mls_order=None):
+    """generateParticles + initWallData + applyLidProfile + applyIC."""
+    cs = case.raw
+    if mls_order is not None:
+        cs.mls_order = int(mls_order)
+    n = C.c_int()
+    host_lib().alh_lattice_size(C.byref(cs), C.byref(n))
+    N = n.value
+    gs = cs.nv ** 3 if cs.dim == 3 else 2 * cs.nv ** 2
+    x, y, z = np.zeros(N), np.zeros(N), np.zeros(N)
+    b = np.zeros(N, np.uint8)
+    wall = np.zeros((N, 7))
+    mom = np.zeros((N, 5))
+    g = np.zeros((N, gs))
+    cfg = ABConfig()
+    rc = host_lib().alh_build_lattice(C.byref(cs), C.c_int(precision), C.byref(cfg),
+                                      _ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp), _ptr(b, _up),
+                                      _ptr(wall, _dp), _ptr(mom, _dp), _ptr(g, _dp))
+    if rc:
+        raise ALEBGKError(f"alh_build_lattice: {ERRORS.get(rc, rc)}")
+    return Cloud(cfg, x, y, z, b, wall, mom, g)
+
+
+class Solver:
+    """One device context (ab_ctx): the cloud resident on the GPU."""
+
+    def __init__(self, cloud: Cloud, precision=None, mls_order=None, no_mass_fix=False,
+                 mls_clamp=False):
+        self._lib = hip_lib()
+        cfg = ABConfig()
+        C.memmove(C.byref(cfg), C.byref(cloud.cfg), C.sizeof(ABConfig))
+        cfg.n_particles = cloud.N
+        if precision is not None:
+            cfg.precision = int(precision)
+        if mls_order is not None:
+            cfg.mls_order = int(mls_order)
+        cfg.no_mass_fix = int(bool(no_mass_fix))
+        cfg.mls_clamp = int(bool(mls_clamp))
+        self.cfg = cfg
+        self.cloud = cloud
+        self.N, self.dim = cloud.N, cfg.dim
+        self.gstride = cloud.gstride
+        self._ctx = C.c_void_p()
+        rc = self._lib.ab_create(C.byref(cfg), C.byref(self._ctx))
+        if rc:
+            raise ALEBGKError(f"ab_create: {ERRORS.get(rc, rc)}")
+        self.upload(cloud)
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self._lib.ab_last_error(self._ctx)
+            raise ALEBGKError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def upload(self, cloud: Cloud):
+        f8 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        keep = dict(x=f8(cloud.x), y=f8(cloud.y), z=f8(cloud.z),
+                    b=np.ascontiguousarray(cloud.boundary, dtype=np.uint8),
+                    w=[f8(cloud.wall[:, k]) for k in range(7)],
+                    m=[f8(cloud.mom[:, k]) for k in range(5)], g=f8(cloud.g))
+        v = ABCloudView()
+        v.x, v.y, v.z = _ptr(keep["x"], _dp), _ptr(keep["y"], _dp), _ptr(keep["z"], _dp)
+        v.boundary = _ptr(keep["b"], _up)
+        v.wnx, v.wny, v.wnz, v.wux, v.wuy, v.wuz, v.wt = [_ptr(a, _dp) for a in keep["w"]]
+        if cloud.mesh is not None:
+            keep["mv"] = [f8(cloud.mesh[:, k]) for k in range(3)]
+            v.mvx, v.mvy, v.mvz = [_ptr(a, _dp) for a in keep["mv"]]
+        else:
+            v.mvx = v.mvy = v.mvz = C.cast(None, _dp)
+        v.rho, v.ux, v.uy, v.uz, v.t = [_ptr(a, _dp) for a in keep["m"]]
+        v.g = _ptr(keep["g"], _dp)
+        if cloud.neigh is not None:
+            keep["nb"] = np.ascontiguousarray(cloud.neigh, dtype=np.int32)
+            keep["nn"] = np.ascontiguousarray(cloud.nneigh, dtype=np.int32)
+            v.neigh, v.nneigh = _ptr(keep["nb"], _ip), _ptr(keep["nn"], _ip)
+        else:
+            v.neigh, v.nneigh = C.cast(None, _ip), C.cast(None, _ip)
+        self._check(self._lib.ab_upload(self._ctx, C.byref(v)), "ab_upload")
+
+    def close(self):
+        if self._ctx:
+            self._lib.ab_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- stepping --------------------------------------------------------
+    def step(self, dt, stats=False):
+        if stats:
+            st = ABStepStats()
+            self._check(self._lib.ab_step(self._ctx, dt, C.byref(st)), "ab_step")
+            return {k: getattr(st, k) for k, _ in ABStepStats._fields_}
+        self._check(self._lib.ab_step(self._ctx, dt, None), "ab_step")
+        return None
+
+    def steps(self, dt, n):
+        self._check(self._lib.ab_steps(self._ctx, dt, int(n)), "ab_steps")
+
+    def transport_only(self, dt):
+        self._check(self._lib.ab_transport_only(self._ctx, dt), "ab_transport_only")
+
+    def synchronize(self):
+        self._check(self._lib.ab_synchronize(self._ctx), "ab_synchronize")
+
+    def rebuild_neighbours(self):
+        self._check(self._lib.ab_rebuild_neighbours(self._ctx), "ab_rebuild_neighbours")
+
+    # ---- downloads --------------------------------------------------------
+    def g(self):
+        out = np.zeros((self.N, self.gstride))
+        self._check(self._lib.ab_download_g(self._ctx, _ptr(out, _dp)), "ab_download_g")
+        return out
+
+    def gt(self):
+        out = np.zeros((self.N, self.gstride))
+        self._check(self._lib.ab_download_gt(self._ctx, _ptr(out, _dp)), "ab_download_gt")
+        return out
+
+    def moments(self, with_limflag=False):
+        a = [np.zeros(self.N) for _ in range(6)]
+        self._check(self._lib.ab_download_moments(self._ctx, *[_ptr(x, _dp) for x in a]),
+                    "ab_download_moments")
+        mom = np.stack(a[:5], axis=1)
+        return (mom, a[5]) if with_limflag else mom
+
+    def neighbours(self):
+        nb = np.zeros((self.N, MAX_NEIGH), np.int32)
+        nn = np.zeros(self.N, np.int32)
+        self._check(self._lib.ab_download_neighbours(self._ctx, _ptr(nb, _ip), _ptr(nn, _ip)),
+                    "ab_download_neighbours")
+        return nb, nn
+
+    def total_mass(self):
+        m = C.c_double()
+        self._check(self._lib.ab_total_mass(self._ctx, C.byref(m)), "ab_total_mass")
+        return m.value
+
+    def body_loads(self, surf_idx, surf_body, centres, areas, wu=None):
+        si = np.ascontiguousarray(surf_idx, dtype=np.int32)
+        sb = np.ascontiguousarray(surf_body, dtype=np.int32)
+        bc = np.ascontiguousarray(centres, dtype=np.float64).reshape(-1)
+        ba = np.ascontiguousarray(areas, dtype=np.float64).reshape(-1)
+        wuu = None if wu is None else np.ascontiguousarray(wu, dtype=np.float64).reshape(-1)
+        nb = len(ba)
+        out = np.zeros(6 * nb)
+        self._check(self._lib.ab_body_loads(self._ctx, C.c_int(len(si)), _ptr(si, _ip),
+                                            _ptr(sb, _ip), C.c_int(nb), _ptr(bc, _dp),
+                                            _ptr(ba, _dp), _ptr(wuu, _dp), _ptr(out, _dp)),
+                    "ab_body_loads")
+        return out.reshape(nb, 6)
+
+    # ---- profiling ------------------------------------------------------------
+    def profile(self, enable=True):
+        self._check(self._lib.ab_profile(self._ctx, C.c_int(int(enable))), "ab_profile")
+
+    def profile_read(self):
+        ms = (C.c_double * 16)()
+        cnt = (C.c_long * 16)()
+        ns = C.c_int()
+        self._check(self._lib.ab_profile_read(self._ctx, ms, cnt, C.byref(ns)), "ab_profile_read")
+        return {self._lib.ab_stage_name(k).decode(): (ms[k], cnt[k]) for k in range(ns.value)}
+
+
+def velocities(cfg):
+    """Node velocities in lin order, shape (nodes, 3)."""
+    Nv = cfg.nv
+    dv = (cfg.vmax - cfg.vmin) / (Nv - 1)
+    v = cfg.vmin + dv * np.arange(Nv)
+    if cfg.dim == 3:
+        kv, jv, iv = np.meshgrid(np.arange(Nv), np.arange(Nv), np.arange(Nv), indexing="ij")
+        return np.stack([v[iv].ravel(), v[jv].ravel(), v[kv].ravel()], axis=1)
+    jv, iv = np.meshgrid(np.arange(Nv), np.arange(Nv), indexing="ij")
+    return np.stack([v[iv].ravel(), v[jv].ravel(), np.zeros(Nv * Nv)], axis=1)

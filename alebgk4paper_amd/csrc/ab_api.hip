@@ -1,0 +1,683 @@
+// ab_api.hip -- the extern "C" entry points of libalebgk_hip.so
+// (include/alebgk_dev.h): context, upload/download, the step driver.
+//
+// The step keeps every global sum on the device: the reference reads the mass
+// back to the host three times per step (sumMassHost, CudaBackend.cu:1384),
+// here the rescale factors are computed and consumed on the device and a step
+// enqueues without a single host synchronisation.
+#include "ab_device.hpp"
+#include "ab_internal.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace ab;
+
+namespace {
+
+#define HIPCHK(ctx, expr)                                                        \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);      \
+            return AB_ERR_HIP;                                                   \
+        }                                                                        \
+    } while (0)
+
+bool table_mode(const ab_ctx *c) { return c->cfg.precision != AB_PREC_FP64; }
+
+template <typename T>
+int dalloc(ab_ctx *ctx, T **p, size_t n)
+{
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess)
+    {
+        ctx->err = "device allocation of " + std::to_string(n * sizeof(T)) + " bytes failed";
+        *p = nullptr;
+        return AB_ERR_NOMEM;
+    }
+    return AB_OK;
+}
+
+void free_cloud(ab_ctx *c)
+{
+    void *ptrs[] = {c->x, c->y, c->z, c->bnd, c->wnx, c->wny, c->wnz, c->wux, c->wuy, c->wuz,
+                    c->wt, c->mvx, c->mvy, c->mvz, c->rho, c->ux, c->uy, c->uz, c->T,
+                    c->neigh, c->nneigh, c->ilist, c->blist, c->g, c->gt, c->limflag,
+                    c->mlsflag, c->mlsovr, c->pmass, c->vox, c->vcount, c->vstart, c->vlist,
+                    c->bmom, c->bidx, c->bwu};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    c->x = c->y = c->z = nullptr; c->bnd = nullptr;
+    c->wnx = c->wny = c->wnz = c->wux = c->wuy = c->wuz = c->wt = nullptr;
+    c->mvx = c->mvy = c->mvz = nullptr;
+    c->rho = c->ux = c->uy = c->uz = c->T = nullptr;
+    c->neigh = c->nneigh = c->ilist = c->blist = nullptr;
+    c->g = c->gt = c->limflag = c->mlsovr = c->pmass = nullptr;
+    c->mlsflag = nullptr;
+    c->vox = c->vcount = c->vstart = c->vlist = nullptr;
+    c->bmom = nullptr; c->bidx = nullptr; c->bwu = nullptr; c->bmom_cap = 0;
+    table_free(c);
+    c->coef_dt = -1.0;
+}
+
+int alloc_cloud(ab_ctx *c)
+{
+    const size_t N = (size_t)c->N;
+    int rc = 0;
+    double **dd[] = {&c->x, &c->y, &c->z, &c->wnx, &c->wny, &c->wnz, &c->wux, &c->wuy, &c->wuz,
+                     &c->wt, &c->mvx, &c->mvy, &c->mvz, &c->rho, &c->ux, &c->uy, &c->uz, &c->T,
+                     &c->limflag, &c->mlsovr, &c->pmass};
+    for (double **p : dd) if ((rc = dalloc(c, p, N))) return rc;
+    if ((rc = dalloc(c, &c->bnd, N))) return rc;
+    if ((rc = dalloc(c, &c->neigh, N * AB_MAX_NEIGH))) return rc;
+    if ((rc = dalloc(c, &c->nneigh, N))) return rc;
+    if ((rc = dalloc(c, &c->ilist, N))) return rc;
+    if ((rc = dalloc(c, &c->blist, N))) return rc;
+    if ((rc = dalloc(c, &c->mlsflag, N))) return rc;
+    if ((rc = dalloc(c, &c->g, N * c->gstride))) return rc;
+    if (!table_mode(c) && (rc = dalloc(c, &c->gt, N * c->gstride))) return rc;
+    return AB_OK;
+}
+
+__global__ void k_fill_int(int *p, int n, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_dbl(double *p, int n, double v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_scatter_wall(int n, const int *__restrict__ idx, const double *__restrict__ src,
+                               int has_xyz, int has_wn, int has_wu, int has_mv, double *x,
+                               double *y, double *z, double *wnx, double *wny, double *wnz,
+                               double *wux, double *wuy, double *wuz, double *mvx, double *mvy,
+                               double *mvz)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int i = idx[k];
+    const double *s = src + (size_t)k * 12;
+    if (has_xyz) { x[i] = s[0]; y[i] = s[1]; z[i] = s[2]; }
+    if (has_wn) { wnx[i] = s[3]; wny[i] = s[4]; wnz[i] = s[5]; }
+    if (has_wu) { wux[i] = s[6]; wuy[i] = s[7]; wuz[i] = s[8]; }
+    if (has_mv) { mvx[i] = s[9]; mvy[i] = s[10]; mvz[i] = s[11]; }
+}
+
+int zero_step_flags(ab_ctx *c)
+{
+    const int nb = (c->N + 255) / 256;
+    k_fill_int<<<nb, 256, 0, c->stream>>>(c->mlsflag, c->N, 0);
+    k_fill_dbl<<<nb, 256, 0, c->stream>>>(c->mlsovr, c->N, 0.0);
+    k_fill_dbl<<<nb, 256, 0, c->stream>>>(c->limflag, c->N, 0.0);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+struct Timer
+{
+    ab_ctx *c;
+    explicit Timer(ab_ctx *ctx) : c(ctx) {}
+    void tic() { prof_begin(c); }
+    void toc(int st) { prof_end(c, st); }
+};
+
+int fp64_transport(ab_ctx *c, double dt)
+{
+    const double tol = c->cfg.mls_tol > 0 ? c->cfg.mls_tol : 1e-12;
+    DevCloud v = c->view();
+    launch_transport_fp64(v, c->g, c->gt, dt, c->cfg.radius, c->cfg.mls_order, tol,
+                          c->cfg.mls_clamp, c->mlsflag, c->mlsovr, c->ilist, c->n_int, false,
+                          c->stream);
+    launch_transport_fp64(v, c->g, c->gt, dt, c->cfg.radius, c->cfg.mls_order, tol,
+                          c->cfg.mls_clamp, c->mlsflag, c->mlsovr, c->blist, c->n_bnd, true,
+                          c->stream);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+int fp64_step(ab_ctx *c, double dt)
+{
+    DevCloud v = c->view();
+    hipStream_t s = c->stream;
+    Timer tm(c);
+    int rc;
+    tm.tic();
+    if (!c->mass_valid) launch_mass_partials(v, c->g, c->pmass, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_OLD, 0, s);
+    if ((rc = zero_step_flags(c))) return rc;
+    tm.toc(ST_REDUCE);
+    tm.tic();
+    if ((rc = fp64_transport(c, dt))) return rc;
+    tm.toc(ST_TRANSPORT);
+    tm.tic();
+    launch_limit_commit(v, c->g, c->gt, c->rho, c->limflag, c->pmass, s);
+    tm.toc(ST_LIMIT_COMMIT);
+    tm.tic();
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s);
+    tm.toc(ST_REDUCE);
+    tm.tic();
+    launch_wall_collide(v, c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux, c->uy, c->uz,
+                        c->T, c->pmass, s);
+    tm.toc(ST_WALL_COLLIDE);
+    tm.tic();
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, c->cfg.no_mass_fix, s);
+    tm.toc(ST_REDUCE);
+    tm.tic();
+    launch_final(v, c->g, c->scal, c->cfg.R, c->rho, c->ux, c->uy, c->uz, c->T, c->pmass, s);
+    tm.toc(ST_FINAL);
+    c->mass_valid = true;
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+int do_step(ab_ctx *c, double dt)
+{
+    if (!c->uploaded) { c->err = "ab_step before ab_upload"; return AB_ERR_STATE; }
+    if (!(dt > 0.0) || !std::isfinite(dt)) { c->err = "dt must be positive and finite"; return AB_ERR_ARG; }
+    if (table_mode(c))
+    {
+        int rc = table_step(c, dt);
+        if (rc && c->err.empty()) c->err = "table step failed";
+        return rc;
+    }
+    return fp64_step(c, dt);
+}
+
+}  // namespace
+
+namespace ab {
+
+constexpr int kEvPairs = 2048;
+
+void prof_flush(ab_ctx *c)
+{
+    if (c->nrec == 0) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (int k = 0; k < c->nrec; ++k)
+    {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->evpool[2 * k], c->evpool[2 * k + 1]) == hipSuccess)
+        {
+            c->prof_ms[c->evstage[k]] += ms;
+            c->prof_n[c->evstage[k]] += 1;
+        }
+    }
+    c->nrec = 0;
+}
+
+void prof_begin(ab_ctx *c)
+{
+    if (!c->prof) return;
+    if (c->evpool.empty())
+    {
+        c->evpool.resize(2 * kEvPairs, nullptr);
+        c->evstage.resize(kEvPairs, 0);
+        for (auto &e : c->evpool)
+            if (hipEventCreate(&e) != hipSuccess) { c->prof = false; return; }
+    }
+    if (c->nrec >= kEvPairs) prof_flush(c);
+    (void)hipEventRecord(c->evpool[2 * c->nrec], c->stream);
+}
+
+void prof_end(ab_ctx *c, int stage)
+{
+    if (!c->prof || c->evpool.empty()) return;
+    (void)hipEventRecord(c->evpool[2 * c->nrec + 1], c->stream);
+    c->evstage[c->nrec] = stage;
+    c->nrec++;
+}
+
+}  // namespace ab
+
+extern "C" {
+
+int ab_create(const ab_config *cfg, ab_ctx **out)
+{
+    if (!cfg || !out) return AB_ERR_ARG;
+    *out = nullptr;
+    if ((cfg->dim != 2 && cfg->dim != 3) || cfg->n_particles <= 0 || cfg->nv < 2 ||
+        cfg->nv > 48 || !(cfg->vmax > cfg->vmin) || !(cfg->R > 0) || !(cfg->tau > 0) ||
+        !(cfg->radius > 0) || (cfg->mls_order != 1 && cfg->mls_order != 2) ||
+        cfg->precision < AB_PREC_FP64 || cfg->precision > AB_PREC_FP64_TABLE)
+        return AB_ERR_ARG;
+    ab_ctx *c = new (std::nothrow) ab_ctx;
+    if (!c) return AB_ERR_NOMEM;
+    c->cfg = *cfg;
+    c->dim = cfg->dim;
+    c->Nv = cfg->nv;
+    c->nodes = (cfg->dim == 3) ? cfg->nv * cfg->nv * cfg->nv : cfg->nv * cfg->nv;
+    c->gstride = (cfg->dim == 3) ? c->nodes : 2 * c->nodes;
+    c->vlo = cfg->vmin;
+    c->dv = (cfg->vmax - cfg->vmin) / (cfg->nv - 1);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipMalloc((void **)&c->scal, sizeof(double) * S_COUNT) != hipSuccess ||
+        hipMemset(c->scal, 0, sizeof(double) * S_COUNT) != hipSuccess)
+    {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return AB_ERR_HIP;
+    }
+    *out = c;
+    return AB_OK;
+}
+
+int ab_destroy(ab_ctx *c)
+{
+    if (!c) return AB_ERR_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_cloud(c);
+    if (c->scal) (void)hipFree(c->scal);
+    for (auto e : c->evpool) if (e) (void)hipEventDestroy(e);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return AB_OK;
+}
+
+const char *ab_last_error(const ab_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int ab_upload(ab_ctx *c, const ab_cloud_view *v)
+{
+    if (!c || !v) return AB_ERR_ARG;
+    if (!v->x || !v->y || !v->z || !v->boundary || !v->wnx || !v->wny || !v->wnz || !v->wux ||
+        !v->wuy || !v->wuz || !v->wt || !v->rho || !v->ux || !v->uy || !v->uz || !v->t || !v->g)
+    { c->err = "ab_upload: a required array is NULL"; return AB_ERR_ARG; }
+    const int N = c->cfg.n_particles;
+    if (c->N != N) { free_cloud(c); c->N = N; }
+    c->N = N;
+    int rc;
+    if (!c->x && (rc = alloc_cloud(c))) { free_cloud(c); return rc; }
+    hipStream_t s = c->stream;
+    const size_t n8 = sizeof(double) * (size_t)N;
+    auto up = [&](double *dst, const double *src) -> bool {
+        if (src) return hipMemcpyAsync(dst, src, n8, hipMemcpyHostToDevice, s) == hipSuccess;
+        return hipMemsetAsync(dst, 0, n8, s) == hipSuccess;
+    };
+    bool ok = up(c->x, v->x) && up(c->y, v->y) && up(c->z, v->z) && up(c->wnx, v->wnx) &&
+              up(c->wny, v->wny) && up(c->wnz, v->wnz) && up(c->wux, v->wux) &&
+              up(c->wuy, v->wuy) && up(c->wuz, v->wuz) && up(c->wt, v->wt) &&
+              up(c->mvx, v->mvx) && up(c->mvy, v->mvy) && up(c->mvz, v->mvz) &&
+              up(c->rho, v->rho) && up(c->ux, v->ux) && up(c->uy, v->uy) && up(c->uz, v->uz) &&
+              up(c->T, v->t);
+    if (!ok) { c->err = "ab_upload: host-to-device copy failed"; return AB_ERR_HIP; }
+    HIPCHK(c, hipMemcpyAsync(c->bnd, v->boundary, N, hipMemcpyHostToDevice, s));
+    // interior / boundary lists
+    std::vector<int> il, bl;
+    il.reserve(N); bl.reserve(N);
+    for (int i = 0; i < N; ++i) (v->boundary[i] ? bl : il).push_back(i);
+    c->n_int = (int)il.size(); c->n_bnd = (int)bl.size();
+    if (c->n_int) HIPCHK(c, hipMemcpyAsync(c->ilist, il.data(), 4 * il.size(), hipMemcpyHostToDevice, s));
+    if (c->n_bnd) HIPCHK(c, hipMemcpyAsync(c->blist, bl.data(), 4 * bl.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->g, v->g, n8 * c->gstride, hipMemcpyHostToDevice, s));
+    if (v->neigh && v->nneigh)
+    {
+        HIPCHK(c, hipMemcpyAsync(c->neigh, v->neigh, sizeof(int) * (size_t)N * AB_MAX_NEIGH,
+                                 hipMemcpyHostToDevice, s));
+        HIPCHK(c, hipMemcpyAsync(c->nneigh, v->nneigh, sizeof(int) * (size_t)N,
+                                 hipMemcpyHostToDevice, s));
+        // host lists are borrowed: finish the copies before returning
+        HIPCHK(c, hipStreamSynchronize(s));
+    }
+    else
+    {
+        HIPCHK(c, hipStreamSynchronize(s));
+        c->uploaded = true;
+        if ((rc = ab_rebuild_neighbours(c))) { c->uploaded = false; return rc; }
+    }
+    c->uploaded = true;
+    c->mass_valid = false;
+    if (table_mode(c))
+    {
+        if ((rc = table_setup(c))) { c->uploaded = false; return rc; }
+        c->coef_dt = -1.0;
+        if ((rc = table_upload_g(c, c->g))) { c->err = "table upload failed"; return rc; }
+    }
+    HIPCHK(c, hipStreamSynchronize(s));
+    return AB_OK;
+}
+
+int ab_rebuild_neighbours(ab_ctx *c)
+{
+    if (!c) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "ab_rebuild_neighbours before ab_upload"; return AB_ERR_STATE; }
+    const ab_config &f = c->cfg;
+    const int nbw = (c->dim == 3) ? f.nbwbox : 1;
+    if (f.nbhbox <= 0 || f.nbvbox <= 0 || nbw <= 0 || f.hbox == 0 || f.vbox == 0 ||
+        (c->dim == 3 && f.wbox == 0))
+    { c->err = "neighbour rebuild needs the voxel grid in ab_config"; return AB_ERR_ARG; }
+    const int nvox = f.nbhbox * f.nbvbox * nbw;
+    if (nvox != c->nvox)
+    {
+        if (c->vcount) (void)hipFree(c->vcount);
+        if (c->vstart) (void)hipFree(c->vstart);
+        c->vcount = c->vstart = nullptr;
+        int rc;
+        if ((rc = dalloc(c, &c->vcount, (size_t)nvox)) || (rc = dalloc(c, &c->vstart, (size_t)nvox + 1)))
+            return rc;
+        c->nvox = nvox;
+    }
+    if (!c->vox)
+    {
+        int rc;
+        if ((rc = dalloc(c, &c->vox, (size_t)c->N)) || (rc = dalloc(c, &c->vlist, (size_t)c->N)))
+            return rc;
+    }
+    Timer tm(c);
+    tm.tic();
+    if (launch_neighbours(c->view(), f, c->vox, c->vcount, c->vstart, c->vlist, nvox, c->neigh,
+                          c->nneigh, c->stream))
+    { c->err = "neighbour rebuild launch failed"; return AB_ERR_HIP; }
+    tm.toc(ST_NEIGH);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_download_neighbours(ab_ctx *c, int *neigh, int *nneigh)
+{
+    if (!c) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    if (neigh) HIPCHK(c, hipMemcpyAsync(neigh, c->neigh, sizeof(int) * (size_t)c->N * AB_MAX_NEIGH,
+                                        hipMemcpyDeviceToHost, c->stream));
+    if (nneigh) HIPCHK(c, hipMemcpyAsync(nneigh, c->nneigh, sizeof(int) * (size_t)c->N,
+                                         hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_step(ab_ctx *c, double dt, ab_step_stats *st)
+{
+    if (!c) return AB_ERR_ARG;
+    int rc = do_step(c, dt);
+    if (rc || !st) return rc;
+    double sc[S_COUNT];
+    HIPCHK(c, hipMemcpyAsync(sc, c->scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> lim(c->N), ovr(c->N);
+    std::vector<int> fl(c->N);
+    HIPCHK(c, hipMemcpyAsync(lim.data(), c->limflag, 8 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+    if (!table_mode(c))
+    {
+        HIPCHK(c, hipMemcpyAsync(ovr.data(), c->mlsovr, 8 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(fl.data(), c->mlsflag, 4 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memset(st, 0, sizeof *st);
+    st->mass_old = table_mode(c) ? sc[S_MASS_START] : sc[S_MASS_OLD];
+    st->mass_after_transport = sc[S_MASS_NEW];
+    st->mass_after_collision = sc[S_MASS_COLL];
+    for (int i = 0; i < c->N; ++i)
+    {
+        int f = (int)(lim[i] + 0.5);
+        if (f & 1) st->n_positivity++;
+        if (f & 2) st->n_nad++;
+    }
+    if (table_mode(c))
+    {
+        // the tables fold degeneracy into the coefficients and do not track
+        // per-value extrapolation
+        st->n_mls_degenerate = st->n_mls_extrapolate = -1;
+        st->mls_ovr_sum = st->mls_ovr_max = -1.0;
+    }
+    else
+        for (int i = 0; i < c->N; ++i)
+        {
+            if (fl[i] & 1) st->n_mls_degenerate++;
+            if (fl[i] & 2) st->n_mls_extrapolate++;
+            st->mls_ovr_sum += ovr[i];
+            if (ovr[i] > st->mls_ovr_max) st->mls_ovr_max = ovr[i];
+        }
+    return AB_OK;
+}
+
+int ab_steps(ab_ctx *c, double dt, int nsteps)
+{
+    if (!c || nsteps < 0) return AB_ERR_ARG;
+    for (int k = 0; k < nsteps; ++k)
+    {
+        int rc = do_step(c, dt);
+        if (rc) return rc;
+    }
+    return AB_OK;
+}
+
+int ab_transport_only(ab_ctx *c, double dt)
+{
+    if (!c) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    if (table_mode(c)) { c->err = "ab_transport_only runs the AB_PREC_FP64 formulation"; return AB_ERR_UNSUPPORTED; }
+    int rc;
+    if ((rc = zero_step_flags(c))) return rc;
+    if ((rc = fp64_transport(c, dt))) { c->err = "transport launch failed"; return rc; }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_download_g(ab_ctx *c, double *g)
+{
+    if (!c || !g) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    if (table_mode(c))
+    {
+        int rc = table_download_g(c, c->g);
+        if (rc) { c->err = "table download failed"; return rc; }
+    }
+    HIPCHK(c, hipMemcpyAsync(g, c->g, sizeof(double) * (size_t)c->N * c->gstride,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_download_gt(ab_ctx *c, double *gt)
+{
+    if (!c || !gt) return AB_ERR_ARG;
+    if (!c->uploaded || table_mode(c)) { c->err = "gt is kept by the AB_PREC_FP64 formulation only"; return AB_ERR_STATE; }
+    HIPCHK(c, hipMemcpyAsync(gt, c->gt, sizeof(double) * (size_t)c->N * c->gstride,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_download_moments(ab_ctx *c, double *rho, double *ux, double *uy, double *uz, double *t,
+                        double *limflag)
+{
+    if (!c) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    const size_t n8 = sizeof(double) * (size_t)c->N;
+    double *dst[] = {rho, ux, uy, uz, t, limflag};
+    const double *src[] = {c->rho, c->ux, c->uy, c->uz, c->T, c->limflag};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k]) HIPCHK(c, hipMemcpyAsync(dst[k], src[k], n8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_total_mass(ab_ctx *c, double *mass)
+{
+    if (!c || !mass) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    double sc[S_COUNT];
+    if (table_mode(c))
+    {
+        // the step already leaves the current mass in S_MASS_OLD
+        HIPCHK(c, hipMemcpyAsync(sc, c->scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *mass = sc[S_MASS_OLD];
+        return AB_OK;
+    }
+    launch_mass_partials(c->view(), c->g, c->pmass, c->stream);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_OLD, 0, c->stream);
+    c->mass_valid = true;
+    HIPCHK(c, hipMemcpyAsync(sc, c->scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *mass = sc[S_MASS_OLD];
+    return AB_OK;
+}
+
+int ab_body_loads(ab_ctx *c, int nsurf, const int *surf_idx, const int *surf_body, int nbodies,
+                  const double *body_c, const double *body_area, const double *wu, double *out)
+{
+    if (!c || nsurf < 0 || nbodies <= 0 || !body_c || !body_area || !out ||
+        (nsurf > 0 && (!surf_idx || !surf_body)))
+        return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    for (int k = 0; k < nsurf; ++k)
+        if (surf_idx[k] < 0 || surf_idx[k] >= c->N || surf_body[k] < 0 || surf_body[k] >= nbodies)
+        { c->err = "ab_body_loads: surface index or body id out of range"; return AB_ERR_ARG; }
+    std::memset(out, 0, sizeof(double) * 6 * (size_t)nbodies);
+    if (nsurf == 0) return AB_OK;
+    if (nsurf > c->bmom_cap)
+    {
+        if (c->bmom) (void)hipFree(c->bmom);
+        if (c->bidx) (void)hipFree(c->bidx);
+        c->bmom = nullptr; c->bidx = nullptr; c->bmom_cap = 0;
+        int rc;
+        if ((rc = dalloc(c, &c->bmom, (size_t)nsurf * 10)) || (rc = dalloc(c, &c->bidx, (size_t)nsurf)))
+            return rc;
+        c->bmom_cap = nsurf;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->bidx, surf_idx, 4 * (size_t)nsurf, hipMemcpyHostToDevice, c->stream));
+    if (table_mode(c))
+        launch_body_moments(c->view(), nullptr, c->gs, c->cfg.precision == AB_PREC_MIXED, c->S,
+                            c->scal + S_LAZY, c->bidx, nsurf, c->bmom, c->stream);
+    else
+        launch_body_moments(c->view(), c->g, nullptr, 0, 0, nullptr, c->bidx, nsurf, c->bmom,
+                            c->stream);
+    std::vector<double> m((size_t)nsurf * 10);
+    HIPCHK(c, hipMemcpyAsync(m.data(), c->bmom, 8 * m.size(), hipMemcpyDeviceToHost, c->stream));
+    // positions, normals and (if not given) wall velocities of the surface
+    const size_t n8 = 8 * (size_t)c->N;
+    std::vector<double> px(c->N), py(c->N), pz(c->N), nx(c->N), ny(c->N), nz(c->N);
+    std::vector<double> ux, uy, uz;
+    const double *src[] = {c->x, c->y, c->z, c->wnx, c->wny, c->wnz};
+    double *dst[] = {px.data(), py.data(), pz.data(), nx.data(), ny.data(), nz.data()};
+    for (int k = 0; k < 6; ++k) HIPCHK(c, hipMemcpyAsync(dst[k], src[k], n8, hipMemcpyDeviceToHost, c->stream));
+    if (!wu)
+    {
+        ux.resize(c->N); uy.resize(c->N); uz.resize(c->N);
+        HIPCHK(c, hipMemcpyAsync(ux.data(), c->wux, n8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(uy.data(), c->wuy, n8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(uz.data(), c->wuz, n8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // computeBodyLoads accumulation (BodyDynamics.cpp:41-94) from the moments
+    for (int k = 0; k < nsurf; ++k)
+    {
+        const int i = surf_idx[k], b = surf_body[k];
+        const double *q = &m[(size_t)k * 10];
+        const double Ux = wu ? wu[3 * k] : ux[i], Uy = wu ? wu[3 * k + 1] : uy[i];
+        const double Uz = (c->dim == 3) ? (wu ? wu[3 * k + 2] : uz[i]) : 0.0;
+        const double area = body_area[b];
+        const double cdx = px[i] - body_c[3 * b], cdy = py[i] - body_c[3 * b + 1],
+                     cdz = pz[i] - body_c[3 * b + 2];
+        // P_ab = m_ab - U_a m_b - U_b m_a + U_a U_b m0
+        const double pxx = q[4] - 2.0 * Ux * q[1] + Ux * Ux * q[0];
+        const double pyy = q[5] - 2.0 * Uy * q[2] + Uy * Uy * q[0];
+        const double pxy = q[7] - Ux * q[2] - Uy * q[1] + Ux * Uy * q[0];
+        double *o = out + 6 * (size_t)b;
+        if (c->dim == 3)
+        {
+            const double pzz = q[6] - 2.0 * Uz * q[3] + Uz * Uz * q[0];
+            const double pxz = q[8] - Ux * q[3] - Uz * q[1] + Ux * Uz * q[0];
+            const double pyz = q[9] - Uy * q[3] - Uz * q[2] + Uy * Uz * q[0];
+            const double tx = -(nx[i] * pxx + ny[i] * pxy + nz[i] * pxz);
+            const double ty = -(nx[i] * pxy + ny[i] * pyy + nz[i] * pyz);
+            const double tz = -(nx[i] * pxz + ny[i] * pyz + nz[i] * pzz);
+            o[0] += area * tx; o[1] += area * ty; o[2] += area * tz;
+            o[3] += area * (cdy * tz - cdz * ty);
+            o[4] += area * (cdz * tx - cdx * tz);
+            o[5] += area * (cdx * ty - cdy * tx);
+        }
+        else
+        {
+            const double valX = -nx[i] * pxx - ny[i] * pxy;
+            const double valY = -nx[i] * pxy - ny[i] * pyy;
+            o[0] += area * valX;
+            o[1] += area * valY;
+            o[5] += area * (cdx * valY - cdy * valX);
+        }
+    }
+    return AB_OK;
+}
+
+int ab_set_wall_state(ab_ctx *c, int n, const int *idx, const double *x, const double *y,
+                      const double *z, const double *wn, const double *wu, const double *mv)
+{
+    if (!c || n < 0 || (n > 0 && !idx)) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    if (n == 0) return AB_OK;
+    const bool hx = x && y && z;
+    std::vector<double> buf((size_t)n * 12, 0.0);
+    for (int k = 0; k < n; ++k)
+    {
+        if (idx[k] < 0 || idx[k] >= c->N) { c->err = "ab_set_wall_state: index out of range"; return AB_ERR_ARG; }
+        double *s = &buf[(size_t)k * 12];
+        if (hx) { s[0] = x[k]; s[1] = y[k]; s[2] = z[k]; }
+        for (int a = 0; a < 3; ++a)
+        {
+            if (wn) s[3 + a] = wn[3 * k + a];
+            if (wu) s[6 + a] = wu[3 * k + a];
+            if (mv) s[9 + a] = mv[3 * k + a];
+        }
+    }
+    int *di = nullptr;
+    double *ds = nullptr;
+    int rc;
+    if ((rc = dalloc(c, &di, (size_t)n)) || (rc = dalloc(c, &ds, buf.size())))
+    { if (di) (void)hipFree(di); return rc; }
+    hipError_t e = hipMemcpyAsync(di, idx, 4 * (size_t)n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(ds, buf.data(), 8 * buf.size(), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+    {
+        k_scatter_wall<<<(n + 255) / 256, 256, 0, c->stream>>>(
+            n, di, ds, hx, wn != nullptr, wu != nullptr, mv != nullptr, c->x, c->y, c->z, c->wnx,
+            c->wny, c->wnz, c->wux, c->wuy, c->wuz, c->mvx, c->mvy, c->mvz);
+        e = hipStreamSynchronize(c->stream);
+    }
+    (void)hipFree(di);
+    (void)hipFree(ds);
+    if (e != hipSuccess) { c->err = std::string("ab_set_wall_state: ") + hipGetErrorString(e); return AB_ERR_HIP; }
+    // moved surfaces change the MLS stencils of the table pipelines
+    if (table_mode(c) && (hx || mv)) c->coef_dt = -1.0;
+    return AB_OK;
+}
+
+int ab_synchronize(ab_ctx *c)
+{
+    if (!c) return AB_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_profile(ab_ctx *c, int enable)
+{
+    if (!c) return AB_ERR_ARG;
+    prof_flush(c);
+    c->prof = enable != 0;
+    for (int k = 0; k < ST_COUNT; ++k) { c->prof_ms[k] = 0.0; c->prof_n[k] = 0; }
+    return AB_OK;
+}
+
+int ab_profile_read(ab_ctx *c, double *ms, long *counts, int *nstages)
+{
+    if (!c) return AB_ERR_ARG;
+    prof_flush(c);
+    if (nstages) *nstages = ST_COUNT;
+    for (int k = 0; k < ST_COUNT; ++k)
+    {
+        if (ms) ms[k] = c->prof_ms[k];
+        if (counts) counts[k] = c->prof_n[k];
+    }
+    return AB_OK;
+}
+
+const char *ab_stage_name(int k)
+{
+    static const char *names[ST_COUNT] = {"transport", "limit_commit", "wall_collide",
+                                          "final", "reduce", "neighbours"};
+    return (k >= 0 && k < ST_COUNT) ? names[k] : "";
+}
+
+}  // extern "C"

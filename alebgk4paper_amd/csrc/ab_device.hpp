@@ -1,0 +1,116 @@
+// ab_device.hpp -- device helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#ifndef AB_PI
+#define AB_PI 3.14159265358979323846
+#endif
+
+namespace ab {
+
+constexpr int kWave = 64;          // CDNA wavefront
+constexpr int kBlock = 256;        // 4 waves per workgroup for the per-particle kernels
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
+// XCD b % 8). Give every XCD a contiguous range of logical block ids so the
+// particles a XCD processes are spatially close and share its L2.
+__device__ __forceinline__ int xcd_remap(int b, int nblocks)
+{
+    int xcd = b & 7, slot = b >> 3;
+    int per = nblocks >> 3, rem = nblocks & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + slot;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Sum over a kBlock-thread block in a fixed order (wave tree, then the four
+// wave totals in wave order). Every thread gets the result. `sh` needs
+// kBlock/kWave slots per value; NV values are reduced together.
+template <int NV, typename T>
+__device__ __forceinline__ void block_sum(T (&v)[NV], T *sh)
+{
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k * (kBlock / kWave) + w] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+    {
+        T s = sh[k * (kBlock / kWave)];
+#pragma unroll
+        for (int q = 1; q < kBlock / kWave; ++q) s += sh[k * (kBlock / kWave) + q];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void block_minmax(T &mn, T &mx, T *sh)
+{
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    __syncthreads();
+    if (lane == 0) { sh[w] = mn; sh[kBlock / kWave + w] = mx; }
+    __syncthreads();
+    T a = sh[0], b = sh[kBlock / kWave];
+#pragma unroll
+    for (int q = 1; q < kBlock / kWave; ++q)
+    {
+        a = fmin(a, sh[q]);
+        b = fmax(b, sh[kBlock / kWave + q]);
+    }
+    mn = a; mx = b;
+    __syncthreads();
+}
+
+// maxwellian3D (MLSTransport/MLSBasis.hpp:112-122), FP64.
+__device__ __forceinline__ double maxwellian3d(double vx, double vy, double vz,
+                                               double ux, double uy, double uz,
+                                               double T, double rho, double R)
+{
+    double twoRT = 2.0 * R * T;
+    double e = -((vx - ux) * (vx - ux) + (vy - uy) * (vy - uy) + (vz - uz) * (vz - uz)) / twoRT;
+    double norm = pow(AB_PI * twoRT, 1.5);
+    return rho / norm * exp(e);
+}
+
+// maxwellianG1 (MLSTransport/MLSBasis.hpp:87-94), FP64.
+__device__ __forceinline__ double maxwellian_g1(double vx, double vy, double ux, double uy,
+                                                double T, double rho, double R)
+{
+    double twoRT = 2.0 * R * T;
+    double e = -((vx - ux) * (vx - ux) + (vy - uy) * (vy - uy)) / twoRT;
+    return rho / (AB_PI * twoRT) * exp(e);
+}
+
+}  // namespace ab

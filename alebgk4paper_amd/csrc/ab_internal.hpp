@@ -1,0 +1,173 @@
+// ab_internal.hpp -- the context behind the C ABI and the launchers each
+// kernel translation unit exports to ab_api.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/alebgk_dev.h"
+
+namespace ab {
+
+// Device view of one cloud: flat SoA arrays, mirroring the fields of the
+// reference's DeviceCloud (Solver/CudaBackend.cu:69-113).
+struct DevCloud
+{
+    int dim, N, Nv;
+    int nodes;        // velocity nodes of the mass-carrying layer: Nv^3 or Nv^2
+    int gstride;      // values per particle: Nv^3 or 2*Nv^2
+    double vlo, dv;
+    const double *x, *y, *z;
+    const unsigned char *bnd;
+    const double *wnx, *wny, *wnz, *wux, *wuy, *wuz, *wt;
+    const double *mvx, *mvy, *mvz;
+    const int *neigh, *nneigh;
+};
+
+// Device scalars of one step (slots of Ctx::scal).
+enum ScalarSlot
+{
+    S_MASS_OLD = 0,     // total mass at the start of the step
+    S_MASS_NEW,         // after transport, limiters and commit
+    S_SCALE1,           // massOld / massNew (1 when skipped)
+    S_MASS_COLL,        // after wall and collision
+    S_SCALE2,           // massOld / massColl (1 when skipped)
+    S_LAZY,             // table pipelines: factor the stored g still carries
+    S_MASS_START,       // mass at the start of the last completed step
+    S_COUNT = 8
+};
+
+enum ReduceMode { R_MASS_OLD = 0, R_MASS_NEW = 1, R_MASS_COLL = 2 };
+
+// Stages timed by ab_profile, in report order.
+enum Stage
+{
+    ST_TRANSPORT = 0,   // MLS transport (the dominant kernel)
+    ST_LIMIT_COMMIT,    // positivity + NAD + commit
+    ST_WALL_COLLIDE,    // mass rescale + diffuse wall + moments + collision
+    ST_FINAL,           // end-of-step rescale + moments
+    ST_REDUCE,          // global mass sums
+    ST_NEIGH,           // neighbour rebuild
+    ST_COUNT
+};
+
+}  // namespace ab
+
+struct ab_ctx
+{
+    ab_config cfg{};
+    int dim = 3, N = 0, Nv = 0, nodes = 0, gstride = 0;
+    double vlo = 0, dv = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool uploaded = false;
+
+    // cloud (device)
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    unsigned char *bnd = nullptr;
+    double *wnx = nullptr, *wny = nullptr, *wnz = nullptr;
+    double *wux = nullptr, *wuy = nullptr, *wuz = nullptr, *wt = nullptr;
+    double *mvx = nullptr, *mvy = nullptr, *mvz = nullptr;
+    double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *T = nullptr;
+    int *neigh = nullptr, *nneigh = nullptr;
+    int *ilist = nullptr, *blist = nullptr;   // interior / boundary particle lists
+    int n_int = 0, n_bnd = 0;
+
+    // FP64 reference-formulation state
+    double *g = nullptr, *gt = nullptr;
+    double *limflag = nullptr;
+    int *mlsflag = nullptr;
+    double *mlsovr = nullptr;
+    double *pmass = nullptr;       // per-particle partial sums
+    double *scal = nullptr;        // ab::ScalarSlot
+    bool mass_valid = false;       // pmass holds the current g's masses
+
+    // neighbour rebuild scratch
+    int *vox = nullptr, *vcount = nullptr, *vstart = nullptr, *vlist = nullptr;
+    int nvox = 0;
+
+    // table pipelines (AB_PREC_MIXED / AB_PREC_FP64_TABLE)
+    int S = 0;                     // padded per-particle stride of the tables
+    void *gs = nullptr;            // stored g, [N][S], FP32 or FP64
+    void *gts = nullptr;           // transported g, [N][S]
+    void *coef = nullptr;          // [ncls][81][S] MLS coefficients
+    int *cls = nullptr;            // representative particle of each class
+    int *lat = nullptr;            // lattice index (ix, iy, iz) per particle, 3*N
+    int ncls = 0;
+    double coef_dt = -1.0;         // dt the tables were built for
+    double *part = nullptr;        // per-particle partials of the table pipeline
+    int *bad = nullptr;            // stencil check result
+
+    // body loads scratch
+    double *bmom = nullptr;
+    int bmom_cap = 0;
+    int *bidx = nullptr;
+    double *bwu = nullptr;
+
+    // profiling: event pairs recorded around stages on the context's stream,
+    // resolved (synchronised and accumulated) only when the pool fills or the
+    // caller reads the totals, so the timed work is not serialised.
+    bool prof = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> evpool;   // 2 per recorded stage
+    std::vector<int> evstage;
+    int nrec = 0;
+    double prof_ms[ab::ST_COUNT] = {0};
+    long prof_n[ab::ST_COUNT] = {0};
+
+    ab::DevCloud view() const
+    {
+        ab::DevCloud c;
+        c.dim = dim; c.N = N; c.Nv = Nv; c.nodes = nodes; c.gstride = gstride;
+        c.vlo = vlo; c.dv = dv;
+        c.x = x; c.y = y; c.z = z; c.bnd = bnd;
+        c.wnx = wnx; c.wny = wny; c.wnz = wnz; c.wux = wux; c.wuy = wuy; c.wuz = wuz;
+        c.wt = wt; c.mvx = mvx; c.mvy = mvy; c.mvz = mvz;
+        c.neigh = neigh; c.nneigh = nneigh;
+        return c;
+    }
+};
+
+namespace ab {
+
+// ---- ab_api.hip: stage timing ------------------------------------------------
+void prof_begin(ab_ctx *c);
+void prof_end(ab_ctx *c, int stage);
+void prof_flush(ab_ctx *c);
+
+// ---- k_fp64.hip ------------------------------------------------------------
+void launch_transport_fp64(const DevCloud &c, const double *g, double *gt, double dt,
+                           double h, int order, double tol, int clamp, int *mlsflag,
+                           double *mlsovr, const int *list, int nlist, bool boundary,
+                           hipStream_t s);
+void launch_mass_partials(const DevCloud &c, const double *g, double *pmass, hipStream_t s);
+void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int nofix,
+                        hipStream_t s);
+void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
+                         double *limflag, double *pmass, hipStream_t s);
+void launch_wall_collide(const DevCloud &c, double *g, const double *scal, double dt,
+                         double tau, double R, double *rho, double *ux, double *uy,
+                         double *uz, double *T, double *pmass, hipStream_t s);
+void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
+                  double *ux, double *uy, double *uz, double *T, double *pmass,
+                  hipStream_t s);
+
+// ---- k_neigh.hip -------------------------------------------------------------
+int launch_neighbours(const DevCloud &c, const ab_config &cfg, int *vox, int *vcount,
+                      int *vstart, int *vlist, int nvox, int *neigh, int *nneigh,
+                      hipStream_t s);
+
+// ---- k_table.hip -------------------------------------------------------------
+int table_setup(ab_ctx *ctx);
+int table_upload_g(ab_ctx *ctx, const double *g_host);
+int table_download_g(ab_ctx *ctx, double *g_host);
+int table_step(ab_ctx *ctx, double dt);
+void table_free(ab_ctx *ctx);
+
+// ---- k_body.hip --------------------------------------------------------------
+void launch_body_moments(const DevCloud &c, const double *g_fp64, const void *g_table,
+                         int table_fp32, int S, const double *lazy, const int *idx, int n,
+                         double *mom, hipStream_t s);
+
+}  // namespace ab

@@ -1,0 +1,758 @@
+// k_fp64.hip -- the reference formulation of the ALE-BGK step in FP64 on gfx950.
+//
+// Transport: one thread per (particle, velocity node), coalesced over the
+// node index; a workgroup covers 256 nodes of one particle and stages that
+// particle's neighbour positions in LDS (every lane reads the same entry: an
+// LDS broadcast). Interior and boundary particles are launched from compact
+// lists, each with its compile-time basis size, instead of sweeping the whole
+// grid twice with early exits as the reference does (CudaBackend.cu:1667-1677).
+//
+// Per-particle stages: one 256-thread workgroup per particle, coalesced over
+// the node index, with fixed-order workgroup reductions. They fuse reference
+// kernels that touch only their own particle:
+//   limit_commit  kPositivity3D + kNAD3D + kCommit  (+ mass partial)
+//   wall_collide  kScaleG + kWallBC3D + kMoments3D + kCollisionNode3D
+//   final         kScaleG + kMoments3D
+// and the global sums stay on the device (no host round trip per rescale).
+#include "ab_device.hpp"
+#include "ab_internal.hpp"
+
+namespace ab {
+
+namespace {
+
+__device__ __forceinline__ void node_velocity(const DevCloud &c, int lin, double &vx,
+                                              double &vy, double &vz)
+{
+    int Nv = c.Nv;
+    int iv = lin % Nv, jv = (lin / Nv) % Nv, kv = lin / (Nv * Nv);
+    vx = c.vlo + iv * c.dv;
+    vy = c.vlo + jv * c.dv;
+    vz = (c.dim == 3) ? c.vlo + kv * c.dv : 0.0;
+}
+
+// kCopyGt + kTransportNode3DT<P, BND> (Solver/CudaBackend.cu:191-197, 699-869).
+template <int P, bool BND>
+__global__ __launch_bounds__(kBlock) void k_transport3d(DevCloud c, const double *__restrict__ g,
+                                                        double *__restrict__ gt, double dt,
+                                                        double h, double tol, int clamp,
+                                                        int *mlsflag, double *mlsovr,
+                                                        const int *__restrict__ list,
+                                                        int nchunk)
+{
+    __shared__ double sx[AB_MAX_NEIGH], sy[AB_MAX_NEIGH], sz[AB_MAX_NEIGH];
+    __shared__ int sj[AB_MAX_NEIGH];
+    __shared__ double sred[2 * (kBlock / kWave)];
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int i = list[b / nchunk];
+    const int lin = (b % nchunk) * kBlock + threadIdx.x;
+    const int nn = c.nneigh[i];
+    for (int t = threadIdx.x; t < nn; t += kBlock)
+    {
+        int j = c.neigh[(size_t)i * AB_MAX_NEIGH + t];
+        sj[t] = j; sx[t] = c.x[j]; sy[t] = c.y[j]; sz[t] = c.z[j];
+    }
+    __syncthreads();
+
+    const size_t gs = (size_t)c.gstride;
+    double excess = 0.0;
+    int flag = 0;
+    if (lin < c.nodes)
+    {
+        double vx, vy, vz;
+        node_velocity(c, lin, vx, vy, vz);
+        const double g_self = g[(size_t)i * gs + lin];
+        double out = g_self;
+        bool gated = false;
+        if (BND)
+        {
+            double cn = (vx - c.wux[i]) * c.wnx[i] + (vy - c.wuy[i]) * c.wny[i]
+                      + (vz - c.wuz[i]) * c.wnz[i];
+            gated = (cn >= 0.0);
+        }
+        if (!gated)
+        {
+            const double xf = c.x[i] - (vx - c.mvx[i]) * dt;
+            const double yf = c.y[i] - (vy - c.mvy[i]) * dt;
+            const double zf = c.z[i] - (vz - c.mvz[i]) * dt;
+            const double hh = h * h;
+            double A[P * P], b1[P];
+#pragma unroll
+            for (int q = 0; q < P * P; ++q) A[q] = 0.0;
+#pragma unroll
+            for (int q = 0; q < P; ++q) b1[q] = 0.0;
+            bool has = false;
+            double sumW = 0.0, sumWG = 0.0, gmin = 1e300, gmax = -1e300;
+            for (int m = 0; m < nn; ++m)
+            {
+                const double dxn = sx[m] - xf, dyn = sy[m] - yf, dzn = sz[m] - zf;
+                const double r2 = dxn * dxn + dyn * dyn + dzn * dzn;
+                const double q2 = r2 / hh;                  // mlsWeightSq
+                double w = 0.0;
+                if (q2 < 1.0) { double t1 = 1.0 - q2; w = t1 * t1; }
+                if (w < 1e-30) continue;
+                double basis[P];
+                const double ssx = dxn / h, ssy = dyn / h, ssz = dzn / h;
+                basis[0] = 1.0; basis[1] = ssx; basis[2] = ssy; basis[3] = ssz;
+                if constexpr (P >= 10)
+                {
+                    basis[4] = ssx * ssx; basis[5] = ssy * ssy; basis[6] = ssz * ssz;
+                    basis[7] = ssx * ssy; basis[8] = ssx * ssz; basis[9] = ssy * ssz;
+                }
+                const double gj = g[(size_t)sj[m] * gs + lin];
+#pragma unroll
+                for (int p1 = 0; p1 < P; ++p1)
+                {
+                    const double wb = w * basis[p1];
+#pragma unroll
+                    for (int p2 = p1; p2 < P; ++p2) A[p1 * P + p2] += wb * basis[p2];
+                    b1[p1] += wb * gj;
+                }
+                sumW += w;
+                sumWG += w * gj;
+                gmin = fmin(gmin, gj);
+                gmax = fmax(gmax, gj);
+                has = true;
+            }
+            if (has)
+            {
+                double dmax = 0.0;
+#pragma unroll
+                for (int k = 0; k < P; ++k) dmax = fmax(dmax, A[k * P + k]);
+                bool degenerate = (dmax <= 0.0);
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                {
+                    double d = A[k * P + k];
+#pragma unroll
+                    for (int j = 0; j < k; ++j) { double l = A[j * P + k]; d -= l * l; }
+                    if (!(d > tol * dmax)) degenerate = true;
+                    d = degenerate ? 1.0 : sqrt(d);
+                    A[k * P + k] = d;
+#pragma unroll
+                    for (int m2 = k + 1; m2 < P; ++m2)
+                    {
+                        double v = A[k * P + m2];
+#pragma unroll
+                        for (int j = 0; j < k; ++j) v -= A[j * P + k] * A[j * P + m2];
+                        A[k * P + m2] = v / d;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < P; ++k)
+                {
+                    double v = b1[k];
+#pragma unroll
+                    for (int j = 0; j < k; ++j) v -= A[j * P + k] * b1[j];
+                    b1[k] = v / A[k * P + k];
+                }
+#pragma unroll
+                for (int k = P - 1; k >= 0; --k)
+                {
+                    double v = b1[k];
+#pragma unroll
+                    for (int j = k + 1; j < P; ++j) v -= A[k * P + j] * b1[j];
+                    b1[k] = v / A[k * P + k];
+                }
+                out = b1[0];
+                if (degenerate || !isfinite(out)) out = (sumW > 0.0) ? (sumWG / sumW) : g_self;
+                if (degenerate || !isfinite(b1[0])) flag |= 1;
+                excess = fmax(fmax(out - gmax, gmin - out), 0.0);
+                bool overshoot = (out < gmin) || (out > gmax);
+                if (overshoot) flag |= 2;
+                if (overshoot && clamp) out = fmin(fmax(out, gmin), gmax);
+            }
+        }
+        gt[(size_t)i * gs + lin] = out;
+    }
+    // One flag word and one overshoot sum per workgroup.
+    double red[1] = {excess};
+    block_sum<1>(red, sred);
+    int f = flag;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) f |= __shfl_xor(f, o, kWave);
+    __shared__ int sflag[kBlock / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) sflag[threadIdx.x / kWave] = f;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int fa = 0;
+        for (int q = 0; q < kBlock / kWave; ++q) fa |= sflag[q];
+        if (fa && mlsflag) atomicOr(&mlsflag[i], fa);
+        if (red[0] > 0.0 && mlsovr)
+        {
+            double dv3 = c.dv * c.dv * c.dv;
+            atomicAdd(&mlsovr[i], red[0] * dv3);
+        }
+    }
+}
+
+// kCopyGt + kTransportNode, 2-D Chu layers (Solver/CudaBackend.cu:199-300):
+// weight through sqrt + mlsWeight, Gauss with partial pivoting on both layers.
+// Row swaps compare unrolled row indices against the pivot so A stays in
+// registers.
+template <int P, bool BND>
+__global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double *__restrict__ g,
+                                                        double *__restrict__ gt, double dt,
+                                                        double h, const int *__restrict__ list,
+                                                        int nchunk)
+{
+    __shared__ double sx[AB_MAX_NEIGH], sy[AB_MAX_NEIGH];
+    __shared__ int sj[AB_MAX_NEIGH];
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int i = list[b / nchunk];
+    const int lin = (b % nchunk) * kBlock + threadIdx.x;
+    const int nn = c.nneigh[i];
+    for (int t = threadIdx.x; t < nn; t += kBlock)
+    {
+        int j = c.neigh[(size_t)i * AB_MAX_NEIGH + t];
+        sj[t] = j; sx[t] = c.x[j]; sy[t] = c.y[j];
+    }
+    __syncthreads();
+    if (lin >= c.nodes) return;
+    const size_t gs = (size_t)c.gstride;
+    const int Nv2 = c.nodes;
+    const double vx = c.vlo + (lin % c.Nv) * c.dv;
+    const double vy = c.vlo + (lin / c.Nv) * c.dv;
+    double o1 = g[(size_t)i * gs + lin], o2 = g[(size_t)i * gs + lin + Nv2];
+    bool gated = false;
+    if (BND)
+    {
+        double cn = (vx - c.wux[i]) * c.wnx[i] + (vy - c.wuy[i]) * c.wny[i];
+        gated = (cn >= 0.0);
+    }
+    if (!gated)
+    {
+        const double xf = c.x[i] - (vx - c.mvx[i]) * dt;
+        const double yf = c.y[i] - (vy - c.mvy[i]) * dt;
+        double A[P * P], bb1[P], bb2[P];
+#pragma unroll
+        for (int q = 0; q < P * P; ++q) A[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < P; ++q) { bb1[q] = 0.0; bb2[q] = 0.0; }
+        bool has = false;
+        for (int m = 0; m < nn; ++m)
+        {
+            const double dxn = sx[m] - xf, dyn = sy[m] - yf;
+            const double r = sqrt(dxn * dxn + dyn * dyn);
+            double w = 0.0;                                   // mlsWeight
+            if (r < h) { double qq = r / h; double t1 = 1.0 - qq * qq; w = t1 * t1; }
+            if (w < 1e-30) continue;
+            double basis[P];
+            const double ssx = dxn / h, ssy = dyn / h;
+            basis[0] = 1.0; basis[1] = ssx; basis[2] = ssy;
+            if constexpr (P >= 6) { basis[3] = ssx * ssx; basis[4] = ssx * ssy; basis[5] = ssy * ssy; }
+#pragma unroll
+            for (int p1 = 0; p1 < P; ++p1)
+#pragma unroll
+                for (int p2 = 0; p2 < P; ++p2) A[p1 * P + p2] += w * basis[p1] * basis[p2];
+            const double *gj = g + (size_t)sj[m] * gs;
+            const double gn1 = gj[lin], gn2 = gj[lin + Nv2];
+#pragma unroll
+            for (int q = 0; q < P; ++q)
+            {
+                bb1[q] += w * basis[q] * gn1;
+                bb2[q] += w * basis[q] * gn2;
+            }
+            has = true;
+        }
+        if (has)
+        {
+#pragma unroll
+            for (int col = 0; col < P; ++col)
+            {
+                int piv = col;
+                double mx = fabs(A[col * P + col]);
+#pragma unroll
+                for (int row = col + 1; row < P; ++row)
+                    if (fabs(A[row * P + col]) > mx) { mx = fabs(A[row * P + col]); piv = row; }
+#pragma unroll
+                for (int row = col + 1; row < P; ++row)
+                {
+                    if (row == piv)
+                    {
+#pragma unroll
+                        for (int q = 0; q < P; ++q)
+                        {
+                            double t1 = A[col * P + q];
+                            A[col * P + q] = A[row * P + q];
+                            A[row * P + q] = t1;
+                        }
+                        double tb = bb1[col]; bb1[col] = bb1[row]; bb1[row] = tb;
+                        tb = bb2[col]; bb2[col] = bb2[row]; bb2[row] = tb;
+                    }
+                }
+                const double d = A[col * P + col];
+                if (fabs(d) < 1e-30) continue;
+#pragma unroll
+                for (int row = col + 1; row < P; ++row)
+                {
+                    const double f = A[row * P + col] / d;
+#pragma unroll
+                    for (int q = col; q < P; ++q) A[row * P + q] -= f * A[col * P + q];
+                    bb1[row] -= f * bb1[col];
+                    bb2[row] -= f * bb2[col];
+                }
+            }
+            double s1[P], s2[P];
+#pragma unroll
+            for (int row = P - 1; row >= 0; --row)
+            {
+                double v1 = bb1[row], v2 = bb2[row];
+#pragma unroll
+                for (int q = row + 1; q < P; ++q)
+                {
+                    v1 -= A[row * P + q] * s1[q];
+                    v2 -= A[row * P + q] * s2[q];
+                }
+                const double d = A[row * P + row];
+                s1[row] = fabs(d) > 1e-30 ? v1 / d : 0.0;
+                s2[row] = fabs(d) > 1e-30 ? v2 / d : 0.0;
+            }
+            o1 = s1[0];
+            o2 = s2[0];
+        }
+    }
+    gt[(size_t)i * gs + lin] = o1;
+    gt[(size_t)i * gs + lin + Nv2] = o2;
+}
+
+// Per-particle mass partial: sum of g*dv^d over the mass-carrying layer
+// (kSumMass3D / kSumMass per-thread part, CudaBackend.cu:391-436).
+__global__ __launch_bounds__(kBlock) void k_mass_partials(DevCloud c, const double *__restrict__ g,
+                                                          double *__restrict__ pmass)
+{
+    __shared__ double sh[kBlock / kWave];
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    const double dvd = (c.dim == 3) ? c.dv * c.dv * c.dv : c.dv * c.dv;
+    const double *p = g + (size_t)i * c.gstride;
+    double m[1] = {0.0};
+    for (int k = threadIdx.x; k < c.nodes; k += kBlock) m[0] += p[k] * dvd;
+    block_sum<1>(m, sh);
+    if (threadIdx.x == 0) pmass[i] = m[0];
+}
+
+// Fixed-order sum of the per-particle partials and the rescale factor
+// (sumMassHost + the scalar logic of cudaStep :1627, :1706-1710, :1773-1780),
+// kept on the device so no host synchronisation is needed.
+__global__ __launch_bounds__(1024) void k_reduce_mass(int N, const double *__restrict__ pmass,
+                                                      double *scal, int mode, int nofix)
+{
+    __shared__ double sh[1024 / kWave];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < N; i += 1024) s += pmass[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0.0;
+        for (int q = 0; q < 1024 / kWave; ++q) t += sh[q];
+        if (mode == R_MASS_OLD)
+            scal[S_MASS_OLD] = t;
+        else if (mode == R_MASS_NEW)
+        {
+            scal[S_MASS_NEW] = t;
+            scal[S_SCALE1] = (t > 1e-30) ? scal[S_MASS_OLD] / t : 1.0;
+        }
+        else
+        {
+            scal[S_MASS_COLL] = t;
+            scal[S_SCALE2] = (!nofix && t > 1e-30) ? scal[S_MASS_OLD] / t : 1.0;
+        }
+    }
+}
+
+// kPositivity3D + kNAD3D + kCommit (CudaBackend.cu:882-954, 328-336) for 3-D;
+// kPositivity + kCommit (:304-336) for 2-D. Writes the committed g, the
+// limiter bits and the particle's mass.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__restrict__ g,
+                                                         const double *__restrict__ gt,
+                                                         const double *__restrict__ rho,
+                                                         double *__restrict__ limflag,
+                                                         double *__restrict__ pmass)
+{
+    __shared__ double sh[4 * (kBlock / kWave)];
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    const size_t gs = (size_t)c.gstride;
+    double *gp = g + (size_t)i * gs;
+    const double *tp = gt + (size_t)i * gs;
+    const int nodes = c.nodes;
+    double flag = 0.0;
+    double sc = 1.0, sc2 = 1.0;
+    bool fires = false, nad = false;
+
+    if constexpr (DIM == 3)
+    {
+        const double dv3 = c.dv * c.dv * c.dv;
+        double r[3] = {0.0, 0.0, 0.0};             // mPos, mNeg, mFallback
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double v = tp[k];
+            if (v >= 0) r[0] += v;
+            else { r[1] += v; r[2] += gp[k]; }
+        }
+        block_sum<3>(r, sh);
+        fires = (r[1] < -1e-30 && (r[0] + r[1]) > 0);
+        if (fires)
+        {
+            flag += 1.0;
+            double target = r[0] + r[1], replaced = r[0] + r[2];
+            sc = (replaced > 1e-30) ? target / replaced : 0.0;
+        }
+        if (!c.bnd[i])
+        {
+            double rn[1] = {0.0};
+            for (int k = threadIdx.x; k < nodes; k += kBlock)
+            {
+                double v = tp[k];
+                if (fires) v = ((v < 0.0) ? gp[k] : v) * sc;
+                rn[0] += v;
+            }
+            block_sum<1>(rn, sh);
+            const double rho_new = rn[0] * dv3;
+            const int nn = c.nneigh[i];
+            double rmin = 1e300, rmax = -1e300;
+            for (int k = threadIdx.x; k < nn; k += kBlock)
+            {
+                double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
+                rmin = fmin(rmin, q);
+                rmax = fmax(rmax, q);
+            }
+            block_minmax(rmin, rmax, sh);
+            if (!(rmin > rmax))
+            {
+                double slack = 0.1 * fmax(rmax - rmin, 1e-12) + 1e-9;
+                double lo = rmin - slack, hi = rmax + slack;
+                if (rho_new > hi && rho_new > 1e-30) { nad = true; sc2 = hi / rho_new; }
+                else if (rho_new < lo && rho_new > 1e-30) { nad = true; sc2 = lo / rho_new; }
+                if (nad) flag += 2.0;
+            }
+        }
+        double m[1] = {0.0};
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double v = tp[k];
+            if (fires) v = ((v < 0.0) ? gp[k] : v) * sc;
+            if (nad) v *= sc2;
+            gp[k] = v;
+            m[0] += v * dv3;
+        }
+        block_sum<1>(m, sh);
+        if (threadIdx.x == 0) { pmass[i] = m[0]; limflag[i] = flag; }
+    }
+    else
+    {
+        double r[2] = {0.0, 0.0};                  // mPos, mNeg of g1
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double v = tp[k];
+            if (v >= 0) r[0] += v; else r[1] += v;
+        }
+        block_sum<2>(r, sh);
+        fires = (r[1] < -1e-30 && r[0] > 1e-30 && (r[0] + r[1]) > 0);
+        if (fires) { flag += 1.0; sc = (r[0] + r[1]) / r[0]; }
+        const double dv2 = c.dv * c.dv;
+        double m[1] = {0.0};
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double v1 = tp[k], v2 = tp[k + nodes];
+            if (fires)
+            {
+                if (v1 < 0) { v1 = 0; v2 = 0; }
+                else { v1 *= sc; v2 *= sc; }
+            }
+            gp[k] = v1;
+            gp[k + nodes] = v2;
+            m[0] += v1 * dv2;
+        }
+        block_sum<1>(m, sh);
+        if (threadIdx.x == 0) { pmass[i] = m[0]; limflag[i] = flag; }
+    }
+}
+
+// kScaleG (massOld/massNew) + kWallBC3D / kMoments3D + kCollisionNode3D
+// (CudaBackend.cu:478-485, 956-1030, 532-569, 1181-1197) and their 2-D
+// twins (:338-389, 124-158, 1165-1179). The scale is applied as the value is
+// read, which rounds exactly as the stored product would.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__restrict__ g,
+                                                         const double *__restrict__ scal,
+                                                         double dt, double tau, double R,
+                                                         double *rho, double *ux, double *uy,
+                                                         double *uz, double *T,
+                                                         double *__restrict__ pmass)
+{
+    __shared__ double sh[5 * (kBlock / kWave)];
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    const double s1 = scal[S_SCALE1];
+    const bool doScale = scal[S_MASS_NEW] > 1e-30;
+    const size_t gs = (size_t)c.gstride;
+    double *gp = g + (size_t)i * gs;
+    const int nodes = c.nodes, Nv = c.Nv;
+    const double dvd = (DIM == 3) ? c.dv * c.dv * c.dv : c.dv * c.dv;
+    double m[1] = {0.0};
+
+    if (c.bnd[i])
+    {
+        const double nx = c.wnx[i], ny = c.wny[i], nz = (DIM == 3) ? c.wnz[i] : 0.0;
+        const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = (DIM == 3) ? c.wuz[i] : 0.0;
+        const double Tw = c.wt[i];
+        const double RT2w = 2.0 * R * Tw;
+        const double norm = (DIM == 3) ? pow(AB_PI * RT2w, 1.5) : (AB_PI * RT2w);
+        double r[2] = {0.0, 0.0};                  // intM, intP
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv;
+            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vz = (DIM == 3) ? c.vlo + (k / (Nv * Nv)) * c.dv : 0.0;
+            double v = doScale ? gp[k] * s1 : gp[k];
+            double cn = (DIM == 3) ? (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz
+                                   : (vx - Uwx) * nx + (vy - Uwy) * ny;
+            if (cn < 0.0) r[0] += cn * v * dvd;
+            else if (cn > 0.0)
+            {
+                double e2 = (DIM == 3) ? (vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
+                                             + (vz - Uwz) * (vz - Uwz)
+                                       : (vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy);
+                double Mw = exp(-(e2) / RT2w) / norm;
+                r[1] += cn * Mw * dvd;
+            }
+        }
+        block_sum<2>(r, sh);
+        const double rho_w = (r[1] > 1e-30) ? (-r[0] / r[1]) : 1.0;
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv;
+            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vz = (DIM == 3) ? c.vlo + (k / (Nv * Nv)) * c.dv : 0.0;
+            double v = doScale ? gp[k] * s1 : gp[k];
+            double v2 = 0.0;
+            if constexpr (DIM == 2) v2 = doScale ? gp[k + nodes] * s1 : gp[k + nodes];
+            double cn = (DIM == 3) ? (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz
+                                   : (vx - Uwx) * nx + (vy - Uwy) * ny;
+            if (cn > 0.0)
+            {
+                double e2 = (DIM == 3) ? (vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
+                                             + (vz - Uwz) * (vz - Uwz)
+                                       : (vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy);
+                v = rho_w * exp(-(e2) / RT2w) / norm;
+                if constexpr (DIM == 2) v2 = 0.5 * R * Tw * v;
+            }
+            gp[k] = v;
+            if constexpr (DIM == 2) gp[k + nodes] = v2;
+            m[0] += v * dvd;
+        }
+        if (threadIdx.x == 0)
+        {
+            rho[i] = rho_w; ux[i] = Uwx; uy[i] = Uwy; T[i] = Tw;
+            if (DIM == 3) uz[i] = Uwz;
+        }
+    }
+    else
+    {
+        double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv;
+            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double f = doScale ? gp[k] * s1 : gp[k];
+            if constexpr (DIM == 3)
+            {
+                double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+                r[0] += f * dvd;
+                r[1] += vx * f * dvd;
+                r[2] += vy * f * dvd;
+                r[3] += vz * f * dvd;
+                r[4] += (vx * vx + vy * vy + vz * vz) * f * dvd;
+            }
+            else
+            {
+                double f2 = doScale ? gp[k + nodes] * s1 : gp[k + nodes];
+                r[0] += f * dvd;
+                r[1] += vx * f * dvd;
+                r[2] += vy * f * dvd;
+                r[4] += (vx * vx + vy * vy) * f * dvd + 2.0 * f2 * dvd;
+            }
+        }
+        block_sum<5>(r, sh);
+        double mrho = rho[i], mux = ux[i], muy = uy[i], muz = (DIM == 3) ? uz[i] : 0.0, mT = T[i];
+        if (r[0] > 1e-30)
+        {
+            mrho = r[0];
+            mux = r[1] / r[0];
+            muy = r[2] / r[0];
+            double u2;
+            if constexpr (DIM == 3) { muz = r[3] / r[0]; u2 = mux * mux + muy * muy + muz * muz; }
+            else u2 = mux * mux + muy * muy;
+            double t1 = (r[4] / r[0] - u2) / (3.0 * R);
+            mT = t1 > 1e-6 ? t1 : 1e-6;
+        }
+        const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
+        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv;
+            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double f = doScale ? gp[k] * s1 : gp[k];
+            if constexpr (DIM == 3)
+            {
+                double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+                double M = maxwellian3d(vx, vy, vz, mux, muy, muz, mT, mrho, R);
+                f = coeff * (f + lam * M);
+            }
+            else
+            {
+                double f2 = doScale ? gp[k + nodes] * s1 : gp[k + nodes];
+                double Mg1 = maxwellian_g1(vx, vy, mux, muy, mT, mrho, R);
+                f = coeff * (f + lam * Mg1);
+                f2 = coeff * (f2 + lam * 0.5 * R * mT * Mg1);
+                gp[k + nodes] = f2;
+            }
+            gp[k] = f;
+            m[0] += f * dvd;
+        }
+        if (threadIdx.x == 0)
+        {
+            rho[i] = mrho; ux[i] = mux; uy[i] = muy; T[i] = mT;
+            if (DIM == 3) uz[i] = muz;
+        }
+    }
+    block_sum<1>(m, sh);
+    if (threadIdx.x == 0) pmass[i] = m[0];
+}
+
+// kScaleG (massOld/massColl) + kMoments3D / kMoments (CudaBackend.cu:1773-1788).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict__ g,
+                                                  const double *__restrict__ scal, double R,
+                                                  double *rho, double *ux, double *uy,
+                                                  double *uz, double *T,
+                                                  double *__restrict__ pmass)
+{
+    __shared__ double sh[5 * (kBlock / kWave)];
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    const double s2 = scal[S_SCALE2];
+    const bool doScale = (s2 != 1.0);
+    const size_t gs = (size_t)c.gstride;
+    double *gp = g + (size_t)i * gs;
+    const int nodes = c.nodes, Nv = c.Nv;
+    const double dvd = (DIM == 3) ? c.dv * c.dv * c.dv : c.dv * c.dv;
+    const bool inner = !c.bnd[i];
+    double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // rho, rux, ruy, ruz, E, mass
+    for (int k = threadIdx.x; k < nodes; k += kBlock)
+    {
+        double f = gp[k];
+        double f2 = 0.0;
+        if constexpr (DIM == 2) f2 = gp[k + nodes];
+        if (doScale)
+        {
+            f *= s2; gp[k] = f;
+            if constexpr (DIM == 2) { f2 *= s2; gp[k + nodes] = f2; }
+        }
+        r[5] += f * dvd;
+        if (inner)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv;
+            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            r[0] += f * dvd;
+            r[1] += vx * f * dvd;
+            r[2] += vy * f * dvd;
+            if constexpr (DIM == 3)
+            {
+                double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+                r[3] += vz * f * dvd;
+                r[4] += (vx * vx + vy * vy + vz * vz) * f * dvd;
+            }
+            else
+                r[4] += (vx * vx + vy * vy) * f * dvd + 2.0 * f2 * dvd;
+        }
+    }
+    block_sum<6>(r, sh);
+    if (threadIdx.x == 0)
+    {
+        pmass[i] = r[5];
+        if (inner && r[0] > 1e-30)
+        {
+            double mux = r[1] / r[0], muy = r[2] / r[0], muz = 0.0, u2;
+            if constexpr (DIM == 3) { muz = r[3] / r[0]; u2 = mux * mux + muy * muy + muz * muz; }
+            else u2 = mux * mux + muy * muy;
+            double t1 = (r[4] / r[0] - u2) / (3.0 * R);
+            rho[i] = r[0]; ux[i] = mux; uy[i] = muy; T[i] = t1 > 1e-6 ? t1 : 1e-6;
+            if (DIM == 3) uz[i] = muz;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_transport_fp64(const DevCloud &c, const double *g, double *gt, double dt, double h,
+                           int order, double tol, int clamp, int *mlsflag, double *mlsovr,
+                           const int *list, int nlist, bool boundary, hipStream_t s)
+{
+    if (nlist <= 0) return;
+    const int nchunk = (c.nodes + kBlock - 1) / kBlock;
+    const int nb = nlist * nchunk;
+    if (c.dim == 3)
+    {
+        if (boundary)
+            k_transport3d<4, true><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, tol, clamp, mlsflag,
+                                                         mlsovr, list, nchunk);
+        else if (order >= 2)
+            k_transport3d<10, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, tol, clamp, mlsflag,
+                                                           mlsovr, list, nchunk);
+        else
+            k_transport3d<4, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, tol, clamp, mlsflag,
+                                                          mlsovr, list, nchunk);
+    }
+    else
+    {
+        if (boundary)
+            k_transport2d<3, true><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk);
+        else if (order >= 2)
+            k_transport2d<6, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk);
+        else
+            k_transport2d<3, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk);
+    }
+}
+
+void launch_mass_partials(const DevCloud &c, const double *g, double *pmass, hipStream_t s)
+{
+    k_mass_partials<<<c.N, kBlock, 0, s>>>(c, g, pmass);
+}
+
+void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int nofix,
+                        hipStream_t s)
+{
+    k_reduce_mass<<<1, 1024, 0, s>>>(N, pmass, scal, mode, nofix);
+}
+
+void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
+                         double *limflag, double *pmass, hipStream_t s)
+{
+    if (c.dim == 3) k_limit_commit<3><<<c.N, kBlock, 0, s>>>(c, g, gt, rho, limflag, pmass);
+    else k_limit_commit<2><<<c.N, kBlock, 0, s>>>(c, g, gt, rho, limflag, pmass);
+}
+
+void launch_wall_collide(const DevCloud &c, double *g, const double *scal, double dt,
+                         double tau, double R, double *rho, double *ux, double *uy, double *uz,
+                         double *T, double *pmass, hipStream_t s)
+{
+    if (c.dim == 3)
+        k_wall_collide<3><<<c.N, kBlock, 0, s>>>(c, g, scal, dt, tau, R, rho, ux, uy, uz, T,
+                                                 pmass);
+    else
+        k_wall_collide<2><<<c.N, kBlock, 0, s>>>(c, g, scal, dt, tau, R, rho, ux, uy, uz, T,
+                                                 pmass);
+}
+
+void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
+                  double *ux, double *uy, double *uz, double *T, double *pmass, hipStream_t s)
+{
+    if (c.dim == 3)
+        k_final<3><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass);
+    else
+        k_final<2><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass);
+}
+
+}  // namespace ab

@@ -1,0 +1,860 @@
+// k_table.hip -- the table pipeline of the ALE-BGK step (AB_PREC_MIXED with
+// FP32 storage and arithmetic, AB_PREC_FP64_TABLE with FP64).
+//
+// MLS transport is linear in g: for every (particle i, node v)
+//     gt[i,v] = sum_o C[i,o,v] g[i+o,v]
+// with C fixed by the cloud geometry (foot point x_i - v dt, the weights and
+// the basis). The reference re-assembles and factorises the normal equations
+// every step (Solver/CudaBackend.cu:699-869). On a lattice every particle's
+// stencil is one of 5^3 = 125 classes (per axis: first, second, interior,
+// second-to-last, last lattice plane), so the setup factorises each class once
+// in FP64 (q = A^-1 e0, C_o = w_o phi_o^T q, the Shepard weights where the
+// pivot collapses, identity on gated boundary nodes) and the step applies it.
+//
+// Stencil: the 81 lattice offsets with ox^2+oy^2+oz^2 <= 6 (radius 2.5 dx
+// includes exactly these; self included for the gated nodes), grouped in 21
+// (ox, oy) columns of 5 or 3 z-offsets.
+//
+// Step (four kernels, no host synchronisation):
+//   k_apply      gt = lazy * sum_o C g     one wave = 64 consecutive nodes of a
+//                lattice column (ix, iy); marches along z in blocks of K
+//                outputs with the class coefficients in registers and each
+//                loaded value reused by up to 5 outputs.
+//   k_post       one wave per particle: positivity (fallback to the
+//                pre-transport value), NAD, then the diffuse wall (boundary)
+//                or moments + BGK collision (interior), written back in place
+//                of g unscaled; FP64 partial sums per particle.
+//   k_reduce     mass sums -> the two global rescale factors, folded into one
+//                lazy factor the stored g carries into the next step.
+//   k_finalize   per-particle rho, u, T of the step's end.
+// Homogeneity makes the lazy factor exact: wall, moments (u, T) and
+// collision are all homogeneous of degree 1 in a uniform rescale of g.
+#include "ab_device.hpp"
+#include "ab_internal.hpp"
+
+#include <cstring>
+#include <utility>
+#include <vector>
+
+namespace ab {
+
+namespace {
+
+constexpr int kOff = 81;      // stencil offsets
+constexpr int kCols = 21;     // (ox, oy) columns
+constexpr int kNcls = 125;    // 5 categories per axis
+
+struct Col { int ox, oy, d, o0; };
+
+// Columns in canonical order; o0 is the index of the column's oz = -d offset.
+__host__ __device__ constexpr Col col_at(int c)
+{
+    constexpr int t[kCols][3] = {
+        {-1, -1, 2}, {0, -1, 2}, {1, -1, 2}, {-1, 0, 2}, {0, 0, 2}, {1, 0, 2},
+        {-1, 1, 2}, {0, 1, 2}, {1, 1, 2},
+        {-2, -1, 1}, {-2, 0, 1}, {-2, 1, 1}, {2, -1, 1}, {2, 0, 1}, {2, 1, 1},
+        {-1, -2, 1}, {0, -2, 1}, {1, -2, 1}, {-1, 2, 1}, {0, 2, 1}, {1, 2, 1}};
+    int o0 = 0;
+    for (int k = 0; k < c; ++k) o0 += 2 * t[k][2] + 1;
+    return Col{t[c][0], t[c][1], t[c][2], o0};
+}
+
+// Canonical index of lattice offset (ox, oy, oz), or -1 outside the stencil.
+__host__ __device__ inline int canon_index(int ox, int oy, int oz)
+{
+    for (int c = 0; c < kCols; ++c)
+    {
+        Col k = col_at(c);
+        if (k.ox == ox && k.oy == oy)
+            return (oz >= -k.d && oz <= k.d) ? k.o0 + oz + k.d : -1;
+    }
+    return -1;
+}
+
+__host__ __device__ inline int category(int i, int n)
+{
+    return i == 0 ? 0 : (i == 1 ? 1 : (i == n - 1 ? 4 : (i == n - 2 ? 3 : 2)));
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// ---- coefficient tables (FP64 factorisation, once per cloud) ---------------
+// Thread per (class, node): the assembly of kTransportNode3DT for the class's
+// representative particle, then q = A^-1 e0 by the same Cholesky and
+// substitutions, then C_m = w_m * phi_m . q for each neighbour m.
+template <typename T, int P, bool BND>
+__device__ void build_class(const DevCloud &c, int i, int lin, const int *lat, double dt,
+                            double h, double tol, T *out, int S)
+{
+    double vx = c.vlo + (lin % c.Nv) * c.dv;
+    double vy = c.vlo + ((lin / c.Nv) % c.Nv) * c.dv;
+    double vz = c.vlo + (lin / (c.Nv * c.Nv)) * c.dv;
+    const int self = canon_index(0, 0, 0);
+    if (BND)
+    {
+        double cn = (vx - c.wux[i]) * c.wnx[i] + (vy - c.wuy[i]) * c.wny[i]
+                  + (vz - c.wuz[i]) * c.wnz[i];
+        if (cn >= 0.0) { out[(size_t)self * S] = T(1); return; }
+    }
+    const double xf = c.x[i] - (vx - c.mvx[i]) * dt;
+    const double yf = c.y[i] - (vy - c.mvy[i]) * dt;
+    const double zf = c.z[i] - (vz - c.mvz[i]) * dt;
+    const double hh = h * h;
+    const int nn = c.nneigh[i];
+    const int *nb = c.neigh + (size_t)i * AB_MAX_NEIGH;
+    double A[P * P];
+#pragma unroll
+    for (int q = 0; q < P * P; ++q) A[q] = 0.0;
+    bool has = false;
+    double sumW = 0.0;
+    auto weight_basis = [&](int j, double *basis) -> double {
+        const double dxn = c.x[j] - xf, dyn = c.y[j] - yf, dzn = c.z[j] - zf;
+        const double r2 = dxn * dxn + dyn * dyn + dzn * dzn;
+        const double q2 = r2 / hh;
+        double w = 0.0;
+        if (q2 < 1.0) { double t1 = 1.0 - q2; w = t1 * t1; }
+        const double sx = dxn / h, sy = dyn / h, sz = dzn / h;
+        basis[0] = 1.0; basis[1] = sx; basis[2] = sy; basis[3] = sz;
+        if constexpr (P >= 10)
+        {
+            basis[4] = sx * sx; basis[5] = sy * sy; basis[6] = sz * sz;
+            basis[7] = sx * sy; basis[8] = sx * sz; basis[9] = sy * sz;
+        }
+        return w;
+    };
+    for (int m = 0; m < nn; ++m)
+    {
+        double basis[P];
+        double w = weight_basis(nb[m], basis);
+        if (w < 1e-30) continue;
+#pragma unroll
+        for (int p1 = 0; p1 < P; ++p1)
+        {
+            const double wb = w * basis[p1];
+#pragma unroll
+            for (int p2 = p1; p2 < P; ++p2) A[p1 * P + p2] += wb * basis[p2];
+        }
+        sumW += w;
+        has = true;
+    }
+    if (!has) { out[(size_t)self * S] = T(1); return; }
+    double dmax = 0.0;
+#pragma unroll
+    for (int k = 0; k < P; ++k) dmax = fmax(dmax, A[k * P + k]);
+    bool degenerate = (dmax <= 0.0);
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+    {
+        double d = A[k * P + k];
+#pragma unroll
+        for (int j = 0; j < k; ++j) { double l = A[j * P + k]; d -= l * l; }
+        if (!(d > tol * dmax)) degenerate = true;
+        d = degenerate ? 1.0 : sqrt(d);
+        A[k * P + k] = d;
+#pragma unroll
+        for (int m2 = k + 1; m2 < P; ++m2)
+        {
+            double v = A[k * P + m2];
+#pragma unroll
+            for (int j = 0; j < k; ++j) v -= A[j * P + k] * A[j * P + m2];
+            A[k * P + m2] = v / d;
+        }
+    }
+    double q[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+    {
+        double v = (k == 0) ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = 0; j < k; ++j) v -= A[j * P + k] * q[j];
+        q[k] = v / A[k * P + k];
+    }
+#pragma unroll
+    for (int k = P - 1; k >= 0; --k)
+    {
+        double v = q[k];
+#pragma unroll
+        for (int j = k + 1; j < P; ++j) v -= A[k * P + j] * q[j];
+        q[k] = v / A[k * P + k];
+    }
+    const int ix = lat[3 * i], iy = lat[3 * i + 1], iz = lat[3 * i + 2];
+    for (int m = 0; m < nn; ++m)
+    {
+        int j = nb[m];
+        double basis[P];
+        double w = weight_basis(j, basis);
+        if (w < 1e-30) continue;
+        double cm;
+        if (degenerate) cm = w / sumW;
+        else
+        {
+            cm = 0.0;
+#pragma unroll
+            for (int p = 0; p < P; ++p) cm += basis[p] * q[p];
+            cm *= w;
+        }
+        int o = canon_index(lat[3 * j] - ix, lat[3 * j + 1] - iy, lat[3 * j + 2] - iz);
+        out[(size_t)o * S] = T(cm);
+    }
+}
+
+template <typename T>
+__global__ void k_build(DevCloud c, const int *__restrict__ rep, const int *__restrict__ lat,
+                        double dt, double h, double tol, int order, T *__restrict__ coef, int S)
+{
+    const int cls = blockIdx.y;
+    const int lin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= S) return;
+    T *out = coef + (size_t)cls * kOff * S + lin;
+    for (int o = 0; o < kOff; ++o) out[(size_t)o * S] = T(0);
+    if (lin >= c.nodes) return;
+    const int i = rep[cls];
+    if (c.bnd[i]) build_class<T, 4, true>(c, i, lin, lat, dt, h, tol, out, S);
+    else if (order >= 2) build_class<T, 10, false>(c, i, lin, lat, dt, h, tol, out, S);
+    else build_class<T, 4, false>(c, i, lin, lat, dt, h, tol, out, S);
+}
+
+// ---- transport apply ---------------------------------------------------------
+struct Lat { int nx, ny, nz, S, nchunk; };
+
+// Addresses: every pointer below is wave-uniform and only the node index
+// varies across lanes. Loads and stores go through raw buffer instructions
+// whose descriptor is built in SGPRs from the uniform row pointer, with the
+// lane's byte offset as the only VGPR operand (a plain global access would
+// carry a 64-bit VGPR address per value in flight).
+template <typename T>
+__device__ __forceinline__ T bload(const T *row, int voff)
+{
+    __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)row, (short)0,
+                                                                 0x7fffffff, 0x00020000);
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore(T *row, int voff, T v)
+{
+    __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)row, (short)0,
+                                                                 0x7fffffff, 0x00020000);
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, 0, 0);
+    else
+    {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, 0, 0);
+    }
+}
+
+// One output of an end-plane particle: per-offset coefficient loads.
+template <typename T>
+__device__ __forceinline__ T apply_one(const T *__restrict__ g, const T *__restrict__ cf,
+                                       const Lat &L, int ix, int iy, int iz, int lin)
+{
+    T acc = T(0);
+#pragma unroll
+    for (int cc = 0; cc < kCols; ++cc)
+    {
+        const Col k = col_at(cc);
+        const int xx = clampi(ix + k.ox, 0, L.nx - 1), yy = clampi(iy + k.oy, 0, L.ny - 1);
+#pragma unroll
+        for (int dz = -k.d; dz <= k.d; ++dz)
+        {
+            const int zz = clampi(iz + dz, 0, L.nz - 1);
+            const T *crow = cf + (size_t)(k.o0 + dz + k.d) * L.S;
+            const T *grow = g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * zz)) * L.S;
+            acc += bload(crow, lin) * bload(grow, lin);
+        }
+        asm volatile("" : "+v"(acc)::"memory");
+    }
+    return acc;
+}
+
+// Accumulate one (ox, oy) column into the K outputs iz0 .. iz0+K-1.
+template <typename T, int K, int CC>
+__device__ __forceinline__ void apply_column(const T *__restrict__ g, const T (&cr)[kOff],
+                                             const Lat &L, int ix, int iy, int iz0, int lin,
+                                             size_t zstride, T (&acc)[K])
+{
+    constexpr Col kc = col_at(CC);
+    constexpr int NV = K + 2 * kc.d;
+    const int xx = clampi(ix + kc.ox, 0, L.nx - 1), yy = clampi(iy + kc.oy, 0, L.ny - 1);
+    const T *gc = g + ((size_t)xx + (size_t)L.nx * yy) * L.S;
+    T vals[NV];
+#pragma clang loop unroll(full)
+    for (int j = 0; j < NV; ++j)
+    {
+        const int zz = clampi(iz0 - kc.d + j, 0, L.nz - 1);
+        vals[j] = bload(gc + (size_t)zz * zstride, lin);
+    }
+#pragma clang loop unroll(full)
+    for (int k = 0; k < K; ++k)
+#pragma clang loop unroll(full)
+        for (int dz = 0; dz <= 2 * kc.d; ++dz) acc[k] += cr[kc.o0 + dz] * vals[k + dz];
+    // Pin the column's FMAs here: without it the optimiser sinks every
+    // column's arithmetic below all the loads and spills the values.
+#pragma clang loop unroll(full)
+    for (int k = 0; k < K; ++k) asm volatile("" : "+v"(acc[k])::"memory");
+}
+
+template <typename T, int K, int... CC>
+__device__ __forceinline__ void apply_columns(const T *__restrict__ g, const T (&cr)[kOff],
+                                              const Lat &L, int ix, int iy, int iz0, int lin,
+                                              size_t zstride, T (&acc)[K],
+                                              std::integer_sequence<int, CC...>)
+{
+    // one column at a time: bounds the loads in flight and the registers
+    ((apply_column<T, K, CC>(g, cr, L, ix, iy, iz0, lin, zstride, acc),
+      __builtin_amdgcn_sched_barrier(0)), ...);
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
+void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef,
+             const double *__restrict__ scal)
+{
+    const int nwave_blk = blockDim.x / kWave;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    // wave-uniform: keep it (and every address part derived from it) in SGPRs
+    const int wv = __builtin_amdgcn_readfirstlane(b * nwave_blk + threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int voff = lane * (int)sizeof(T);          // the lane's byte offset
+    const int ncol = L.nx * L.ny;
+    if (wv >= ncol * L.nchunk) return;
+    const int chunk = wv % L.nchunk, col = wv / L.nchunk;
+    const int ix = col % L.nx, iy = col / L.nx;
+    const T lazy = T(scal[S_LAZY]);
+    const int cx = category(ix, L.nx), cy = category(iy, L.ny);
+    const size_t plane = (size_t)L.nx * L.ny;
+    // this wave's chunk of nodes: offset every base pointer once
+    const size_t c0 = (size_t)chunk * kWave;
+    const T *gw = g + c0;
+    T *gtw = gt + c0;
+
+    // The four end planes: their own classes, coefficients loaded per offset.
+#pragma unroll 1
+    for (int e = 0; e < 4; ++e)
+    {
+        const int iz = (e < 2) ? e : L.nz - 4 + e;
+        const int cls = cx + 5 * (cy + 5 * category(iz, L.nz));
+        const T *cf = coef + (size_t)cls * kOff * L.S + c0;
+        T acc = apply_one(gw, cf, L, ix, iy, iz, voff);
+        bstore(gtw + ((size_t)col + plane * iz) * L.S, voff, acc * lazy);
+    }
+
+    // Interior planes iz = 2 .. nz-3 share one class: coefficients in registers.
+    T cr[kOff];
+    {
+        const T *cf = coef + (size_t)(cx + 5 * (cy + 5 * 2)) * kOff * L.S + c0;
+#pragma unroll
+        for (int o = 0; o < kOff; ++o) cr[o] = bload(cf + (size_t)o * L.S, voff);
+    }
+    for (int iz0 = 2; iz0 <= L.nz - 3; iz0 += K)
+    {
+        T acc[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = T(0);
+        apply_columns<T, K>(gw, cr, L, ix, iy, iz0, voff, plane * L.S, acc,
+                            std::make_integer_sequence<int, kCols>{});
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (iz0 + k <= L.nz - 3)
+                bstore(gtw + ((size_t)col + plane * (iz0 + k)) * L.S, voff, acc[k] * lazy);
+    }
+}
+
+// ---- per-particle limiters, wall, moments, collision -------------------------
+// Partial sums written per particle (FP64):
+//   0  mass after limiters and commit (before the first rescale)
+//   1  mass after wall / collision, unscaled
+//   2..4  sum v_a h dv^3 (interior)   | 2 = rho_w unscaled (boundary)
+//   5  sum |v|^2 h dv^3 (interior)
+//   6  limiter bits (1 positivity, 2 NAD)
+constexpr int kPart = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__ g,
+                                              const T *__restrict__ gt,
+                                              const double *__restrict__ rho,
+                                              const double *__restrict__ scal, double dt,
+                                              double tau, double R, double *__restrict__ part)
+{
+    const int i = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (i >= c.N) return;
+    const T lazy = T(scal[S_LAZY]);
+    T *gp = g + (size_t)i * L.S;
+    const T *tp = gt + (size_t)i * L.S;
+    const int nodes = c.nodes, Nv = c.Nv;
+    const double dv3 = c.dv * c.dv * c.dv;
+
+    // positivity: negatives fall back to the pre-transport value (lazy * g)
+    double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
+    for (int k = lane; k < nodes; k += kWave)
+    {
+        T v = tp[k];
+        if (v >= T(0)) mPos += (double)v;
+        else { mNeg += (double)v; mFb += (double)(gp[k] * lazy); }
+    }
+    mPos = wave_sum(mPos); mNeg = wave_sum(mNeg); mFb = wave_sum(mFb);
+    const bool fires = (mNeg < -1e-30 && (mPos + mNeg) > 0);
+    double sc = 1.0;
+    if (fires)
+    {
+        double replaced = mPos + mFb;
+        sc = (replaced > 1e-30) ? (mPos + mNeg) / replaced : 0.0;
+    }
+    const T scT = T(sc);
+    auto limited = [&](int k) -> T {
+        T v = tp[k];
+        if (fires) v = ((v < T(0)) ? gp[k] * lazy : v) * scT;
+        return v;
+    };
+    double flag = fires ? 1.0 : 0.0;
+    double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    if (!c.bnd[i])
+    {
+        // moments of the limited field; the NAD scale multiplies them all
+        double r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+            double f = (double)limited(k);
+            r0 += f; r1 += vx * f; r2 += vy * f; r3 += vz * f;
+            r4 += (vx * vx + vy * vy + vz * vz) * f;
+        }
+        r0 = wave_sum(r0); r1 = wave_sum(r1); r2 = wave_sum(r2); r3 = wave_sum(r3);
+        r4 = wave_sum(r4);
+        // NAD against the neighbours' current densities
+        const int nn = c.nneigh[i];
+        double rmin = 1e300, rmax = -1e300;
+        for (int k = lane; k < nn; k += kWave)
+        {
+            double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
+            rmin = fmin(rmin, q); rmax = fmax(rmax, q);
+        }
+        rmin = wave_min(rmin); rmax = wave_max(rmax);
+        const double rho_new = r0 * dv3;
+        double sc2 = 1.0;
+        if (!(rmin > rmax))
+        {
+            double slack = 0.1 * fmax(rmax - rmin, 1e-12) + 1e-9;
+            double lo = rmin - slack, hi = rmax + slack;
+            bool nad = false;
+            if (rho_new > hi && rho_new > 1e-30) { sc2 = hi / rho_new; nad = true; }
+            else if (rho_new < lo && rho_new > 1e-30) { sc2 = lo / rho_new; nad = true; }
+            if (nad) flag += 2.0;
+        }
+        out[0] = sc2 * rho_new;                              // committed mass
+        // moments (unscaled by the global rescale; u and T are invariant)
+        double mr = sc2 * rho_new, mux = 0, muy = 0, muz = 0, mT = 270.0;
+        bool okm = mr > 1e-30;
+        if (okm)
+        {
+            mux = r1 / r0; muy = r2 / r0; muz = r3 / r0;
+            double u2 = mux * mux + muy * muy + muz * muz;
+            double t1 = (r4 / r0 - u2) / (3.0 * R);
+            mT = t1 > 1e-6 ? t1 : 1e-6;
+        }
+        const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
+        const double twoRT = 2.0 * R * mT;
+        const T pref = T(okm ? coeff * lam * mr / pow(AB_PI * twoRT, 1.5) : 0.0);
+        const T inv2RT = T(1.0 / twoRT);
+        const T cT = T(coeff * sc2);
+        const T tux = T(mux), tuy = T(muy), tuz = T(muz);
+        double h0 = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            const T vx = T(c.vlo + (k % Nv) * c.dv), vy = T(c.vlo + ((k / Nv) % Nv) * c.dv);
+            const T vz = T(c.vlo + (k / (Nv * Nv)) * c.dv);
+            const T ex = -((vx - tux) * (vx - tux) + (vy - tuy) * (vy - tuy)
+                           + (vz - tuz) * (vz - tuz)) * inv2RT;
+            const T hv = cT * limited(k) + pref * exp(ex);
+            gp[k] = hv;
+            const double hd = (double)hv;
+            const double dvx = (double)vx, dvy = (double)vy, dvz = (double)vz;
+            h0 += hd; h1 += dvx * hd; h2 += dvy * hd; h3 += dvz * hd;
+            h4 += (dvx * dvx + dvy * dvy + dvz * dvz) * hd;
+        }
+        out[1] = wave_sum(h0) * dv3;
+        out[2] = wave_sum(h1) * dv3;
+        out[3] = wave_sum(h2) * dv3;
+        out[4] = wave_sum(h3) * dv3;
+        out[5] = wave_sum(h4) * dv3;
+    }
+    else
+    {
+        const double nx = c.wnx[i], ny = c.wny[i], nz = c.wnz[i];
+        const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = c.wuz[i], Tw = c.wt[i];
+        const double RT2w = 2.0 * R * Tw;
+        const double norm = pow(AB_PI * RT2w, 1.5);
+        double m0 = 0, intM = 0, intP = 0;
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+            double f = (double)limited(k);
+            m0 += f;
+            double cn = (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz;
+            if (cn < 0.0) intM += cn * f * dv3;
+            else if (cn > 0.0)
+            {
+                double Mw = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
+                                  + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm;
+                intP += cn * Mw * dv3;
+            }
+        }
+        m0 = wave_sum(m0); intM = wave_sum(intM); intP = wave_sum(intP);
+        const double rho_w = (intP > 1e-30) ? (-intM / intP) : 1.0;
+        out[0] = m0 * dv3;
+        out[2] = rho_w;
+        const T pw = T(rho_w / norm), inv = T(1.0 / RT2w);
+        const T tUx = T(Uwx), tUy = T(Uwy), tUz = T(Uwz);
+        double h0 = 0;
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+            double cn = (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz;
+            T hv;
+            if (cn > 0.0)
+            {
+                const T ex = -((T(vx) - tUx) * (T(vx) - tUx) + (T(vy) - tUy) * (T(vy) - tUy)
+                               + (T(vz) - tUz) * (T(vz) - tUz)) * inv;
+                hv = pw * exp(ex);
+            }
+            else hv = limited(k);
+            gp[k] = hv;
+            h0 += (double)hv;
+        }
+        out[1] = wave_sum(h0) * dv3;
+    }
+    out[6] = flag;
+    if (lane < kPart) part[(size_t)i * kPart + lane] = out[lane];
+}
+
+// Global sums of the partials -> the two rescale factors of cudaStep
+// (:1706-1710, :1773-1780), folded into the lazy factor.
+__global__ __launch_bounds__(1024) void k_reduce(int N, const double *__restrict__ part,
+                                                 double *scal, int nofix)
+{
+    __shared__ double sh[2][1024 / kWave];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < N; i += 1024) { a += part[(size_t)i * kPart]; b += part[(size_t)i * kPart + 1]; }
+    a = wave_sum(a); b = wave_sum(b);
+    if ((threadIdx.x & (kWave - 1)) == 0) { sh[0][threadIdx.x / kWave] = a; sh[1][threadIdx.x / kWave] = b; }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double mNew = 0.0, hsum = 0.0;
+        for (int q = 0; q < 1024 / kWave; ++q) { mNew += sh[0][q]; hsum += sh[1][q]; }
+        const double mOld = scal[S_MASS_OLD];
+        const double s1 = (mNew > 1e-30) ? mOld / mNew : 1.0;
+        const double mColl = s1 * hsum;
+        const double s2 = (!nofix && mColl > 1e-30) ? mOld / mColl : 1.0;
+        scal[S_MASS_START] = mOld;
+        scal[S_MASS_NEW] = mNew;
+        scal[S_SCALE1] = s1;
+        scal[S_MASS_COLL] = mColl;
+        scal[S_SCALE2] = s2;
+        scal[S_LAZY] = s1 * s2;
+        scal[S_MASS_OLD] = s2 * mColl;          // the next step's starting mass
+    }
+}
+
+__global__ void k_finalize(DevCloud c, const double *__restrict__ part,
+                           const double *__restrict__ scal, double R, double *rho, double *ux,
+                           double *uy, double *uz, double *T, double *limflag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    const double *p = part + (size_t)i * kPart;
+    limflag[i] = p[6];
+    if (c.bnd[i])
+    {
+        rho[i] = scal[S_SCALE1] * p[2];
+        ux[i] = c.wux[i]; uy[i] = c.wuy[i]; uz[i] = c.wuz[i]; T[i] = c.wt[i];
+        return;
+    }
+    const double r = scal[S_LAZY] * p[1];
+    if (r > 1e-30)
+    {
+        const double mux = p[2] / p[1], muy = p[3] / p[1], muz = p[4] / p[1];
+        const double u2 = mux * mux + muy * muy + muz * muz;
+        const double t1 = (p[5] / p[1] - u2) / (3.0 * R);
+        rho[i] = r; ux[i] = mux; uy[i] = muy; uz[i] = muz; T[i] = t1 > 1e-6 ? t1 : 1e-6;
+    }
+}
+
+// Mass of the stored field (lazy factor applied).
+template <typename T>
+__global__ void k_table_mass(int N, int nodes, int S, const T *__restrict__ g, double dv3,
+                             double *__restrict__ part)
+{
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (i >= N) return;
+    double m = 0.0;
+    for (int k = lane; k < nodes; k += kWave) m += (double)g[(size_t)i * S + k];
+    m = wave_sum(m);
+    if (lane == 0) part[(size_t)i * kPart + 7] = m * dv3;
+}
+
+__global__ __launch_bounds__(1024) void k_mass_from_part(int N, const double *__restrict__ part,
+                                                         double *scal)
+{
+    __shared__ double sh[1024 / kWave];
+    double a = 0.0;
+    for (int i = threadIdx.x; i < N; i += 1024) a += part[(size_t)i * kPart + 7];
+    a = wave_sum(a);
+    if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = a;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double t = 0.0;
+        for (int q = 0; q < 1024 / kWave; ++q) t += sh[q];
+        scal[S_MASS_OLD] = t * scal[S_LAZY];
+    }
+}
+
+template <typename T>
+__global__ void k_pack(int N, int nodes, int gstride, int S, const double *__restrict__ src,
+                       T *__restrict__ dst)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)N * S) return;
+    size_t i = k / S, lin = k % S;
+    dst[k] = (lin < (size_t)nodes) ? T(src[i * gstride + lin]) : T(0);
+}
+
+template <typename T>
+__global__ void k_unpack(int N, int gstride, int S, const T *__restrict__ src,
+                         const double *__restrict__ scal, double *__restrict__ dst)
+{
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (size_t)N * gstride) return;
+    size_t i = k / gstride, lin = k % gstride;
+    dst[k] = (double)src[i * S + lin] * scal[S_LAZY];
+}
+
+__global__ void k_set_scalar(double *scal, int slot, double v) { scal[slot] = v; }
+
+bool is_fp32(const ab_ctx *ctx) { return ctx->cfg.precision == AB_PREC_MIXED; }
+
+template <typename T>
+int apply_launch(ab_ctx *ctx, const Lat &L)
+{
+    const int nwaves = ctx->cfg.nx * ctx->cfg.ny * L.nchunk;
+    const int nb = (nwaves + 3) / 4;
+    const int run = ctx->cfg.nz - 4;
+    const T *g = (const T *)ctx->gs;
+    T *gt = (T *)ctx->gts;
+    const T *cf = (const T *)ctx->coef;
+    if (run % 9 == 0) k_apply<T, 9><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
+    else if (run % 7 == 0) k_apply<T, 7><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
+    else k_apply<T, 8><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
+    return 0;
+}
+
+}  // namespace
+
+void table_free(ab_ctx *ctx)
+{
+    void *ptrs[] = {ctx->gs, ctx->gts, ctx->coef, ctx->cls, ctx->lat, ctx->part, ctx->bad};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    ctx->gs = ctx->gts = ctx->coef = nullptr;
+    ctx->cls = ctx->lat = ctx->bad = nullptr;
+    ctx->part = nullptr;
+}
+
+// Validate that the cloud is the lattice the config names (every particle's
+// neighbour set is its class's stencil and its wall state is its class's),
+// then build the tables. Host-side check of the uploaded arrays.
+int table_setup(ab_ctx *ctx)
+{
+    const ab_config &cfg = ctx->cfg;
+    const int N = ctx->N;
+    if (cfg.dim != 3)
+    { ctx->err = "table pipelines need dim 3 (2-D runs with AB_PREC_FP64)"; return AB_ERR_UNSUPPORTED; }
+    if (cfg.nx < 5 || cfg.ny < 5 || cfg.nz < 5 || (long)cfg.nx * cfg.ny * cfg.nz != N)
+    { ctx->err = "table pipelines need a lattice cloud: nx, ny, nz >= 5 and N = nx*ny*nz"; return AB_ERR_UNSUPPORTED; }
+    if (cfg.mls_clamp)
+    { ctx->err = "mls_clamp is nonlinear in g; run it with AB_PREC_FP64"; return AB_ERR_UNSUPPORTED; }
+
+    std::vector<double> x(N), y(N), z(N), w(10 * (size_t)N);
+    std::vector<unsigned char> b(N);
+    std::vector<int> nb((size_t)N * AB_MAX_NEIGH), nn(N);
+    hipStream_t s = ctx->stream;
+    auto dl = [&](void *dst, const void *src, size_t n) {
+        return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s) == hipSuccess;
+    };
+    bool ok = dl(x.data(), ctx->x, 8 * (size_t)N) && dl(y.data(), ctx->y, 8 * (size_t)N)
+           && dl(z.data(), ctx->z, 8 * (size_t)N) && dl(b.data(), ctx->bnd, N)
+           && dl(nb.data(), ctx->neigh, 4 * (size_t)N * AB_MAX_NEIGH)
+           && dl(nn.data(), ctx->nneigh, 4 * (size_t)N);
+    const double *wsrc[10] = {ctx->wnx, ctx->wny, ctx->wnz, ctx->wux, ctx->wuy, ctx->wuz,
+                              ctx->wt, ctx->mvx, ctx->mvy, ctx->mvz};
+    for (int k = 0; k < 10 && ok; ++k) ok = dl(&w[(size_t)k * N], wsrc[k], 8 * (size_t)N);
+    if (!ok || hipStreamSynchronize(s) != hipSuccess)
+    { ctx->err = "table setup: device-to-host copy failed"; return AB_ERR_HIP; }
+
+    const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+    std::vector<int> lat(3 * (size_t)N), cls(N), rep(kNcls, -1);
+    std::vector<unsigned long long> mlo(N, 0), mhi(N, 0);
+    for (int p = 0; p < N; ++p)
+    {
+        int ix = p % nx, iy = (p / nx) % ny, iz = p / (nx * ny);
+        lat[3 * p] = ix; lat[3 * p + 1] = iy; lat[3 * p + 2] = iz;
+        cls[p] = category(ix, nx) + 5 * (category(iy, ny) + 5 * category(iz, nz));
+        if (rep[cls[p]] < 0) rep[cls[p]] = p;
+    }
+    for (int p = 0; p < N; ++p)
+    {
+        for (int m = 0; m < nn[p]; ++m)
+        {
+            int j = nb[(size_t)p * AB_MAX_NEIGH + m];
+            int jx = j % nx, jy = (j / nx) % ny, jz = j / (nx * ny);
+            int o = canon_index(jx - lat[3 * p], jy - lat[3 * p + 1], jz - lat[3 * p + 2]);
+            if (o < 0)
+            {
+                ctx->err = "table setup: particle " + std::to_string(p) +
+                           " has a neighbour outside the lattice stencil";
+                return AB_ERR_UNSUPPORTED;
+            }
+            if (o < 64) mlo[p] |= 1ull << o; else mhi[p] |= 1ull << (o - 64);
+        }
+    }
+    for (int p = 0; p < N; ++p)
+    {
+        int r = rep[cls[p]];
+        bool same = (mlo[p] == mlo[r]) && (mhi[p] == mhi[r]) && (nn[p] == nn[r]) && (b[p] == b[r]);
+        for (int k = 0; k < 10 && same; ++k) same = (w[(size_t)k * N + p] == w[(size_t)k * N + r]);
+        if (!same)
+        {
+            ctx->err = "table setup: particle " + std::to_string(p) +
+                       " differs from its lattice class (stencil or wall state)";
+            return AB_ERR_UNSUPPORTED;
+        }
+    }
+
+    const bool f32 = is_fp32(ctx);
+    const size_t es = f32 ? sizeof(float) : sizeof(double);
+    ctx->S = ((ctx->nodes + kWave - 1) / kWave) * kWave;
+    ctx->ncls = kNcls;
+    const size_t gsz = (size_t)N * ctx->S * es;
+    if (!ctx->gs)
+    {
+        bool a = hipMalloc(&ctx->gs, gsz) == hipSuccess && hipMalloc(&ctx->gts, gsz) == hipSuccess
+              && hipMalloc(&ctx->coef, (size_t)kNcls * kOff * ctx->S * es) == hipSuccess
+              && hipMalloc((void **)&ctx->lat, 3 * sizeof(int) * (size_t)N) == hipSuccess
+              && hipMalloc((void **)&ctx->cls, sizeof(int) * kNcls) == hipSuccess
+              && hipMalloc((void **)&ctx->part, sizeof(double) * kPart * (size_t)N) == hipSuccess;
+        if (!a) { table_free(ctx); ctx->err = "table setup: device allocation failed"; return AB_ERR_NOMEM; }
+        if (hipMemsetAsync(ctx->gts, 0, gsz, s) != hipSuccess)
+        { ctx->err = "table setup: memset failed"; return AB_ERR_HIP; }
+    }
+    if (hipMemcpyAsync(ctx->lat, lat.data(), 3 * sizeof(int) * (size_t)N, hipMemcpyHostToDevice, s) != hipSuccess
+        || hipMemcpyAsync(ctx->cls, rep.data(), sizeof(int) * kNcls, hipMemcpyHostToDevice, s) != hipSuccess)
+    { ctx->err = "table setup: host-to-device copy failed"; return AB_ERR_HIP; }
+    return AB_OK;
+}
+
+// Pack g (FP64, particle-major) into the padded table layout and (re)build
+// the coefficient tables for time step dt.
+static int table_build(ab_ctx *ctx, double dt)
+{
+    const double tol = ctx->cfg.mls_tol > 0 ? ctx->cfg.mls_tol : 1e-12;
+    dim3 grid((ctx->S + 255) / 256, kNcls);
+    ab::DevCloud c = ctx->view();
+    if (is_fp32(ctx))
+        k_build<float><<<grid, 256, 0, ctx->stream>>>(c, ctx->cls, ctx->lat, dt, ctx->cfg.radius,
+                                                      tol, ctx->cfg.mls_order, (float *)ctx->coef,
+                                                      ctx->S);
+    else
+        k_build<double><<<grid, 256, 0, ctx->stream>>>(c, ctx->cls, ctx->lat, dt,
+                                                       ctx->cfg.radius, tol, ctx->cfg.mls_order,
+                                                       (double *)ctx->coef, ctx->S);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+int table_upload_g(ab_ctx *ctx, const double *g_dev)
+{
+    const size_t n = (size_t)ctx->N * ctx->S;
+    const int bs = 256;
+    const unsigned nb = (unsigned)((n + bs - 1) / bs);
+    k_set_scalar<<<1, 1, 0, ctx->stream>>>(ctx->scal, S_LAZY, 1.0);
+    if (is_fp32(ctx))
+        k_pack<float><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->gstride, ctx->S, g_dev,
+                                                  (float *)ctx->gs);
+    else
+        k_pack<double><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->gstride, ctx->S, g_dev,
+                                                   (double *)ctx->gs);
+    const int nbm = (ctx->N * kWave + 255) / 256;
+    const double dv3 = ctx->dv * ctx->dv * ctx->dv;
+    if (is_fp32(ctx))
+        k_table_mass<float><<<nbm, 256, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->S,
+                                                          (const float *)ctx->gs, dv3, ctx->part);
+    else
+        k_table_mass<double><<<nbm, 256, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->S,
+                                                           (const double *)ctx->gs, dv3, ctx->part);
+    k_mass_from_part<<<1, 1024, 0, ctx->stream>>>(ctx->N, ctx->part, ctx->scal);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+int table_download_g(ab_ctx *ctx, double *g_dev)
+{
+    const size_t n = (size_t)ctx->N * ctx->gstride;
+    const int bs = 256;
+    const unsigned nb = (unsigned)((n + bs - 1) / bs);
+    if (is_fp32(ctx))
+        k_unpack<float><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->gstride, ctx->S,
+                                                    (const float *)ctx->gs, ctx->scal, g_dev);
+    else
+        k_unpack<double><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->gstride, ctx->S,
+                                                     (const double *)ctx->gs, ctx->scal, g_dev);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+int table_step(ab_ctx *ctx, double dt)
+{
+    // The coefficients depend on dt through the foot points: rebuild when it
+    // changes (the last step of a run is shortened to land on tfinal).
+    if (ctx->coef_dt != dt)
+    {
+        int rc = table_build(ctx, dt);
+        if (rc) { ctx->err = "table build failed"; return rc; }
+        ctx->coef_dt = dt;
+    }
+    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave};
+    ab::DevCloud c = ctx->view();
+    hipStream_t s = ctx->stream;
+    auto tic = [&]() { prof_begin(ctx); };
+    auto toc = [&](int st) { prof_end(ctx, st); };
+    const int nbp = (ctx->N * kWave + 255) / 256;
+    tic();
+    if (is_fp32(ctx)) apply_launch<float>(ctx, L);
+    else apply_launch<double>(ctx, L);
+    toc(ST_TRANSPORT);
+    tic();
+    if (is_fp32(ctx))
+        k_post<float><<<nbp, 256, 0, s>>>(c, L, (float *)ctx->gs, (const float *)ctx->gts,
+                                          ctx->rho, ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R,
+                                          ctx->part);
+    else
+        k_post<double><<<nbp, 256, 0, s>>>(c, L, (double *)ctx->gs, (const double *)ctx->gts,
+                                           ctx->rho, ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R,
+                                           ctx->part);
+    toc(ST_WALL_COLLIDE);
+    tic();
+    k_reduce<<<1, 1024, 0, s>>>(ctx->N, ctx->part, ctx->scal, ctx->cfg.no_mass_fix);
+    k_finalize<<<(ctx->N + 255) / 256, 256, 0, s>>>(c, ctx->part, ctx->scal, ctx->cfg.R,
+                                                    ctx->rho, ctx->ux, ctx->uy, ctx->uz,
+                                                    ctx->T, ctx->limflag);
+    toc(ST_FINAL);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+}  // namespace ab

@@ -1,0 +1,190 @@
+"""Benchmark: phase-space updates/s of the ALE-BGK time step on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): cavity3d,
+1 um cube, argon 270 K, density 1.0, uniform lid 1 m/s at z = zfront,
+Nx = 40 (64,000 particles) x Nv = 10 (1,000 velocity nodes, 6.4e7 values),
+quadratic MLS (10-term basis), dt = 1e-11 s, synthetic initial state (the
+reference's: gas at rest at the wall temperature).
+
+One step = one full time step of the mixed-precision pipeline: transport
+(MLS coefficients factorised once in FP64, FP32 contraction), positivity,
+NAD, diffuse wall, moments, BGK collision, both mass rescales. The FP64 step
+of the reference formulation is run from the same state for the same number
+of steps; the JSON line reports both and the relative moment error between
+them.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def moment_errors(m, ref, inner):
+    import numpy as np
+    d = {}
+    d["rho"] = float(np.max(np.abs(m[inner, 0] - ref[inner, 0]) / ref[inner, 0]))
+    umax = float(np.max(np.abs(ref[inner, 1:4])))
+    d["u"] = float(np.max(np.abs(m[inner, 1:4] - ref[inner, 1:4])) / max(umax, 1e-300))
+    d["T"] = float(np.max(np.abs(m[inner, 4] - ref[inner, 4]) / ref[inner, 4]))
+    return d
+
+
+def cpu_baseline(nx=28, nv=10, order=2):
+    """The reference CPU step (oracle/_ref, built from the reference sources)
+    on a bounded sample: cavity3d at nx^3, Nv=10, quadratic MLS, two steps;
+    the steady second step is timed (step 1 carries the MOOD re-transport at
+    uniform density). Falls back to the oracle port on a 12^3 sample."""
+    threads = os.cpu_count() or 1
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    threads = min(threads, 16)
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_v3")
+    if os.path.exists(exe):
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        t0 = time.time()
+        r = subprocess.run([exe, "cpustep", str(nx), str(nv), str(order), "2"], env=env,
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
+                           timeout=600)
+        if r.returncode == 0:
+            out = json.loads(r.stdout.strip().splitlines()[-1])
+            sec = out["step_seconds"][-1]
+            upd = out["N"] * nv ** 3
+            return {"value": upd / sec, "unit": "phase-space updates/s",
+                    "cores": out["threads"], "kind": "reference",
+                    "sample": (f"reference CPU step (OpenMP, x86-64-v3) on cavity3d {nx}^3 "
+                               f"(N={out['N']}), Nv={nv}, quadratic MLS; steady step 2 "
+                               f"timed: {sec:.2f} s; wall {time.time() - t0:.1f} s")}
+        log(f"[bench] reference CPU baseline failed: {r.stderr[-400:]}")
+    # oracle port: single thread, small sample
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ctypes as O
+    from helpers import setup
+    case, cl, ocl, neigh, nn = setup("cavity3d", 12, nv, order=order, perturb=False)
+    st = O.State(ocl, cl.g, cl.mom)
+    O.step(ocl, st, case.dt, case.tau, case.R, cl.cfg.radius, order)
+    t0 = time.perf_counter()
+    O.step(ocl, st, case.dt, case.tau, case.R, cl.cfg.radius, order)
+    sec = time.perf_counter() - t0
+    return {"value": cl.N * nv ** 3 / sec, "unit": "phase-space updates/s", "cores": 1,
+            "kind": "port", "sample": f"oracle port, cavity3d 12^3, Nv={nv}, quadratic, step 2"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=40)
+    ap.add_argument("--nv", type=int, default=10)
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="FP64 steps for the comparison (default: warmup + steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import alebgk4paper_amd as A
+
+    case = A.configure("cavity3d", nx=args.nx, nv=args.nv, mls_order=2)
+    cloud = A.build_lattice(case, precision=A.PREC_MIXED, mls_order=2)
+    N, nodes = cloud.N, args.nv ** 3
+    updates = N * nodes
+    dt = case.dt
+    log(f"[bench] cavity3d {args.nx}^3 Nv={args.nv}: N={N}, {updates:.3e} updates/step")
+
+    # ---- mixed pipeline: the measured step --------------------------------
+    mix = A.Solver(cloud, precision=A.PREC_MIXED)
+    mix.steps(dt, args.warmup)
+    mix.synchronize()
+    mix.profile(True)
+    t0 = time.perf_counter()
+    mix.steps(dt, args.steps)
+    mix.synchronize()
+    t1 = time.perf_counter()
+    prof = mix.profile_read()
+    mix.profile(False)
+    sec = t1 - t0
+    value = updates * args.steps / sec
+    log(f"[bench] mixed: {sec * 1e3 / args.steps:.4f} ms/step, {value:.4e} updates/s")
+    log(f"[bench] stage times (ms total, launches): {prof}")
+    tr_ms, tr_n = prof["transport"]
+    tr_avg_s = tr_ms / max(tr_n, 1) * 1e-3
+    alg_bytes = updates * 2 * 4           # read g once + write gt once, FP32
+    achieved = alg_bytes / tr_avg_s / 1e9
+    m_mix = mix.moments()
+    mass_mix = mix.total_mass()
+
+    # ---- FP64 reference formulation from the same state --------------------
+    n64 = args.fp64_steps if args.fp64_steps is not None else args.warmup + args.steps
+    ref = A.Solver(cloud, precision=A.PREC_FP64)
+    ref.step(dt)
+    ref.synchronize()
+    t0 = time.perf_counter()
+    ref.steps(dt, n64 - 1)
+    ref.synchronize()
+    sec64 = (time.perf_counter() - t0) / max(n64 - 1, 1)
+    value64 = updates / sec64
+    log(f"[bench] fp64: {sec64 * 1e3:.3f} ms/step, {value64:.4e} updates/s")
+    err = None
+    if n64 == args.warmup + args.steps:
+        err = moment_errors(m_mix, ref.moments(), cloud.boundary == 0)
+        log(f"[bench] mixed vs fp64 moment errors after {n64} steps: {err}")
+
+    cpu = None if args.no_cpu else cpu_baseline(nv=args.nv)
+    if cpu:
+        log(f"[bench] cpu baseline: {cpu}")
+
+    line = {
+        "metric": "phase-space updates/s, cavity3d Nx40 Nv10: mixed vs FP64 GPU @1e-5 moment err",
+        "value": value,
+        "unit": "phase-space updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: the reference's initial state (gas at rest, 270 K) on the cavity3d lattice",
+        "config": {"workload": f"cavity3d Nx={args.nx} Nv={args.nv} quadratic MLS, dt=1e-11, "
+                               f"mixed precision (FP32 transport+collision, FP64 factorisation, "
+                               f"conservation and wall scaling)",
+                   "particles": N, "velocity_nodes": nodes, "updates_per_step": updates},
+        "fp64_value": value64,
+        "fp64_ms_per_step": sec64 * 1e3,
+        "speedup_vs_fp64": value / value64,
+        "moment_err_vs_fp64": err,
+        "mass_drift": None if mass_mix is None else abs(mass_mix - mix.total_mass()),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_apply (MLS transport contraction)",
+                     "kernel_avg_ms": tr_avg_s * 1e3,
+                     "algorithmic_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

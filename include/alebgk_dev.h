@@ -1,0 +1,178 @@
+/*
+ * alebgk_dev.h -- C ABI of the MI355X (gfx950) ALE-BGK time-step library
+ * (libalebgk_hip.so).
+ *
+ * Replaces the reference's GPU driver boundary, the free functions declared
+ * at Solver/Solver.cpp:67-89 and defined in Solver/CudaBackend.cu, plus the
+ * host neighbour search (Initialization/NeighbourSearch.cpp:17) and the
+ * body-load reduction (RigidBody/BodyDynamics.cpp:17).
+ *
+ * Conventions
+ *   - Every call returns an int status: AB_OK (0) or an AB_ERR_* code; the
+ *     message is ab_last_error(ctx). Nothing calls exit().
+ *   - Host arrays are borrowed for the duration of the call only; device
+ *     buffers are owned by the context.
+ *   - One context = one cloud = one HIP stream; calls on one context must be
+ *     serialised by the caller. There are no hidden globals.
+ *   - FP64 at the interface whatever the internal precision.
+ *   - g is particle-major: g[i*gstride + lin], lin = iv + Nv*(jv + Nv*kv) in
+ *     3-D (gstride = Nv^3); in 2-D g1 at lin = iv + Nv*jv, g2 at lin + Nv^2
+ *     (gstride = 2*Nv^2). Neighbour lists are N x AB_MAX_NEIGH int32.
+ */
+#ifndef ALEBGK_DEV_H
+#define ALEBGK_DEV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AB_MAX_NEIGH 128          /* Core/Types.hpp:22 MAX_NEIGH */
+
+#define AB_OK 0
+#define AB_ERR_ARG 1              /* invalid argument */
+#define AB_ERR_HIP 2              /* HIP runtime error (message has details) */
+#define AB_ERR_STATE 3            /* call out of order (e.g. step before upload) */
+#define AB_ERR_UNSUPPORTED 4      /* configuration this build does not run */
+#define AB_ERR_NOMEM 5            /* device allocation failed */
+
+/* Arithmetic of the step. */
+#define AB_PREC_FP64 0        /* the reference formulation in FP64: per
+                                 (particle, node) normal equations and
+                                 Cholesky every step (kTransportNode3DT) */
+#define AB_PREC_MIXED 1       /* MLS coefficients factorised once in FP64,
+                                 transport contraction, collision and storage
+                                 in FP32; conservation sums, wall scaling and
+                                 body loads in FP64 */
+#define AB_PREC_FP64_TABLE 2  /* the MIXED pipeline with FP64 storage and
+                                 arithmetic */
+
+typedef struct ab_ctx ab_ctx;
+
+typedef struct ab_config {
+    int dim;                  /* 2 or 3 */
+    int n_particles;          /* N */
+    int nv;                   /* velocity nodes per axis (SimParameters::Nv) */
+    double vmin, vmax;        /* velocity grid, dv = (vmax-vmin)/(nv-1) */
+    double R;                 /* gas constant (GasConstants::R) */
+    double tau;               /* BGK relaxation time (SimParameters::tao) */
+    double radius;            /* MLS support = neighbour radius (cp.radius) */
+    int mls_order;            /* 1 linear, 2 quadratic (SimParameters::mlsOrder) */
+    double mls_tol;           /* relative Cholesky pivot tolerance; <= 0 -> 1e-12
+                                 (ALEBGK_MLS_TOL, CudaBackend.cu:680) */
+    int mls_clamp;            /* clamp reconstructions to the neighbour range
+                                 (ALEBGK_MLS_CLAMP, CudaBackend.cu:691) */
+    int no_mass_fix;          /* skip the end-of-step mass restore
+                                 (ALEBGK_NO_MASS_FIX, CudaBackend.cu:1768) */
+    int precision;            /* AB_PREC_* */
+    /* Lattice the cloud was generated on, particle p = ix + nx*(iy + ny*iz)
+     * (Initialization/ParticleGeneration.cpp:119-185). Required by the table
+     * pipelines (MIXED, FP64_TABLE); 0 when the cloud is not a pure lattice. */
+    int nx, ny, nz;
+    /* Voxel grid of the neighbour search (CalcParameters hBox.. nbwBox). */
+    double xleft, ybottom, zfront;
+    double hbox, vbox, wbox;
+    int nbhbox, nbvbox, nbwbox;
+} ab_config;
+
+/* Borrowed host view of a cloud (the SoA fields cudaUploadCloud gathers,
+ * Solver/CudaBackend.cu:1223-1381). Any of mvx/mvy/mvz may be NULL (zero
+ * mesh velocity). neigh/nneigh may be NULL: then ab_upload rebuilds the lists
+ * on the device. */
+typedef struct ab_cloud_view {
+    const double *x, *y, *z;
+    const unsigned char *boundary;
+    const double *wnx, *wny, *wnz;
+    const double *wux, *wuy, *wuz;
+    const double *wt;
+    const double *mvx, *mvy, *mvz;
+    const double *rho, *ux, *uy, *uz, *t;
+    const double *g;
+    const int *neigh;
+    const int *nneigh;
+} ab_cloud_view;
+
+/* Per-step report (cudaLimiterCounts / cudaMlsCounts / cudaMlsOvershoot /
+ * cudaMassBudget, Solver/CudaBackend.cu:1986-2030). Filling it costs one
+ * host synchronisation. */
+typedef struct ab_step_stats {
+    double mass_old;               /* total mass at the start of the step */
+    double mass_after_transport;   /* after limiters and commit, before rescale */
+    double mass_after_collision;   /* after wall + collision, before rescale */
+    int n_positivity;              /* particles the positivity repair touched */
+    int n_nad;                     /* particles the NAD clamp touched */
+    int n_mls_degenerate;          /* particles with a collapsed MLS pivot */
+    int n_mls_extrapolate;         /* particles with an overshooting value */
+    double mls_ovr_sum, mls_ovr_max;
+} ab_step_stats;
+
+/* Lifecycle. Replaces the file-static DeviceCloud (CudaBackend.cu:115) with
+ * an explicit handle. */
+int ab_create(const ab_config *cfg, ab_ctx **out);
+int ab_destroy(ab_ctx *ctx);
+const char *ab_last_error(const ab_ctx *ctx);
+
+/* cudaUploadCloud (Solver/CudaBackend.cu:1223). Re-upload with the same N
+ * reuses the device buffers. For the table pipelines this also builds the
+ * MLS coefficient tables (the FP64 factorisation). */
+int ab_upload(ab_ctx *ctx, const ab_cloud_view *view);
+
+/* updateVoxelsAndNeighbours (Initialization/NeighbourSearch.cpp:17) on the
+ * device: integer counting sort into voxels, then the 27-voxel scan in the
+ * reference order; lists are identical to the reference's. */
+int ab_rebuild_neighbours(ab_ctx *ctx);
+int ab_download_neighbours(ab_ctx *ctx, int *neigh, int *nneigh);
+
+/* cudaStep (Solver/CudaBackend.cu:1505). stats may be NULL (no host sync). */
+int ab_step(ab_ctx *ctx, double dt, ab_step_stats *stats);
+/* nsteps calls of ab_step without stats, enqueued back to back. */
+int ab_steps(ab_ctx *ctx, double dt, int nsteps);
+
+/* The transport stage alone (kCopyGt + kTransportNode3DT / kTransportNode,
+ * CudaBackend.cu:191-300, 699-869): gt from the current g, g untouched.
+ * AB_PREC_FP64 only. */
+int ab_transport_only(ab_ctx *ctx, double dt);
+
+/* cudaDownloadFull / cudaDownloadMoments (CudaBackend.cu:2056, 1917).
+ * ab_download_moments: any pointer may be NULL; limflag gets the per-particle
+ * limiter bits of the last step (1 positivity, 2 NAD). */
+int ab_download_g(ab_ctx *ctx, double *g);
+int ab_download_gt(ab_ctx *ctx, double *gt);
+int ab_download_moments(ab_ctx *ctx, double *rho, double *ux, double *uy,
+                        double *uz, double *t, double *limflag);
+
+/* cudaTotalMass (CudaBackend.cu:2018). */
+int ab_total_mass(ab_ctx *ctx, double *mass);
+
+/* Surface-stress loads on rigid bodies, computeBodyLoads
+ * (RigidBody/BodyDynamics.cpp:17-97), reduced on the device in FP64.
+ * surf_idx[k] is a particle index, surf_body[k] its body id (< nbodies);
+ * body_c is 3*nbodies centres, body_area the per-particle surface area of
+ * each body; wu (3*nsurf) is the wall velocity of each surface particle, or
+ * NULL to use the uploaded wall velocity. out = 6*nbodies
+ * (Fx, Fy, Fz, Tx, Ty, Tz per body; 2-D: Fx, Fy, 0, 0, 0, Tz). */
+int ab_body_loads(ab_ctx *ctx, int nsurf, const int *surf_idx,
+                  const int *surf_body, int nbodies, const double *body_c,
+                  const double *body_area, const double *wu, double *out);
+
+/* Update wall state and mesh velocity of selected particles after a body
+ * moved (the uploads of cudaSyncBodies, CudaBackend.cu:1884-1915). Any array
+ * may be NULL. */
+int ab_set_wall_state(ab_ctx *ctx, int n, const int *idx, const double *x,
+                      const double *y, const double *z, const double *wn,
+                      const double *wu, const double *mv);
+
+int ab_synchronize(ab_ctx *ctx);
+
+/* Per-stage device timing with HIP events on the context's stream (the
+ * [GPU-STEADY] table of CudaBackend.cu:1518-1623). After ab_profile(ctx, 1),
+ * every step records events around its stages; ab_profile_read returns the
+ * accumulated milliseconds and launch counts per stage, in the order of
+ * ab_stage_name(k) for k < *nstages. */
+int ab_profile(ab_ctx *ctx, int enable);
+int ab_profile_read(ab_ctx *ctx, double *ms, long *counts, int *nstages);
+const char *ab_stage_name(int k);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle.
+
+Tolerances (stated per check):
+  neighbour lists                 bit-exact
+  FP64 transport                  1e-12 of each particle's largest |g|
+  FP64 step (g, rho, u, T)        1e-10 relative (north star: FP64 moments
+                                  to 1e-10 of the oracle)
+  FP64_TABLE vs FP64              1e-10 (same arithmetic, coefficients
+                                  factorised once)
+  MIXED vs FP64                   1e-5 relative on rho, u, T (north star)
+  body loads                      1e-12 relative to the reference golden
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import alebgk4paper_amd as A
+import oracle_ctypes as O
+from conftest import load_fixture
+from helpers import moment_errors, rel_err, setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+# ------------------------------------------------------------- neighbours ----
+
+@pytest.mark.parametrize("name,case", [("cavity3d_n6_v5", "cavity3d"),
+                                       ("cavity3d_n9_v3", "cavity3d"),
+                                       ("cavity2d_n10_v8", "cavity2d")])
+def test_device_neighbours_match_reference(name, case):
+    fx = load_fixture(name)
+    m = fx.meta
+    _say(f"[neigh] {name}")
+    cl = A.build_lattice(A.configure(case, nx=m["Nx"], nv=m["Nv"]))
+    sol = A.Solver(cl)                       # no lists given: device rebuild
+    nb, nn = sol.neighbours()
+    assert np.array_equal(nn, fx["nneigh"])
+    assert np.array_equal(nb, fx.neigh_padded())
+
+
+def test_device_neighbours_match_oracle_metric_size():
+    _say("[neigh] cavity3d 40^3")
+    case, cl, ocl, neigh, nn = setup("cavity3d", 40, 3, perturb=False)
+    sol = A.Solver(cl)
+    nb, dnn = sol.neighbours()
+    assert np.array_equal(dnn, nn)
+    assert np.array_equal(nb, neigh)
+    assert nn.max() == 80
+
+
+# -------------------------------------------------------------- transport ----
+
+@pytest.mark.parametrize("nx,nv,order", [(6, 5, 1), (6, 5, 2), (9, 4, 2), (12, 6, 2)])
+def test_fp64_transport_matches_oracle(nx, nv, order):
+    _say(f"[transport] cavity3d nx={nx} nv={nv} order={order}")
+    case, cl, ocl, neigh, nn = setup("cavity3d", nx, nv, order=order)
+    cl.neigh, cl.nneigh = neigh, nn
+    dx = 1e-6 / (nx - 1)
+    dt = 0.5 * dx / 1303.0
+    sol = A.Solver(cl, precision=A.PREC_FP64)
+    sol.transport_only(dt)
+    gt = sol.gt()
+    want, fl, _ = O.transport(ocl, cl.g, dt, cl.cfg.radius, order, flags=True)
+    scale = np.abs(cl.g).max(axis=1, keepdims=True)
+    err = np.abs(gt - want) / scale
+    assert err.max() < 1e-12, err.max()
+
+
+@pytest.mark.parametrize("nx,nv,order", [(10, 6, 1), (10, 6, 2)])
+def test_fp64_transport_2d_matches_oracle(nx, nv, order):
+    _say(f"[transport2d] cavity2d nx={nx} nv={nv} order={order}")
+    case, cl, ocl, neigh, nn = setup("cavity2d", nx, nv, order=order)
+    dx = 1e-6 / (nx - 1)
+    dt = 0.5 * dx / 1303.0
+    sol = A.Solver(cl, precision=A.PREC_FP64)
+    sol.transport_only(dt)
+    gt = sol.gt()
+    want = O.transport(ocl, cl.g, dt, cl.cfg.radius, order)
+    scale = np.abs(cl.g).max(axis=1, keepdims=True)
+    assert (np.abs(gt - want) / scale).max() < 1e-12
+
+
+# ------------------------------------------------------------------ steps ----
+
+@pytest.mark.parametrize("name,nx,nv,order", [("cavity3d", 8, 5, 2), ("cavity3d", 10, 6, 1),
+                                              ("cavity2d", 12, 8, 2)])
+def test_fp64_step_matches_oracle(name, nx, nv, order):
+    _say(f"[step] {name} nx={nx} nv={nv} order={order}")
+    case, cl, ocl, neigh, nn = setup(name, nx, nv, order=order)
+    dx = 1e-6 / (nx - 1)
+    dt = 0.5 * dx / 1303.0
+    sol = A.Solver(cl, precision=A.PREC_FP64)
+    st = O.State(ocl, cl.g, cl.mom)
+    for k in range(3):
+        stats = sol.step(dt, stats=True)
+        m_old = O.step(ocl, st, dt, case.tau, case.R, cl.cfg.radius, order)
+        assert abs(stats["mass_old"] - m_old) <= 1e-13 * m_old
+    g = sol.g()
+    mom, lim = sol.moments(with_limflag=True)
+    assert rel_err(g, st.g) < 1e-10
+    om = st.mom
+    cols = [0, 1, 2, 3, 4] if cl.cfg.dim == 3 else [0, 1, 2, 4]
+    for c in cols:
+        assert rel_err(mom[:, c], om[:, c], floor=1e-300) < 1e-10, c
+    assert np.array_equal(lim, st.limflag)
+
+
+def test_fp64_step_conserves_mass_and_is_deterministic():
+    _say("[step] determinism cavity3d 10^3 nv6")
+    case, cl, ocl, neigh, nn = setup("cavity3d", 10, 6, order=2)
+    dt = 0.5 * (1e-6 / 9) / 1303.0
+    runs = []
+    for _ in range(2):
+        sol = A.Solver(cl, precision=A.PREC_FP64)
+        m0 = sol.total_mass()
+        sol.steps(dt, 4)
+        runs.append(sol.g())
+        assert abs(sol.total_mass() - m0) < 1e-13 * m0
+        sol.close()
+    assert np.array_equal(runs[0], runs[1])
+
+
+# ------------------------------------------------------- table pipelines ----
+
+def test_fp64_table_matches_fp64_direct():
+    _say("[table] fp64-table vs fp64, cavity3d 10^3 nv6")
+    case, cl, ocl, neigh, nn = setup("cavity3d", 10, 6, order=2)
+    dt = 0.5 * (1e-6 / 9) / 1303.0
+    ref = A.Solver(cl, precision=A.PREC_FP64)
+    tab = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    for _ in range(3):
+        ref.step(dt)
+        tab.step(dt)
+    assert rel_err(tab.g(), ref.g()) < 1e-10
+    mr, mt = ref.moments(), tab.moments()
+    for c in range(5):
+        assert rel_err(mt[:, c], mr[:, c], floor=1e-300) < 1e-10, c
+    assert abs(tab.total_mass() - ref.total_mass()) < 1e-12 * ref.total_mass()
+
+
+@pytest.mark.parametrize("nx,nv,nsteps", [(10, 6, 20), (16, 8, 10)])
+def test_mixed_moments_within_1e5_of_fp64(nx, nv, nsteps):
+    _say(f"[mixed] vs fp64 cavity3d nx={nx} nv={nv} steps={nsteps}")
+    case, cl, ocl, neigh, nn = setup("cavity3d", nx, nv, order=2)
+    dt = case.dt
+    ref = A.Solver(cl, precision=A.PREC_FP64)
+    mix = A.Solver(cl, precision=A.PREC_MIXED)
+    for _ in range(nsteps):
+        ref.step(dt)
+        mix.step(dt)
+    inner = cl.boundary == 0
+    e = moment_errors(mix.moments(), ref.moments(), inner)
+    _say(f"  errors {e}")
+    assert max(e.values()) < 1e-5, e
+    assert abs(mix.total_mass() - ref.total_mass()) < 1e-9 * ref.total_mass()
+
+
+def test_mixed_metric_size_properties():
+    """cavity3d 40^3, Nv 10, quadratic: the benchmark configuration. Mass is
+    restored to the FP64 sum, the field stays finite and positive, and the
+    uniform state at rest with resting walls stays uniform."""
+    _say("[mixed] cavity3d 40^3 nv10 properties")
+    case = A.configure("cavity3d", nx=40, nv=10, mls_order=2)
+    cl = A.build_lattice(case, precision=A.PREC_MIXED, mls_order=2)
+    sol = A.Solver(cl)
+    m0 = sol.total_mass()
+    for _ in range(5):
+        stats = sol.step(case.dt, stats=True)
+    assert abs(sol.total_mass() - m0) < 1e-12 * m0
+    g = sol.g()
+    assert np.all(np.isfinite(g))
+    mom = sol.moments()
+    assert np.all(mom[:, 0] > 0) and np.all(np.isfinite(mom))
+    assert stats["n_positivity"] >= 0
+
+
+# ------------------------------------------------------------- body loads ----
+
+def test_body_loads_match_reference(cube):
+    _say("[body] cube loads")
+    m = cube.meta
+    surf = cube["surf_idx"]
+    N = m["N"]
+    g = np.zeros((N, m["gStride"]))
+    g[surf] = cube["g_surf"]
+    wall = cube["wall"].copy()
+    wall[surf, 3:6] = cube["surf_wU"]
+    cfg = A.ABConfig()
+    cfg.dim, cfg.n_particles, cfg.nv = 3, N, m["Nv"]
+    cfg.vmin, cfg.vmax, cfg.R, cfg.tau = m["vmin"], m["vmax"], m["R"], m["tau"]
+    cfg.radius, cfg.mls_order, cfg.precision = m["radius"], 1, A.PREC_FP64
+    mom = np.tile([1.0, 0, 0, 0, 270.0], (N, 1))
+    cl = A.Cloud(cfg, cube["x"], cube["y"], cube["z"], cube["boundary"], wall, mom, g,
+                 neigh=cube.neigh_padded(), nneigh=cube["nneigh"])
+    sol = A.Solver(cl)
+    b = cube["body"]
+    out = sol.body_loads(surf, np.zeros(len(surf), np.int32), b[None, :3], [b[3]])
+    ref = cube["loads"]
+    np.testing.assert_allclose(out[0], ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    # the same loads for a trial wall velocity given explicitly (Heun corrector)
+    out2 = sol.body_loads(surf, np.zeros(len(surf), np.int32), b[None, :3], [b[3]],
+                          wu=cube["surf_wU"])
+    np.testing.assert_allclose(out2[0], ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())

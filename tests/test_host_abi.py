@@ -1,0 +1,92 @@
+"""CPU tests of the product's host side and of the C ABI's presence.
+
+No GPU calls: the HIP library is only loaded and its exports checked against
+include/alebgk_dev.h; the host library builds the reference's cases and is
+compared bit for bit with the reference goldens."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import alebgk4paper_amd as A
+from conftest import ROOT, load_fixture
+
+
+def _declared(header, prefix):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_hip_library_exports_every_declared_symbol():
+    names = _declared("alebgk_dev.h", "ab_")
+    assert len(names) >= 15
+    lib = C.CDLL(A.HIP_LIB)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(A.HIP_EXPORTS)
+
+
+def test_host_library_exports_every_declared_symbol():
+    names = _declared("alebgk_host.h", "alh_")
+    lib = C.CDLL(A.HOST_LIB)
+    assert all(hasattr(lib, n) for n in names)
+    assert set(names) == set(A.HOST_EXPORTS)
+
+
+def test_case_table():
+    """configureCase values (TestCases.cpp:113-228) and the overrides."""
+    c = A.configure("cavity3d")
+    assert (c.id, c.dim, c.nx, c.nv, c.dt, c.tau, c.rho0) == (2, 3, 31, 20, 1e-11, 3.7142e-10, 1.0)
+    assert c.mls_order == 1                      # Types.hpp:148 default
+    c = A.configure("6", nx=50)
+    assert (c.name, c.nx, c.nz, c.nv, c.rho0, c.dt, c.lid_speed) == \
+        ("cavity3d-cube", 50, 50, 10, 4.4, 1.5e-11, 10.0)
+    c = A.configure("cavity2d-circle")
+    assert (c.dim, c.nx, c.nv, c.vmax, c.tau) == (2, 60, 30, 1500, 3.37e-11)
+    with pytest.raises(A.ALEBGKError):
+        A.configure("cavity4d")
+    with pytest.raises(A.ALEBGKError):
+        A.configure("7")
+
+
+@pytest.mark.parametrize("name,case", [("cavity3d_n6_v5", "cavity3d"),
+                                       ("cavity3d_n9_v3", "cavity3d"),
+                                       ("cavity2d_n10_v8", "cavity2d")])
+def test_host_lattice_matches_reference(name, case):
+    fx = load_fixture(name)
+    m = fx.meta
+    cs = A.configure(case, nx=m["Nx"], nv=m["Nv"])
+    cl = A.build_lattice(cs)
+    for k in ("x", "y", "z"):
+        assert np.array_equal(getattr(cl, k), fx[k]), k
+    assert np.array_equal(cl.boundary, fx["boundary"])
+    b = cl.boundary == 1
+    assert np.array_equal(cl.wall[b], fx["wall"][b])
+    assert np.array_equal(cl.g, fx["g_ic"])
+    cfg = cl.cfg
+    assert (cfg.hbox, cfg.vbox, cfg.radius) == (m["hBox"], m["vBox"], m["radius"])
+    assert (cfg.nbhbox, cfg.nbvbox) == (m["nbhBox"], m["nbvBox"])
+    if m["dim"] == 3:
+        assert (cfg.wbox, cfg.nbwbox) == (m["wBox"], m["nbwBox"])
+
+
+def test_host_lattice_rejects_body_cases():
+    with pytest.raises(A.ALEBGKError):
+        A.build_lattice(A.configure("cavity3d-cube", nx=8, nv=4))
+
+
+def test_solver_fails_loudly_without_gpu():
+    """No CPU fallback: without a device the context cannot be created."""
+    try:
+        import torch  # noqa: F401
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    cl = A.build_lattice(A.configure("cavity3d", nx=6, nv=4))
+    with pytest.raises(A.ALEBGKError):
+        A.Solver(cl)
