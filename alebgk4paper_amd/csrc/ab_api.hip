@@ -541,10 +541,10 @@ int ab_body_loads(ab_ctx *c, int nsurf, const int *surf_idx, const int *surf_bod
     HIPCHK(c, hipMemcpyAsync(c->bidx, surf_idx, 4 * (size_t)nsurf, hipMemcpyHostToDevice, c->stream));
     if (table_mode(c))
         launch_body_moments(c->view(), nullptr, c->gs, c->cfg.precision == AB_PREC_MIXED, c->S,
-                            c->scal + S_LAZY, c->bidx, nsurf, c->bmom, c->stream);
+                            c->scal + S_LAZY, c->m0, c->bidx, nsurf, c->bmom, c->stream);
     else
-        launch_body_moments(c->view(), c->g, nullptr, 0, 0, nullptr, c->bidx, nsurf, c->bmom,
-                            c->stream);
+        launch_body_moments(c->view(), c->g, nullptr, 0, 0, nullptr, nullptr, c->bidx, nsurf,
+                            c->bmom, c->stream);
     std::vector<double> m((size_t)nsurf * 10);
     HIPCHK(c, hipMemcpyAsync(m.data(), c->bmom, 8 * m.size(), hipMemcpyDeviceToHost, c->stream));
     // positions, normals and (if not given) wall velocities of the surface

@@ -93,6 +93,27 @@ __device__ __forceinline__ void block_minmax(T &mn, T &mx, T *sh)
     __syncthreads();
 }
 
+// c_n = (v - U_w) . n, the sign that gates transport and splits the diffuse
+// wall's fluxes (CudaBackend.cu:715-719, 974-986). Evaluated without FMA
+// contraction, as the reference writes it: on an edge particle the two
+// products are exact negatives for the nodes with v_x = v_y and c_n is then
+// exactly 0 (neither incoming nor outgoing); a fused multiply-add leaves a
+// +-1e-13 residue that flips those nodes.
+__device__ __forceinline__ double wall_cn3(double vx, double vy, double vz, double Ux,
+                                           double Uy, double Uz, double nx, double ny,
+                                           double nz)
+{
+#pragma clang fp contract(off)
+    return (vx - Ux) * nx + (vy - Uy) * ny + (vz - Uz) * nz;
+}
+
+__device__ __forceinline__ double wall_cn2(double vx, double vy, double Ux, double Uy,
+                                           double nx, double ny)
+{
+#pragma clang fp contract(off)
+    return (vx - Ux) * nx + (vy - Uy) * ny;
+}
+
 // maxwellian3D (MLSTransport/MLSBasis.hpp:112-122), FP64.
 __device__ __forceinline__ double maxwellian3d(double vx, double vy, double vz,
                                                double ux, double uy, double uz,

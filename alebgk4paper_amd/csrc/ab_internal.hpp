@@ -96,6 +96,9 @@ struct ab_ctx
     int *lat = nullptr;            // lattice index (ix, iy, iz) per particle, 3*N
     int ncls = 0;
     double coef_dt = -1.0;         // dt the tables were built for
+    double *m0 = nullptr;          // equilibrium the stored deviation is taken from, S
+    double m0mom[5] = {0};         // its moments: sum M0 dv^3, sum v M0 dv^3, sum |v|^2 M0 dv^3
+    double m0rho = 1.0, m0T = 270.0;
     double *part = nullptr;        // per-particle partials of the table pipeline
     int *bad = nullptr;            // stencil check result
 
@@ -167,7 +170,7 @@ void table_free(ab_ctx *ctx);
 
 // ---- k_body.hip --------------------------------------------------------------
 void launch_body_moments(const DevCloud &c, const double *g_fp64, const void *g_table,
-                         int table_fp32, int S, const double *lazy, const int *idx, int n,
-                         double *mom, hipStream_t s);
+                         int table_fp32, int S, const double *lazy, const double *m0,
+                         const int *idx, int n, double *mom, hipStream_t s);
 
 }  // namespace ab

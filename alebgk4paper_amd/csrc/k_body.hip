@@ -15,9 +15,11 @@ namespace ab {
 
 namespace {
 
+// Table storage holds g = L (M0 + d); m0 == nullptr means g is stored as is.
 template <typename T>
 __global__ void k_body_moments(DevCloud c, const T *__restrict__ g, int stride, const double *lazy,
-                               const int *__restrict__ idx, int n, double *__restrict__ mom)
+                               const double *__restrict__ m0, const int *__restrict__ idx, int n,
+                               double *__restrict__ mom)
 {
     const int k = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
@@ -33,7 +35,7 @@ __global__ void k_body_moments(DevCloud c, const T *__restrict__ g, int stride, 
         const double vx = c.vlo + (q % Nv) * c.dv;
         const double vy = c.vlo + ((q / Nv) % Nv) * c.dv;
         const double vz = (c.dim == 3) ? c.vlo + (q / (Nv * Nv)) * c.dv : 0.0;
-        const double f = (double)p[q] * scale * dvd;
+        const double f = ((double)p[q] + (m0 ? m0[q] : 0.0)) * scale * dvd;
         m[0] += f;
         m[1] += f * vx; m[2] += f * vy; m[3] += f * vz;
         m[4] += f * vx * vx; m[5] += f * vy * vy; m[6] += f * vz * vz;
@@ -47,18 +49,20 @@ __global__ void k_body_moments(DevCloud c, const T *__restrict__ g, int stride, 
 }  // namespace
 
 void launch_body_moments(const DevCloud &c, const double *g_fp64, const void *g_table,
-                         int table_fp32, int S, const double *lazy, const int *idx, int n,
-                         double *mom, hipStream_t s)
+                         int table_fp32, int S, const double *lazy, const double *m0,
+                         const int *idx, int n, double *mom, hipStream_t s)
 {
     if (n <= 0) return;
     const int nb = (n * kWave + 255) / 256;
     if (g_fp64)
-        k_body_moments<double><<<nb, 256, 0, s>>>(c, g_fp64, c.gstride, nullptr, idx, n, mom);
-    else if (table_fp32)
-        k_body_moments<float><<<nb, 256, 0, s>>>(c, (const float *)g_table, S, lazy, idx, n, mom);
-    else
-        k_body_moments<double><<<nb, 256, 0, s>>>(c, (const double *)g_table, S, lazy, idx, n,
+        k_body_moments<double><<<nb, 256, 0, s>>>(c, g_fp64, c.gstride, nullptr, nullptr, idx, n,
                                                   mom);
+    else if (table_fp32)
+        k_body_moments<float><<<nb, 256, 0, s>>>(c, (const float *)g_table, S, lazy, m0, idx, n,
+                                                 mom);
+    else
+        k_body_moments<double><<<nb, 256, 0, s>>>(c, (const double *)g_table, S, lazy, m0, idx,
+                                                  n, mom);
 }
 
 }  // namespace ab

@@ -66,8 +66,8 @@ __global__ __launch_bounds__(kBlock) void k_transport3d(DevCloud c, const double
         bool gated = false;
         if (BND)
         {
-            double cn = (vx - c.wux[i]) * c.wnx[i] + (vy - c.wuy[i]) * c.wny[i]
-                      + (vz - c.wuz[i]) * c.wnz[i];
+            double cn = wall_cn3(vx, vy, vz, c.wux[i], c.wuy[i], c.wuz[i], c.wnx[i], c.wny[i],
+                                 c.wnz[i]);
             gated = (cn >= 0.0);
         }
         if (!gated)
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double
     bool gated = false;
     if (BND)
     {
-        double cn = (vx - c.wux[i]) * c.wnx[i] + (vy - c.wuy[i]) * c.wny[i];
+        double cn = wall_cn2(vx, vy, c.wux[i], c.wuy[i], c.wnx[i], c.wny[i]);
         gated = (cn >= 0.0);
     }
     if (!gated)
@@ -508,8 +508,8 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
             double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
             double vz = (DIM == 3) ? c.vlo + (k / (Nv * Nv)) * c.dv : 0.0;
             double v = doScale ? gp[k] * s1 : gp[k];
-            double cn = (DIM == 3) ? (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz
-                                   : (vx - Uwx) * nx + (vy - Uwy) * ny;
+            double cn = (DIM == 3) ? wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz)
+                                   : wall_cn2(vx, vy, Uwx, Uwy, nx, ny);
             if (cn < 0.0) r[0] += cn * v * dvd;
             else if (cn > 0.0)
             {
@@ -530,8 +530,8 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
             double v = doScale ? gp[k] * s1 : gp[k];
             double v2 = 0.0;
             if constexpr (DIM == 2) v2 = doScale ? gp[k + nodes] * s1 : gp[k + nodes];
-            double cn = (DIM == 3) ? (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz
-                                   : (vx - Uwx) * nx + (vy - Uwy) * ny;
+            double cn = (DIM == 3) ? wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz)
+                                   : wall_cn2(vx, vy, Uwx, Uwy, nx, ny);
             if (cn > 0.0)
             {
                 double e2 = (DIM == 3) ? (vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
                                                   double *uz, double *T,
                                                   double *__restrict__ pmass)
 {
-    __shared__ double sh[5 * (kBlock / kWave)];
+    __shared__ double sh[6 * (kBlock / kWave)];
     const int i = xcd_remap(blockIdx.x, gridDim.x);
     const double s2 = scal[S_SCALE2];
     const bool doScale = (s2 != 1.0);

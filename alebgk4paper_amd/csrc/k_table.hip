@@ -16,7 +16,7 @@
 // (ox, oy) columns of 5 or 3 z-offsets.
 //
 // Step (four kernels, no host synchronisation):
-//   k_apply      gt = lazy * sum_o C g     one wave = 64 consecutive nodes of a
+//   k_apply      dt = sum_o C d  (stored deviation, below) one wave = 64 nodes of a
 //                lattice column (ix, iy); marches along z in blocks of K
 //                outputs with the class coefficients in registers and each
 //                loaded value reused by up to 5 outputs.
@@ -32,6 +32,7 @@
 #include "ab_device.hpp"
 #include "ab_internal.hpp"
 
+#include <cmath>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -92,8 +93,8 @@ __device__ void build_class(const DevCloud &c, int i, int lin, const int *lat, d
     const int self = canon_index(0, 0, 0);
     if (BND)
     {
-        double cn = (vx - c.wux[i]) * c.wnx[i] + (vy - c.wuy[i]) * c.wny[i]
-                  + (vz - c.wuz[i]) * c.wnz[i];
+        double cn = wall_cn3(vx, vy, vz, c.wux[i], c.wuy[i], c.wuz[i], c.wnx[i], c.wny[i],
+                             c.wnz[i]);
         if (cn >= 0.0) { out[(size_t)self * S] = T(1); return; }
     }
     const double xf = c.x[i] - (vx - c.mvx[i]) * dt;
@@ -194,7 +195,7 @@ __device__ void build_class(const DevCloud &c, int i, int lin, const int *lat, d
             cm *= w;
         }
         int o = canon_index(lat[3 * j] - ix, lat[3 * j + 1] - iy, lat[3 * j + 2] - iz);
-        out[(size_t)o * S] = T(cm);
+        if (o >= 0) out[(size_t)o * S] = T(cm);
     }
 }
 
@@ -324,7 +325,7 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
     if (wv >= ncol * L.nchunk) return;
     const int chunk = wv % L.nchunk, col = wv / L.nchunk;
     const int ix = col % L.nx, iy = col / L.nx;
-    const T lazy = T(scal[S_LAZY]);
+    (void)scal;
     const int cx = category(ix, L.nx), cy = category(iy, L.ny);
     const size_t plane = (size_t)L.nx * L.ny;
     // this wave's chunk of nodes: offset every base pointer once
@@ -340,7 +341,7 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
         const int cls = cx + 5 * (cy + 5 * category(iz, L.nz));
         const T *cf = coef + (size_t)cls * kOff * L.S + c0;
         T acc = apply_one(gw, cf, L, ix, iy, iz, voff);
-        bstore(gtw + ((size_t)col + plane * iz) * L.S, voff, acc * lazy);
+        bstore(gtw + ((size_t)col + plane * iz) * L.S, voff, acc);
     }
 
     // Interior planes iz = 2 .. nz-3 share one class: coefficients in registers.
@@ -360,22 +361,38 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (iz0 + k <= L.nz - 3)
-                bstore(gtw + ((size_t)col + plane * (iz0 + k)) * L.S, voff, acc[k] * lazy);
+                bstore(gtw + ((size_t)col + plane * (iz0 + k)) * L.S, voff, acc[k]);
     }
 }
 
 // ---- per-particle limiters, wall, moments, collision -------------------------
-// Partial sums written per particle (FP64):
+// Storage: g = L * (M0 + d), d in the stored type, M0 the equilibrium
+// Maxwellian of the reference state (rho_ref, u = 0, T_ref) in FP64 and L the
+// step's pending rescale. A low-Mach flow is a small deviation from M0, so
+// FP32 keeps ~7 digits of the deviation itself instead of ~7 digits of g
+// (the velocity moment of a 1 m/s flow is 3e-3 of the thermal moments).
+// MLS rows sum to 1, so the transport of M0 is M0 and only d is applied.
+// Everything below works in L-units (values / L); partials are written in
+// true units.
+//
+// Partial sums written per particle (FP64, true units):
 //   0  mass after limiters and commit (before the first rescale)
-//   1  mass after wall / collision, unscaled
-//   2..4  sum v_a h dv^3 (interior)   | 2 = rho_w unscaled (boundary)
+//   1  mass after wall / collision, before the rescales
+//   2..4  sum v_a h dv^3 (interior)   | 2 = rho_w before the rescales (boundary)
 //   5  sum |v|^2 h dv^3 (interior)
 //   6  limiter bits (1 positivity, 2 NAD)
 constexpr int kPart = 8;
 
+struct Ref
+{
+    const double *m0;       // M0 per node, S entries
+    double mom[5];          // sum M0 dv^3, sum v_a M0 dv^3, sum |v|^2 M0 dv^3
+    double rho, T;          // reference state of M0
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__ g,
-                                              const T *__restrict__ gt,
+                                              const T *__restrict__ gt, Ref ref,
                                               const double *__restrict__ rho,
                                               const double *__restrict__ scal, double dt,
                                               double tau, double R, double *__restrict__ part)
@@ -383,52 +400,58 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
     const int i = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (i >= c.N) return;
-    const T lazy = T(scal[S_LAZY]);
+    const double Lo = scal[S_LAZY];
     T *gp = g + (size_t)i * L.S;
     const T *tp = gt + (size_t)i * L.S;
+    const double *m0 = ref.m0;
     const int nodes = c.nodes, Nv = c.Nv;
     const double dv3 = c.dv * c.dv * c.dv;
 
-    // positivity: negatives fall back to the pre-transport value (lazy * g)
+    // positivity: negatives fall back to the pre-transport value
     double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
     for (int k = lane; k < nodes; k += kWave)
     {
-        T v = tp[k];
-        if (v >= T(0)) mPos += (double)v;
-        else { mNeg += (double)v; mFb += (double)(gp[k] * lazy); }
+        const double v = m0[k] + (double)tp[k];
+        if (v >= 0.0) mPos += v;
+        else { mNeg += v; mFb += m0[k] + (double)gp[k]; }
     }
     mPos = wave_sum(mPos); mNeg = wave_sum(mNeg); mFb = wave_sum(mFb);
-    const bool fires = (mNeg < -1e-30 && (mPos + mNeg) > 0);
+    const bool fires = (Lo * mNeg < -1e-30 && (mPos + mNeg) > 0);
     double sc = 1.0;
     if (fires)
     {
-        double replaced = mPos + mFb;
-        sc = (replaced > 1e-30) ? (mPos + mNeg) / replaced : 0.0;
+        const double replaced = mPos + mFb;
+        sc = (Lo * replaced > 1e-30) ? (mPos + mNeg) / replaced : 0.0;
     }
-    const T scT = T(sc);
-    auto limited = [&](int k) -> T {
-        T v = tp[k];
-        if (fires) v = ((v < T(0)) ? gp[k] * lazy : v) * scT;
-        return v;
+    // deviation after the positivity repair, FP64
+    auto limited = [&](int k) -> double {
+        const double dg = (double)tp[k];
+        if (!fires) return dg;
+        const double sel = (m0[k] + dg < 0.0) ? (double)gp[k] : dg;
+        return sc * sel + (sc - 1.0) * m0[k];
     };
     double flag = fires ? 1.0 : 0.0;
     double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const T Lt = T(Lo);
 
     if (!c.bnd[i])
     {
-        // moments of the limited field; the NAD scale multiplies them all
+        // moments of the limited field = moments of M0 + moments of d
         double r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
         for (int k = lane; k < nodes; k += kWave)
         {
-            double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
-            double f = (double)limited(k);
-            r0 += f; r1 += vx * f; r2 += vy * f; r3 += vz * f;
-            r4 += (vx * vx + vy * vy + vz * vz) * f;
+            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+            const double d = limited(k);
+            r0 += d; r1 += vx * d; r2 += vy * d; r3 += vz * d;
+            r4 += (vx * vx + vy * vy + vz * vz) * d;
         }
-        r0 = wave_sum(r0); r1 = wave_sum(r1); r2 = wave_sum(r2); r3 = wave_sum(r3);
-        r4 = wave_sum(r4);
-        // NAD against the neighbours' current densities
+        const double rhoL = ref.mom[0] + wave_sum(r0) * dv3;
+        const double pxL = ref.mom[1] + wave_sum(r1) * dv3;
+        const double pyL = ref.mom[2] + wave_sum(r2) * dv3;
+        const double pzL = ref.mom[3] + wave_sum(r3) * dv3;
+        const double EL = ref.mom[4] + wave_sum(r4) * dv3;
+        // NAD against the neighbours' current densities (true units)
         const int nn = c.nneigh[i];
         double rmin = 1e300, rmax = -1e300;
         for (int k = lane; k < nn; k += kWave)
@@ -437,7 +460,7 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
             rmin = fmin(rmin, q); rmax = fmax(rmax, q);
         }
         rmin = wave_min(rmin); rmax = wave_max(rmax);
-        const double rho_new = r0 * dv3;
+        const double rho_new = Lo * rhoL;
         double sc2 = 1.0;
         if (!(rmin > rmax))
         {
@@ -448,42 +471,48 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
             else if (rho_new < lo && rho_new > 1e-30) { sc2 = lo / rho_new; nad = true; }
             if (nad) flag += 2.0;
         }
-        out[0] = sc2 * rho_new;                              // committed mass
-        // moments (unscaled by the global rescale; u and T are invariant)
-        double mr = sc2 * rho_new, mux = 0, muy = 0, muz = 0, mT = 270.0;
-        bool okm = mr > 1e-30;
+        out[0] = sc2 * rho_new;                                // committed mass
+        // moments of the committed field (L-units); u and T are ratios
+        const double mr = sc2 * rhoL;
+        double mux = 0, muy = 0, muz = 0, mT = ref.T;
+        const bool okm = Lo * mr > 1e-30;
         if (okm)
         {
-            mux = r1 / r0; muy = r2 / r0; muz = r3 / r0;
-            double u2 = mux * mux + muy * muy + muz * muz;
-            double t1 = (r4 / r0 - u2) / (3.0 * R);
+            mux = pxL / rhoL; muy = pyL / rhoL; muz = pzL / rhoL;
+            const double u2 = mux * mux + muy * muy + muz * muz;
+            const double t1 = (EL / rhoL - u2) / (3.0 * R);
             mT = t1 > 1e-6 ? t1 : 1e-6;
         }
+        // collision: h = coeff (g'' + lam M)  =>  dh = coeff (d'' + lam (M - M0))
+        // with M - M0 = M0 expm1(D),
+        // D = ln(rho/rho_ref) - 1.5 ln(T/T_ref) - |u|^2/2RT + |v|^2 (1/2RT_ref - 1/2RT) + v.u/RT
         const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
-        const double twoRT = 2.0 * R * mT;
-        const T pref = T(okm ? coeff * lam * mr / pow(AB_PI * twoRT, 1.5) : 0.0);
-        const T inv2RT = T(1.0 / twoRT);
-        const T cT = T(coeff * sc2);
-        const T tux = T(mux), tuy = T(muy), tuz = T(muz);
+        const double A0 = okm ? log(mr / ref.rho) - 1.5 * log(mT / ref.T)
+                                    - (mux * mux + muy * muy + muz * muz) / (2.0 * R * mT)
+                              : 0.0;
+        const double Bq = 1.0 / (2.0 * R * ref.T) - 1.0 / (2.0 * R * mT);
+        const double Cx = mux / (R * mT), Cy = muy / (R * mT), Cz = muz / (R * mT);
+        const T cT = T(coeff), lamT = T(lam);
         double h0 = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
         for (int k = lane; k < nodes; k += kWave)
         {
-            const T vx = T(c.vlo + (k % Nv) * c.dv), vy = T(c.vlo + ((k / Nv) % Nv) * c.dv);
-            const T vz = T(c.vlo + (k / (Nv * Nv)) * c.dv);
-            const T ex = -((vx - tux) * (vx - tux) + (vy - tuy) * (vy - tuy)
-                           + (vz - tuz) * (vz - tuz)) * inv2RT;
-            const T hv = cT * limited(k) + pref * exp(ex);
-            gp[k] = hv;
-            const double hd = (double)hv;
-            const double dvx = (double)vx, dvy = (double)vy, dvz = (double)vz;
-            h0 += hd; h1 += dvx * hd; h2 += dvy * hd; h3 += dvz * hd;
-            h4 += (dvx * dvx + dvy * dvy + dvz * dvz) * hd;
+            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+            const double d2 = sc2 * limited(k) + (sc2 - 1.0) * m0[k];
+            const double D = A0 + Bq * (vx * vx + vy * vy + vz * vz) + Cx * vx + Cy * vy + Cz * vz;
+            const T mm = okm ? T(m0[k]) * expm1(T(D)) : T(-m0[k]);
+            const T dh = cT * (T(d2) + lamT * mm);
+            // fold the pending rescale of this step's input into the stored value
+            gp[k] = Lt * dh + T((Lo - 1.0) * m0[k]);
+            const double hd = (double)dh;
+            h0 += hd; h1 += vx * hd; h2 += vy * hd; h3 += vz * hd;
+            h4 += (vx * vx + vy * vy + vz * vz) * hd;
         }
-        out[1] = wave_sum(h0) * dv3;
-        out[2] = wave_sum(h1) * dv3;
-        out[3] = wave_sum(h2) * dv3;
-        out[4] = wave_sum(h3) * dv3;
-        out[5] = wave_sum(h4) * dv3;
+        out[1] = Lo * (ref.mom[0] + wave_sum(h0) * dv3);
+        out[2] = Lo * (ref.mom[1] + wave_sum(h1) * dv3);
+        out[3] = Lo * (ref.mom[2] + wave_sum(h2) * dv3);
+        out[4] = Lo * (ref.mom[3] + wave_sum(h3) * dv3);
+        out[5] = Lo * (ref.mom[4] + wave_sum(h4) * dv3);
     }
     else
     {
@@ -491,46 +520,43 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
         const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = c.wuz[i], Tw = c.wt[i];
         const double RT2w = 2.0 * R * Tw;
         const double norm = pow(AB_PI * RT2w, 1.5);
-        double m0 = 0, intM = 0, intP = 0;
+        double m0s = 0, intM = 0, intP = 0;
         for (int k = lane; k < nodes; k += kWave)
         {
-            double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
-            double f = (double)limited(k);
-            m0 += f;
-            double cn = (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz;
-            if (cn < 0.0) intM += cn * f * dv3;
+            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+            const double d = limited(k);
+            m0s += d;
+            const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
+            if (cn < 0.0) intM += cn * (m0[k] + d) * dv3;
             else if (cn > 0.0)
             {
-                double Mw = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
-                                  + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm;
+                const double Mw = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
+                                        + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm;
                 intP += cn * Mw * dv3;
             }
         }
-        m0 = wave_sum(m0); intM = wave_sum(intM); intP = wave_sum(intP);
-        const double rho_w = (intP > 1e-30) ? (-intM / intP) : 1.0;
-        out[0] = m0 * dv3;
-        out[2] = rho_w;
-        const T pw = T(rho_w / norm), inv = T(1.0 / RT2w);
-        const T tUx = T(Uwx), tUy = T(Uwy), tUz = T(Uwz);
+        m0s = wave_sum(m0s); intM = wave_sum(intM); intP = wave_sum(intP);
+        // intM is in L-units; the reference's fallback rho_w = 1 is in true units
+        const double rhoWL = (intP > 1e-30) ? (-intM / intP) : 1.0 / Lo;
+        out[0] = Lo * (ref.mom[0] + m0s * dv3);
+        out[2] = Lo * rhoWL;
         double h0 = 0;
         for (int k = lane; k < nodes; k += kWave)
         {
-            double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
-            double cn = (vx - Uwx) * nx + (vy - Uwy) * ny + (vz - Uwz) * nz;
-            T hv;
+            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+            const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
+            double dh;
             if (cn > 0.0)
-            {
-                const T ex = -((T(vx) - tUx) * (T(vx) - tUx) + (T(vy) - tUy) * (T(vy) - tUy)
-                               + (T(vz) - tUz) * (T(vz) - tUz)) * inv;
-                hv = pw * exp(ex);
-            }
-            else hv = limited(k);
-            gp[k] = hv;
-            h0 += (double)hv;
+                dh = rhoWL * exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
+                                   + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm - m0[k];
+            else dh = limited(k);
+            const T dhT = T(dh);
+            gp[k] = Lt * dhT + T((Lo - 1.0) * m0[k]);
+            h0 += (double)dhT;
         }
-        out[1] = wave_sum(h0) * dv3;
+        out[1] = Lo * (ref.mom[0] + wave_sum(h0) * dv3);
     }
     out[6] = flag;
     if (lane < kPart) part[(size_t)i * kPart + lane] = out[lane];
@@ -589,16 +615,17 @@ __global__ void k_finalize(DevCloud c, const double *__restrict__ part,
     }
 }
 
-// Mass of the stored field (lazy factor applied).
+// Mass of the stored field, sum L (M0 + d) dv^3 (L applied by k_mass_from_part).
 template <typename T>
-__global__ void k_table_mass(int N, int nodes, int S, const T *__restrict__ g, double dv3,
+__global__ void k_table_mass(int N, int nodes, int S, const T *__restrict__ g,
+                             const double *__restrict__ m0, double dv3,
                              double *__restrict__ part)
 {
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (i >= N) return;
     double m = 0.0;
-    for (int k = lane; k < nodes; k += kWave) m += (double)g[(size_t)i * S + k];
+    for (int k = lane; k < nodes; k += kWave) m += m0[k] + (double)g[(size_t)i * S + k];
     m = wave_sum(m);
     if (lane == 0) part[(size_t)i * kPart + 7] = m * dv3;
 }
@@ -620,24 +647,26 @@ __global__ __launch_bounds__(1024) void k_mass_from_part(int N, const double *__
     }
 }
 
+// FP64 g -> stored deviation d = g - M0 (L = 1), padded rows.
 template <typename T>
 __global__ void k_pack(int N, int nodes, int gstride, int S, const double *__restrict__ src,
-                       T *__restrict__ dst)
+                       const double *__restrict__ m0, T *__restrict__ dst)
 {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (size_t)N * S) return;
     size_t i = k / S, lin = k % S;
-    dst[k] = (lin < (size_t)nodes) ? T(src[i * gstride + lin]) : T(0);
+    dst[k] = (lin < (size_t)nodes) ? T(src[i * gstride + lin] - m0[lin]) : T(0);
 }
 
 template <typename T>
 __global__ void k_unpack(int N, int gstride, int S, const T *__restrict__ src,
-                         const double *__restrict__ scal, double *__restrict__ dst)
+                         const double *__restrict__ m0, const double *__restrict__ scal,
+                         double *__restrict__ dst)
 {
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (size_t)N * gstride) return;
     size_t i = k / gstride, lin = k % gstride;
-    dst[k] = (double)src[i * S + lin] * scal[S_LAZY];
+    dst[k] = (m0[lin] + (double)src[i * S + lin]) * scal[S_LAZY];
 }
 
 __global__ void k_set_scalar(double *scal, int slot, double v) { scal[slot] = v; }
@@ -663,9 +692,11 @@ int apply_launch(ab_ctx *ctx, const Lat &L)
 
 void table_free(ab_ctx *ctx)
 {
-    void *ptrs[] = {ctx->gs, ctx->gts, ctx->coef, ctx->cls, ctx->lat, ctx->part, ctx->bad};
+    void *ptrs[] = {ctx->gs, ctx->gts, ctx->coef, ctx->cls, ctx->lat, ctx->part, ctx->bad,
+                    ctx->m0};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     ctx->gs = ctx->gts = ctx->coef = nullptr;
+    ctx->m0 = nullptr;
     ctx->cls = ctx->lat = ctx->bad = nullptr;
     ctx->part = nullptr;
 }
@@ -684,7 +715,7 @@ int table_setup(ab_ctx *ctx)
     if (cfg.mls_clamp)
     { ctx->err = "mls_clamp is nonlinear in g; run it with AB_PREC_FP64"; return AB_ERR_UNSUPPORTED; }
 
-    std::vector<double> x(N), y(N), z(N), w(10 * (size_t)N);
+    std::vector<double> x(N), y(N), z(N), w(10 * (size_t)N), hr(N), hT(N);
     std::vector<unsigned char> b(N);
     std::vector<int> nb((size_t)N * AB_MAX_NEIGH), nn(N);
     hipStream_t s = ctx->stream;
@@ -698,6 +729,7 @@ int table_setup(ab_ctx *ctx)
     const double *wsrc[10] = {ctx->wnx, ctx->wny, ctx->wnz, ctx->wux, ctx->wuy, ctx->wuz,
                               ctx->wt, ctx->mvx, ctx->mvy, ctx->mvz};
     for (int k = 0; k < 10 && ok; ++k) ok = dl(&w[(size_t)k * N], wsrc[k], 8 * (size_t)N);
+    ok = ok && dl(hr.data(), ctx->rho, 8 * (size_t)N) && dl(hT.data(), ctx->T, 8 * (size_t)N);
     if (!ok || hipStreamSynchronize(s) != hipSuccess)
     { ctx->err = "table setup: device-to-host copy failed"; return AB_ERR_HIP; }
 
@@ -751,10 +783,41 @@ int table_setup(ab_ctx *ctx)
               && hipMalloc(&ctx->coef, (size_t)kNcls * kOff * ctx->S * es) == hipSuccess
               && hipMalloc((void **)&ctx->lat, 3 * sizeof(int) * (size_t)N) == hipSuccess
               && hipMalloc((void **)&ctx->cls, sizeof(int) * kNcls) == hipSuccess
-              && hipMalloc((void **)&ctx->part, sizeof(double) * kPart * (size_t)N) == hipSuccess;
+              && hipMalloc((void **)&ctx->part, sizeof(double) * kPart * (size_t)N) == hipSuccess
+              && hipMalloc((void **)&ctx->m0, sizeof(double) * ctx->S) == hipSuccess;
         if (!a) { table_free(ctx); ctx->err = "table setup: device allocation failed"; return AB_ERR_NOMEM; }
         if (hipMemsetAsync(ctx->gts, 0, gsz, s) != hipSuccess)
         { ctx->err = "table setup: memset failed"; return AB_ERR_HIP; }
+    }
+    // The equilibrium the deviation is stored against: the mean state of the
+    // uploaded cloud at rest (any reference is exact; a close one keeps the
+    // stored deviation small).
+    double rs = 0.0, ts = 0.0;
+    for (int p = 0; p < N; ++p) { rs += hr[p]; ts += hT[p]; }
+    ctx->m0rho = rs / N;
+    ctx->m0T = ts / N;
+    {
+        const int Nv = ctx->Nv;
+        const double R = cfg.R, twoRT = 2.0 * R * ctx->m0T;
+        const double norm = std::pow(AB_PI * twoRT, 1.5);
+        const double dv3 = ctx->dv * ctx->dv * ctx->dv;
+        std::vector<double> m0(ctx->S, 0.0);
+        for (int q = 0; q < 5; ++q) ctx->m0mom[q] = 0.0;
+        for (int k = 0; k < ctx->nodes; ++k)
+        {
+            const double vx = ctx->vlo + (k % Nv) * ctx->dv;
+            const double vy = ctx->vlo + ((k / Nv) % Nv) * ctx->dv;
+            const double vz = ctx->vlo + (k / (Nv * Nv)) * ctx->dv;
+            m0[k] = ctx->m0rho / norm * std::exp(-(vx * vx + vy * vy + vz * vz) / twoRT);
+            ctx->m0mom[0] += m0[k] * dv3;
+            ctx->m0mom[1] += vx * m0[k] * dv3;
+            ctx->m0mom[2] += vy * m0[k] * dv3;
+            ctx->m0mom[3] += vz * m0[k] * dv3;
+            ctx->m0mom[4] += (vx * vx + vy * vy + vz * vz) * m0[k] * dv3;
+        }
+        if (hipMemcpyAsync(ctx->m0, m0.data(), sizeof(double) * ctx->S, hipMemcpyHostToDevice, s)
+                != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        { ctx->err = "table setup: host-to-device copy failed"; return AB_ERR_HIP; }
     }
     if (hipMemcpyAsync(ctx->lat, lat.data(), 3 * sizeof(int) * (size_t)N, hipMemcpyHostToDevice, s) != hipSuccess
         || hipMemcpyAsync(ctx->cls, rep.data(), sizeof(int) * kNcls, hipMemcpyHostToDevice, s) != hipSuccess)
@@ -787,19 +850,21 @@ int table_upload_g(ab_ctx *ctx, const double *g_dev)
     const unsigned nb = (unsigned)((n + bs - 1) / bs);
     k_set_scalar<<<1, 1, 0, ctx->stream>>>(ctx->scal, S_LAZY, 1.0);
     if (is_fp32(ctx))
-        k_pack<float><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->gstride, ctx->S, g_dev,
+        k_pack<float><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->gstride, ctx->S, g_dev, ctx->m0,
                                                   (float *)ctx->gs);
     else
-        k_pack<double><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->gstride, ctx->S, g_dev,
+        k_pack<double><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->gstride, ctx->S, g_dev, ctx->m0,
                                                    (double *)ctx->gs);
     const int nbm = (ctx->N * kWave + 255) / 256;
     const double dv3 = ctx->dv * ctx->dv * ctx->dv;
     if (is_fp32(ctx))
         k_table_mass<float><<<nbm, 256, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->S,
-                                                          (const float *)ctx->gs, dv3, ctx->part);
+                                                          (const float *)ctx->gs, ctx->m0, dv3,
+                                                          ctx->part);
     else
         k_table_mass<double><<<nbm, 256, 0, ctx->stream>>>(ctx->N, ctx->nodes, ctx->S,
-                                                           (const double *)ctx->gs, dv3, ctx->part);
+                                                           (const double *)ctx->gs, ctx->m0, dv3,
+                                                           ctx->part);
     k_mass_from_part<<<1, 1024, 0, ctx->stream>>>(ctx->N, ctx->part, ctx->scal);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
@@ -811,10 +876,12 @@ int table_download_g(ab_ctx *ctx, double *g_dev)
     const unsigned nb = (unsigned)((n + bs - 1) / bs);
     if (is_fp32(ctx))
         k_unpack<float><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->gstride, ctx->S,
-                                                    (const float *)ctx->gs, ctx->scal, g_dev);
+                                                    (const float *)ctx->gs, ctx->m0, ctx->scal,
+                                                    g_dev);
     else
         k_unpack<double><<<nb, bs, 0, ctx->stream>>>(ctx->N, ctx->gstride, ctx->S,
-                                                     (const double *)ctx->gs, ctx->scal, g_dev);
+                                                     (const double *)ctx->gs, ctx->m0, ctx->scal,
+                                                     g_dev);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 
@@ -834,17 +901,22 @@ int table_step(ab_ctx *ctx, double dt)
     auto tic = [&]() { prof_begin(ctx); };
     auto toc = [&](int st) { prof_end(ctx, st); };
     const int nbp = (ctx->N * kWave + 255) / 256;
+    Ref ref;
+    ref.m0 = ctx->m0;
+    for (int q = 0; q < 5; ++q) ref.mom[q] = ctx->m0mom[q];
+    ref.rho = ctx->m0rho;
+    ref.T = ctx->m0T;
     tic();
     if (is_fp32(ctx)) apply_launch<float>(ctx, L);
     else apply_launch<double>(ctx, L);
     toc(ST_TRANSPORT);
     tic();
     if (is_fp32(ctx))
-        k_post<float><<<nbp, 256, 0, s>>>(c, L, (float *)ctx->gs, (const float *)ctx->gts,
+        k_post<float><<<nbp, 256, 0, s>>>(c, L, (float *)ctx->gs, (const float *)ctx->gts, ref,
                                           ctx->rho, ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R,
                                           ctx->part);
     else
-        k_post<double><<<nbp, 256, 0, s>>>(c, L, (double *)ctx->gs, (const double *)ctx->gts,
+        k_post<double><<<nbp, 256, 0, s>>>(c, L, (double *)ctx->gs, (const double *)ctx->gts, ref,
                                            ctx->rho, ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R,
                                            ctx->part);
     toc(ST_WALL_COLLIDE);

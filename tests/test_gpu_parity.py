@@ -160,6 +160,22 @@ def test_mixed_moments_within_1e5_of_fp64(nx, nv, nsteps):
     assert abs(mix.total_mass() - ref.total_mass()) < 1e-9 * ref.total_mass()
 
 
+def test_mixed_metric_size_moment_error():
+    """The benchmark configuration (cavity3d 40^3, Nv 10, quadratic, lid
+    1 m/s): after 50 steps the mixed path's rho, u and T agree with the FP64
+    reference formulation to 1e-5 relative (|du| relative to max |u|)."""
+    _say("[mixed] vs fp64 cavity3d 40^3 nv10, 50 steps")
+    case = A.configure("cavity3d", nx=40, nv=10, mls_order=2)
+    cl = A.build_lattice(case, mls_order=2)
+    ref = A.Solver(cl, precision=A.PREC_FP64)
+    mix = A.Solver(cl, precision=A.PREC_MIXED)
+    ref.steps(case.dt, 50)
+    mix.steps(case.dt, 50)
+    e = moment_errors(mix.moments(), ref.moments(), cl.boundary == 0)
+    _say(f"  errors {e}")
+    assert max(e.values()) < 1e-5, e
+
+
 def test_mixed_metric_size_properties():
     """cavity3d 40^3, Nv 10, quadratic: the benchmark configuration. Mass is
     restored to the FP64 sum, the field stays finite and positive, and the
