@@ -93,6 +93,16 @@ __device__ __forceinline__ void block_minmax(T &mn, T &mx, T *sh)
     __syncthreads();
 }
 
+// Velocity node vlo + idx*dv, rounded as the reference's CPU code rounds it
+// (product, then sum). A fused multiply-add differs in the last bit for some
+// nodes, and on a corner particle (n = (1,1)/sqrt2) that bit decides whether
+// the node with v_x = -v_y counts as incoming (c_n < 0) or not.
+__host__ __device__ __forceinline__ double vnode(double vlo, double dv, int idx)
+{
+#pragma clang fp contract(off)
+    return vlo + idx * dv;
+}
+
 // c_n = (v - U_w) . n, the sign that gates transport and splits the diffuse
 // wall's fluxes (CudaBackend.cu:715-719, 974-986). Evaluated without FMA
 // contraction, as the reference writes it: on an edge particle the two

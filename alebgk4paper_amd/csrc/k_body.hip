@@ -32,9 +32,9 @@ __global__ void k_body_moments(DevCloud c, const T *__restrict__ g, int stride, 
     double m[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int q = lane; q < c.nodes; q += kWave)
     {
-        const double vx = c.vlo + (q % Nv) * c.dv;
-        const double vy = c.vlo + ((q / Nv) % Nv) * c.dv;
-        const double vz = (c.dim == 3) ? c.vlo + (q / (Nv * Nv)) * c.dv : 0.0;
+        const double vx = vnode(c.vlo, c.dv, q % Nv);
+        const double vy = vnode(c.vlo, c.dv, (q / Nv) % Nv);
+        const double vz = (c.dim == 3) ? vnode(c.vlo, c.dv, q / (Nv * Nv)) : 0.0;
         const double f = ((double)p[q] + (m0 ? m0[q] : 0.0)) * scale * dvd;
         m[0] += f;
         m[1] += f * vx; m[2] += f * vy; m[3] += f * vz;

@@ -26,9 +26,9 @@ __device__ __forceinline__ void node_velocity(const DevCloud &c, int lin, double
 {
     int Nv = c.Nv;
     int iv = lin % Nv, jv = (lin / Nv) % Nv, kv = lin / (Nv * Nv);
-    vx = c.vlo + iv * c.dv;
-    vy = c.vlo + jv * c.dv;
-    vz = (c.dim == 3) ? c.vlo + kv * c.dv : 0.0;
+    vx = vnode(c.vlo, c.dv, iv);
+    vy = vnode(c.vlo, c.dv, jv);
+    vz = (c.dim == 3) ? vnode(c.vlo, c.dv, kv) : 0.0;
 }
 
 // kCopyGt + kTransportNode3DT<P, BND> (Solver/CudaBackend.cu:191-197, 699-869).
@@ -212,8 +212,8 @@ __global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double
     if (lin >= c.nodes) return;
     const size_t gs = (size_t)c.gstride;
     const int Nv2 = c.nodes;
-    const double vx = c.vlo + (lin % c.Nv) * c.dv;
-    const double vy = c.vlo + (lin / c.Nv) * c.dv;
+    const double vx = vnode(c.vlo, c.dv, lin % c.Nv);
+    const double vy = vnode(c.vlo, c.dv, lin / c.Nv);
     double o1 = g[(size_t)i * gs + lin], o2 = g[(size_t)i * gs + lin + Nv2];
     bool gated = false;
     if (BND)
@@ -504,9 +504,9 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
         double r[2] = {0.0, 0.0};                  // intM, intP
         for (int k = threadIdx.x; k < nodes; k += kBlock)
         {
-            double vx = c.vlo + (k % Nv) * c.dv;
-            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            double vz = (DIM == 3) ? c.vlo + (k / (Nv * Nv)) * c.dv : 0.0;
+            double vx = vnode(c.vlo, c.dv, k % Nv);
+            double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
+            double vz = (DIM == 3) ? vnode(c.vlo, c.dv, k / (Nv * Nv)) : 0.0;
             double v = doScale ? gp[k] * s1 : gp[k];
             double cn = (DIM == 3) ? wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz)
                                    : wall_cn2(vx, vy, Uwx, Uwy, nx, ny);
@@ -524,9 +524,9 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
         const double rho_w = (r[1] > 1e-30) ? (-r[0] / r[1]) : 1.0;
         for (int k = threadIdx.x; k < nodes; k += kBlock)
         {
-            double vx = c.vlo + (k % Nv) * c.dv;
-            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            double vz = (DIM == 3) ? c.vlo + (k / (Nv * Nv)) * c.dv : 0.0;
+            double vx = vnode(c.vlo, c.dv, k % Nv);
+            double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
+            double vz = (DIM == 3) ? vnode(c.vlo, c.dv, k / (Nv * Nv)) : 0.0;
             double v = doScale ? gp[k] * s1 : gp[k];
             double v2 = 0.0;
             if constexpr (DIM == 2) v2 = doScale ? gp[k + nodes] * s1 : gp[k + nodes];
@@ -555,12 +555,12 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
         double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         for (int k = threadIdx.x; k < nodes; k += kBlock)
         {
-            double vx = c.vlo + (k % Nv) * c.dv;
-            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vx = vnode(c.vlo, c.dv, k % Nv);
+            double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
             double f = doScale ? gp[k] * s1 : gp[k];
             if constexpr (DIM == 3)
             {
-                double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+                double vz = vnode(c.vlo, c.dv, k / (Nv * Nv));
                 r[0] += f * dvd;
                 r[1] += vx * f * dvd;
                 r[2] += vy * f * dvd;
@@ -592,12 +592,12 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
         const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
         for (int k = threadIdx.x; k < nodes; k += kBlock)
         {
-            double vx = c.vlo + (k % Nv) * c.dv;
-            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vx = vnode(c.vlo, c.dv, k % Nv);
+            double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
             double f = doScale ? gp[k] * s1 : gp[k];
             if constexpr (DIM == 3)
             {
-                double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+                double vz = vnode(c.vlo, c.dv, k / (Nv * Nv));
                 double M = maxwellian3d(vx, vy, vz, mux, muy, muz, mT, mrho, R);
                 f = coeff * (f + lam * M);
             }
@@ -653,14 +653,14 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
         r[5] += f * dvd;
         if (inner)
         {
-            double vx = c.vlo + (k % Nv) * c.dv;
-            double vy = c.vlo + ((k / Nv) % Nv) * c.dv;
+            double vx = vnode(c.vlo, c.dv, k % Nv);
+            double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
             r[0] += f * dvd;
             r[1] += vx * f * dvd;
             r[2] += vy * f * dvd;
             if constexpr (DIM == 3)
             {
-                double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
+                double vz = vnode(c.vlo, c.dv, k / (Nv * Nv));
                 r[3] += vz * f * dvd;
                 r[4] += (vx * vx + vy * vy + vz * vz) * f * dvd;
             }

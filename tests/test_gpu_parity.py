@@ -71,7 +71,7 @@ def test_fp64_transport_matches_oracle(nx, nv, order):
     assert err.max() < 1e-12, err.max()
 
 
-@pytest.mark.parametrize("nx,nv,order", [(10, 6, 1), (10, 6, 2)])
+@pytest.mark.parametrize("nx,nv,order", [(10, 6, 1), (10, 6, 2), (12, 8, 1), (12, 8, 2)])
 def test_fp64_transport_2d_matches_oracle(nx, nv, order):
     _say(f"[transport2d] cavity2d nx={nx} nv={nv} order={order}")
     case, cl, ocl, neigh, nn = setup("cavity2d", nx, nv, order=order)
