@@ -36,6 +36,7 @@ ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "call out of order",
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _up = C.POINTER(C.c_ubyte)
+_fp = C.POINTER(C.c_float)
 
 
 class ABConfig(C.Structure):
@@ -46,7 +47,8 @@ class ABConfig(C.Structure):
                 ("precision", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("xleft", C.c_double), ("ybottom", C.c_double), ("zfront", C.c_double),
                 ("hbox", C.c_double), ("vbox", C.c_double), ("wbox", C.c_double),
-                ("nbhbox", C.c_int), ("nbvbox", C.c_int), ("nbwbox", C.c_int)]
+                ("nbhbox", C.c_int), ("nbvbox", C.c_int), ("nbwbox", C.c_int),
+                ("collision", C.c_int), ("dm_tol", C.c_double), ("dm_maxit", C.c_int)]
 
 
 class ABCloudView(C.Structure):
@@ -62,7 +64,9 @@ class ABStepStats(C.Structure):
                 ("mass_after_collision", C.c_double), ("n_positivity", C.c_int),
                 ("n_nad", C.c_int), ("n_mls_degenerate", C.c_int),
                 ("n_mls_extrapolate", C.c_int), ("mls_ovr_sum", C.c_double),
-                ("mls_ovr_max", C.c_double)]
+                ("mls_ovr_max", C.c_double), ("dm_particles", C.c_int),
+                ("dm_unconverged", C.c_int), ("dm_passes", C.c_longlong),
+                ("dm_res_max", C.c_double)]
 
 
 class ALHCase(C.Structure):
@@ -94,7 +98,13 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_download_neighbours", "ab_step", "ab_steps", "ab_transport_only",
                "ab_download_g", "ab_download_gt", "ab_download_moments", "ab_total_mass",
                "ab_body_loads", "ab_set_wall_state", "ab_synchronize", "ab_profile",
-               "ab_profile_read", "ab_stage_name"]
+               "ab_profile_read", "ab_stage_name", "ab_dm_download", "ab_dm_record",
+               "ab_dm_rows", "ab_mlp_init", "ab_mlp_train", "ab_mlp_enable", "ab_mlp_get",
+               "ab_mlp_set", "ab_mlp_forward", "ab_mlp_loss_grad"]
+MLP_TRAINABLE = 4869
+MLP_PARAMS = 4889
+COLLISION_CONTINUOUS = 0
+COLLISION_DISCRETE = 1
 HOST_EXPORTS = ["alh_configure", "alh_lattice_size", "alh_build_lattice"]
 
 
@@ -111,6 +121,8 @@ def hip_lib():
         L.ab_step.argtypes = [C.c_void_p, C.c_double, C.c_void_p]
         L.ab_steps.argtypes = [C.c_void_p, C.c_double, C.c_int]
         L.ab_transport_only.argtypes = [C.c_void_p, C.c_double]
+        L.ab_mlp_train.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_ulonglong,
+                                   C.c_void_p, C.c_void_p]
         _hip = L
     return _hip
 
@@ -199,7 +211,7 @@ class Solver:
     """One device context (ab_ctx): the cloud resident on the GPU."""
 
     def __init__(self, cloud: Cloud, precision=None, mls_order=None, no_mass_fix=False,
-                 mls_clamp=False):
+                 mls_clamp=False, collision=None, dm_tol=None, dm_maxit=None):
         self._lib = hip_lib()
         cfg = ABConfig()
         C.memmove(C.byref(cfg), C.byref(cloud.cfg), C.sizeof(ABConfig))
@@ -210,6 +222,12 @@ class Solver:
             cfg.mls_order = int(mls_order)
         cfg.no_mass_fix = int(bool(no_mass_fix))
         cfg.mls_clamp = int(bool(mls_clamp))
+        if collision is not None:
+            cfg.collision = int(collision)
+        if dm_tol is not None:
+            cfg.dm_tol = float(dm_tol)
+        if dm_maxit is not None:
+            cfg.dm_maxit = int(dm_maxit)
         self.cfg = cfg
         self.cloud = cloud
         self.N, self.dim = cloud.N, cfg.dim
@@ -281,6 +299,63 @@ class Solver:
 
     def rebuild_neighbours(self):
         self._check(self._lib.ab_rebuild_neighbours(self._ctx), "ab_rebuild_neighbours")
+
+    # ---- discrete-Maxwellian collision and the MLP warm start --------------
+    def dm_download(self):
+        """(alpha N x 5, passes N, residual N) of the last step."""
+        a = np.zeros((self.N, 5)); p = np.zeros(self.N, np.int32); r = np.zeros(self.N)
+        self._check(self._lib.ab_dm_download(self._ctx, _ptr(a, _dp), _ptr(p, _ip), _ptr(r, _dp)),
+                    "ab_dm_download")
+        return a, p, r
+
+    def dm_record(self, steps):
+        self._check(self._lib.ab_dm_record(self._ctx, int(steps)), "ab_dm_record")
+
+    def dm_rows(self):
+        n, v = C.c_int(0), C.c_int(0)
+        self._check(self._lib.ab_dm_rows(self._ctx, C.byref(n), C.byref(v)), "ab_dm_rows")
+        return n.value, v.value
+
+    def mlp_init(self, seed=1):
+        self._check(self._lib.ab_mlp_init(self._ctx, C.c_ulonglong(seed)), "ab_mlp_init")
+
+    def mlp_train(self, iters, batch=4096, lr=1e-3, seed=7):
+        a, b = C.c_float(0), C.c_float(0)
+        self._check(self._lib.ab_mlp_train(self._ctx, int(iters), int(batch), C.c_float(lr),
+                                           C.c_ulonglong(seed), C.byref(a), C.byref(b)),
+                    "ab_mlp_train")
+        return a.value, b.value
+
+    def mlp_enable(self, on=True):
+        self._check(self._lib.ab_mlp_enable(self._ctx, int(bool(on))), "ab_mlp_enable")
+
+    def mlp_get(self):
+        p = np.zeros(MLP_PARAMS, np.float32)
+        self._check(self._lib.ab_mlp_get(self._ctx, _ptr(p, _fp)), "ab_mlp_get")
+        return p
+
+    def mlp_set(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_PARAMS
+        self._check(self._lib.ab_mlp_set(self._ctx, _ptr(p, _fp)), "ab_mlp_set")
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], 5), np.float32)
+        self._check(self._lib.ab_mlp_forward(self._ctx, _ptr(x, _fp), x.shape[0], _ptr(y, _fp)),
+                    "ab_mlp_forward")
+        return y
+
+    def mlp_loss_grad(self, x, t, w):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        loss = C.c_float(0)
+        grad = np.zeros(MLP_TRAINABLE, np.float32)
+        self._check(self._lib.ab_mlp_loss_grad(self._ctx, _ptr(x, _fp), _ptr(t, _fp), _ptr(w, _fp),
+                                               x.shape[0], C.byref(loss), _ptr(grad, _fp)),
+                    "ab_mlp_loss_grad")
+        return loss.value, grad
 
     # ---- downloads --------------------------------------------------------
     def g(self):

@@ -48,8 +48,10 @@ void free_cloud(ab_ctx *c)
                     c->wt, c->mvx, c->mvy, c->mvz, c->rho, c->ux, c->uy, c->uz, c->T,
                     c->neigh, c->nneigh, c->ilist, c->blist, c->g, c->gt, c->limflag,
                     c->mlsflag, c->mlsovr, c->pmass, c->vox, c->vcount, c->vstart, c->vlist,
-                    c->bmom, c->bidx, c->bwu};
+                    c->bmom, c->bidx, c->bwu, c->dm_alpha, c->dm_pass, c->dm_res, c->dm_rec};
     for (void *p : ptrs) if (p) (void)hipFree(p);
+    c->dm_alpha = c->dm_res = nullptr; c->dm_pass = nullptr; c->dm_rec = nullptr;
+    c->dm_rec_slots = c->dm_rec_next = c->dm_rec_filled = c->dm_rec_left = 0;
     c->x = c->y = c->z = nullptr; c->bnd = nullptr;
     c->wnx = c->wny = c->wnz = c->wux = c->wuy = c->wuz = c->wt = nullptr;
     c->mvx = c->mvy = c->mvz = nullptr;
@@ -79,6 +81,16 @@ int alloc_cloud(ab_ctx *c)
     if ((rc = dalloc(c, &c->mlsflag, N))) return rc;
     if ((rc = dalloc(c, &c->g, N * c->gstride))) return rc;
     if (!table_mode(c) && (rc = dalloc(c, &c->gt, N * c->gstride))) return rc;
+    if (c->cfg.collision == AB_COLLISION_DISCRETE)
+    {
+        if ((rc = dalloc(c, &c->dm_alpha, N * 5)) || (rc = dalloc(c, &c->dm_pass, N)) ||
+            (rc = dalloc(c, &c->dm_res, N)))
+            return rc;
+        if (hipMemset(c->dm_pass, 0, sizeof(int) * N) != hipSuccess ||
+            hipMemset(c->dm_res, 0, sizeof(double) * N) != hipSuccess ||
+            hipMemset(c->dm_alpha, 0, sizeof(double) * N * 5) != hipSuccess)
+        { c->err = "DM state: memset failed"; return AB_ERR_HIP; }
+    }
     return AB_OK;
 }
 
@@ -159,9 +171,10 @@ int fp64_step(ab_ctx *c, double dt)
     launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s);
     tm.toc(ST_REDUCE);
     tm.tic();
-    launch_wall_collide(v, c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux, c->uy, c->uz,
-                        c->T, c->pmass, s);
+    launch_wall_collide(v, dm_args(c), c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux,
+                        c->uy, c->uz, c->T, c->pmass, s);
     tm.toc(ST_WALL_COLLIDE);
+    dm_step_done(c);
     tm.tic();
     launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, c->cfg.no_mass_fix, s);
     tm.toc(ST_REDUCE);
@@ -188,6 +201,57 @@ int do_step(ab_ctx *c, double dt)
 }  // namespace
 
 namespace ab {
+
+constexpr int kRecSlots = 16;     // steps the training-row ring holds
+
+__global__ void k_fill_f(float *p, size_t n, float v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+bool dm_on(const ab_ctx *c) { return c->cfg.collision == AB_COLLISION_DISCRETE && c->dim == 3; }
+
+double dm_scale(const ab_ctx *c)
+{
+    const double vhi = c->vlo + (c->Nv - 1) * c->dv;
+    return std::fmax(std::fabs(c->vlo), std::fabs(vhi));
+}
+
+// Per-step settings of the discrete-Maxwellian collision. When rows are
+// being recorded, the slot the step writes is cleared first (weight 0 for
+// every particle the kernels do not fill).
+DmArgs dm_args(ab_ctx *c)
+{
+    DmArgs d{};
+    if (!dm_on(c) || !c->dm_alpha) return d;
+    d.on = 1;
+    d.cs = dm_scale(c);
+    d.tol = c->cfg.dm_tol > 0 ? c->cfg.dm_tol : 1e-12;
+    d.maxit = c->cfg.dm_maxit > 0 ? c->cfg.dm_maxit : 8;
+    d.alpha = c->dm_alpha;
+    d.passes = c->dm_pass;
+    d.res = c->dm_res;
+    d.mlp = (c->mlp_on && c->mlp_ready) ? 1 : 0;
+    d.wb = c->mlp_wb;
+    d.pf = c->mlp_p;
+    if (c->dm_rec_left > 0 && c->dm_rec)
+    {
+        float *slot = c->dm_rec + (size_t)c->dm_rec_next * c->N * kMlpRow;
+        const size_t n = (size_t)c->N * kMlpRow;
+        k_fill_f<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(slot, n, 0.0f);
+        d.rec = slot;
+    }
+    return d;
+}
+
+void dm_step_done(ab_ctx *c)
+{
+    if (!dm_on(c) || c->dm_rec_left <= 0 || !c->dm_rec) return;
+    c->dm_rec_next = (c->dm_rec_next + 1) % c->dm_rec_slots;
+    if (c->dm_rec_filled < c->dm_rec_slots) c->dm_rec_filled++;
+    c->dm_rec_left--;
+}
 
 constexpr int kEvPairs = 2048;
 
@@ -240,8 +304,10 @@ int ab_create(const ab_config *cfg, ab_ctx **out)
     if ((cfg->dim != 2 && cfg->dim != 3) || cfg->n_particles <= 0 || cfg->nv < 2 ||
         cfg->nv > 48 || !(cfg->vmax > cfg->vmin) || !(cfg->R > 0) || !(cfg->tau > 0) ||
         !(cfg->radius > 0) || (cfg->mls_order != 1 && cfg->mls_order != 2) ||
-        cfg->precision < AB_PREC_FP64 || cfg->precision > AB_PREC_FP64_TABLE)
+        cfg->precision < AB_PREC_FP64 || cfg->precision > AB_PREC_FP64_TABLE ||
+        (cfg->collision != AB_COLLISION_CONTINUOUS && cfg->collision != AB_COLLISION_DISCRETE))
         return AB_ERR_ARG;
+    if (cfg->collision == AB_COLLISION_DISCRETE && cfg->dim != 3) return AB_ERR_UNSUPPORTED;
     ab_ctx *c = new (std::nothrow) ab_ctx;
     if (!c) return AB_ERR_NOMEM;
     c->cfg = *cfg;
@@ -269,6 +335,7 @@ int ab_destroy(ab_ctx *c)
     if (!c) return AB_ERR_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_cloud(c);
+    mlp_free(c);
     if (c->scal) (void)hipFree(c->scal);
     for (auto e : c->evpool) if (e) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -429,6 +496,69 @@ int ab_step(ab_ctx *c, double dt, ab_step_stats *st)
             st->mls_ovr_sum += ovr[i];
             if (ovr[i] > st->mls_ovr_max) st->mls_ovr_max = ovr[i];
         }
+    if (dm_on(c) && c->dm_pass)
+    {
+        std::vector<int> pass(c->N);
+        std::vector<double> res(c->N);
+        HIPCHK(c, hipMemcpyAsync(pass.data(), c->dm_pass, 4 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(res.data(), c->dm_res, 8 * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        const int maxit = c->cfg.dm_maxit > 0 ? c->cfg.dm_maxit : 8;
+        for (int i = 0; i < c->N; ++i)
+        {
+            if (pass[i] <= 0) continue;
+            st->dm_particles++;
+            st->dm_passes += pass[i];
+            if (pass[i] > maxit + 1) st->dm_unconverged++;
+            if (res[i] > st->dm_res_max) st->dm_res_max = res[i];
+        }
+    }
+    return AB_OK;
+}
+
+int ab_dm_download(ab_ctx *c, double *alpha, int *passes, double *residual)
+{
+    if (!c) return AB_ERR_ARG;
+    if (!dm_on(c) || !c->dm_alpha) { c->err = "the context does not run the discrete-Maxwellian collision"; return AB_ERR_STATE; }
+    const size_t N = (size_t)c->N;
+    if (alpha) HIPCHK(c, hipMemcpyAsync(alpha, c->dm_alpha, 8 * 5 * N, hipMemcpyDeviceToHost, c->stream));
+    if (passes) HIPCHK(c, hipMemcpyAsync(passes, c->dm_pass, 4 * N, hipMemcpyDeviceToHost, c->stream));
+    if (residual) HIPCHK(c, hipMemcpyAsync(residual, c->dm_res, 8 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_dm_record(ab_ctx *c, int steps)
+{
+    if (!c || steps < 0) return AB_ERR_ARG;
+    if (!dm_on(c) || !c->uploaded) { c->err = "ab_dm_record needs an uploaded discrete-Maxwellian context"; return AB_ERR_STATE; }
+    if (!c->dm_rec)
+    {
+        int rc;
+        if ((rc = dalloc(c, &c->dm_rec, (size_t)kRecSlots * c->N * kMlpRow))) return rc;
+        c->dm_rec_slots = kRecSlots;
+        c->dm_rec_next = c->dm_rec_filled = 0;
+    }
+    c->dm_rec_left = steps;
+    return AB_OK;
+}
+
+int ab_dm_rows(ab_ctx *c, int *rows, int *valid)
+{
+    if (!c) return AB_ERR_ARG;
+    const int n = c->dm_rec_filled * c->N;
+    if (rows) *rows = n;
+    if (valid)
+    {
+        *valid = 0;
+        if (n > 0)
+        {
+            std::vector<float> r((size_t)n * kMlpRow);
+            HIPCHK(c, hipMemcpyAsync(r.data(), c->dm_rec, sizeof(float) * r.size(), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            for (int k = 0; k < n; ++k) *valid += r[(size_t)k * kMlpRow + kMlpIn + kMlpOut] > 0.0f;
+        }
+    }
     return AB_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/alebgk_dev.h"
+#include "ab_dm.hpp"
 
 namespace ab {
 
@@ -23,6 +24,22 @@ struct DevCloud
     const double *wnx, *wny, *wnz, *wux, *wuy, *wuz, *wt;
     const double *mvx, *mvy, *mvz;
     const int *neigh, *nneigh;
+};
+
+// Discrete-Maxwellian collision settings and outputs (3-D interior
+// particles). Pointers are per particle; rec/wb/pf may be null.
+struct DmArgs
+{
+    int on;                   // relax to the discrete Maxwellian
+    int mlp;                  // warm-start Newton from the MLP's prediction
+    double cs, tol;           // velocity scale; relative residual to stop at
+    int maxit;                // Newton updates at most
+    double *alpha;            // N x 5 solution
+    int *passes;              // N: grid passes (> maxit + 1: not converged)
+    double *res;              // N: final relative residual
+    float *rec;               // N x kMlpRow training rows, or null
+    const unsigned short *wb; // MLP weights, BF16
+    const float *pf;          // MLP biases and normalisation, FP32
 };
 
 // Device scalars of one step (slots of Ctx::scal).
@@ -100,7 +117,29 @@ struct ab_ctx
     double m0mom[5] = {0};         // its moments: sum M0 dv^3, sum v M0 dv^3, sum |v|^2 M0 dv^3
     double m0rho = 1.0, m0T = 270.0;
     double *part = nullptr;        // per-particle partials of the table pipeline
+    double *red = nullptr;         // block sums of the partials (2 per reduction block)
     int *bad = nullptr;            // stencil check result
+
+    // discrete-Maxwellian collision (cfg.collision) and its MLP warm start
+    double *dm_alpha = nullptr;    // N x 5 solution of the last step
+    int *dm_pass = nullptr;        // N grid passes of the last step
+    double *dm_res = nullptr;      // N final relative residuals
+    float *dm_rec = nullptr;       // recorded training rows, dm_rec_slots x N x kMlpRow
+    int dm_rec_slots = 0;          // steps the ring holds
+    int dm_rec_next = 0;           // slot the next recorded step writes
+    int dm_rec_filled = 0;         // slots holding rows
+    int dm_rec_left = 0;           // steps still to record
+    float *mlp_p = nullptr;        // FP32 parameters (kMlpParams)
+    unsigned short *mlp_wb = nullptr;   // BF16 copy of the trainable block
+    float *mlp_m = nullptr, *mlp_v = nullptr, *mlp_g = nullptr;   // Adam moments, gradient
+    float *mlp_work = nullptr;     // per-sample activations of a training batch
+    int mlp_work_cap = 0;          // samples the work buffer holds
+    int *mlp_idx = nullptr;        // rows of a batch
+    float *mlp_rows = nullptr;     // caller rows (forward / gradient check)
+    int mlp_rows_cap = 0;
+    long mlp_step = 0;             // Adam step count
+    bool mlp_ready = false;        // parameters initialised
+    bool mlp_on = false;           // warm start enabled
 
     // body loads scratch
     double *bmom = nullptr;
@@ -149,7 +188,7 @@ void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int 
                         hipStream_t s);
 void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
                          double *limflag, double *pmass, hipStream_t s);
-void launch_wall_collide(const DevCloud &c, double *g, const double *scal, double dt,
+void launch_wall_collide(const DevCloud &c, const DmArgs &dm, double *g, const double *scal, double dt,
                          double tau, double R, double *rho, double *ux, double *uy,
                          double *uz, double *T, double *pmass, hipStream_t s);
 void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
@@ -166,6 +205,12 @@ int table_setup(ab_ctx *ctx);
 int table_upload_g(ab_ctx *ctx, const double *g_host);
 int table_download_g(ab_ctx *ctx, double *g_host);
 int table_step(ab_ctx *ctx, double dt);
+
+// Discrete-Maxwellian settings of the next step (records training rows when
+// asked); defined in ab_api.hip.
+DmArgs dm_args(ab_ctx *ctx);
+void dm_step_done(ab_ctx *ctx);
+void mlp_free(ab_ctx *ctx);   // k_mlp.hip
 void table_free(ab_ctx *ctx);
 
 // ---- k_body.hip --------------------------------------------------------------

@@ -15,6 +15,7 @@
 //   final         kScaleG + kMoments3D
 // and the global sums stay on the device (no host round trip per rescale).
 #include "ab_device.hpp"
+#include "ab_dm.hpp"
 #include "ab_internal.hpp"
 
 namespace ab {
@@ -477,14 +478,14 @@ __global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__r
 // twins (:338-389, 124-158, 1165-1179). The scale is applied as the value is
 // read, which rounds exactly as the stored product would.
 template <int DIM>
-__global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__restrict__ g,
+__global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, double *__restrict__ g,
                                                          const double *__restrict__ scal,
                                                          double dt, double tau, double R,
                                                          double *rho, double *ux, double *uy,
                                                          double *uz, double *T,
                                                          double *__restrict__ pmass)
 {
-    __shared__ double sh[5 * (kBlock / kWave)];
+    __shared__ double sh[15 * (kBlock / kWave)];
     const int i = xcd_remap(blockIdx.x, gridDim.x);
     const double s1 = scal[S_SCALE1];
     const bool doScale = scal[S_MASS_NEW] > 1e-30;
@@ -590,6 +591,78 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
             mT = t1 > 1e-6 ? t1 : 1e-6;
         }
         const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
+        // discrete-Maxwellian target: Newton on a from the continuous guess
+        double a[5] = {0, 0, 0, 0, 0};
+        bool useDm = false;
+        if constexpr (DIM == 3)
+        {
+            if (dm.on && r[0] > 1e-30)
+            {
+                useDm = true;
+                const double cs = dm.cs;
+                double ac[5];
+                dm_guess(mrho, mux, muy, muz, mT, R, cs, ac);
+                const float feat[kMlpIn] = {float(mux / cs), float(muy / cs), float(muz / cs),
+                                            float(R * mT / (cs * cs)), float(c.dv / cs)};
+                float da[kMlpOut] = {0, 0, 0, 0, 0};
+                if (dm.mlp) mlp_wave_forward(dm.wb, dm.pf, feat, da);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) a[q] = ac[q] + (double)da[q];
+                const double ms[5] = {r[0], r[1] / cs, r[2] / cs, r[3] / cs, r[4] / (cs * cs)};
+                int pass = 0;
+                double res = 0.0;
+                bool ok = true;
+                for (;;)
+                {
+                    ++pass;
+                    double S[15];
+#pragma unroll
+                    for (int q = 0; q < 15; ++q) S[q] = 0.0;
+                    for (int k = threadIdx.x; k < nodes; k += kBlock)
+                    {
+                        const double xx = vnode(c.vlo, c.dv, k % Nv) / cs;
+                        const double yy = vnode(c.vlo, c.dv, (k / Nv) % Nv) / cs;
+                        const double zz = vnode(c.vlo, c.dv, k / (Nv * Nv)) / cs;
+                        const double e = exp(a[0] + a[1] * xx + a[2] * yy + a[3] * zz
+                                             + a[4] * (xx * xx + yy * yy + zz * zz)) * dvd;
+                        dm_accum(S, e, xx, yy, zz);
+                    }
+                    block_sum<15>(S, sh);
+                    double F[5];
+                    res = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) { F[q] = S[q] - ms[q]; res = fmax(res, fabs(F[q])); }
+                    res /= fabs(ms[0]);
+                    if (res <= dm.tol || pass > dm.maxit) break;
+                    double d[5];
+                    if (!dm_solve5(S, F, d)) { ok = false; break; }
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) a[q] -= d[q];
+                }
+                bool conv = ok && res <= dm.tol;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) conv = conv && isfinite(a[q]);
+                if (!conv)
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) a[q] = ac[q];       // continuous fallback
+                if (threadIdx.x == 0)
+                {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) dm.alpha[(size_t)i * 5 + q] = a[q];
+                    dm.passes[i] = conv ? pass : dm.maxit + 2;
+                    dm.res[i] = res;
+                    if (dm.rec)
+                    {
+                        float *row = dm.rec + (size_t)i * kMlpRow;
+#pragma unroll
+                        for (int q = 0; q < kMlpIn; ++q) row[q] = feat[q];
+#pragma unroll
+                        for (int q = 0; q < kMlpOut; ++q) row[kMlpIn + q] = float(a[q] - ac[q]);
+                        row[kMlpIn + kMlpOut] = conv ? 1.0f : 0.0f;
+                    }
+                }
+            }
+        }
         for (int k = threadIdx.x; k < nodes; k += kBlock)
         {
             double vx = vnode(c.vlo, c.dv, k % Nv);
@@ -598,7 +671,14 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, double *__r
             if constexpr (DIM == 3)
             {
                 double vz = vnode(c.vlo, c.dv, k / (Nv * Nv));
-                double M = maxwellian3d(vx, vy, vz, mux, muy, muz, mT, mrho, R);
+                double M;
+                if (useDm)
+                {
+                    const double xx = vx / dm.cs, yy = vy / dm.cs, zz = vz / dm.cs;
+                    M = exp(a[0] + a[1] * xx + a[2] * yy + a[3] * zz
+                            + a[4] * (xx * xx + yy * yy + zz * zz));
+                }
+                else M = maxwellian3d(vx, vy, vz, mux, muy, muz, mT, mrho, R);
                 f = coeff * (f + lam * M);
             }
             else
@@ -734,15 +814,15 @@ void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const d
     else k_limit_commit<2><<<c.N, kBlock, 0, s>>>(c, g, gt, rho, limflag, pmass);
 }
 
-void launch_wall_collide(const DevCloud &c, double *g, const double *scal, double dt,
-                         double tau, double R, double *rho, double *ux, double *uy, double *uz,
-                         double *T, double *pmass, hipStream_t s)
+void launch_wall_collide(const DevCloud &c, const DmArgs &dm, double *g, const double *scal,
+                         double dt, double tau, double R, double *rho, double *ux, double *uy,
+                         double *uz, double *T, double *pmass, hipStream_t s)
 {
     if (c.dim == 3)
-        k_wall_collide<3><<<c.N, kBlock, 0, s>>>(c, g, scal, dt, tau, R, rho, ux, uy, uz, T,
+        k_wall_collide<3><<<c.N, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
                                                  pmass);
     else
-        k_wall_collide<2><<<c.N, kBlock, 0, s>>>(c, g, scal, dt, tau, R, rho, ux, uy, uz, T,
+        k_wall_collide<2><<<c.N, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
                                                  pmass);
 }
 

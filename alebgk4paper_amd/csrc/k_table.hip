@@ -87,9 +87,9 @@ template <typename T, int P, bool BND>
 __device__ void build_class(const DevCloud &c, int i, int lin, const int *lat, double dt,
                             double h, double tol, T *out, int S)
 {
-    double vx = c.vlo + (lin % c.Nv) * c.dv;
-    double vy = c.vlo + ((lin / c.Nv) % c.Nv) * c.dv;
-    double vz = c.vlo + (lin / (c.Nv * c.Nv)) * c.dv;
+    double vx = vnode(c.vlo, c.dv, lin % c.Nv);
+    double vy = vnode(c.vlo, c.dv, (lin / c.Nv) % c.Nv);
+    double vz = vnode(c.vlo, c.dv, lin / (c.Nv * c.Nv));
     const int self = canon_index(0, 0, 0);
     if (BND)
     {
@@ -248,55 +248,102 @@ __device__ __forceinline__ void bstore(T *row, int voff, T v)
     }
 }
 
-// One output of an end-plane particle: per-offset coefficient loads.
-template <typename T>
+// The loads of column CC+1 are issued before the FMAs of column CC (two
+// register buffers), so every column's memory latency overlaps the previous
+// column's arithmetic. Each step ends in a register pin and a scheduling
+// barrier: without them the optimiser sinks all the arithmetic below all
+// the loads and spills the values.
+
+// End-plane output: coefficients and values of one column, per offset.
+template <typename T, int CC>
+__device__ __forceinline__ void end_load(const T *__restrict__ g, const T *__restrict__ cf,
+                                         const Lat &L, int ix, int iy, int iz, int lin,
+                                         T (&cv)[5], T (&gv)[5])
+{
+    constexpr Col k = col_at(CC);
+    const int xx = clampi(ix + k.ox, 0, L.nx - 1), yy = clampi(iy + k.oy, 0, L.ny - 1);
+#pragma clang loop unroll(full)
+    for (int j = 0; j <= 2 * k.d; ++j)
+    {
+        const int zz = clampi(iz - k.d + j, 0, L.nz - 1);
+        cv[j] = bload(cf + (size_t)(k.o0 + j) * L.S, lin);
+        gv[j] = bload(g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * zz)) * L.S, lin);
+    }
+}
+
+template <typename T, int CC>
+__device__ __forceinline__ void end_step(const T *__restrict__ g, const T *__restrict__ cf,
+                                         const Lat &L, int ix, int iy, int iz, int lin,
+                                         T (&ca)[5], T (&ga)[5], T (&cb)[5], T (&gb)[5], T &acc)
+{
+    constexpr Col k = col_at(CC);
+    if constexpr (CC + 1 < kCols)
+    {
+        if constexpr ((CC & 1) == 0) end_load<T, CC + 1>(g, cf, L, ix, iy, iz, lin, cb, gb);
+        else end_load<T, CC + 1>(g, cf, L, ix, iy, iz, lin, ca, ga);
+    }
+#pragma clang loop unroll(full)
+    for (int j = 0; j <= 2 * k.d; ++j)
+    {
+        if constexpr ((CC & 1) == 0) acc += ca[j] * ga[j];
+        else acc += cb[j] * gb[j];
+    }
+    asm volatile("" : "+v"(acc)::"memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <typename T, int... CC>
 __device__ __forceinline__ T apply_one(const T *__restrict__ g, const T *__restrict__ cf,
-                                       const Lat &L, int ix, int iy, int iz, int lin)
+                                       const Lat &L, int ix, int iy, int iz, int lin,
+                                       std::integer_sequence<int, CC...>)
 {
     T acc = T(0);
-#pragma unroll
-    for (int cc = 0; cc < kCols; ++cc)
-    {
-        const Col k = col_at(cc);
-        const int xx = clampi(ix + k.ox, 0, L.nx - 1), yy = clampi(iy + k.oy, 0, L.ny - 1);
-#pragma unroll
-        for (int dz = -k.d; dz <= k.d; ++dz)
-        {
-            const int zz = clampi(iz + dz, 0, L.nz - 1);
-            const T *crow = cf + (size_t)(k.o0 + dz + k.d) * L.S;
-            const T *grow = g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * zz)) * L.S;
-            acc += bload(crow, lin) * bload(grow, lin);
-        }
-        asm volatile("" : "+v"(acc)::"memory");
-    }
+    T ca[5], ga[5], cb[5], gb[5];
+    end_load<T, 0>(g, cf, L, ix, iy, iz, lin, ca, ga);
+    (end_step<T, CC>(g, cf, L, ix, iy, iz, lin, ca, ga, cb, gb, acc), ...);
     return acc;
 }
 
-// Accumulate one (ox, oy) column into the K outputs iz0 .. iz0+K-1.
+// Interior: the values of column CC for outputs iz0 .. iz0+K-1.
 template <typename T, int K, int CC>
-__device__ __forceinline__ void apply_column(const T *__restrict__ g, const T (&cr)[kOff],
-                                             const Lat &L, int ix, int iy, int iz0, int lin,
-                                             size_t zstride, T (&acc)[K])
+__device__ __forceinline__ void col_load(const T *__restrict__ g, const Lat &L, int ix, int iy,
+                                         int iz0, int lin, size_t zstride, T (&v)[K + 4])
 {
     constexpr Col kc = col_at(CC);
     constexpr int NV = K + 2 * kc.d;
     const int xx = clampi(ix + kc.ox, 0, L.nx - 1), yy = clampi(iy + kc.oy, 0, L.ny - 1);
     const T *gc = g + ((size_t)xx + (size_t)L.nx * yy) * L.S;
-    T vals[NV];
 #pragma clang loop unroll(full)
     for (int j = 0; j < NV; ++j)
     {
         const int zz = clampi(iz0 - kc.d + j, 0, L.nz - 1);
-        vals[j] = bload(gc + (size_t)zz * zstride, lin);
+        v[j] = bload(gc + (size_t)zz * zstride, lin);
+    }
+}
+
+template <typename T, int K, int CC>
+__device__ __forceinline__ void col_step(const T *__restrict__ g, const T (&cr)[kOff],
+                                         const Lat &L, int ix, int iy, int iz0, int lin,
+                                         size_t zstride, T (&va)[K + 4], T (&vb)[K + 4],
+                                         T (&acc)[K])
+{
+    constexpr Col kc = col_at(CC);
+    if constexpr (CC + 1 < kCols)
+    {
+        if constexpr ((CC & 1) == 0) col_load<T, K, CC + 1>(g, L, ix, iy, iz0, lin, zstride, vb);
+        else col_load<T, K, CC + 1>(g, L, ix, iy, iz0, lin, zstride, va);
     }
 #pragma clang loop unroll(full)
     for (int k = 0; k < K; ++k)
 #pragma clang loop unroll(full)
-        for (int dz = 0; dz <= 2 * kc.d; ++dz) acc[k] += cr[kc.o0 + dz] * vals[k + dz];
-    // Pin the column's FMAs here: without it the optimiser sinks every
-    // column's arithmetic below all the loads and spills the values.
+        for (int dz = 0; dz <= 2 * kc.d; ++dz)
+        {
+            if constexpr ((CC & 1) == 0) acc[k] += cr[kc.o0 + dz] * va[k + dz];
+            else acc[k] += cr[kc.o0 + dz] * vb[k + dz];
+        }
 #pragma clang loop unroll(full)
     for (int k = 0; k < K; ++k) asm volatile("" : "+v"(acc[k])::"memory");
+    __builtin_amdgcn_sched_barrier(0);
 }
 
 template <typename T, int K, int... CC>
@@ -305,9 +352,9 @@ __device__ __forceinline__ void apply_columns(const T *__restrict__ g, const T (
                                               size_t zstride, T (&acc)[K],
                                               std::integer_sequence<int, CC...>)
 {
-    // one column at a time: bounds the loads in flight and the registers
-    ((apply_column<T, K, CC>(g, cr, L, ix, iy, iz0, lin, zstride, acc),
-      __builtin_amdgcn_sched_barrier(0)), ...);
+    T va[K + 4], vb[K + 4];
+    col_load<T, K, 0>(g, L, ix, iy, iz0, lin, zstride, va);
+    (col_step<T, K, CC>(g, cr, L, ix, iy, iz0, lin, zstride, va, vb, acc), ...);
 }
 
 template <typename T, int K>
@@ -323,7 +370,10 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
     const int voff = lane * (int)sizeof(T);          // the lane's byte offset
     const int ncol = L.nx * L.ny;
     if (wv >= ncol * L.nchunk) return;
-    const int chunk = wv % L.nchunk, col = wv / L.nchunk;
+    // chunk-major: the waves of one XCD (a contiguous range after the remap)
+    // share one or two 64-node chunks, so the chunk's coefficient rows and
+    // the neighbouring columns' values stay in that XCD's L2
+    const int chunk = wv / ncol, col = wv % ncol;
     const int ix = col % L.nx, iy = col / L.nx;
     (void)scal;
     const int cx = category(ix, L.nx), cy = category(iy, L.ny);
@@ -340,7 +390,7 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
         const int iz = (e < 2) ? e : L.nz - 4 + e;
         const int cls = cx + 5 * (cy + 5 * category(iz, L.nz));
         const T *cf = coef + (size_t)cls * kOff * L.S + c0;
-        T acc = apply_one(gw, cf, L, ix, iy, iz, voff);
+        T acc = apply_one(gw, cf, L, ix, iy, iz, voff, std::make_integer_sequence<int, kCols>{});
         bstore(gtw + ((size_t)col + plane * iz) * L.S, voff, acc);
     }
 
@@ -390,24 +440,61 @@ struct Ref
     double rho, T;          // reference state of M0
 };
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__ g,
-                                              const T *__restrict__ gt, Ref ref,
-                                              const double *__restrict__ rho,
-                                              const double *__restrict__ scal, double dt,
-                                              double tau, double R, double *__restrict__ part)
+// Walks one lane's nodes k = lane, lane + 64, ... carrying (iv, jv, kv) of
+// lin = iv + Nv (jv + Nv kv) incrementally (no integer division per node).
+struct NodeWalk
 {
-    const int i = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (i >= c.N) return;
-    const double Lo = scal[S_LAZY];
-    T *gp = g + (size_t)i * L.S;
-    const T *tp = gt + (size_t)i * L.S;
+    int iv, jv, kv, a, b, c, Nv;
+    __device__ NodeWalk(int lane, int nv) : Nv(nv)
+    {
+        iv = lane % nv; jv = (lane / nv) % nv; kv = lane / (nv * nv);
+        a = kWave % nv; b = (kWave / nv) % nv; c = kWave / (nv * nv);
+        // opaque start: each pass recomputes its velocities instead of the
+        // compiler keeping every pass's values live across the others
+        asm volatile("" : "+v"(iv), "+v"(jv), "+v"(kv));
+    }
+    __device__ __forceinline__ void step()
+    {
+        iv += a; jv += b; kv += c;
+        if (iv >= Nv) { iv -= Nv; ++jv; }
+        if (jv >= Nv) { jv -= Nv; ++kv; }
+    }
+};
+
+// Calls f(j, k, walk) for the lane's nodes k = lane + 64 j < nodes. With
+// NJ > 0 the loop is unrolled (j is a compile-time index into the lane's
+// registers, nodes <= 64 NJ); NJ == 0 walks any row length.
+template <int NJ, typename F>
+__device__ __forceinline__ void for_nodes(int lane, int nodes, int Nv, F &&f)
+{
+    NodeWalk w(lane, Nv);
+    if constexpr (NJ > 0)
+    {
+#pragma unroll
+        for (int j = 0; j < NJ; ++j)
+        {
+            const int k = lane + kWave * j;
+            if (k < nodes) f(j, k, w);
+            w.step();
+        }
+    }
+    else
+    {
+        for (int j = 0, k = lane; k < nodes; ++j, k += kWave) { f(j, k, w); w.step(); }
+    }
+}
+
+// Boundary particle (one wave): positivity, then the diffuse wall in FP64
+// (kWallBC3D, CudaBackend.cu:956-1030), written back as a deviation.
+// Streams the row from memory (14 % of the particles at 40^3).
+template <typename T>
+__device__ __forceinline__ void post_boundary(const DevCloud &c, int i, int lane, T *__restrict__ gp,
+                                              const T *__restrict__ tp, const Ref &ref, double Lo,
+                                              double R, double out[kPart])
+{
     const double *m0 = ref.m0;
     const int nodes = c.nodes, Nv = c.Nv;
-    const double dv3 = c.dv * c.dv * c.dv;
-
-    // positivity: negatives fall back to the pre-transport value
+    const double dv3 = c.dv * c.dv * c.dv, vlo = c.vlo, dv = c.dv;
     double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
     for (int k = lane; k < nodes; k += kWave)
     {
@@ -423,160 +510,335 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
         const double replaced = mPos + mFb;
         sc = (Lo * replaced > 1e-30) ? (mPos + mNeg) / replaced : 0.0;
     }
-    // deviation after the positivity repair, FP64
     auto limited = [&](int k) -> double {
         const double dg = (double)tp[k];
         if (!fires) return dg;
         const double sel = (m0[k] + dg < 0.0) ? (double)gp[k] : dg;
         return sc * sel + (sc - 1.0) * m0[k];
     };
-    double flag = fires ? 1.0 : 0.0;
-    double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const T Lt = T(Lo);
-
-    if (!c.bnd[i])
+    const double nx = c.wnx[i], ny = c.wny[i], nz = c.wnz[i];
+    const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = c.wuz[i], Tw = c.wt[i];
+    const double RT2w = 2.0 * R * Tw;
+    const double norm = pow(AB_PI * RT2w, 1.5);
+    double m0s = 0, intM = 0, intP = 0;
+    NodeWalk w(lane, Nv);
+    for (int k = lane; k < nodes; k += kWave, w.step())
     {
-        // moments of the limited field = moments of M0 + moments of d
-        double r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0;
-        for (int k = lane; k < nodes; k += kWave)
+        const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
+        const double d = limited(k);
+        m0s += d;
+        const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
+        if (cn < 0.0) intM += cn * (m0[k] + d) * dv3;
+        else if (cn > 0.0)
         {
-            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
-            const double d = limited(k);
-            r0 += d; r1 += vx * d; r2 += vy * d; r3 += vz * d;
-            r4 += (vx * vx + vy * vy + vz * vz) * d;
+            const double Mw = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
+                                    + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm;
+            intP += cn * Mw * dv3;
         }
-        const double rhoL = ref.mom[0] + wave_sum(r0) * dv3;
-        const double pxL = ref.mom[1] + wave_sum(r1) * dv3;
-        const double pyL = ref.mom[2] + wave_sum(r2) * dv3;
-        const double pzL = ref.mom[3] + wave_sum(r3) * dv3;
-        const double EL = ref.mom[4] + wave_sum(r4) * dv3;
-        // NAD against the neighbours' current densities (true units)
-        const int nn = c.nneigh[i];
-        double rmin = 1e300, rmax = -1e300;
-        for (int k = lane; k < nn; k += kWave)
-        {
-            double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
-            rmin = fmin(rmin, q); rmax = fmax(rmax, q);
-        }
-        rmin = wave_min(rmin); rmax = wave_max(rmax);
-        const double rho_new = Lo * rhoL;
-        double sc2 = 1.0;
-        if (!(rmin > rmax))
-        {
-            double slack = 0.1 * fmax(rmax - rmin, 1e-12) + 1e-9;
-            double lo = rmin - slack, hi = rmax + slack;
-            bool nad = false;
-            if (rho_new > hi && rho_new > 1e-30) { sc2 = hi / rho_new; nad = true; }
-            else if (rho_new < lo && rho_new > 1e-30) { sc2 = lo / rho_new; nad = true; }
-            if (nad) flag += 2.0;
-        }
-        out[0] = sc2 * rho_new;                                // committed mass
-        // moments of the committed field (L-units); u and T are ratios
-        const double mr = sc2 * rhoL;
-        double mux = 0, muy = 0, muz = 0, mT = ref.T;
-        const bool okm = Lo * mr > 1e-30;
-        if (okm)
-        {
-            mux = pxL / rhoL; muy = pyL / rhoL; muz = pzL / rhoL;
-            const double u2 = mux * mux + muy * muy + muz * muz;
-            const double t1 = (EL / rhoL - u2) / (3.0 * R);
-            mT = t1 > 1e-6 ? t1 : 1e-6;
-        }
-        // collision: h = coeff (g'' + lam M)  =>  dh = coeff (d'' + lam (M - M0))
-        // with M - M0 = M0 expm1(D),
-        // D = ln(rho/rho_ref) - 1.5 ln(T/T_ref) - |u|^2/2RT + |v|^2 (1/2RT_ref - 1/2RT) + v.u/RT
-        const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
-        const double A0 = okm ? log(mr / ref.rho) - 1.5 * log(mT / ref.T)
-                                    - (mux * mux + muy * muy + muz * muz) / (2.0 * R * mT)
-                              : 0.0;
-        const double Bq = 1.0 / (2.0 * R * ref.T) - 1.0 / (2.0 * R * mT);
-        const double Cx = mux / (R * mT), Cy = muy / (R * mT), Cz = muz / (R * mT);
-        const T cT = T(coeff), lamT = T(lam);
-        double h0 = 0, h1 = 0, h2 = 0, h3 = 0, h4 = 0;
-        for (int k = lane; k < nodes; k += kWave)
-        {
-            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
-            const double d2 = sc2 * limited(k) + (sc2 - 1.0) * m0[k];
-            const double D = A0 + Bq * (vx * vx + vy * vy + vz * vz) + Cx * vx + Cy * vy + Cz * vz;
-            const T mm = okm ? T(m0[k]) * expm1(T(D)) : T(-m0[k]);
-            const T dh = cT * (T(d2) + lamT * mm);
-            // fold the pending rescale of this step's input into the stored value
-            gp[k] = Lt * dh + T((Lo - 1.0) * m0[k]);
-            const double hd = (double)dh;
-            h0 += hd; h1 += vx * hd; h2 += vy * hd; h3 += vz * hd;
-            h4 += (vx * vx + vy * vy + vz * vz) * hd;
-        }
-        out[1] = Lo * (ref.mom[0] + wave_sum(h0) * dv3);
-        out[2] = Lo * (ref.mom[1] + wave_sum(h1) * dv3);
-        out[3] = Lo * (ref.mom[2] + wave_sum(h2) * dv3);
-        out[4] = Lo * (ref.mom[3] + wave_sum(h3) * dv3);
-        out[5] = Lo * (ref.mom[4] + wave_sum(h4) * dv3);
     }
-    else
+    m0s = wave_sum(m0s); intM = wave_sum(intM); intP = wave_sum(intP);
+    // intM is in L-units; the reference's fallback rho_w = 1 is in true units
+    const double rhoWL = (intP > 1e-30) ? (-intM / intP) : 1.0 / Lo;
+    out[0] = Lo * (ref.mom[0] + m0s * dv3);
+    out[2] = Lo * rhoWL;
+    const T Lt = T(Lo);
+    double h0 = 0;
+    NodeWalk w2(lane, Nv);
+    for (int k = lane; k < nodes; k += kWave, w2.step())
     {
-        const double nx = c.wnx[i], ny = c.wny[i], nz = c.wnz[i];
-        const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = c.wuz[i], Tw = c.wt[i];
-        const double RT2w = 2.0 * R * Tw;
-        const double norm = pow(AB_PI * RT2w, 1.5);
-        double m0s = 0, intM = 0, intP = 0;
-        for (int k = lane; k < nodes; k += kWave)
+        const double vx = vnode(vlo, dv, w2.iv), vy = vnode(vlo, dv, w2.jv), vz = vnode(vlo, dv, w2.kv);
+        const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
+        double dh;
+        if (cn > 0.0)
+            dh = rhoWL * exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
+                               + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm - m0[k];
+        else dh = limited(k);
+        const T dhT = T(dh);
+        gp[k] = Lt * dhT + T((Lo - 1.0) * m0[k]);
+        h0 += (double)dhT;
+    }
+    out[1] = Lo * (ref.mom[0] + wave_sum(h0) * dv3);
+    out[6] = fires ? 1.0 : 0.0;
+}
+
+// One wave per particle. Interior particles: positivity, NAD, moments, the
+// BGK collision (continuous target, or the discrete Maxwellian when DM) and
+// the partial sums, with the row held in registers when NJ > 0 (NJ = the
+// lane's node count, nodes <= 64 NJ, only the last one guarded); NJ == 0
+// streams the row from memory in every pass. The raw moments are summed in
+// the positivity pass and redone only when the repair fires.
+template <typename T, int NJ, bool DM>
+__global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *__restrict__ g,
+                                              T *__restrict__ gt, Ref ref,
+                                              const double *__restrict__ rho,
+                                              const double *__restrict__ scal, double dt,
+                                              double tau, double R, double *__restrict__ part)
+{
+    const int i = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (i >= c.N) return;
+    const double Lo = scal[S_LAZY];
+    T *gp = g + (size_t)i * L.S;
+    T *tp = gt + (size_t)i * L.S;
+    double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (c.bnd[i])
+    {
+        post_boundary<T>(c, i, lane, gp, tp, ref, Lo, R, out);
+        if (lane < kPart) part[(size_t)i * kPart + lane] = out[lane];
+        return;
+    }
+    const double *m0 = ref.m0;
+    const int nodes = c.nodes, Nv = c.Nv;
+    const double dv3 = c.dv * c.dv * c.dv;
+    const double vlo = c.vlo, dv = c.dv;
+
+    constexpr int NR = NJ > 0 ? NJ : 1;
+    T tv[NR];
+    double mv[NR];
+    // the lane's nodes k = lane + 64 j; with NJ > 0 only j = NJ-1 can run
+    // past the row
+    auto each = [&](auto &&f) {
+        NodeWalk w(lane, Nv);
+        if constexpr (NJ > 0)
         {
-            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
-            const double d = limited(k);
-            m0s += d;
-            const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
-            if (cn < 0.0) intM += cn * (m0[k] + d) * dv3;
-            else if (cn > 0.0)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j)
             {
-                const double Mw = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
-                                        + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm;
-                intP += cn * Mw * dv3;
+                const int k = lane + kWave * j;
+                if (j < NJ - 1 || k < nodes) f(j, k, w);
+                w.step();
             }
         }
-        m0s = wave_sum(m0s); intM = wave_sum(intM); intP = wave_sum(intP);
-        // intM is in L-units; the reference's fallback rho_w = 1 is in true units
-        const double rhoWL = (intP > 1e-30) ? (-intM / intP) : 1.0 / Lo;
-        out[0] = Lo * (ref.mom[0] + m0s * dv3);
-        out[2] = Lo * rhoWL;
-        double h0 = 0;
-        for (int k = lane; k < nodes; k += kWave)
+        else
         {
-            const double vx = c.vlo + (k % Nv) * c.dv, vy = c.vlo + ((k / Nv) % Nv) * c.dv;
-            const double vz = c.vlo + (k / (Nv * Nv)) * c.dv;
-            const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
-            double dh;
-            if (cn > 0.0)
-                dh = rhoWL * exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
-                                   + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm - m0[k];
-            else dh = limited(k);
-            const T dhT = T(dh);
-            gp[k] = Lt * dhT + T((Lo - 1.0) * m0[k]);
-            h0 += (double)dhT;
+            for (int j = 0, k = lane; k < nodes; ++j, k += kWave) { f(j, k, w); w.step(); }
         }
-        out[1] = Lo * (ref.mom[0] + wave_sum(h0) * dv3);
+    };
+    auto tval = [&](int j, int k) -> double {
+        if constexpr (NJ > 0) return (double)tv[j]; else return (double)tp[k];
+    };
+    auto mval = [&](int j, int k) -> double {
+        if constexpr (NJ > 0) return mv[j]; else return m0[k];
+    };
+
+    // pass 1: positivity sums and the raw moments of M0 + d (L-units)
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};      // mPos mNeg mFb r0..r4
+    auto moments_of = [&](int j, int k, const NodeWalk &w, double d) {
+        const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
+        s[3] += d; s[4] += vx * d; s[5] += vy * d; s[6] += vz * d;
+        s[7] += (vx * vx + vy * vy + vz * vz) * d;
+    };
+    each([&](int j, int k, const NodeWalk &w) {
+        const T t = tp[k];
+        const double m = m0[k];
+        if constexpr (NJ > 0) { tv[j] = t; mv[j] = m; }
+        const double d = (double)t, v = m + d;
+        if (v >= 0.0) s[0] += v;
+        else { s[1] += v; s[2] += m + (double)gp[k]; }
+        moments_of(j, k, w, d);
+    });
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] = wave_sum(s[q]);
+    const bool fires = (Lo * s[1] < -1e-30 && (s[0] + s[1]) > 0);
+    double flag = 0.0;
+    if (fires)
+    {
+        // negatives fall back to the pre-transport value, then the rescale
+        // to the transported mass; the limited row replaces the transported
+        // one and its moments are summed again
+        flag = 1.0;
+        const double replaced = s[0] + s[2];
+        const double sc = (Lo * replaced > 1e-30) ? (s[0] + s[1]) / replaced : 0.0;
+#pragma unroll
+        for (int q = 3; q < 8; ++q) s[q] = 0.0;
+        each([&](int j, int k, const NodeWalk &w) {
+            const double dg = tval(j, k), m = mval(j, k);
+            const double sel = (m + dg < 0.0) ? (double)gp[k] : dg;
+            const T lim = T(sc * sel + (sc - 1.0) * m);
+            if constexpr (NJ > 0) tv[j] = lim; else tp[k] = lim;
+            moments_of(j, k, w, (double)lim);
+        });
+#pragma unroll
+        for (int q = 3; q < 8; ++q) s[q] = wave_sum(s[q]);
     }
+    const double rhoL = ref.mom[0] + s[3] * dv3;
+    const double pxL = ref.mom[1] + s[4] * dv3;
+    const double pyL = ref.mom[2] + s[5] * dv3;
+    const double pzL = ref.mom[3] + s[6] * dv3;
+    const double EL = ref.mom[4] + s[7] * dv3;
+    // NAD against the neighbours' current densities (true units)
+    const int nn = c.nneigh[i];
+    double rmin = 1e300, rmax = -1e300;
+    for (int k = lane; k < nn; k += kWave)
+    {
+        const double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
+        rmin = fmin(rmin, q); rmax = fmax(rmax, q);
+    }
+    rmin = wave_min(rmin); rmax = wave_max(rmax);
+    const double rho_new = Lo * rhoL;
+    double sc2 = 1.0;
+    if (!(rmin > rmax))
+    {
+        const double slack = 0.1 * fmax(rmax - rmin, 1e-12) + 1e-9;
+        const double lo = rmin - slack, hi = rmax + slack;
+        bool nad = false;
+        if (rho_new > hi && rho_new > 1e-30) { sc2 = hi / rho_new; nad = true; }
+        else if (rho_new < lo && rho_new > 1e-30) { sc2 = lo / rho_new; nad = true; }
+        if (nad) flag += 2.0;
+    }
+    out[0] = sc2 * rho_new;                                    // committed mass
+    // moments of the committed field (L-units); u and T are ratios
+    const double mr = sc2 * rhoL;
+    double mux = 0, muy = 0, muz = 0, mT = ref.T;
+    const bool okm = Lo * mr > 1e-30;
+    if (okm)
+    {
+        mux = pxL / rhoL; muy = pyL / rhoL; muz = pzL / rhoL;
+        const double u2 = mux * mux + muy * muy + muz * muz;
+        const double t1 = (EL / rhoL - u2) / (3.0 * R);
+        mT = t1 > 1e-6 ? t1 : 1e-6;
+    }
+    // collision: h = coeff (g'' + lam M)  =>  dh = coeff (d'' + lam (M - M0))
+    // with M - M0 = M0 expm1(D),
+    // D = ln(rho/rho_ref) - 1.5 ln(T/T_ref) - |u|^2/2RT + |v|^2 (1/2RT_ref - 1/2RT) + v.u/RT
+    const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
+    double A0 = okm ? log(mr / ref.rho) - 1.5 * log(mT / ref.T)
+                          - (mux * mux + muy * muy + muz * muz) / (2.0 * R * mT)
+                    : 0.0;
+    double Bq = 1.0 / (2.0 * R * ref.T) - 1.0 / (2.0 * R * mT);
+    double Cx = mux / (R * mT), Cy = muy / (R * mT), Cz = muz / (R * mT);
+    if constexpr (DM)
+    {
+        if (okm)
+        {
+            // discrete Maxwellian with the grid moments of the committed
+            // field (L-units): Newton in FP64
+            const double cs = dm.cs, ics = 1.0 / cs;
+            double ac[5], a[5];
+            dm_guess(mr, mux, muy, muz, mT, R, cs, ac);
+            const float feat[kMlpIn] = {float(mux * ics), float(muy * ics), float(muz * ics),
+                                        float(R * mT * ics * ics), float(dv * ics)};
+            float da[kMlpOut] = {0, 0, 0, 0, 0};
+            if (dm.mlp) mlp_wave_forward(dm.wb, dm.pf, feat, da);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) a[q] = ac[q] + (double)da[q];
+            const double ms[5] = {mr, sc2 * pxL * ics, sc2 * pyL * ics, sc2 * pzL * ics,
+                                  sc2 * EL * ics * ics};
+            int pass = 0;
+            double res = 0.0;
+            bool ok = true;
+            for (;;)
+            {
+                ++pass;
+                double S[15];
+#pragma unroll
+                for (int q = 0; q < 15; ++q) S[q] = 0.0;
+                NodeWalk w(lane, Nv);
+                for (int k = lane; k < nodes; k += kWave, w.step())
+                {
+                    const double xx = vnode(vlo, dv, w.iv) * ics, yy = vnode(vlo, dv, w.jv) * ics;
+                    const double zz = vnode(vlo, dv, w.kv) * ics;
+                    const double e = exp(a[0] + a[1] * xx + a[2] * yy + a[3] * zz
+                                         + a[4] * (xx * xx + yy * yy + zz * zz)) * dv3;
+                    dm_accum(S, e, xx, yy, zz);
+                }
+#pragma unroll
+                for (int q = 0; q < 15; ++q) S[q] = wave_sum(S[q]);
+                double F[5];
+                res = 0.0;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) { F[q] = S[q] - ms[q]; res = fmax(res, fabs(F[q])); }
+                res /= fabs(ms[0]);
+                if (res <= dm.tol || pass > dm.maxit) break;
+                double d[5];
+                if (!dm_solve5(S, F, d)) { ok = false; break; }
+#pragma unroll
+                for (int q = 0; q < 5; ++q) a[q] -= d[q];
+            }
+            bool conv = ok && res <= dm.tol;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) conv = conv && isfinite(a[q]);
+            if (!conv)
+#pragma unroll
+                for (int q = 0; q < 5; ++q) a[q] = ac[q];           // continuous fallback
+            if (lane == 0)
+            {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) dm.alpha[(size_t)i * 5 + q] = a[q];
+                dm.passes[i] = conv ? pass : dm.maxit + 2;
+                dm.res[i] = res;
+                if (dm.rec)
+                {
+                    float *row = dm.rec + (size_t)i * kMlpRow;
+#pragma unroll
+                    for (int q = 0; q < kMlpIn; ++q) row[q] = feat[q];
+#pragma unroll
+                    for (int q = 0; q < kMlpOut; ++q) row[kMlpIn + q] = float(a[q] - ac[q]);
+                    row[kMlpIn + kMlpOut] = conv ? 1.0f : 0.0f;
+                }
+            }
+            // D = (a - a_ref) . psi in unscaled velocities, a_ref the
+            // parameters of M0
+            double aref[5];
+            dm_guess(ref.rho, 0.0, 0.0, 0.0, ref.T, R, cs, aref);
+            A0 = a[0] - aref[0];
+            Cx = a[1] * ics; Cy = a[2] * ics; Cz = a[3] * ics;
+            Bq = (a[4] - aref[4]) * ics * ics;
+        }
+    }
+    const T cT = T(coeff), lamT = T(lam), Lt = T(Lo);
+    double h[5] = {0, 0, 0, 0, 0};
+    each([&](int j, int k, const NodeWalk &w) {
+        const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
+        const double m = mval(j, k);
+        const double v2 = vx * vx + vy * vy + vz * vz;
+        const double d2 = sc2 * tval(j, k) + (sc2 - 1.0) * m;
+        const double D = A0 + Bq * v2 + Cx * vx + Cy * vy + Cz * vz;
+        const T mm = okm ? T(m) * expm1(T(D)) : T(-m);
+        const T dh = cT * (T(d2) + lamT * mm);
+        // fold the pending rescale of this step's input into the stored value
+        gp[k] = Lt * dh + T((Lo - 1.0) * m);
+        const double hd = (double)dh;
+        h[0] += hd; h[1] += vx * hd; h[2] += vy * hd; h[3] += vz * hd; h[4] += v2 * hd;
+    });
+#pragma unroll
+    for (int q = 0; q < 5; ++q) out[1 + q] = Lo * (ref.mom[q] + wave_sum(h[q]) * dv3);
     out[6] = flag;
     if (lane < kPart) part[(size_t)i * kPart + lane] = out[lane];
 }
 
 // Global sums of the partials -> the two rescale factors of cudaStep
-// (:1706-1710, :1773-1780), folded into the lazy factor.
-__global__ __launch_bounds__(1024) void k_reduce(int N, const double *__restrict__ part,
-                                                 double *scal, int nofix)
+// (:1706-1710, :1773-1780), folded into the lazy factor. Two stages with a
+// fixed partition (deterministic): kRedBlocks blocks sum contiguous particle
+// ranges, then one block combines them.
+constexpr int kRedBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_reduce_stage(int N, const double *__restrict__ part,
+                                                      double *__restrict__ red)
 {
-    __shared__ double sh[2][1024 / kWave];
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < N; i += 1024) { a += part[(size_t)i * kPart]; b += part[(size_t)i * kPart + 1]; }
-    a = wave_sum(a); b = wave_sum(b);
-    if ((threadIdx.x & (kWave - 1)) == 0) { sh[0][threadIdx.x / kWave] = a; sh[1][threadIdx.x / kWave] = b; }
-    __syncthreads();
+    __shared__ double sh[2 * (256 / kWave)];
+    const int per = (N + kRedBlocks - 1) / kRedBlocks;
+    const int i0 = blockIdx.x * per, i1 = min(N, i0 + per);
+    double v[2] = {0.0, 0.0};
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += 256)
+    {
+        v[0] += part[(size_t)i * kPart];
+        v[1] += part[(size_t)i * kPart + 1];
+    }
+    block_sum<2>(v, sh);
+    if (threadIdx.x == 0) { red[2 * blockIdx.x] = v[0]; red[2 * blockIdx.x + 1] = v[1]; }
+}
+
+__global__ __launch_bounds__(256) void k_reduce(const double *__restrict__ red, double *scal,
+                                                int nofix)
+{
+    __shared__ double sh[2 * (256 / kWave)];
+    double v[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < kRedBlocks; b += 256) { v[0] += red[2 * b]; v[1] += red[2 * b + 1]; }
+    block_sum<2>(v, sh);
     if (threadIdx.x == 0)
     {
-        double mNew = 0.0, hsum = 0.0;
-        for (int q = 0; q < 1024 / kWave; ++q) { mNew += sh[0][q]; hsum += sh[1][q]; }
+        const double mNew = v[0], hsum = v[1];
         const double mOld = scal[S_MASS_OLD];
         const double s1 = (mNew > 1e-30) ? mOld / mNew : 1.0;
         const double mColl = s1 * hsum;
@@ -693,10 +955,10 @@ int apply_launch(ab_ctx *ctx, const Lat &L)
 void table_free(ab_ctx *ctx)
 {
     void *ptrs[] = {ctx->gs, ctx->gts, ctx->coef, ctx->cls, ctx->lat, ctx->part, ctx->bad,
-                    ctx->m0};
+                    ctx->m0, ctx->red};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     ctx->gs = ctx->gts = ctx->coef = nullptr;
-    ctx->m0 = nullptr;
+    ctx->m0 = ctx->red = nullptr;
     ctx->cls = ctx->lat = ctx->bad = nullptr;
     ctx->part = nullptr;
 }
@@ -784,7 +1046,8 @@ int table_setup(ab_ctx *ctx)
               && hipMalloc((void **)&ctx->lat, 3 * sizeof(int) * (size_t)N) == hipSuccess
               && hipMalloc((void **)&ctx->cls, sizeof(int) * kNcls) == hipSuccess
               && hipMalloc((void **)&ctx->part, sizeof(double) * kPart * (size_t)N) == hipSuccess
-              && hipMalloc((void **)&ctx->m0, sizeof(double) * ctx->S) == hipSuccess;
+              && hipMalloc((void **)&ctx->m0, sizeof(double) * ctx->S) == hipSuccess
+              && hipMalloc((void **)&ctx->red, sizeof(double) * 2 * kRedBlocks) == hipSuccess;
         if (!a) { table_free(ctx); ctx->err = "table setup: device allocation failed"; return AB_ERR_NOMEM; }
         if (hipMemsetAsync(ctx->gts, 0, gsz, s) != hipSuccess)
         { ctx->err = "table setup: memset failed"; return AB_ERR_HIP; }
@@ -805,9 +1068,9 @@ int table_setup(ab_ctx *ctx)
         for (int q = 0; q < 5; ++q) ctx->m0mom[q] = 0.0;
         for (int k = 0; k < ctx->nodes; ++k)
         {
-            const double vx = ctx->vlo + (k % Nv) * ctx->dv;
-            const double vy = ctx->vlo + ((k / Nv) % Nv) * ctx->dv;
-            const double vz = ctx->vlo + (k / (Nv * Nv)) * ctx->dv;
+            const double vx = vnode(ctx->vlo, ctx->dv, k % Nv);
+            const double vy = vnode(ctx->vlo, ctx->dv, (k / Nv) % Nv);
+            const double vz = vnode(ctx->vlo, ctx->dv, k / (Nv * Nv));
             m0[k] = ctx->m0rho / norm * std::exp(-(vx * vx + vy * vy + vz * vz) / twoRT);
             ctx->m0mom[0] += m0[k] * dv3;
             ctx->m0mom[1] += vx * m0[k] * dv3;
@@ -911,17 +1174,27 @@ int table_step(ab_ctx *ctx, double dt)
     else apply_launch<double>(ctx, L);
     toc(ST_TRANSPORT);
     tic();
+    // the row in registers when the lane's node count is 16 or 8 exactly
+    const int nj = (ctx->nodes + kWave - 1) / kWave;
+    const DmArgs dm = dm_args(ctx);
     if (is_fp32(ctx))
-        k_post<float><<<nbp, 256, 0, s>>>(c, L, (float *)ctx->gs, (const float *)ctx->gts, ref,
-                                          ctx->rho, ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R,
-                                          ctx->part);
+    {
+        auto k = dm.on ? (nj == 16 ? k_post<float, 16, true> : nj == 8 ? k_post<float, 8, true> : k_post<float, 0, true>)
+                       : (nj == 16 ? k_post<float, 16, false> : nj == 8 ? k_post<float, 8, false> : k_post<float, 0, false>);
+        k<<<nbp, 256, 0, s>>>(c, dm, L, (float *)ctx->gs, (float *)ctx->gts, ref, ctx->rho,
+                              ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R, ctx->part);
+    }
     else
-        k_post<double><<<nbp, 256, 0, s>>>(c, L, (double *)ctx->gs, (const double *)ctx->gts, ref,
-                                           ctx->rho, ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R,
-                                           ctx->part);
+    {
+        auto k = dm.on ? k_post<double, 0, true> : (nj == 8 ? k_post<double, 8, false> : k_post<double, 0, false>);
+        k<<<nbp, 256, 0, s>>>(c, dm, L, (double *)ctx->gs, (double *)ctx->gts, ref, ctx->rho,
+                              ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R, ctx->part);
+    }
     toc(ST_WALL_COLLIDE);
+    dm_step_done(ctx);
     tic();
-    k_reduce<<<1, 1024, 0, s>>>(ctx->N, ctx->part, ctx->scal, ctx->cfg.no_mass_fix);
+    k_reduce_stage<<<kRedBlocks, 256, 0, s>>>(ctx->N, ctx->part, ctx->red);
+    k_reduce<<<1, 256, 0, s>>>(ctx->red, ctx->scal, ctx->cfg.no_mass_fix);
     k_finalize<<<(ctx->N + 255) / 256, 256, 0, s>>>(c, ctx->part, ctx->scal, ctx->cfg.R,
                                                     ctx->rho, ctx->ux, ctx->uy, ctx->uz,
                                                     ctx->T, ctx->limflag);

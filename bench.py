@@ -8,10 +8,13 @@ reference's: gas at rest at the wall temperature).
 
 One step = one full time step of the mixed-precision pipeline: transport
 (MLS coefficients factorised once in FP64, FP32 contraction), positivity,
-NAD, diffuse wall, moments, BGK collision, both mass rescales. The FP64 step
-of the reference formulation is run from the same state for the same number
-of steps; the JSON line reports both and the relative moment error between
-them.
+NAD, diffuse wall, moments, the BGK collision towards the discrete Maxwellian
+(FP64 Newton per particle, warm-started by an MLP trained on the GPU on the
+run's first steps), both mass rescales. The plain FP64 step of the
+reference formulation (continuous Maxwellian) is timed for the speedup; the
+FP64 formulation with the same discrete collision is run from the same state
+for the same number of steps and the JSON line reports the relative moment
+error between it and the measured path.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -93,6 +96,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nx", type=int, default=40)
     ap.add_argument("--nv", type=int, default=10)
+    ap.add_argument("--record", type=int, default=8,
+                    help="warmup steps whose discrete-Maxwellian solves train the MLP")
+    ap.add_argument("--train-iters", type=int, default=600)
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="FP64 steps for the comparison (default: warmup + steps)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -104,18 +110,30 @@ def main():
 
     import numpy as np
     import alebgk4paper_amd as A
+    DISC = A.COLLISION_DISCRETE
 
     case = A.configure("cavity3d", nx=args.nx, nv=args.nv, mls_order=2)
     cloud = A.build_lattice(case, precision=A.PREC_MIXED, mls_order=2)
     N, nodes = cloud.N, args.nv ** 3
     updates = N * nodes
     dt = case.dt
+    rec = min(args.record, args.warmup)
     log(f"[bench] cavity3d {args.nx}^3 Nv={args.nv}: N={N}, {updates:.3e} updates/step")
 
-    # ---- mixed pipeline: the measured step --------------------------------
-    mix = A.Solver(cloud, precision=A.PREC_MIXED)
-    mix.steps(dt, args.warmup)
+    # ---- the measured step: mixed precision, discrete-Maxwellian collision,
+    # Newton warm-started by the MLP trained on this run's own solves -------
+    mix = A.Solver(cloud, precision=A.PREC_MIXED, collision=DISC)
+    mix.steps(dt, args.warmup - rec)
+    mix.dm_record(rec)
+    mix.steps(dt, rec)
     mix.synchronize()
+    t0 = time.perf_counter()
+    mix.mlp_init(seed=1)
+    loss0, loss1 = mix.mlp_train(args.train_iters, batch=4096, lr=2e-3, seed=5)
+    mix.mlp_enable(True)
+    train_s = time.perf_counter() - t0
+    log(f"[bench] MLP: {args.train_iters} Adam steps on {rec * N} rows in {train_s:.2f} s, "
+        f"loss {loss0:.3e} -> {loss1:.3e}")
     mix.profile(True)
     t0 = time.perf_counter()
     mix.steps(dt, args.steps)
@@ -125,18 +143,35 @@ def main():
     mix.profile(False)
     sec = t1 - t0
     value = updates * args.steps / sec
-    log(f"[bench] mixed: {sec * 1e3 / args.steps:.4f} ms/step, {value:.4e} updates/s")
+    log(f"[bench] mixed+DM+MLP: {sec * 1e3 / args.steps:.4f} ms/step, {value:.4e} updates/s")
     log(f"[bench] stage times (ms total, launches): {prof}")
     tr_ms, tr_n = prof["transport"]
     tr_avg_s = tr_ms / max(tr_n, 1) * 1e-3
-    alg_bytes = updates * 2 * 4           # read g once + write gt once, FP32
+    alg_bytes = updates * 2 * 4           # read d once + write gt once, FP32
     achieved = alg_bytes / tr_avg_s / 1e9
     m_mix = mix.moments()
-    mass_mix = mix.total_mass()
+    st_l = mix.step(dt, stats=True)       # one more step: Newton passes with the warm start
+    m_mix_n1 = mix.moments()
 
-    # ---- FP64 reference formulation from the same state --------------------
+    # ---- the same without the warm start (plain Newton) --------------------
+    pln = A.Solver(cloud, precision=A.PREC_MIXED, collision=DISC)
+    pln.steps(dt, args.warmup)
+    pln.synchronize()
+    t0 = time.perf_counter()
+    pln.steps(dt, args.steps)
+    pln.synchronize()
+    sec_plain = time.perf_counter() - t0
+    st_p = pln.step(dt, stats=True)
+    value_plain = updates * args.steps / sec_plain
+    log(f"[bench] mixed+DM (no warm start): {sec_plain * 1e3 / args.steps:.4f} ms/step")
+    passes = {"plain": st_p["dm_passes"] / max(st_p["dm_particles"], 1),
+              "learned": st_l["dm_passes"] / max(st_l["dm_particles"], 1)}
+    log(f"[bench] Newton grid passes per solve: {passes}")
+    pln.close()
+
+    # ---- the reference formulation in FP64 ----------------------------------
     n64 = args.fp64_steps if args.fp64_steps is not None else args.warmup + args.steps
-    ref = A.Solver(cloud, precision=A.PREC_FP64)
+    ref = A.Solver(cloud, precision=A.PREC_FP64)      # the plain FP64 step (continuous BGK)
     ref.step(dt)
     ref.synchronize()
     t0 = time.perf_counter()
@@ -145,10 +180,15 @@ def main():
     sec64 = (time.perf_counter() - t0) / max(n64 - 1, 1)
     value64 = updates / sec64
     log(f"[bench] fp64: {sec64 * 1e3:.3f} ms/step, {value64:.4e} updates/s")
+    ref.close()
     err = None
     if n64 == args.warmup + args.steps:
-        err = moment_errors(m_mix, ref.moments(), cloud.boundary == 0)
-        log(f"[bench] mixed vs fp64 moment errors after {n64} steps: {err}")
+        # accuracy within the same collision mode: FP64 with the discrete target
+        ref_dm = A.Solver(cloud, precision=A.PREC_FP64, collision=DISC)
+        ref_dm.steps(dt, n64)
+        err = moment_errors(m_mix, ref_dm.moments(), cloud.boundary == 0)
+        log(f"[bench] mixed+DM+MLP vs FP64+DM moment errors after {n64} steps: {err}")
+        ref_dm.close()
 
     cpu = None if args.no_cpu else cpu_baseline(nv=args.nv)
     if cpu:
@@ -168,14 +208,18 @@ def main():
         "dtype": "fp32",
         "data": "synthetic: the reference's initial state (gas at rest, 270 K) on the cavity3d lattice",
         "config": {"workload": f"cavity3d Nx={args.nx} Nv={args.nv} quadratic MLS, dt=1e-11, "
-                               f"mixed precision (FP32 transport+collision, FP64 factorisation, "
-                               f"conservation and wall scaling)",
+                               f"mixed precision (FP32 transport and collision; FP64 MLS "
+                               f"factorisation, conservation, wall scaling and discrete-Maxwellian "
+                               f"Newton), MLP warm start trained on the run's first {rec} steps",
                    "particles": N, "velocity_nodes": nodes, "updates_per_step": updates},
         "fp64_value": value64,
         "fp64_ms_per_step": sec64 * 1e3,
         "speedup_vs_fp64": value / value64,
         "moment_err_vs_fp64": err,
-        "mass_drift": None if mass_mix is None else abs(mass_mix - mix.total_mass()),
+        "without_warm_start": {"value": value_plain, "ms_per_step": sec_plain * 1e3 / args.steps},
+        "newton_passes_per_solve": passes,
+        "mlp": {"train_seconds": train_s, "adam_steps": args.train_iters, "batch": 4096,
+                "rows": rec * N, "loss_first": loss0, "loss_last": loss1},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_apply (MLS transport contraction)",

@@ -72,7 +72,18 @@ typedef struct ab_config {
     double xleft, ybottom, zfront;
     double hbox, vbox, wbox;
     int nbhbox, nbvbox, nbwbox;
+    /* Collision target (3-D). AB_COLLISION_CONTINUOUS is the reference's
+     * continuous Maxwellian of (rho, u, T) (kCollisionNode3D,
+     * CudaBackend.cu:1181-1197); AB_COLLISION_DISCRETE relaxes to the
+     * discrete Maxwellian with the grid moments of g (no counterpart in the
+     * reference). dm_tol <= 0 -> 1e-12, dm_maxit <= 0 -> 8. */
+    int collision;
+    double dm_tol;
+    int dm_maxit;
 } ab_config;
+
+#define AB_COLLISION_CONTINUOUS 0
+#define AB_COLLISION_DISCRETE 1
 
 /* Borrowed host view of a cloud (the SoA fields cudaUploadCloud gathers,
  * Solver/CudaBackend.cu:1223-1381). Any of mvx/mvy/mvz may be NULL (zero
@@ -103,6 +114,13 @@ typedef struct ab_step_stats {
     int n_mls_degenerate;          /* particles with a collapsed MLS pivot */
     int n_mls_extrapolate;         /* particles with an overshooting value */
     double mls_ovr_sum, mls_ovr_max;
+    /* discrete-Maxwellian collision: particles solved, those that did not
+     * converge (continuous fallback), grid passes summed over particles,
+     * largest final relative residual */
+    int dm_particles;
+    int dm_unconverged;
+    long long dm_passes;
+    double dm_res_max;
 } ab_step_stats;
 
 /* Lifecycle. Replaces the file-static DeviceCloud (CudaBackend.cu:115) with
@@ -162,6 +180,50 @@ int ab_set_wall_state(ab_ctx *ctx, int n, const int *idx, const double *x,
                       const double *wu, const double *mv);
 
 int ab_synchronize(ab_ctx *ctx);
+
+/* ---- Discrete-Maxwellian collision and its learned warm start ----------
+ * No counterpart in the reference. With cfg.collision = AB_COLLISION_DISCRETE
+ * every interior particle solves, in FP64, for a (5) such that
+ * M_d(v) = exp(a . (1, v/cs, |v|^2/cs^2)) has the grid moments (1, v, |v|^2)
+ * of g; cs is the largest |v| of the grid. Newton starts from the continuous
+ * Maxwellian's parameters, or from those plus a small MLP's prediction of
+ * the difference (ab_mlp_enable); Newton always runs to dm_tol, so the MLP
+ * only changes how many grid passes that takes. */
+
+/* The last step's solution: alpha N x 5, passes N (> dm_maxit + 1: did not
+ * converge, continuous fallback), residual N; any pointer may be NULL. */
+int ab_dm_download(ab_ctx *ctx, double *alpha, int *passes, double *residual);
+
+/* Record MLP training rows during the next `steps` steps (a ring of up to 16
+ * steps of N rows: 5 features u_x/cs, u_y/cs, u_z/cs, R T/cs^2, dv/cs;
+ * 5 targets a_discrete - a_continuous; a weight, 0 for boundary particles
+ * and unconverged solves). ab_dm_rows reports the rows held and how many
+ * carry weight. */
+int ab_dm_record(ab_ctx *ctx, int steps);
+int ab_dm_rows(ab_ctx *ctx, int *rows, int *valid);
+
+/* MLP 5 -> 64 -> 64 -> 5 (tanh), FP32 master parameters, BF16 weights in the
+ * warm start. Parameter block of AB_MLP_PARAMS floats: W1[64][5] b1[64]
+ * W2[64][64] b2[64] W3[5][64] b3[5] (AB_MLP_TRAINABLE), then input and
+ * target normalisation xm[5] xs[5] ym[5] ys[5]. */
+#define AB_MLP_TRAINABLE 4869
+#define AB_MLP_PARAMS 4889
+/* Random initialisation (seeded) and normalisation from the recorded rows. */
+int ab_mlp_init(ab_ctx *ctx, unsigned long long seed);
+/* Adam on the recorded rows, on the device: iters minibatches of `batch`
+ * rows drawn with the seed; loss = sum w |y - t|^2 / (5 batch) in
+ * normalised units. loss_first / loss_last may be NULL. */
+int ab_mlp_train(ab_ctx *ctx, int iters, int batch, float lr, unsigned long long seed,
+                 float *loss_first, float *loss_last);
+int ab_mlp_enable(ab_ctx *ctx, int on);
+int ab_mlp_get(ab_ctx *ctx, float *params);
+int ab_mlp_set(ab_ctx *ctx, const float *params);
+/* Forward pass with the FP32 parameters: x n x 5 -> y n x 5 (de-normalised). */
+int ab_mlp_forward(ab_ctx *ctx, const float *x, int n, float *y);
+/* Loss and its gradient (AB_MLP_TRAINABLE floats) over rows x (n x 5),
+ * targets t (n x 5, de-normalised), weights w (n), FP32 parameters. */
+int ab_mlp_loss_grad(ab_ctx *ctx, const float *x, const float *t, const float *w, int n,
+                     float *loss, float *grad);
 
 /* Per-stage device timing with HIP events on the context's stream (the
  * [GPU-STEADY] table of CudaBackend.cu:1518-1623). After ab_profile(ctx, 1),

@@ -832,12 +832,164 @@ void orc_collision2d(const orc_cloud *c, double *g, double dt, double tau,
 }
 
 /* ======================================================================== */
+/* Discrete-Maxwellian collision (not in the reference)                     */
+/* ======================================================================== */
+
+double orc_dm_scale(int Nv, double vlo, double dv)
+{
+    double vhi = vlo + (Nv - 1) * dv;
+    return fmax(fabs(vlo), fabs(vhi));
+}
+
+/* -|v-u|^2/2RT + ln(rho/(2 pi R T)^1.5) in the scaled basis. */
+void orc_dm_guess(double rho, double ux, double uy, double uz, double T,
+                  double R, double cs, double a[5])
+{
+    double RT = R * T;
+    a[0] = log(rho) - 1.5 * log(2.0 * M_PI * RT) - (ux * ux + uy * uy + uz * uz) / (2.0 * RT);
+    a[1] = cs * ux / RT;
+    a[2] = cs * uy / RT;
+    a[3] = cs * uz / RT;
+    a[4] = -cs * cs / (2.0 * RT);
+}
+
+/* Solve J x = F for symmetric positive definite J (5x5, full storage) by
+ * Cholesky; returns 0 when a pivot is not positive. */
+static int chol5(const double J[25], const double F[5], double x[5])
+{
+    double L[25] = {0}, y[5];
+    for (int i = 0; i < 5; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double s = J[i * 5 + j];
+            for (int k = 0; k < j; ++k) s -= L[i * 5 + k] * L[j * 5 + k];
+            if (i == j) {
+                if (!(s > 0.0)) return 0;
+                L[i * 5 + i] = sqrt(s);
+            } else {
+                L[i * 5 + j] = s / L[j * 5 + j];
+            }
+        }
+    }
+    for (int i = 0; i < 5; ++i) {
+        double s = F[i];
+        for (int k = 0; k < i; ++k) s -= L[i * 5 + k] * y[k];
+        y[i] = s / L[i * 5 + i];
+    }
+    for (int i = 4; i >= 0; --i) {
+        double s = y[i];
+        for (int k = i + 1; k < 5; ++k) s -= L[k * 5 + i] * x[k];
+        x[i] = s / L[i * 5 + i];
+    }
+    return 1;
+}
+
+int orc_dm_solve(int Nv, double vlo, double dv, double cs, const double m[5],
+                 double a[5], double tol, int maxit, double *res)
+{
+    double dv3 = dv * dv * dv;
+    double r = INFINITY;
+    int pass;
+    for (pass = 1; pass <= maxit + 1; ++pass) {
+        double J[25] = {0};
+        for (int kv = 0; kv < Nv; ++kv) {
+            double zz = (vlo + kv * dv) / cs;
+            for (int jv = 0; jv < Nv; ++jv) {
+                double yy = (vlo + jv * dv) / cs;
+                for (int iv = 0; iv < Nv; ++iv) {
+                    double xx = (vlo + iv * dv) / cs;
+                    double psi[5] = {1.0, xx, yy, zz, xx * xx + yy * yy + zz * zz};
+                    double e = exp(a[0] + a[1] * xx + a[2] * yy + a[3] * zz + a[4] * psi[4]) * dv3;
+                    for (int p = 0; p < 5; ++p)
+                        for (int q = p; q < 5; ++q) J[p * 5 + q] += e * psi[p] * psi[q];
+                }
+            }
+        }
+        for (int p = 0; p < 5; ++p)
+            for (int q = 0; q < p; ++q) J[p * 5 + q] = J[q * 5 + p];
+        double F[5];
+        r = 0.0;
+        for (int p = 0; p < 5; ++p) {
+            F[p] = J[p] - m[p];          /* row 0 of J holds the moments of M_d */
+            r = fmax(r, fabs(F[p]));
+        }
+        r /= fabs(m[0]);
+        if (r <= tol || pass == maxit + 1) break;
+        double d[5];
+        if (!chol5(J, F, d)) { pass = maxit + 1; break; }
+        for (int p = 0; p < 5; ++p) a[p] -= d[p];
+    }
+    if (res) *res = r;
+    return (r <= tol) ? pass : maxit + 2;
+}
+
+void orc_collision3d_dm(const orc_cloud *c, double *g, double dt, double tau,
+                        double R, double tol, int maxit, const double *a_init,
+                        double *a_out, int *passes, double *res)
+{
+    int Nv = c->Nv;
+    size_t gs = (size_t)c->gstride;
+    double dv3 = c->dv * c->dv * c->dv;
+    double cs = orc_dm_scale(Nv, c->vlo, c->dv);
+    double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
+    for (int i = 0; i < c->N; i++) {
+        if (c->boundary[i]) continue;
+        double *p = g + (size_t)i * gs;
+        double m[5] = {0, 0, 0, 0, 0};
+        for (int kv = 0; kv < Nv; ++kv) {
+            double vz = c->vlo + kv * c->dv;
+            for (int jv = 0; jv < Nv; ++jv) {
+                double vy = c->vlo + jv * c->dv;
+                for (int iv = 0; iv < Nv; ++iv) {
+                    double vx = c->vlo + iv * c->dv;
+                    double f = p[iv + Nv * (jv + Nv * kv)] * dv3;
+                    m[0] += f;
+                    m[1] += vx * f;
+                    m[2] += vy * f;
+                    m[3] += vz * f;
+                    m[4] += (vx * vx + vy * vy + vz * vz) * f;
+                }
+            }
+        }
+        if (!(m[0] > 1e-30)) continue;
+        double rho = m[0], ux = m[1] / rho, uy = m[2] / rho, uz = m[3] / rho;
+        double t = (m[4] / rho - (ux * ux + uy * uy + uz * uz)) / (3.0 * R);
+        double T = t > 1e-6 ? t : 1e-6;
+        double ms[5] = {m[0], m[1] / cs, m[2] / cs, m[3] / cs, m[4] / (cs * cs)};
+        double a[5], a0[5];
+        orc_dm_guess(rho, ux, uy, uz, T, R, cs, a0);
+        for (int q = 0; q < 5; ++q) a[q] = a_init ? a_init[(size_t)i * 5 + q] : a0[q];
+        double r;
+        int np = orc_dm_solve(Nv, c->vlo, c->dv, cs, ms, a, tol, maxit, &r);
+        int ok = np <= maxit + 1;
+        for (int q = 0; q < 5 && ok; ++q) ok = isfinite(a[q]);
+        if (!ok) for (int q = 0; q < 5; ++q) a[q] = a0[q];   /* continuous fallback */
+        if (a_out) for (int q = 0; q < 5; ++q) a_out[(size_t)i * 5 + q] = a[q];
+        if (passes) passes[i] = np;
+        if (res) res[i] = r;
+        for (int kv = 0; kv < Nv; ++kv) {
+            double zz = (c->vlo + kv * c->dv) / cs;
+            for (int jv = 0; jv < Nv; ++jv) {
+                double yy = (c->vlo + jv * c->dv) / cs;
+                for (int iv = 0; iv < Nv; ++iv) {
+                    double xx = (c->vlo + iv * c->dv) / cs;
+                    double M = exp(a[0] + a[1] * xx + a[2] * yy + a[3] * zz
+                                   + a[4] * (xx * xx + yy * yy + zz * zz));
+                    int lin = iv + Nv * (jv + Nv * kv);
+                    p[lin] = coeff * (p[lin] + lam * M);
+                }
+            }
+        }
+    }
+}
+
+/* ======================================================================== */
 /* One time step                                                             */
 /* ======================================================================== */
 
 /* cudaStep (Solver/CudaBackend.cu:1505-1870) without its diagnostics. */
-double orc_step(const orc_cloud *c, orc_state *s, double dt, double tau,
-                double R, double h, int mlsOrder, double tol, int nofix)
+static double step_impl(const orc_cloud *c, orc_state *s, double dt, double tau,
+                        double R, double h, int mlsOrder, double tol, int nofix,
+                        int dm, double dm_tol, int dm_maxit)
 {
     int N = c->N;
     for (int i = 0; i < N; i++) {
@@ -860,7 +1012,8 @@ double orc_step(const orc_cloud *c, orc_state *s, double dt, double tau,
     if (c->dim == 3) {
         orc_wallbc3d(c, s->g, R, s->rho, s->ux, s->uy, s->uz, s->T);
         orc_moments3d(c, s->g, R, s->rho, s->ux, s->uy, s->uz, s->T);
-        orc_collision3d(c, s->g, dt, tau, R, s->rho, s->ux, s->uy, s->uz, s->T);
+        if (dm) orc_collision3d_dm(c, s->g, dt, tau, R, dm_tol, dm_maxit, NULL, NULL, NULL, NULL);
+        else orc_collision3d(c, s->g, dt, tau, R, s->rho, s->ux, s->uy, s->uz, s->T);
     } else {
         orc_wallbc2d(c, s->g, R, s->rho, s->ux, s->uy, s->T);
         orc_moments2d(c, s->g, R, s->rho, s->ux, s->uy, s->T);
@@ -873,6 +1026,19 @@ double orc_step(const orc_cloud *c, orc_state *s, double dt, double tau,
     if (c->dim == 3) orc_moments3d(c, s->g, R, s->rho, s->ux, s->uy, s->uz, s->T);
     else orc_moments2d(c, s->g, R, s->rho, s->ux, s->uy, s->T);
     return massOld;
+}
+
+double orc_step(const orc_cloud *c, orc_state *s, double dt, double tau,
+                double R, double h, int mlsOrder, double tol, int nofix)
+{
+    return step_impl(c, s, dt, tau, R, h, mlsOrder, tol, nofix, 0, 0.0, 0);
+}
+
+double orc_step_dm(const orc_cloud *c, orc_state *s, double dt, double tau,
+                   double R, double h, int mlsOrder, double tol, int nofix,
+                   double dm_tol, int dm_maxit)
+{
+    return step_impl(c, s, dt, tau, R, h, mlsOrder, tol, nofix, 1, dm_tol, dm_maxit);
 }
 
 /* ======================================================================== */

@@ -117,10 +117,38 @@ void orc_collision2d(const orc_cloud *c, double *g, double dt, double tau,
                      double R, const double *rho, const double *ux,
                      const double *uy, const double *T);
 
+/* Discrete-Maxwellian collision (not in the reference). The BGK target is
+ * M_d(v) = exp(a0 + a1 xi_x + a2 xi_y + a3 xi_z + a4 |xi|^2), xi = v / cs,
+ * whose grid moments sum_k M_d(v_k) psi_k dv^3, psi = (1, xi, |xi|^2), equal
+ * those of g, so the collision conserves mass, momentum and energy on the
+ * velocity grid exactly (discrete-velocity BGK, Mieussens 2000). Newton in
+ * FP64 on the 5 unknowns from the continuous Maxwellian of (rho, u, T).
+ * cs: velocity scale (largest |v| of the grid). */
+double orc_dm_scale(int Nv, double vlo, double dv);
+void orc_dm_guess(double rho, double ux, double uy, double uz, double T,
+                  double R, double cs, double a[5]);
+/* m: target moments in scaled form (sum g psi dv^3). Each pass over the
+ * grid evaluates the residual F = sum M_d psi dv^3 - m and the Jacobian at
+ * the current a; a pass with max|F_j| <= tol * m0 ends the solve. Returns
+ * the number of passes (> maxit when it did not converge), *res the last
+ * relative residual. */
+int orc_dm_solve(int Nv, double vlo, double dv, double cs, const double m[5],
+                 double a[5], double tol, int maxit, double *res);
+/* Collision with M_d for every interior particle. a_init (N x 5, may be
+ * NULL: continuous guess) warm-starts Newton; a_out (N x 5), passes (N) and
+ * res (N) may be NULL. */
+void orc_collision3d_dm(const orc_cloud *c, double *g, double dt, double tau,
+                        double R, double tol, int maxit, const double *a_init,
+                        double *a_out, int *passes, double *res);
+
 /* The whole cudaStep sequence. Returns massOld. nofix disables the second
  * mass restore (ALEBGK_NO_MASS_FIX). */
 double orc_step(const orc_cloud *c, orc_state *s, double dt, double tau,
                 double R, double h, int mlsOrder, double tol, int nofix);
+/* The same step with the discrete-Maxwellian collision (3-D). */
+double orc_step_dm(const orc_cloud *c, orc_state *s, double dt, double tau,
+                   double R, double h, int mlsOrder, double tol, int nofix,
+                   double dm_tol, int dm_maxit);
 
 /* Surface-stress loads of one body, computeBodyLoads
  * (RigidBody/BodyDynamics.cpp:17-97). out6 = Fx,Fy,Fz,Tx,Ty,Tz. */

@@ -68,6 +68,9 @@ def lib():
         L.orc_neighbours.restype = C.c_int
         L.orc_sum_mass.restype = C.c_double
         L.orc_step.restype = C.c_double
+        L.orc_step_dm.restype = C.c_double
+        L.orc_dm_scale.restype = C.c_double
+        L.orc_dm_scale.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_maxwellian3d.restype = C.c_double
         L.orc_maxwellian3d.argtypes = [C.c_double] * 9
         L.orc_maxwellian_g1.restype = C.c_double
@@ -279,6 +282,47 @@ def step(cloud, state, dt, tau, R, h, order, tol=1e-12, nofix=0):
     return lib().orc_step(cloud.ref, C.byref(state._s), C.c_double(dt), C.c_double(tau),
                           C.c_double(R), C.c_double(h), C.c_int(order), C.c_double(tol),
                           C.c_int(nofix))
+
+
+def step_dm(cloud, state, dt, tau, R, h, order, tol=1e-12, nofix=0, dm_tol=1e-12, dm_maxit=8):
+    return lib().orc_step_dm(cloud.ref, C.byref(state._s), C.c_double(dt), C.c_double(tau),
+                             C.c_double(R), C.c_double(h), C.c_int(order), C.c_double(tol),
+                             C.c_int(nofix), C.c_double(dm_tol), C.c_int(dm_maxit))
+
+
+def dm_scale(Nv, vlo, dv):
+    return lib().orc_dm_scale(Nv, vlo, dv)
+
+
+def dm_guess(rho, u, T, R, cs):
+    a = np.zeros(5)
+    lib().orc_dm_guess(C.c_double(rho), C.c_double(u[0]), C.c_double(u[1]), C.c_double(u[2]),
+                       C.c_double(T), C.c_double(R), C.c_double(cs), _ptr(a, _dp))
+    return a
+
+
+def dm_solve(Nv, vlo, dv, cs, m, a, tol=1e-12, maxit=8):
+    """Returns (a, passes, residual)."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    r = C.c_double(0.0)
+    n = lib().orc_dm_solve(C.c_int(Nv), C.c_double(vlo), C.c_double(dv), C.c_double(cs),
+                           _ptr(m, _dp), _ptr(a, _dp), C.c_double(tol), C.c_int(maxit),
+                           C.byref(r))
+    return a, n, r.value
+
+
+def collision_dm(cloud, g, dt, tau, R, tol=1e-12, maxit=8, a_init=None):
+    """Returns (g, alpha N x 5, passes N, residual N)."""
+    g = np.ascontiguousarray(g, dtype=np.float64).copy()
+    N = cloud.N
+    a_out = np.zeros((N, 5)); passes = np.zeros(N, np.int32); res = np.zeros(N)
+    ainit = None if a_init is None else np.ascontiguousarray(a_init, dtype=np.float64)
+    ai = None if ainit is None else _ptr(ainit, _dp)
+    lib().orc_collision3d_dm(cloud.ref, _ptr(g, _dp), C.c_double(dt), C.c_double(tau),
+                             C.c_double(R), C.c_double(tol), C.c_int(maxit), ai,
+                             _ptr(a_out, _dp), _ptr(passes, _ip), _ptr(res, _dp))
+    return g, a_out, passes, res
 
 
 def body_loads(cloud, g, bodyid, body, centre, area):
