@@ -100,7 +100,8 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_body_loads", "ab_set_wall_state", "ab_synchronize", "ab_profile",
                "ab_profile_read", "ab_stage_name", "ab_dm_download", "ab_dm_record",
                "ab_dm_rows", "ab_mlp_init", "ab_mlp_train", "ab_mlp_enable", "ab_mlp_get",
-               "ab_mlp_set", "ab_mlp_forward", "ab_mlp_loss_grad"]
+               "ab_mlp_set", "ab_mlp_forward", "ab_mlp_loss_grad", "ab_graph_create",
+               "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
@@ -411,6 +412,41 @@ class Solver:
         ns = C.c_int()
         self._check(self._lib.ab_profile_read(self._ctx, ms, cnt, C.byref(ns)), "ab_profile_read")
         return {self._lib.ab_stage_name(k).decode(): (ms[k], cnt[k]) for k in range(ns.value)}
+
+
+class Graph:
+    """Several solvers' steps captured into one HIP graph (ab_graph_*): each
+    solver is an independent branch; launch() replays nsteps steps of all."""
+
+    def __init__(self, solvers, dt, nsteps):
+        self._lib = hip_lib()
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s._ctx.value for s in self.solvers])
+        self._g = C.c_void_p()
+        rc = self._lib.ab_graph_create(arr, len(self.solvers), C.c_double(dt), int(nsteps),
+                                       C.byref(self._g))
+        if rc:
+            msgs = "; ".join(s._lib.ab_last_error(s._ctx).decode() for s in self.solvers)
+            raise ALEBGKError(f"ab_graph_create: {ERRORS.get(rc, rc)}: {msgs}")
+
+    def launch(self):
+        if self._lib.ab_graph_launch(self._g):
+            raise ALEBGKError("ab_graph_launch failed")
+
+    def synchronize(self):
+        if self._lib.ab_graph_synchronize(self._g):
+            raise ALEBGKError("ab_graph_synchronize failed")
+
+    def close(self):
+        if self._g:
+            self._lib.ab_graph_destroy(self._g)
+            self._g = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def velocities(cfg):

@@ -202,6 +202,23 @@ int do_step(ab_ctx *c, double dt)
 
 namespace ab {
 
+// The step without host work, for graph capture; graph_prepare refuses what
+// a capture cannot hold and builds the table coefficients for dt eagerly.
+int graph_prepare(ab_ctx *c, double dt)
+{
+    if (!c->uploaded) { c->err = "graph capture before ab_upload"; return AB_ERR_STATE; }
+    if (c->prof) { c->err = "graph capture with profiling on"; return AB_ERR_STATE; }
+    if (c->dm_rec_left > 0) { c->err = "graph capture while recording training rows"; return AB_ERR_STATE; }
+    if (table_mode(c) && c->coef_dt != dt)
+    {
+        int rc = table_prepare(c, dt);
+        if (rc) return rc;
+    }
+    return AB_OK;
+}
+
+int graph_step(ab_ctx *c, double dt) { return do_step(c, dt); }
+
 constexpr int kRecSlots = 16;     // steps the training-row ring holds
 
 __global__ void k_fill_f(float *p, size_t n, float v)

@@ -1,0 +1,118 @@
+// ab_graph.hip -- several contexts' steps captured into one HIP graph (no
+// counterpart in the reference, whose time loop launches ~20 kernels per
+// step from the host with three host synchronisations).
+//
+// The capture forks from a private stream into every context's stream and
+// joins back: each instance is an independent branch of the graph, so a
+// replay runs the instances concurrently and a small instance no longer
+// pays a host launch per kernel. The captured step is exactly the eager
+// step (same kernels, arguments and order), so a replay gives bit-identical
+// results; what the capture cannot hold is refused up front: a step with
+// host synchronisation (stats, profiling), training-row recording, or a
+// table pipeline whose coefficients would be rebuilt for a new dt.
+#include "ab_internal.hpp"
+
+#include <new>
+#include <string>
+#include <vector>
+
+struct ab_graph
+{
+    std::vector<ab_ctx *> ctxs;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> events;
+};
+
+namespace ab {
+int graph_step(ab_ctx *c, double dt);   // ab_api.hip: the eager step, no host work
+int graph_prepare(ab_ctx *c, double dt);
+}  // namespace ab
+
+namespace {
+
+void graph_free(ab_graph *g)
+{
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (auto e : g->events) if (e) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph **out)
+{
+    if (!ctxs || n <= 0 || nsteps <= 0 || !out || !(dt > 0.0)) return AB_ERR_ARG;
+    *out = nullptr;
+    for (int k = 0; k < n; ++k)
+    {
+        if (!ctxs[k]) return AB_ERR_ARG;
+        for (int q = 0; q < k; ++q) if (ctxs[q] == ctxs[k]) return AB_ERR_ARG;
+        int rc = ab::graph_prepare(ctxs[k], dt);
+        if (rc) return rc;
+    }
+    ab_graph *g = new (std::nothrow) ab_graph;
+    if (!g) return AB_ERR_NOMEM;
+    g->ctxs.assign(ctxs, ctxs + n);
+    g->events.assign(n + 1, nullptr);
+    bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
+    for (auto &e : g->events)
+        ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    // everything enqueued so far completes before the capture starts
+    for (int k = 0; k < n && ok; ++k) ok = hipStreamSynchronize(ctxs[k]->stream) == hipSuccess;
+    if (!ok) { graph_free(g); return AB_ERR_HIP; }
+
+    int rc = AB_OK;
+    ok = hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal) == hipSuccess &&
+         hipEventRecord(g->events[n], g->stream) == hipSuccess;
+    for (int k = 0; k < n && ok; ++k)
+    {
+        ab_ctx *c = ctxs[k];
+        ok = hipStreamWaitEvent(c->stream, g->events[n], 0) == hipSuccess;
+        for (int s = 0; s < nsteps && ok && !rc; ++s) rc = ab::graph_step(c, dt);
+        ok = ok && !rc && hipEventRecord(g->events[k], c->stream) == hipSuccess &&
+             hipStreamWaitEvent(g->stream, g->events[k], 0) == hipSuccess;
+    }
+    hipGraph_t graph = nullptr;
+    const bool ended = hipStreamEndCapture(g->stream, &graph) == hipSuccess;
+    g->graph = graph;
+    if (!ok || rc || !ended || !graph)
+    {
+        graph_free(g);
+        return rc ? rc : AB_ERR_HIP;
+    }
+    if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    {
+        graph_free(g);
+        return AB_ERR_HIP;
+    }
+    *out = g;
+    return AB_OK;
+}
+
+int ab_graph_launch(ab_graph *g)
+{
+    if (!g || !g->exec) return AB_ERR_ARG;
+    return hipGraphLaunch(g->exec, g->stream) == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+int ab_graph_synchronize(ab_graph *g)
+{
+    if (!g) return AB_ERR_ARG;
+    return hipStreamSynchronize(g->stream) == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+int ab_graph_destroy(ab_graph *g)
+{
+    if (!g) return AB_ERR_ARG;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    graph_free(g);
+    return AB_OK;
+}
+
+}  // extern "C"

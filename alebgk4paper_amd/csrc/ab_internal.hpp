@@ -205,6 +205,7 @@ int table_setup(ab_ctx *ctx);
 int table_upload_g(ab_ctx *ctx, const double *g_host);
 int table_download_g(ab_ctx *ctx, double *g_host);
 int table_step(ab_ctx *ctx, double dt);
+int table_prepare(ab_ctx *ctx, double dt);   // coefficients for dt
 
 // Discrete-Maxwellian settings of the next step (records training rows when
 // asked); defined in ab_api.hip.

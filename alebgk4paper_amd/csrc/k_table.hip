@@ -234,6 +234,24 @@ __device__ __forceinline__ T bload(const T *row, int voff)
         return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0));
 }
 
+// A descriptor built once for a base and reused with a wave-uniform byte
+// offset (SGPR soffset) per row: one descriptor per column and z block
+// instead of one per load. Offsets stay below 2^31 bytes (a z block spans at
+// most 13 lattice planes).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T bload_so(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 template <typename T>
 __device__ __forceinline__ void bstore(T *row, int voff, T v)
 {
@@ -256,31 +274,37 @@ __device__ __forceinline__ void bstore(T *row, int voff, T v)
 
 // End-plane output: coefficients and values of one column, per offset.
 template <typename T, int CC>
-__device__ __forceinline__ void end_load(const T *__restrict__ g, const T *__restrict__ cf,
+__device__ __forceinline__ void end_load(const T *__restrict__ g, __amdgpu_buffer_rsrc_t rc,
                                          const Lat &L, int ix, int iy, int iz, int lin,
                                          T (&cv)[5], T (&gv)[5])
 {
     constexpr Col k = col_at(CC);
     const int xx = clampi(ix + k.ox, 0, L.nx - 1), yy = clampi(iy + k.oy, 0, L.ny - 1);
+    // the column from plane z0 = max(iz - 2, 0): every row it reads is within
+    // 4 planes of it
+    const int z0 = iz >= 2 ? iz - 2 : 0;
+    const __amdgpu_buffer_rsrc_t rg =
+        brsrc(g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * z0)) * L.S);
+    const int pb = L.nx * L.ny * L.S * (int)sizeof(T);
 #pragma clang loop unroll(full)
     for (int j = 0; j <= 2 * k.d; ++j)
     {
         const int zz = clampi(iz - k.d + j, 0, L.nz - 1);
-        cv[j] = bload(cf + (size_t)(k.o0 + j) * L.S, lin);
-        gv[j] = bload(g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * zz)) * L.S, lin);
+        cv[j] = bload_so<T>(rc, lin, (k.o0 + j) * L.S * (int)sizeof(T));
+        gv[j] = bload_so<T>(rg, lin, (zz - z0) * pb);
     }
 }
 
 template <typename T, int CC>
-__device__ __forceinline__ void end_step(const T *__restrict__ g, const T *__restrict__ cf,
+__device__ __forceinline__ void end_step(const T *__restrict__ g, __amdgpu_buffer_rsrc_t rc,
                                          const Lat &L, int ix, int iy, int iz, int lin,
                                          T (&ca)[5], T (&ga)[5], T (&cb)[5], T (&gb)[5], T &acc)
 {
     constexpr Col k = col_at(CC);
     if constexpr (CC + 1 < kCols)
     {
-        if constexpr ((CC & 1) == 0) end_load<T, CC + 1>(g, cf, L, ix, iy, iz, lin, cb, gb);
-        else end_load<T, CC + 1>(g, cf, L, ix, iy, iz, lin, ca, ga);
+        if constexpr ((CC & 1) == 0) end_load<T, CC + 1>(g, rc, L, ix, iy, iz, lin, cb, gb);
+        else end_load<T, CC + 1>(g, rc, L, ix, iy, iz, lin, ca, ga);
     }
 #pragma clang loop unroll(full)
     for (int j = 0; j <= 2 * k.d; ++j)
@@ -299,8 +323,9 @@ __device__ __forceinline__ T apply_one(const T *__restrict__ g, const T *__restr
 {
     T acc = T(0);
     T ca[5], ga[5], cb[5], gb[5];
-    end_load<T, 0>(g, cf, L, ix, iy, iz, lin, ca, ga);
-    (end_step<T, CC>(g, cf, L, ix, iy, iz, lin, ca, ga, cb, gb, acc), ...);
+    const __amdgpu_buffer_rsrc_t rc = brsrc(cf);
+    end_load<T, 0>(g, rc, L, ix, iy, iz, lin, ca, ga);
+    (end_step<T, CC>(g, rc, L, ix, iy, iz, lin, ca, ga, cb, gb, acc), ...);
     return acc;
 }
 
@@ -312,13 +337,14 @@ __device__ __forceinline__ void col_load(const T *__restrict__ g, const Lat &L, 
     constexpr Col kc = col_at(CC);
     constexpr int NV = K + 2 * kc.d;
     const int xx = clampi(ix + kc.ox, 0, L.nx - 1), yy = clampi(iy + kc.oy, 0, L.ny - 1);
-    const T *gc = g + ((size_t)xx + (size_t)L.nx * yy) * L.S;
+    // the column from plane iz0 - d (>= 0 in the interior march)
+    const int z0 = iz0 - kc.d;
+    const __amdgpu_buffer_rsrc_t rg =
+        brsrc(g + ((size_t)xx + (size_t)L.nx * yy) * L.S + (size_t)z0 * zstride);
+    const int pb = (int)(zstride * sizeof(T));
+    const int zmax = L.nz - 1 - z0;
 #pragma clang loop unroll(full)
-    for (int j = 0; j < NV; ++j)
-    {
-        const int zz = clampi(iz0 - kc.d + j, 0, L.nz - 1);
-        v[j] = bload(gc + (size_t)zz * zstride, lin);
-    }
+    for (int j = 0; j < NV; ++j) v[j] = bload_so<T>(rg, lin, (j < zmax ? j : zmax) * pb);
 }
 
 template <typename T, int K, int CC>
@@ -461,29 +487,6 @@ struct NodeWalk
     }
 };
 
-// Calls f(j, k, walk) for the lane's nodes k = lane + 64 j < nodes. With
-// NJ > 0 the loop is unrolled (j is a compile-time index into the lane's
-// registers, nodes <= 64 NJ); NJ == 0 walks any row length.
-template <int NJ, typename F>
-__device__ __forceinline__ void for_nodes(int lane, int nodes, int Nv, F &&f)
-{
-    NodeWalk w(lane, Nv);
-    if constexpr (NJ > 0)
-    {
-#pragma unroll
-        for (int j = 0; j < NJ; ++j)
-        {
-            const int k = lane + kWave * j;
-            if (k < nodes) f(j, k, w);
-            w.step();
-        }
-    }
-    else
-    {
-        for (int j = 0, k = lane; k < nodes; ++j, k += kWave) { f(j, k, w); w.step(); }
-    }
-}
-
 // Boundary particle (one wave): positivity, then the diffuse wall in FP64
 // (kWallBC3D, CudaBackend.cu:956-1030), written back as a deviation.
 // Streams the row from memory (14 % of the particles at 40^3).
@@ -563,11 +566,11 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, int lane
 
 // One wave per particle. Interior particles: positivity, NAD, moments, the
 // BGK collision (continuous target, or the discrete Maxwellian when DM) and
-// the partial sums, with the row held in registers when NJ > 0 (NJ = the
-// lane's node count, nodes <= 64 NJ, only the last one guarded); NJ == 0
-// streams the row from memory in every pass. The raw moments are summed in
-// the positivity pass and redone only when the repair fires.
-template <typename T, int NJ, bool DM>
+// the partial sums. Each pass streams the row (the re-reads hit L2; holding
+// the row in registers measured slower: the occupancy it costs is worth more
+// than the loads it saves). The raw moments are summed in the positivity
+// pass and redone only when the repair fires.
+template <typename T, bool DM>
 __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *__restrict__ g,
                                               T *__restrict__ gt, Ref ref,
                                               const double *__restrict__ rho,
@@ -592,50 +595,24 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
     const double dv3 = c.dv * c.dv * c.dv;
     const double vlo = c.vlo, dv = c.dv;
 
-    constexpr int NR = NJ > 0 ? NJ : 1;
-    T tv[NR];
-    double mv[NR];
-    // the lane's nodes k = lane + 64 j; with NJ > 0 only j = NJ-1 can run
-    // past the row
+    // the lane's nodes k = lane, lane + 64, ...
     auto each = [&](auto &&f) {
         NodeWalk w(lane, Nv);
-        if constexpr (NJ > 0)
-        {
-#pragma unroll
-            for (int j = 0; j < NJ; ++j)
-            {
-                const int k = lane + kWave * j;
-                if (j < NJ - 1 || k < nodes) f(j, k, w);
-                w.step();
-            }
-        }
-        else
-        {
-            for (int j = 0, k = lane; k < nodes; ++j, k += kWave) { f(j, k, w); w.step(); }
-        }
-    };
-    auto tval = [&](int j, int k) -> double {
-        if constexpr (NJ > 0) return (double)tv[j]; else return (double)tp[k];
-    };
-    auto mval = [&](int j, int k) -> double {
-        if constexpr (NJ > 0) return mv[j]; else return m0[k];
+        for (int k = lane; k < nodes; k += kWave) { f(k, w); w.step(); }
     };
 
     // pass 1: positivity sums and the raw moments of M0 + d (L-units)
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};      // mPos mNeg mFb r0..r4
-    auto moments_of = [&](int j, int k, const NodeWalk &w, double d) {
+    auto moments_of = [&](const NodeWalk &w, double d) {
         const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
         s[3] += d; s[4] += vx * d; s[5] += vy * d; s[6] += vz * d;
         s[7] += (vx * vx + vy * vy + vz * vz) * d;
     };
-    each([&](int j, int k, const NodeWalk &w) {
-        const T t = tp[k];
-        const double m = m0[k];
-        if constexpr (NJ > 0) { tv[j] = t; mv[j] = m; }
-        const double d = (double)t, v = m + d;
+    each([&](int k, const NodeWalk &w) {
+        const double m = m0[k], d = (double)tp[k], v = m + d;
         if (v >= 0.0) s[0] += v;
         else { s[1] += v; s[2] += m + (double)gp[k]; }
-        moments_of(j, k, w, d);
+        moments_of(w, d);
     });
 #pragma unroll
     for (int q = 0; q < 8; ++q) s[q] = wave_sum(s[q]);
@@ -651,12 +628,12 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
         const double sc = (Lo * replaced > 1e-30) ? (s[0] + s[1]) / replaced : 0.0;
 #pragma unroll
         for (int q = 3; q < 8; ++q) s[q] = 0.0;
-        each([&](int j, int k, const NodeWalk &w) {
-            const double dg = tval(j, k), m = mval(j, k);
+        each([&](int k, const NodeWalk &w) {
+            const double dg = (double)tp[k], m = m0[k];
             const double sel = (m + dg < 0.0) ? (double)gp[k] : dg;
             const T lim = T(sc * sel + (sc - 1.0) * m);
-            if constexpr (NJ > 0) tv[j] = lim; else tp[k] = lim;
-            moments_of(j, k, w, (double)lim);
+            tp[k] = lim;
+            moments_of(w, (double)lim);
         });
 #pragma unroll
         for (int q = 3; q < 8; ++q) s[q] = wave_sum(s[q]);
@@ -788,11 +765,11 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
     }
     const T cT = T(coeff), lamT = T(lam), Lt = T(Lo);
     double h[5] = {0, 0, 0, 0, 0};
-    each([&](int j, int k, const NodeWalk &w) {
+    each([&](int k, const NodeWalk &w) {
         const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
-        const double m = mval(j, k);
+        const double m = m0[k];
         const double v2 = vx * vx + vy * vy + vz * vz;
-        const double d2 = sc2 * tval(j, k) + (sc2 - 1.0) * m;
+        const double d2 = sc2 * (double)tp[k] + (sc2 - 1.0) * m;
         const double D = A0 + Bq * v2 + Cx * vx + Cy * vy + Cz * vz;
         const T mm = okm ? T(m) * expm1(T(D)) : T(-m);
         const T dh = cT * (T(d2) + lamT * mm);
@@ -944,6 +921,12 @@ int apply_launch(ab_ctx *ctx, const Lat &L)
     const T *g = (const T *)ctx->gs;
     T *gt = (T *)ctx->gts;
     const T *cf = (const T *)ctx->coef;
+    // buffer offsets within a z block (up to 13 planes) must stay below 2^31
+    if (13.0 * ctx->cfg.nx * ctx->cfg.ny * (double)ctx->S * sizeof(T) >= 2147483648.0)
+    {
+        ctx->err = "table transport: a z block of the lattice exceeds 2 GiB";
+        return AB_ERR_UNSUPPORTED;
+    }
     if (run % 9 == 0) k_apply<T, 9><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
     else if (run % 7 == 0) k_apply<T, 7><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
     else k_apply<T, 8><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
@@ -1148,15 +1131,22 @@ int table_download_g(ab_ctx *ctx, double *g_dev)
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 
+int table_prepare(ab_ctx *ctx, double dt)
+{
+    int rc = table_build(ctx, dt);
+    if (rc) { ctx->err = "table build failed"; return rc; }
+    ctx->coef_dt = dt;
+    return AB_OK;
+}
+
 int table_step(ab_ctx *ctx, double dt)
 {
     // The coefficients depend on dt through the foot points: rebuild when it
     // changes (the last step of a run is shortened to land on tfinal).
     if (ctx->coef_dt != dt)
     {
-        int rc = table_build(ctx, dt);
-        if (rc) { ctx->err = "table build failed"; return rc; }
-        ctx->coef_dt = dt;
+        int rc = table_prepare(ctx, dt);
+        if (rc) return rc;
     }
     Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave};
     ab::DevCloud c = ctx->view();
@@ -1170,23 +1160,20 @@ int table_step(ab_ctx *ctx, double dt)
     ref.rho = ctx->m0rho;
     ref.T = ctx->m0T;
     tic();
-    if (is_fp32(ctx)) apply_launch<float>(ctx, L);
-    else apply_launch<double>(ctx, L);
+    const int arc = is_fp32(ctx) ? apply_launch<float>(ctx, L) : apply_launch<double>(ctx, L);
+    if (arc) return arc;
     toc(ST_TRANSPORT);
     tic();
-    // the row in registers when the lane's node count is 16 or 8 exactly
-    const int nj = (ctx->nodes + kWave - 1) / kWave;
     const DmArgs dm = dm_args(ctx);
     if (is_fp32(ctx))
     {
-        auto k = dm.on ? (nj == 16 ? k_post<float, 16, true> : nj == 8 ? k_post<float, 8, true> : k_post<float, 0, true>)
-                       : (nj == 16 ? k_post<float, 16, false> : nj == 8 ? k_post<float, 8, false> : k_post<float, 0, false>);
+        auto k = dm.on ? k_post<float, true> : k_post<float, false>;
         k<<<nbp, 256, 0, s>>>(c, dm, L, (float *)ctx->gs, (float *)ctx->gts, ref, ctx->rho,
                               ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R, ctx->part);
     }
     else
     {
-        auto k = dm.on ? k_post<double, 0, true> : (nj == 8 ? k_post<double, 8, false> : k_post<double, 0, false>);
+        auto k = dm.on ? k_post<double, true> : k_post<double, false>;
         k<<<nbp, 256, 0, s>>>(c, dm, L, (double *)ctx->gs, (double *)ctx->gts, ref, ctx->rho,
                               ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R, ctx->part);
     }

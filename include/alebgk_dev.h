@@ -181,6 +181,22 @@ int ab_set_wall_state(ab_ctx *ctx, int n, const int *idx, const double *x,
 
 int ab_synchronize(ab_ctx *ctx);
 
+/* ---- Several contexts in one HIP graph ----------------------------------
+ * No counterpart in the reference (its time loop launches every kernel from
+ * the host). ab_graph_create captures nsteps steps of each of the n
+ * contexts (uploaded, distinct) into one graph in which every context is an
+ * independent branch; ab_graph_launch enqueues one replay (asynchronous, on
+ * the graph's own stream) and ab_graph_synchronize waits for it. A replay
+ * gives exactly the eager steps' results. Refused (AB_ERR_STATE) while a
+ * context profiles or records training rows; table coefficients are built
+ * for dt before the capture. Do not step a captured context eagerly while
+ * a replay is in flight. */
+typedef struct ab_graph ab_graph;
+int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph **out);
+int ab_graph_launch(ab_graph *graph);
+int ab_graph_synchronize(ab_graph *graph);
+int ab_graph_destroy(ab_graph *graph);
+
 /* ---- Discrete-Maxwellian collision and its learned warm start ----------
  * No counterpart in the reference. With cfg.collision = AB_COLLISION_DISCRETE
  * every interior particle solves, in FP64, for a (5) such that
