@@ -101,7 +101,8 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_profile_read", "ab_stage_name", "ab_dm_download", "ab_dm_record",
                "ab_dm_rows", "ab_mlp_init", "ab_mlp_train", "ab_mlp_enable", "ab_mlp_get",
                "ab_mlp_set", "ab_mlp_forward", "ab_mlp_loss_grad", "ab_graph_create",
-               "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy"]
+               "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy", "ab_autotune",
+               "ab_set_transport_shape"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
@@ -300,6 +301,22 @@ class Solver:
 
     def rebuild_neighbours(self):
         self._check(self._lib.ab_rebuild_neighbours(self._ctx), "ab_rebuild_neighbours")
+
+    # ---- transport launch shape ------------------------------------------
+    def autotune(self, dt, reps=3):
+        """Times every transport launch shape, keeps the fastest; returns
+        ({name: ms}, chosen name)."""
+        cap = 32
+        names = ((C.c_char * 32) * cap)()
+        ms = (C.c_float * cap)()
+        n, ch = C.c_int(0), C.c_int(-1)
+        self._check(self._lib.ab_autotune(self._ctx, C.c_double(dt), int(reps), cap, names, ms,
+                                          C.byref(n), C.byref(ch)), "ab_autotune")
+        table = {names[i].value.decode(): ms[i] for i in range(n.value)}
+        return table, names[ch.value].value.decode()
+
+    def set_transport_shape(self, index):
+        self._check(self._lib.ab_set_transport_shape(self._ctx, int(index)), "ab_set_transport_shape")
 
     # ---- discrete-Maxwellian collision and the MLP warm start --------------
     def dm_download(self):

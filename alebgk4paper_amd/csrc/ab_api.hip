@@ -533,6 +533,22 @@ int ab_step(ab_ctx *c, double dt, ab_step_stats *st)
     return AB_OK;
 }
 
+int ab_autotune(ab_ctx *c, double dt, int reps, int cap, char (*names)[32], float *ms, int *n,
+                int *chosen)
+{
+    if (!c || reps < 1 || cap < 0 || !(dt > 0.0)) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "ab_autotune before ab_upload"; return AB_ERR_STATE; }
+    if (!table_mode(c)) { c->err = "ab_autotune tunes the table pipelines' transport"; return AB_ERR_UNSUPPORTED; }
+    return table_autotune(c, dt, reps, cap, names, ms, n, chosen);
+}
+
+int ab_set_transport_shape(ab_ctx *c, int shape)
+{
+    if (!c) return AB_ERR_ARG;
+    if (!table_mode(c)) { c->err = "transport shapes exist in the table pipelines"; return AB_ERR_UNSUPPORTED; }
+    return table_set_shape(c, shape);
+}
+
 int ab_dm_download(ab_ctx *c, double *alpha, int *passes, double *residual)
 {
     if (!c) return AB_ERR_ARG;

@@ -30,6 +30,16 @@ __device__ __forceinline__ T wave_sum(T v)
     return v;
 }
 
+// A value every lane of the wave holds (a wave reduction's result) moved to
+// scalar registers: per-particle quantities then cost no VGPRs.
+__device__ __forceinline__ double uniform(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_min(T v)
 {

@@ -119,6 +119,7 @@ struct ab_ctx
     double *part = nullptr;        // per-particle partials of the table pipeline
     double *red = nullptr;         // block sums of the partials (2 per reduction block)
     int *bad = nullptr;            // stencil check result
+    int apply_shape = -1;          // transport launch shape (-1: default for nz)
 
     // discrete-Maxwellian collision (cfg.collision) and its MLP warm start
     double *dm_alpha = nullptr;    // N x 5 solution of the last step
@@ -206,6 +207,9 @@ int table_upload_g(ab_ctx *ctx, const double *g_host);
 int table_download_g(ab_ctx *ctx, double *g_host);
 int table_step(ab_ctx *ctx, double dt);
 int table_prepare(ab_ctx *ctx, double dt);   // coefficients for dt
+int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32], float *ms,
+                   int *n, int *chosen);
+int table_set_shape(ab_ctx *ctx, int shape);
 
 // Discrete-Maxwellian settings of the next step (records training rows when
 // asked); defined in ab_api.hip.

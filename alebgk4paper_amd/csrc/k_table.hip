@@ -33,6 +33,7 @@
 #include "ab_internal.hpp"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -383,8 +384,8 @@ __device__ __forceinline__ void apply_columns(const T *__restrict__ g, const T (
     (col_step<T, K, CC>(g, cr, L, ix, iy, iz0, lin, zstride, va, vb, acc), ...);
 }
 
-template <typename T, int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2)))
+template <typename T, int K, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
 void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef,
              const double *__restrict__ scal)
 {
@@ -615,7 +616,7 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
         moments_of(w, d);
     });
 #pragma unroll
-    for (int q = 0; q < 8; ++q) s[q] = wave_sum(s[q]);
+    for (int q = 0; q < 8; ++q) s[q] = uniform(wave_sum(s[q]));
     const bool fires = (Lo * s[1] < -1e-30 && (s[0] + s[1]) > 0);
     double flag = 0.0;
     if (fires)
@@ -636,7 +637,7 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
             moments_of(w, (double)lim);
         });
 #pragma unroll
-        for (int q = 3; q < 8; ++q) s[q] = wave_sum(s[q]);
+        for (int q = 3; q < 8; ++q) s[q] = uniform(wave_sum(s[q]));
     }
     const double rhoL = ref.mom[0] + s[3] * dv3;
     const double pxL = ref.mom[1] + s[4] * dv3;
@@ -651,7 +652,7 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
         const double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
         rmin = fmin(rmin, q); rmax = fmax(rmax, q);
     }
-    rmin = wave_min(rmin); rmax = wave_max(rmax);
+    rmin = uniform(wave_min(rmin)); rmax = uniform(wave_max(rmax));
     const double rho_new = Lo * rhoL;
     double sc2 = 1.0;
     if (!(rmin > rmax))
@@ -684,6 +685,9 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
                     : 0.0;
     double Bq = 1.0 / (2.0 * R * ref.T) - 1.0 / (2.0 * R * mT);
     double Cx = mux / (R * mT), Cy = muy / (R * mT), Cz = muz / (R * mT);
+    const T cT = T(coeff), lamT = T(lam), Lt = T(Lo);
+    double h[5] = {0, 0, 0, 0, 0};
+    bool written = false;        // the discrete collision already wrote g (FP64 pass)
     if constexpr (DM)
     {
         if (okm)
@@ -691,58 +695,123 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
             // discrete Maxwellian with the grid moments of the committed
             // field (L-units): Newton in FP64
             const double cs = dm.cs, ics = 1.0 / cs;
-            double ac[5], a[5];
+            double ac[5], a[5], aref[5];
             dm_guess(mr, mux, muy, muz, mT, R, cs, ac);
+            dm_guess(ref.rho, 0.0, 0.0, 0.0, ref.T, R, cs, aref);
             const float feat[kMlpIn] = {float(mux * ics), float(muy * ics), float(muz * ics),
                                         float(R * mT * ics * ics), float(dv * ics)};
             float da[kMlpOut] = {0, 0, 0, 0, 0};
             if (dm.mlp) mlp_wave_forward(dm.wb, dm.pf, feat, da);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) a[q] = ac[q] + (double)da[q];
+            for (int q = 0; q < 5; ++q)
+            {
+                ac[q] = uniform(ac[q]);
+                aref[q] = uniform(aref[q]);
+                a[q] = uniform(ac[q] + (double)da[q]);
+            }
             const double ms[5] = {mr, sc2 * pxL * ics, sc2 * pyL * ics, sc2 * pzL * ics,
                                   sc2 * EL * ics * ics};
-            int pass = 0;
-            double res = 0.0;
+            auto residual = [&](const double (&S)[5], double (&F)[5]) -> double {
+                double r = 0.0;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) { F[q] = S[q] - ms[q]; r = fmax(r, fabs(F[q])); }
+                return r / fabs(ms[0]);
+            };
+            // one grid pass: the residual and the Jacobian at a; a Newton
+            // update unless converged. Returns the residual before the update.
             bool ok = true;
-            for (;;)
-            {
-                ++pass;
+            auto newton = [&]() -> double {
                 double S[15];
 #pragma unroll
                 for (int q = 0; q < 15; ++q) S[q] = 0.0;
-                NodeWalk w(lane, Nv);
-                for (int k = lane; k < nodes; k += kWave, w.step())
-                {
+                each([&](int, const NodeWalk &w) {
                     const double xx = vnode(vlo, dv, w.iv) * ics, yy = vnode(vlo, dv, w.jv) * ics;
                     const double zz = vnode(vlo, dv, w.kv) * ics;
                     const double e = exp(a[0] + a[1] * xx + a[2] * yy + a[3] * zz
                                          + a[4] * (xx * xx + yy * yy + zz * zz)) * dv3;
                     dm_accum(S, e, xx, yy, zz);
-                }
+                });
 #pragma unroll
-                for (int q = 0; q < 15; ++q) S[q] = wave_sum(S[q]);
+                for (int q = 0; q < 15; ++q) S[q] = uniform(wave_sum(S[q]));
                 double F[5];
-                res = 0.0;
+                const double r = residual(*reinterpret_cast<const double(*)[5]>(S), F);
+                if (r > dm.tol)
+                {
+                    double d[5];
+                    if (!dm_solve5(S, F, d)) ok = false;
+                    else
 #pragma unroll
-                for (int q = 0; q < 5; ++q) { F[q] = S[q] - ms[q]; res = fmax(res, fabs(F[q])); }
-                res /= fabs(ms[0]);
-                if (res <= dm.tol || pass > dm.maxit) break;
-                double d[5];
-                if (!dm_solve5(S, F, d)) { ok = false; break; }
+                        for (int q = 0; q < 5; ++q) a[q] = uniform(a[q] - d[q]);
+                }
+                return r;
+            };
+            // the collision towards M_d(a) = M0 exp((a - a_ref) . psi) in FP64,
+            // written to g, with the grid moments of M_d as the check
+            auto collide = [&]() -> double {
+                const double A1 = a[0] - aref[0], Bx = a[1] * ics, By = a[2] * ics, Bz = a[3] * ics;
+                const double B2 = (a[4] - aref[4]) * ics * ics;
+                double f[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-                for (int q = 0; q < 5; ++q) a[q] -= d[q];
+                for (int q = 0; q < 5; ++q) h[q] = 0.0;
+                each([&](int k, const NodeWalk &w) {
+                    const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
+                    const double m = m0[k];
+                    const double v2 = vx * vx + vy * vy + vz * vz;
+                    const double d2 = sc2 * (double)tp[k] + (sc2 - 1.0) * m;
+                    const double Md = m * exp(A1 + B2 * v2 + Bx * vx + By * vy + Bz * vz);
+                    const T dh = T(coeff * (d2 + lam * (Md - m)));
+                    gp[k] = Lt * dh + T((Lo - 1.0) * m);
+                    const double hd = (double)dh;
+                    h[0] += hd; h[1] += vx * hd; h[2] += vy * hd; h[3] += vz * hd; h[4] += v2 * hd;
+                    f[0] += Md; f[1] += Md * vx; f[2] += Md * vy; f[3] += Md * vz; f[4] += Md * v2;
+                });
+#pragma unroll
+                for (int q = 0; q < 5; ++q) f[q] = uniform(wave_sum(f[q])) * dv3;
+                const double Fs[5] = {f[0], f[1] * ics, f[2] * ics, f[3] * ics, f[4] * ics * ics};
+                double F[5];
+                return residual(Fs, F);
+            };
+            int pass = 0;
+            double res = 0.0;
+            if (dm.mlp)
+            {
+                // warm start: one Newton update, verified by the collision
+                // pass itself (FP64); another update only when it fails
+                for (;;)
+                {
+                    ++pass;
+                    res = newton();
+                    if (!ok) break;
+                    ++pass;
+                    res = collide();
+                    written = true;
+                    if (res <= dm.tol || pass > 2 * dm.maxit) break;
+                }
+            }
+            else
+            {
+                // Newton until the residual passes, then the FP32 collision
+                for (;;)
+                {
+                    ++pass;
+                    res = newton();
+                    if (!ok || res <= dm.tol || pass > dm.maxit) break;
+                }
             }
             bool conv = ok && res <= dm.tol;
 #pragma unroll
             for (int q = 0; q < 5; ++q) conv = conv && isfinite(a[q]);
             if (!conv)
+            {
 #pragma unroll
                 for (int q = 0; q < 5; ++q) a[q] = ac[q];           // continuous fallback
+                written = false;
+            }
             if (lane == 0)
             {
 #pragma unroll
                 for (int q = 0; q < 5; ++q) dm.alpha[(size_t)i * 5 + q] = a[q];
-                dm.passes[i] = conv ? pass : dm.maxit + 2;
+                dm.passes[i] = conv ? pass : 2 * dm.maxit + 2;
                 dm.res[i] = res;
                 if (dm.rec)
                 {
@@ -756,28 +825,29 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
             }
             // D = (a - a_ref) . psi in unscaled velocities, a_ref the
             // parameters of M0
-            double aref[5];
-            dm_guess(ref.rho, 0.0, 0.0, 0.0, ref.T, R, cs, aref);
             A0 = a[0] - aref[0];
             Cx = a[1] * ics; Cy = a[2] * ics; Cz = a[3] * ics;
             Bq = (a[4] - aref[4]) * ics * ics;
         }
     }
-    const T cT = T(coeff), lamT = T(lam), Lt = T(Lo);
-    double h[5] = {0, 0, 0, 0, 0};
-    each([&](int k, const NodeWalk &w) {
-        const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
-        const double m = m0[k];
-        const double v2 = vx * vx + vy * vy + vz * vz;
-        const double d2 = sc2 * (double)tp[k] + (sc2 - 1.0) * m;
-        const double D = A0 + Bq * v2 + Cx * vx + Cy * vy + Cz * vz;
-        const T mm = okm ? T(m) * expm1(T(D)) : T(-m);
-        const T dh = cT * (T(d2) + lamT * mm);
-        // fold the pending rescale of this step's input into the stored value
-        gp[k] = Lt * dh + T((Lo - 1.0) * m);
-        const double hd = (double)dh;
-        h[0] += hd; h[1] += vx * hd; h[2] += vy * hd; h[3] += vz * hd; h[4] += v2 * hd;
-    });
+    if (!written)
+    {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) h[q] = 0.0;
+        each([&](int k, const NodeWalk &w) {
+            const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
+            const double m = m0[k];
+            const double v2 = vx * vx + vy * vy + vz * vz;
+            const double d2 = sc2 * (double)tp[k] + (sc2 - 1.0) * m;
+            const double D = A0 + Bq * v2 + Cx * vx + Cy * vy + Cz * vz;
+            const T mm = okm ? T(m) * expm1(T(D)) : T(-m);
+            const T dh = cT * (T(d2) + lamT * mm);
+            // fold the pending rescale of this step's input into the stored value
+            gp[k] = Lt * dh + T((Lo - 1.0) * m);
+            const double hd = (double)dh;
+            h[0] += hd; h[1] += vx * hd; h[2] += vy * hd; h[3] += vz * hd; h[4] += v2 * hd;
+        });
+    }
 #pragma unroll
     for (int q = 0; q < 5; ++q) out[1 + q] = Lo * (ref.mom[q] + wave_sum(h[q]) * dv3);
     out[6] = flag;
@@ -912,25 +982,74 @@ __global__ void k_set_scalar(double *scal, int slot, double v) { scal[slot] = v;
 
 bool is_fp32(const ab_ctx *ctx) { return ctx->cfg.precision == AB_PREC_MIXED; }
 
+// Launch shapes of the transport contraction the autotuner chooses from:
+// K outputs per z block (values reused across K + 4 planes), W the waves per
+// SIMD the register allocation targets. Every shape does the same products
+// in the same order per output, so the results do not depend on the choice.
+struct ApplyShape { int K, W; };
+constexpr ApplyShape kShapes32[] = {{9, 3}, {8, 3}, {7, 3}, {6, 3}, {12, 3}, {10, 3}, {5, 3}, {4, 3}};
+constexpr ApplyShape kShapes64[] = {{9, 2}, {8, 2}, {7, 2}, {6, 2}};
+constexpr int kMaxBlockPlanes = 12 + 4;
+
 template <typename T>
-int apply_launch(ab_ctx *ctx, const Lat &L)
+constexpr int nshapes() { return sizeof(T) == 4 ? 8 : 4; }
+
+template <typename T>
+ApplyShape shape_at(int i) { return sizeof(T) == 4 ? kShapes32[i] : kShapes64[i]; }
+
+template <typename T>
+void launch_shape(int i, const Lat &L, const T *g, T *gt, const T *cf, const double *scal,
+                  int nb, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 4)
+    {
+        switch (i)
+        {
+        case 0: k_apply<T, 9, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 1: k_apply<T, 8, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 2: k_apply<T, 7, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 3: k_apply<T, 6, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 4: k_apply<T, 12, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 5: k_apply<T, 10, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 6: k_apply<T, 5, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        default: k_apply<T, 4, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        }
+    }
+    else
+    {
+        switch (i)
+        {
+        case 0: k_apply<T, 9, 2><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 1: k_apply<T, 8, 2><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        case 2: k_apply<T, 7, 2><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        default: k_apply<T, 6, 2><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
+        }
+    }
+}
+
+// Default shape before tuning: the z block that divides the interior run.
+template <typename T>
+int default_shape(int nz)
+{
+    const int run = nz - 4;
+    return run % 9 == 0 ? 0 : (run % 7 == 0 ? 2 : 1);
+}
+
+template <typename T>
+int apply_launch(ab_ctx *ctx, const Lat &L, int shape)
 {
     const int nwaves = ctx->cfg.nx * ctx->cfg.ny * L.nchunk;
     const int nb = (nwaves + 3) / 4;
-    const int run = ctx->cfg.nz - 4;
-    const T *g = (const T *)ctx->gs;
-    T *gt = (T *)ctx->gts;
-    const T *cf = (const T *)ctx->coef;
-    // buffer offsets within a z block (up to 13 planes) must stay below 2^31
-    if (13.0 * ctx->cfg.nx * ctx->cfg.ny * (double)ctx->S * sizeof(T) >= 2147483648.0)
+    // buffer offsets within a z block must stay below 2^31
+    if ((double)kMaxBlockPlanes * ctx->cfg.nx * ctx->cfg.ny * (double)ctx->S * sizeof(T) >= 2147483648.0)
     {
         ctx->err = "table transport: a z block of the lattice exceeds 2 GiB";
         return AB_ERR_UNSUPPORTED;
     }
-    if (run % 9 == 0) k_apply<T, 9><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
-    else if (run % 7 == 0) k_apply<T, 7><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
-    else k_apply<T, 8><<<nb, 256, 0, ctx->stream>>>(L, g, gt, cf, ctx->scal);
-    return 0;
+    if (shape < 0 || shape >= nshapes<T>()) shape = default_shape<T>(ctx->cfg.nz);
+    launch_shape<T>(shape, L, (const T *)ctx->gs, (T *)ctx->gts, (const T *)ctx->coef, ctx->scal,
+                    nb, ctx->stream);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 
 }  // namespace
@@ -1160,7 +1279,8 @@ int table_step(ab_ctx *ctx, double dt)
     ref.rho = ctx->m0rho;
     ref.T = ctx->m0T;
     tic();
-    const int arc = is_fp32(ctx) ? apply_launch<float>(ctx, L) : apply_launch<double>(ctx, L);
+    const int arc = is_fp32(ctx) ? apply_launch<float>(ctx, L, ctx->apply_shape)
+                                 : apply_launch<double>(ctx, L, ctx->apply_shape);
     if (arc) return arc;
     toc(ST_TRANSPORT);
     tic();
@@ -1187,6 +1307,64 @@ int table_step(ab_ctx *ctx, double dt)
                                                     ctx->T, ctx->limflag);
     toc(ST_FINAL);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+// Times every launch shape of the transport on the current state (the
+// transport writes only its scratch output gt) and keeps the fastest.
+int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32], float *ms,
+                   int *n, int *chosen)
+{
+    if (ctx->coef_dt != dt)
+    {
+        int rc = table_prepare(ctx, dt);
+        if (rc) return rc;
+    }
+    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave};
+    const bool f32 = is_fp32(ctx);
+    const int ns = f32 ? nshapes<float>() : nshapes<double>();
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    { ctx->err = "autotune: event creation failed"; return AB_ERR_HIP; }
+    int best = -1;
+    float tbest = 0.0f;
+    int rc = AB_OK;
+    for (int i = 0; i < ns && !rc; ++i)
+    {
+        float tmin = 0.0f;
+        for (int r = 0; r <= reps && !rc; ++r)           // r == 0 warms up
+        {
+            (void)hipEventRecord(e0, ctx->stream);
+            rc = f32 ? apply_launch<float>(ctx, L, i) : apply_launch<double>(ctx, L, i);
+            (void)hipEventRecord(e1, ctx->stream);
+            float t = 0.0f;
+            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            { ctx->err = "autotune: timing failed"; rc = AB_ERR_HIP; }
+            if (r == 1 || (r > 1 && t < tmin)) tmin = t;
+        }
+        if (rc) break;
+        if (ms && i < cap) ms[i] = tmin;
+        if (names && i < cap)
+        {
+            const ApplyShape sh = f32 ? shape_at<float>(i) : shape_at<double>(i);
+            snprintf(names[i], 32, "K%d/W%d", sh.K, sh.W);
+        }
+        if (best < 0 || tmin < tbest) { best = i; tbest = tmin; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    ctx->apply_shape = best;
+    if (n) *n = ns;
+    if (chosen) *chosen = best;
+    return AB_OK;
+}
+
+int table_set_shape(ab_ctx *ctx, int shape)
+{
+    const int ns = is_fp32(ctx) ? nshapes<float>() : nshapes<double>();
+    if (shape < -1 || shape >= ns) { ctx->err = "transport shape index out of range"; return AB_ERR_ARG; }
+    ctx->apply_shape = shape;
+    return AB_OK;
 }
 
 }  // namespace ab

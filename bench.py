@@ -47,6 +47,19 @@ def moment_errors(m, ref, inner):
     return d
 
 
+def measured_traffic(nx, nv):
+    """HBM bytes per transport launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    the counter reads for a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_apply_cavity3d_n{nx}_nv{nv}_pmc.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(nx=28, nv=10, order=2):
     """The reference CPU step (oracle/_ref, built from the reference sources)
     on a bounded sample: cavity3d at nx^3, Nv=10, quadratic MLS, two steps;
@@ -132,6 +145,8 @@ def main():
     loss0, loss1 = mix.mlp_train(args.train_iters, batch=4096, lr=2e-3, seed=5)
     mix.mlp_enable(True)
     train_s = time.perf_counter() - t0
+    shapes, chosen = mix.autotune(dt, reps=3)
+    log(f"[bench] transport shapes (ms): {shapes} -> {chosen}")
     log(f"[bench] MLP: {args.train_iters} Adam steps on {rec * N} rows in {train_s:.2f} s, "
         f"loss {loss0:.3e} -> {loss1:.3e}")
     mix.profile(True)
@@ -156,6 +171,7 @@ def main():
     # ---- the same without the warm start (plain Newton) --------------------
     pln = A.Solver(cloud, precision=A.PREC_MIXED, collision=DISC)
     pln.steps(dt, args.warmup)
+    pln.set_transport_shape(list(shapes).index(chosen))
     pln.synchronize()
     t0 = time.perf_counter()
     pln.steps(dt, args.steps)
@@ -190,6 +206,7 @@ def main():
         log(f"[bench] mixed+DM+MLP vs FP64+DM moment errors after {n64} steps: {err}")
         ref_dm.close()
 
+    traffic, traffic_src = measured_traffic(args.nx, args.nv)
     cpu = None if args.no_cpu else cpu_baseline(nv=args.nv)
     if cpu:
         log(f"[bench] cpu baseline: {cpu}")
@@ -220,8 +237,10 @@ def main():
         "newton_passes_per_solve": passes,
         "mlp": {"train_seconds": train_s, "adam_steps": args.train_iters, "batch": 4096,
                 "rows": rec * N, "loss_first": loss0, "loss_last": loss1},
+        "autotune": {"transport_shapes_ms": shapes, "chosen": chosen},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_apply (MLS transport contraction)",
                      "kernel_avg_ms": tr_avg_s * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},

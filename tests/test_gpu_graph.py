@@ -55,3 +55,23 @@ def test_graph_refuses_profiling_context():
     s.profile(False)
     g = A.Graph([s], 1e-11, 1)
     g.launch(); g.synchronize(); g.close()
+
+
+def test_autotune_keeps_results_and_picks_a_shape():
+    """Every transport launch shape gives bit-identical steps; the autotuner
+    times them all and keeps one."""
+    _say("[autotune] cavity3d 12^3 nv8 mixed")
+    case, cl, ocl, neigh, nn = setup("cavity3d", 12, 8, order=2, precision=A.PREC_MIXED)
+    dt = 1e-11
+    base = A.Solver(cl, precision=A.PREC_MIXED)
+    table, chosen = base.autotune(dt, reps=2)
+    _say(f"  {table} -> {chosen}")
+    assert chosen in table and len(table) >= 4 and all(t > 0 for t in table.values())
+    base.steps(dt, 3)
+    ref = base.g()
+    for shape in range(len(table)):
+        s = A.Solver(cl, precision=A.PREC_MIXED)
+        s.set_transport_shape(shape)
+        s.steps(dt, 3)
+        assert np.array_equal(s.g(), ref), shape
+        s.close()
