@@ -79,6 +79,20 @@ class ALHCase(C.Structure):
                 ("save_every", C.c_int), ("mls_order", C.c_int), ("dom", C.c_double * 6)]
 
 
+class CkptInfo(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_longlong), ("gstride", C.c_longlong),
+                ("step", C.c_longlong), ("t", C.c_double), ("nbodies", C.c_int)]
+
+
+class CkptFields(C.Structure):
+    _fields_ = [("x", _dp), ("y", _dp), ("z", _dp), ("mom", _dp), ("mesh", _dp), ("wall", _dp),
+                ("grid", _dp), ("ints", _ip), ("flags", _up), ("g", _dp), ("bodies", _dp)]
+
+
+CKPT_SHAPES = {"x": None, "y": None, "z": None, "mom": 6, "mesh": 3, "wall": 7, "grid": 3,
+               "ints": 4, "flags": 3}
+
+
 class ALEBGKError(RuntimeError):
     pass
 
@@ -102,12 +116,13 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_dm_rows", "ab_mlp_init", "ab_mlp_train", "ab_mlp_enable", "ab_mlp_get",
                "ab_mlp_set", "ab_mlp_forward", "ab_mlp_loss_grad", "ab_graph_create",
                "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy", "ab_autotune",
-               "ab_set_transport_shape"]
+               "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
 COLLISION_DISCRETE = 1
-HOST_EXPORTS = ["alh_configure", "alh_lattice_size", "alh_build_lattice"]
+HOST_EXPORTS = ["alh_configure", "alh_lattice_size", "alh_build_lattice", "alh_ckpt_info_read",
+                "alh_ckpt_read", "alh_ckpt_write"]
 
 
 def hip_lib():
@@ -176,6 +191,7 @@ class Cloud:
     neigh: np.ndarray | None = None
     nneigh: np.ndarray | None = None
     mesh: np.ndarray | None = None   # N x 3 ALE mesh velocity
+    record: dict | None = None       # restart-file fields not held elsewhere (p, bodyId, flags)
 
     @property
     def N(self):
@@ -301,6 +317,37 @@ class Solver:
 
     def rebuild_neighbours(self):
         self._check(self._lib.ab_rebuild_neighbours(self._ctx), "ab_rebuild_neighbours")
+
+    # ---- restart files ----------------------------------------------------
+    def voxels(self):
+        v = np.zeros(self.N, np.int32)
+        self._check(self._lib.ab_download_voxels(self._ctx, _ptr(v, _ip)), "ab_download_voxels")
+        return v
+
+    def save_checkpoint(self, path, step, t):
+        """ckpt::save of the device state: g and rho, u, T from the GPU,
+        p = rho R T, the geometry and wall state of the uploaded cloud, the
+        voxels of the device rebuild; fields a restart file carried over
+        (bodyId, validg, moodFlag, bodies) are kept."""
+        cl, cfg = self.cloud, self.cfg
+        N = self.N
+        mom = self.moments()
+        p = mom[:, 0] * cfg.R * mom[:, 4]
+        rec = cl.record or {}
+        dv = (cfg.vmax - cfg.vmin) / (cfg.nv - 1)
+        grid = rec.get("grid", np.tile([cfg.vmin, cfg.vmax, dv], (N, 1)))
+        ints = np.array(rec["ints"]) if "ints" in rec else \
+            np.column_stack([np.full(N, cfg.nv), np.full(N, cfg.dim), np.full(N, -1),
+                             np.zeros(N)]).astype(np.int32)
+        ints[:, 3] = self.voxels()
+        flags = np.array(rec["flags"]) if "flags" in rec else \
+            np.column_stack([cl.boundary, np.ones(N), np.zeros(N)]).astype(np.uint8)
+        flags[:, 0] = cl.boundary
+        fields = {"x": cl.x, "y": cl.y, "z": cl.z, "mom": np.column_stack([mom, p]),
+                  "mesh": cl.mesh if cl.mesh is not None else np.zeros((N, 3)),
+                  "wall": cl.wall, "grid": grid, "ints": ints, "flags": flags, "g": self.g(),
+                  "bodies": rec.get("bodies", np.zeros((0, 30)))}
+        checkpoint_write(path, step, t, fields, dim=cfg.dim)
 
     # ---- transport launch shape ------------------------------------------
     def autotune(self, dt, reps=3):
@@ -429,6 +476,81 @@ class Solver:
         ns = C.c_int()
         self._check(self._lib.ab_profile_read(self._ctx, ms, cnt, C.byref(ns)), "ab_profile_read")
         return {self._lib.ab_stage_name(k).decode(): (ms[k], cnt[k]) for k in range(ns.value)}
+
+
+# ---- restart files (the reference's checkpoint format v1) -------------------
+
+def cloud_from_checkpoint(path, case, precision=PREC_FP64, mls_order=None):
+    """A cloud configured for `case` (lattice, voxel grid, neighbour radius)
+    with every field the restart file holds restored. The neighbour lists are
+    left to the device rebuild at upload, as the reference rebuilds them on
+    load (Solver/Solver.cpp:281)."""
+    base = build_lattice(case, precision, mls_order)
+    d = checkpoint_read(path, n=base.N, gstride=base.gstride)
+    rec = {"p": d["mom"][:, 5].copy(), "ints": d["ints"].copy(), "flags": d["flags"].copy(),
+           "grid": d["grid"].copy(), "bodies": d["bodies"].copy(), "step": d["step"], "t": d["t"]}
+    return Cloud(base.cfg, d["x"], d["y"], d["z"], d["flags"][:, 0].copy(), d["wall"],
+                 d["mom"][:, :5].copy(), d["g"], mesh=d["mesh"], record=rec)
+
+def checkpoint_info(path):
+    info = CkptInfo()
+    rc = host_lib().alh_ckpt_info_read(str(path).encode(), C.byref(info))
+    if rc:
+        raise ALEBGKError(f"alh_ckpt_info_read({path}): {ERRORS.get(rc, rc)}")
+    return {k: getattr(info, k) for k, _ in CkptInfo._fields_}
+
+
+def _ckpt_fields(d):
+    f = CkptFields()
+    for k in ("x", "y", "z", "mom", "mesh", "wall", "grid", "g", "bodies"):
+        setattr(f, k, _ptr(d[k], _dp) if d.get(k) is not None and d[k].size else C.cast(None, _dp))
+    f.ints = _ptr(d["ints"], _ip)
+    f.flags = _ptr(d["flags"], _up)
+    return f
+
+
+def checkpoint_read(path, n=None, gstride=None, nbodies=None):
+    """The record fields of a restart file as arrays (mom: rho ux uy uz T p;
+    ints: Nv_local dim bodyId voxel; flags: boundary validg moodFlag). The
+    file is refused if it does not match the given n / gstride / nbodies."""
+    info = checkpoint_info(path)
+    exp = CkptInfo(info["dim"], info["n"] if n is None else n,
+                   info["gstride"] if gstride is None else gstride, info["step"], info["t"],
+                   info["nbodies"] if nbodies is None else nbodies)
+    N, gs, nb = exp.n, exp.gstride, exp.nbodies
+    d = {"x": np.zeros(N), "y": np.zeros(N), "z": np.zeros(N), "mom": np.zeros((N, 6)),
+         "mesh": np.zeros((N, 3)), "wall": np.zeros((N, 7)), "grid": np.zeros((N, 3)),
+         "ints": np.zeros((N, 4), np.int32), "flags": np.zeros((N, 3), np.uint8),
+         "g": np.zeros((N, gs)), "bodies": np.zeros((nb, 30))}
+    rc = host_lib().alh_ckpt_read(str(path).encode(), C.byref(exp), C.byref(_ckpt_fields(d)))
+    if rc:
+        raise ALEBGKError(f"alh_ckpt_read({path}): {ERRORS.get(rc, rc)}")
+    d.update(step=info["step"], t=info["t"], dim=info["dim"])
+    return d
+
+
+def checkpoint_write(path, step, t, fields, dim=3):
+    """Writes a restart file from arrays shaped as checkpoint_read returns."""
+    f8 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    d = {k: f8(fields[k]) for k in ("x", "y", "z", "mom", "mesh", "wall", "grid", "g")}
+    d["ints"] = np.ascontiguousarray(fields["ints"], dtype=np.int32)
+    d["flags"] = np.ascontiguousarray(fields["flags"], dtype=np.uint8)
+    d["bodies"] = f8(fields.get("bodies", np.zeros((0, 30)))).reshape(-1, 30)
+    N, gs = d["g"].shape
+    info = CkptInfo(dim, N, gs, int(step), float(t), d["bodies"].shape[0])
+    rc = host_lib().alh_ckpt_write(str(path).encode(), C.byref(info), C.byref(_ckpt_fields(d)))
+    if rc:
+        raise ALEBGKError(f"alh_ckpt_write({path}): {ERRORS.get(rc, rc)}")
+
+
+def copy_calibration(nbytes=256 << 20, reps=2):
+    """A copy kernel over nbytes (read once, written once): the reference
+    point that scales a profiler's HBM byte counters."""
+    L = hip_lib()
+    L.ab_copy_calibration.argtypes = [C.c_ulonglong, C.c_int]
+    rc = L.ab_copy_calibration(nbytes, reps)
+    if rc:
+        raise ALEBGKError(f"ab_copy_calibration: {ERRORS.get(rc, rc)}")
 
 
 class Graph:

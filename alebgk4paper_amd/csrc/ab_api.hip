@@ -472,6 +472,15 @@ int ab_download_neighbours(ab_ctx *c, int *neigh, int *nneigh)
     return AB_OK;
 }
 
+int ab_download_voxels(ab_ctx *c, int *voxel)
+{
+    if (!c || !voxel) return AB_ERR_ARG;
+    if (!c->uploaded || !c->vox) { c->err = "no device neighbour rebuild has run"; return AB_ERR_STATE; }
+    HIPCHK(c, hipMemcpyAsync(voxel, c->vox, sizeof(int) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
 int ab_step(ab_ctx *c, double dt, ab_step_stats *st)
 {
     if (!c) return AB_ERR_ARG;
@@ -531,6 +540,32 @@ int ab_step(ab_ctx *c, double dt, ab_step_stats *st)
         }
     }
     return AB_OK;
+}
+
+__global__ void k_copy_calibration(const float4 *__restrict__ a, float4 *__restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+int ab_copy_calibration(unsigned long long bytes, int reps)
+{
+    if (bytes < 16 || reps < 1) return AB_ERR_ARG;
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void **)&a, bytes) != hipSuccess) return AB_ERR_NOMEM;
+    if (hipMalloc((void **)&b, bytes) != hipSuccess) { (void)hipFree(a); return AB_ERR_NOMEM; }
+    bool ok = hipMemset(a, 0, bytes) == hipSuccess && hipMemset(b, 0, bytes) == hipSuccess;
+    const size_t n = bytes / 16;
+    for (int r = 0; r < reps && ok; ++r)
+    {
+        k_copy_calibration<<<4096, 256>>>(a, b, n);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    ok = ok && hipDeviceSynchronize() == hipSuccess;
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return ok ? AB_OK : AB_ERR_HIP;
 }
 
 int ab_autotune(ab_ctx *c, double dt, int reps, int cap, char (*names)[32], float *ms, int *n,

@@ -207,13 +207,15 @@ __global__ void k_build(DevCloud c, const int *__restrict__ rep, const int *__re
     const int cls = blockIdx.y;
     const int lin = blockIdx.x * blockDim.x + threadIdx.x;
     if (lin >= S) return;
-    T *out = coef + (size_t)cls * kOff * S + lin;
-    for (int o = 0; o < kOff; ++o) out[(size_t)o * S] = T(0);
+    // layout [class][64-node chunk][81 offsets][64 nodes]: a wave's 81
+    // coefficient rows are one contiguous 81 x 64 block (constant offsets)
+    T *out = coef + ((size_t)cls * (S / kWave) + lin / kWave) * kOff * kWave + (lin % kWave);
+    for (int o = 0; o < kOff; ++o) out[(size_t)o * kWave] = T(0);
     if (lin >= c.nodes) return;
     const int i = rep[cls];
-    if (c.bnd[i]) build_class<T, 4, true>(c, i, lin, lat, dt, h, tol, out, S);
-    else if (order >= 2) build_class<T, 10, false>(c, i, lin, lat, dt, h, tol, out, S);
-    else build_class<T, 4, false>(c, i, lin, lat, dt, h, tol, out, S);
+    if (c.bnd[i]) build_class<T, 4, true>(c, i, lin, lat, dt, h, tol, out, kWave);
+    else if (order >= 2) build_class<T, 10, false>(c, i, lin, lat, dt, h, tol, out, kWave);
+    else build_class<T, 4, false>(c, i, lin, lat, dt, h, tol, out, kWave);
 }
 
 // ---- transport apply ---------------------------------------------------------
@@ -273,61 +275,90 @@ __device__ __forceinline__ void bstore(T *row, int voff, T v)
 // barrier: without them the optimiser sinks all the arithmetic below all
 // the loads and spills the values.
 
-// End-plane output: coefficients and values of one column, per offset.
-template <typename T, int CC>
-__device__ __forceinline__ void end_load(const T *__restrict__ g, __amdgpu_buffer_rsrc_t rc,
-                                         const Lat &L, int ix, int iy, int iz, int lin,
-                                         T (&cv)[5], T (&gv)[5])
+// End planes in pairs: planes 0 and 1 (and nz-2, nz-1) read the same four
+// lattice planes z_lo .. z_lo+3 of each stencil column, so one load of the
+// column's values serves both outputs; the two classes' coefficients are
+// loaded per offset. TOP selects the upper pair (z_lo = nz - 4).
+template <bool TOP>
+__host__ __device__ constexpr int pair_idx(int which, int dz)
+{
+    // index into the column's four planes for output `which` (0: lower
+    // plane of the pair) and offset dz, with the lattice clamping
+    if (!TOP)
+    {
+        const int z = which + dz;                  // planes 0, 1
+        return z < 0 ? 0 : z;
+    }
+    const int z = 2 + which + dz;                  // relative to nz - 4
+    return z > 3 ? 3 : z;
+}
+
+template <typename T, bool TOP, int CC>
+__device__ __forceinline__ void pair_load(const T *__restrict__ g, __amdgpu_buffer_rsrc_t ra,
+                                          __amdgpu_buffer_rsrc_t rb, const Lat &L, int ix, int iy,
+                                          int zlo, int lin, T (&gv)[4], T (&ca)[5], T (&cb)[5])
 {
     constexpr Col k = col_at(CC);
     const int xx = clampi(ix + k.ox, 0, L.nx - 1), yy = clampi(iy + k.oy, 0, L.ny - 1);
-    // the column from plane z0 = max(iz - 2, 0): every row it reads is within
-    // 4 planes of it
-    const int z0 = iz >= 2 ? iz - 2 : 0;
     const __amdgpu_buffer_rsrc_t rg =
-        brsrc(g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * z0)) * L.S);
+        brsrc(g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * zlo)) * L.S);
     const int pb = L.nx * L.ny * L.S * (int)sizeof(T);
+    // only the planes this column's offsets reach
+    constexpr int lo = TOP ? 2 - k.d : 0, hi = TOP ? 3 : 1 + k.d;
+#pragma clang loop unroll(full)
+    for (int j = lo; j <= hi; ++j) gv[j] = bload_so<T>(rg, lin, j * pb);
 #pragma clang loop unroll(full)
     for (int j = 0; j <= 2 * k.d; ++j)
     {
-        const int zz = clampi(iz - k.d + j, 0, L.nz - 1);
-        cv[j] = bload_so<T>(rc, lin, (k.o0 + j) * L.S * (int)sizeof(T));
-        gv[j] = bload_so<T>(rg, lin, (zz - z0) * pb);
+        ca[j] = bload_so<T>(ra, lin, (k.o0 + j) * kWave * (int)sizeof(T));
+        cb[j] = bload_so<T>(rb, lin, (k.o0 + j) * kWave * (int)sizeof(T));
     }
 }
 
-template <typename T, int CC>
-__device__ __forceinline__ void end_step(const T *__restrict__ g, __amdgpu_buffer_rsrc_t rc,
-                                         const Lat &L, int ix, int iy, int iz, int lin,
-                                         T (&ca)[5], T (&ga)[5], T (&cb)[5], T (&gb)[5], T &acc)
+template <typename T, bool TOP, int CC>
+__device__ __forceinline__ void pair_fma(const T (&gv)[4], const T (&ca)[5], const T (&cb)[5],
+                                         T &acc0, T &acc1)
 {
     constexpr Col k = col_at(CC);
-    if constexpr (CC + 1 < kCols)
-    {
-        if constexpr ((CC & 1) == 0) end_load<T, CC + 1>(g, rc, L, ix, iy, iz, lin, cb, gb);
-        else end_load<T, CC + 1>(g, rc, L, ix, iy, iz, lin, ca, ga);
-    }
 #pragma clang loop unroll(full)
     for (int j = 0; j <= 2 * k.d; ++j)
     {
-        if constexpr ((CC & 1) == 0) acc += ca[j] * ga[j];
-        else acc += cb[j] * gb[j];
+        acc0 += ca[j] * gv[pair_idx<TOP>(0, j - k.d)];
+        acc1 += cb[j] * gv[pair_idx<TOP>(1, j - k.d)];
     }
-    asm volatile("" : "+v"(acc)::"memory");
+}
+
+template <typename T, bool TOP, int CC>
+__device__ __forceinline__ void pair_step(const T *__restrict__ g, __amdgpu_buffer_rsrc_t ra,
+                                          __amdgpu_buffer_rsrc_t rb, const Lat &L, int ix, int iy,
+                                          int zlo, int lin, T (&g0)[4], T (&a0)[5], T (&b0)[5],
+                                          T (&g1)[4], T (&a1)[5], T (&b1)[5], T &acc0, T &acc1)
+{
+    if constexpr (CC + 1 < kCols)
+    {
+        if constexpr ((CC & 1) == 0) pair_load<T, TOP, CC + 1>(g, ra, rb, L, ix, iy, zlo, lin, g1, a1, b1);
+        else pair_load<T, TOP, CC + 1>(g, ra, rb, L, ix, iy, zlo, lin, g0, a0, b0);
+    }
+    if constexpr ((CC & 1) == 0) pair_fma<T, TOP, CC>(g0, a0, b0, acc0, acc1);
+    else pair_fma<T, TOP, CC>(g1, a1, b1, acc0, acc1);
+    asm volatile("" : "+v"(acc0), "+v"(acc1)::"memory");
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <typename T, int... CC>
-__device__ __forceinline__ T apply_one(const T *__restrict__ g, const T *__restrict__ cf,
-                                       const Lat &L, int ix, int iy, int iz, int lin,
-                                       std::integer_sequence<int, CC...>)
+// Outputs of the pair's two planes (classes cfa, cfb).
+template <typename T, bool TOP, int... CC>
+__device__ __forceinline__ void apply_pair(const T *__restrict__ g, const T *__restrict__ cfa,
+                                           const T *__restrict__ cfb, const Lat &L, int ix, int iy,
+                                           int lin, T &acc0, T &acc1,
+                                           std::integer_sequence<int, CC...>)
 {
-    T acc = T(0);
-    T ca[5], ga[5], cb[5], gb[5];
-    const __amdgpu_buffer_rsrc_t rc = brsrc(cf);
-    end_load<T, 0>(g, rc, L, ix, iy, iz, lin, ca, ga);
-    (end_step<T, CC>(g, rc, L, ix, iy, iz, lin, ca, ga, cb, gb, acc), ...);
-    return acc;
+    const int zlo = TOP ? L.nz - 4 : 0;
+    const __amdgpu_buffer_rsrc_t ra = brsrc(cfa), rb = brsrc(cfb);
+    T g0[4], a0[5], b0[5], g1[4], a1[5], b1[5];
+    acc0 = T(0);
+    acc1 = T(0);
+    pair_load<T, TOP, 0>(g, ra, rb, L, ix, iy, zlo, lin, g0, a0, b0);
+    (pair_step<T, TOP, CC>(g, ra, rb, L, ix, iy, zlo, lin, g0, a0, b0, g1, a1, b1, acc0, acc1), ...);
 }
 
 // Interior: the values of column CC for outputs iz0 .. iz0+K-1.
@@ -410,23 +441,30 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
     const T *gw = g + c0;
     T *gtw = gt + c0;
 
-    // The four end planes: their own classes, coefficients loaded per offset.
-#pragma unroll 1
-    for (int e = 0; e < 4; ++e)
-    {
-        const int iz = (e < 2) ? e : L.nz - 4 + e;
-        const int cls = cx + 5 * (cy + 5 * category(iz, L.nz));
-        const T *cf = coef + (size_t)cls * kOff * L.S + c0;
-        T acc = apply_one(gw, cf, L, ix, iy, iz, voff, std::make_integer_sequence<int, kCols>{});
-        bstore(gtw + ((size_t)col + plane * iz) * L.S, voff, acc);
-    }
+    // The four end planes, in pairs (0, 1) and (nz-2, nz-1): their own
+    // classes, coefficients loaded per offset, column values shared.
+    auto end_pair = [&](auto top) {
+        constexpr bool TOP = decltype(top)::value;
+        const int iz = TOP ? L.nz - 2 : 0;
+        const T *cfa = coef + ((size_t)(cx + 5 * (cy + 5 * category(iz, L.nz))) * L.nchunk + chunk) * kOff * kWave;
+        const T *cfb = coef + ((size_t)(cx + 5 * (cy + 5 * category(iz + 1, L.nz))) * L.nchunk + chunk) * kOff * kWave;
+        T a0, a1;
+        apply_pair<T, TOP>(gw, cfa, cfb, L, ix, iy, voff, a0, a1,
+                           std::make_integer_sequence<int, kCols>{});
+        bstore(gtw + ((size_t)col + plane * iz) * L.S, voff, a0);
+        bstore(gtw + ((size_t)col + plane * (iz + 1)) * L.S, voff, a1);
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    end_pair(std::false_type{});
+    end_pair(std::true_type{});
 
     // Interior planes iz = 2 .. nz-3 share one class: coefficients in registers.
     T cr[kOff];
     {
-        const T *cf = coef + (size_t)(cx + 5 * (cy + 5 * 2)) * kOff * L.S + c0;
+        const T *cf = coef + ((size_t)(cx + 5 * (cy + 5 * 2)) * L.nchunk + chunk) * kOff * kWave;
+        const __amdgpu_buffer_rsrc_t rc = brsrc(cf);
 #pragma unroll
-        for (int o = 0; o < kOff; ++o) cr[o] = bload(cf + (size_t)o * L.S, voff);
+        for (int o = 0; o < kOff; ++o) cr[o] = bload_so<T>(rc, voff, o * kWave * (int)sizeof(T));
     }
     for (int iz0 = 2; iz0 <= L.nz - 3; iz0 += K)
     {

@@ -115,6 +115,10 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None,
                     help="FP64 steps for the comparison (default: warmup + steps)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--counters", action="store_true",
+                    help="for rocprofv3 --pmc: the measured step only (warmup, training, "
+                         "autotune, --steps steps), then a 256 MiB copy kernel that calibrates "
+                         "the byte counters; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -149,6 +153,12 @@ def main():
     log(f"[bench] transport shapes (ms): {shapes} -> {chosen}")
     log(f"[bench] MLP: {args.train_iters} Adam steps on {rec * N} rows in {train_s:.2f} s, "
         f"loss {loss0:.3e} -> {loss1:.3e}")
+    if args.counters:
+        mix.steps(dt, args.steps)
+        mix.synchronize()
+        A.copy_calibration(256 << 20, 2)
+        log("[bench] counters pass done")
+        return
     mix.profile(True)
     t0 = time.perf_counter()
     mix.steps(dt, args.steps)

@@ -139,6 +139,9 @@ int ab_upload(ab_ctx *ctx, const ab_cloud_view *view);
  * reference order; lists are identical to the reference's. */
 int ab_rebuild_neighbours(ab_ctx *ctx);
 int ab_download_neighbours(ab_ctx *ctx, int *neigh, int *nneigh);
+/* The voxel of every particle from the last device rebuild (Particle::voxel,
+ * which restart files record). */
+int ab_download_voxels(ab_ctx *ctx, int *voxel);
 
 /* cudaStep (Solver/CudaBackend.cu:1505). stats may be NULL (no host sync). */
 int ab_step(ab_ctx *ctx, double dt, ab_step_stats *stats);
@@ -194,6 +197,12 @@ int ab_synchronize(ab_ctx *ctx);
 int ab_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32], float *ms,
                 int *n, int *chosen);
 int ab_set_transport_shape(ab_ctx *ctx, int shape);
+
+/* Measurement helper (no counterpart in the reference): `reps` launches of a
+ * plain copy kernel over `bytes` bytes (read once, written once), so a
+ * profiler's HBM counters (FETCH_SIZE, WRITE_SIZE) can be scaled to bytes.
+ * Allocates and frees its own buffers. */
+int ab_copy_calibration(unsigned long long bytes, int reps);
 
 /* ---- Several contexts in one HIP graph ----------------------------------
  * No counterpart in the reference (its time loop launches every kernel from

@@ -57,6 +57,41 @@ int alh_lattice_size(const alh_case *cs, int *n);
 int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double *x, double *y,
                       double *z, unsigned char *boundary, double *wall, double *mom, double *g);
 
+/* ---- Restart files (Core/Checkpoint.hpp:37-234, format v1) -------------
+ * The reference's checkpoint, read and written byte for byte: a 56-byte
+ * header, N particle records of 200 bytes, N x gstride FP64 g, nbodies body
+ * records of 232 bytes, little-endian. */
+typedef struct alh_ckpt_info {
+    int dim;
+    long long n, gstride, step;
+    double t;
+    int nbodies;
+} alh_ckpt_info;
+
+/* The record fields as arrays (N entries per field unless noted). */
+typedef struct alh_ckpt_fields {
+    double *x, *y, *z;
+    double *mom;           /* N x 6: rho, ux, uy, uz, T, p */
+    double *mesh;          /* N x 3: meshVx, meshVy, meshVz */
+    double *wall;          /* N x 7: wnx, wny, wnz, wUx, wUy, wUz, wT */
+    double *grid;          /* N x 3: vlo, vhi, dv_local */
+    int *ints;             /* N x 4: Nv_local, dim, bodyId, voxel */
+    unsigned char *flags;  /* N x 3: boundary, validg, moodFlag */
+    double *g;             /* N x gstride */
+    double *bodies;        /* nbodies x 30: cx cy cz velx vely velz angularVel
+                              omx omy omz forcex forcey forcez torque tqx tqy
+                              tqz ox oy, Rm[9], dim, rotates */
+} alh_ckpt_fields;
+
+/* The header only (AB_ERR_ARG: unreadable or not a v1 file). */
+int alh_ckpt_info_read(const char *path, alh_ckpt_info *info);
+/* ckpt::restore (Checkpoint.hpp:161): a file whose N, gstride or body count
+ * differ from *expect is refused with AB_ERR_UNSUPPORTED; a short file is
+ * AB_ERR_ARG. */
+int alh_ckpt_read(const char *path, const alh_ckpt_info *expect, alh_ckpt_fields *out);
+/* ckpt::save (Checkpoint.hpp:78): written to path.tmp, then renamed. */
+int alh_ckpt_write(const char *path, const alh_ckpt_info *info, const alh_ckpt_fields *in);
+
 #ifdef __cplusplus
 }
 #endif

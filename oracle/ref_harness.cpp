@@ -10,6 +10,8 @@
 //   ref_harness case2d  <outdir> <Nx> <Nv>         cavity2d fixtures
 //   ref_harness cube    <outdir> <Nx> <Nv> <scale> cavity3d-cube body loads
 //   ref_harness cpustep <Nx> <Nv> <order> <nsteps> time the reference CPU step
+//   ref_harness ckpt    <outdir> <case> <Nx> <Nv>   a restart file + its fields
+//   ref_harness ckptread <file> <outdir> <case> <Nx> <Nv>  restore a file, dump fields
 //
 // Functions called (reference file:line):
 //   configureCase            TestCases/TestCases.cpp:113
@@ -26,6 +28,7 @@
 //   applyBoundaryBC          BoundaryConditions/DiffuseWall.cpp:19
 //   sumMass / scaleG         MomentCalculations/Conservation.cpp:48,92
 //   computeBodyLoads (static) RigidBody/BodyDynamics.cpp:17 (TU included below)
+//   ckpt::save / ckpt::restore Core/Checkpoint.hpp:78, 161
 // =============================================================================
 #include "Config.hpp"
 #include "Types.hpp"
@@ -36,6 +39,7 @@
 #include "MLSTransport.hpp"
 #include "Moments.hpp"
 #include "TestCases.hpp"
+#include "Checkpoint.hpp"
 
 #include <chrono>
 #include <cstdint>
@@ -407,6 +411,96 @@ int modeCpuStep(int Nx, int Nv, int order, int nsteps)
 
 }  // namespace
 
+// Every field a restart record carries, as arrays (for the format tests).
+void writeRecordFields(const std::string &dir, const Setup &S)
+{
+    const auto &P = S.P;
+    size_t N = P.size(), gs = P[0].g.size();
+    writeCloud(dir, S);
+    std::vector<double> m(N * 6), mesh(N * 3), grid(N * 3);
+    std::vector<int32_t> ints(N * 2);
+    std::vector<uint8_t> flags(N * 2);
+    for (size_t i = 0; i < N; ++i)
+    {
+        const Particle &q = P[i];
+        double mm[6] = {q.rho, q.ux, q.uy, q.uz, q.T, q.p};
+        std::memcpy(&m[6 * i], mm, sizeof mm);
+        mesh[3 * i] = q.meshVx; mesh[3 * i + 1] = q.meshVy; mesh[3 * i + 2] = q.meshVz;
+        grid[3 * i] = q.vlo; grid[3 * i + 1] = q.vhi; grid[3 * i + 2] = q.dv_local;
+        ints[2 * i] = q.Nv_local; ints[2 * i + 1] = q.dim;
+        flags[2 * i] = q.validg ? 1 : 0; flags[2 * i + 1] = q.moodFlag ? 1 : 0;
+    }
+    npyF8(dir + "/rec_mom.npy", m, {N, 6});
+    npyF8(dir + "/rec_mesh.npy", mesh, {N, 3});
+    npyF8(dir + "/rec_grid.npy", grid, {N, 3});
+    npyI4(dir + "/rec_ints.npy", ints, {N, 2});
+    npyU1(dir + "/rec_flags.npy", flags, {N, 2});
+    npyF8(dir + "/g.npy", gatherG(P, false), {N, gs});
+    std::vector<double> b;
+    for (const auto &rb : S.bodies)
+    {
+        double v[19] = {rb->cx, rb->cy, rb->cz, rb->velx, rb->vely, rb->velz, rb->angularVel,
+                        rb->omx, rb->omy, rb->omz, rb->forcex, rb->forcey, rb->forcez,
+                        rb->torque, rb->tqx, rb->tqy, rb->tqz, rb->ox, rb->oy};
+        b.insert(b.end(), v, v + 19);
+        const double *Rm = rb->orientation();
+        double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        b.insert(b.end(), Rm ? Rm : I, (Rm ? Rm : I) + 9);
+        b.push_back(rb->dim);
+        b.push_back(rb->rotates ? 1.0 : 0.0);
+    }
+    npyF8(dir + "/bodies.npy", b, {S.bodies.size(), 30});
+}
+
+TestCase caseOf(const std::string &name)
+{
+    if (name == "cube") return TestCase::Cavity3D_Cube;
+    if (name == "cavity2d") return TestCase::Cavity2D;
+    return TestCase::Cavity3D;
+}
+
+// A restart file of a perturbed state with moments, mesh velocity and a
+// moving body, written by the reference's ckpt::save.
+int modeCkpt(const std::string &dir, const std::string &cname, int Nx, int Nv)
+{
+    Setup S;
+    build(S, caseOf(cname), Nx, Nv, 1.0);
+    perturb(S);
+    computeMoments(S.P, S.gc.R);
+    for (auto &q : S.P)
+    {
+        q.p = q.rho * S.gc.R * q.T;
+        q.meshVx = 1e-3 * q.x / 1e-6; q.meshVy = -2e-3; q.meshVz = 0.5e-3 * q.z / 1e-6;
+        q.moodFlag = (q.voxel % 3) == 0;
+    }
+    for (auto &rb : S.bodies)
+    {
+        rb->velx = 0.25; rb->vely = -0.5; rb->velz = 0.125;
+        rb->omx = 1e5; rb->omy = -2e5; rb->omz = 3e5; rb->angularVel = 4e5;
+        rb->forcex = 1e-9; rb->tqz = 2e-15;
+    }
+    fs::create_directories(dir);
+    writeMeta(dir, S, 0.0);
+    writeRecordFields(dir, S);
+    if (!ckpt::save(dir + "/restart.ckpt", 7, 1.5e-10, S.P, S.bodies)) return 3;
+    return 0;
+}
+
+int modeCkptRead(const std::string &file, const std::string &dir, const std::string &cname,
+                 int Nx, int Nv)
+{
+    Setup S;
+    build(S, caseOf(cname), Nx, Nv, 1.0);
+    long step = 0;
+    double t = 0.0;
+    if (!ckpt::restore(file, &step, &t, S.P, S.bodies)) return 3;
+    fs::create_directories(dir);
+    writeRecordFields(dir, S);
+    std::vector<double> st{(double)step, t};
+    npyF8(dir + "/step_t.npy", st, {2});
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 1; }
@@ -420,6 +514,10 @@ int main(int argc, char **argv)
     if (mode == "cpustep" && argc == 6)
         return modeCpuStep(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]),
                            std::atoi(argv[5]));
+    if (mode == "ckpt" && argc == 6)
+        return modeCkpt(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]));
+    if (mode == "ckptread" && argc == 7)
+        return modeCkptRead(argv[2], argv[3], argv[4], std::atoi(argv[5]), std::atoi(argv[6]));
     std::fprintf(stderr, "bad arguments\n");
     return 1;
 }

@@ -14,5 +14,9 @@ H="$ROOT/oracle/_ref/ref_harness"
 "$H" case3d "$HERE/cavity3d_n9_v3" 9 3
 "$H" case2d "$HERE/cavity2d_n10_v8" 10 8
 "$H" cube   "$HERE/cube_n12_v5" 12 5 1.0
-( cd "$HERE" && find . -name '*.npy' -o -name 'meta.json' | sort | xargs sha256sum > MANIFEST.sha256 )
+# restart files written by the reference's ckpt::save, with the fields they hold
+"$H" ckpt "$HERE/ckpt_cavity3d_n6_v5" cavity3d 6 5
+"$H" ckpt "$HERE/ckpt_cube_n8_v4" cube 8 4
+rm -f "$HERE"/ckpt_*/neigh_flat.npy "$HERE"/ckpt_*/nneigh.npy
+( cd "$HERE" && find . -name '*.npy' -o -name 'meta.json' -o -name '*.ckpt' | sort | xargs sha256sum > MANIFEST.sha256 )
 echo "fixtures written under $HERE"
