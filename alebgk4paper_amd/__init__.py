@@ -351,16 +351,16 @@ class Solver:
 
     # ---- transport launch shape ------------------------------------------
     def autotune(self, dt, reps=3):
-        """Times every transport launch shape, keeps the fastest; returns
-        ({name: ms}, chosen name)."""
+        """Times every transport launch shape that fits the lattice, keeps the
+        fastest; returns ([(index, name, ms)], chosen index)."""
         cap = 32
         names = ((C.c_char * 32) * cap)()
         ms = (C.c_float * cap)()
         n, ch = C.c_int(0), C.c_int(-1)
         self._check(self._lib.ab_autotune(self._ctx, C.c_double(dt), int(reps), cap, names, ms,
                                           C.byref(n), C.byref(ch)), "ab_autotune")
-        table = {names[i].value.decode(): ms[i] for i in range(n.value)}
-        return table, names[ch.value].value.decode()
+        table = [(i, names[i].value.decode(), ms[i]) for i in range(n.value) if ms[i] >= 0]
+        return table, ch.value
 
     def set_transport_shape(self, index):
         self._check(self._lib.ab_set_transport_shape(self._ctx, int(index)), "ab_set_transport_shape")

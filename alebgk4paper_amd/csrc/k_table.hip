@@ -219,7 +219,7 @@ __global__ void k_build(DevCloud c, const int *__restrict__ rep, const int *__re
 }
 
 // ---- transport apply ---------------------------------------------------------
-struct Lat { int nx, ny, nz, S, nchunk; };
+struct Lat { int nx, ny, nz, S, nchunk, edges; };   // edges: only the columns outside [2, n-3]^2
 
 // Addresses: every pointer below is wave-uniform and only the node index
 // varies across lanes. Loads and stores go through raw buffer instructions
@@ -426,13 +426,29 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
     const int wv = __builtin_amdgcn_readfirstlane(b * nwave_blk + threadIdx.x / kWave);
     const int lane = threadIdx.x & (kWave - 1);
     const int voff = lane * (int)sizeof(T);          // the lane's byte offset
-    const int ncol = L.nx * L.ny;
+    // all columns, or only the edge columns (ix or iy in {0, 1, n-2, n-1})
+    // when k_apply_blk takes the interior ones
+    const int ncol = L.edges ? L.nx * L.ny - (L.nx - 4) * (L.ny - 4) : L.nx * L.ny;
     if (wv >= ncol * L.nchunk) return;
     // chunk-major: the waves of one XCD (a contiguous range after the remap)
     // share one or two 64-node chunks, so the chunk's coefficient rows and
     // the neighbouring columns' values stay in that XCD's L2
-    const int chunk = wv / ncol, col = wv % ncol;
-    const int ix = col % L.nx, iy = col / L.nx;
+    const int chunk = wv / ncol, e = wv % ncol;
+    int ix, iy;
+    if (!L.edges) { ix = e % L.nx; iy = e / L.nx; }
+    else if (e < 4 * L.nx)
+    {
+        const int r = e / L.nx;                  // rows 0, 1, ny-2, ny-1
+        iy = r < 2 ? r : L.ny - 4 + r;
+        ix = e % L.nx;
+    }
+    else
+    {
+        const int q = e - 4 * L.nx;              // columns 0, 1, nx-2, nx-1 of rows 2 .. ny-3
+        iy = 2 + q / 4;
+        ix = (q % 4) < 2 ? q % 4 : L.nx - 4 + q % 4;
+    }
+    const int col = ix + L.nx * iy;
     (void)scal;
     const int cx = category(ix, L.nx), cy = category(iy, L.ny);
     const size_t plane = (size_t)L.nx * L.ny;
@@ -477,6 +493,240 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
         for (int k = 0; k < K; ++k)
             if (iz0 + k <= L.nz - 3)
                 bstore(gtw + ((size_t)col + plane * (iz0 + k)) * L.S, voff, acc[k]);
+    }
+}
+
+// ---- blocks of interior columns ------------------------------------------
+// Columns with ix, iy in [2, n-3] all belong to the classes (2, 2, cz): one
+// wave can compute a TX x TY block of them with one set of coefficients.
+// Stencil columns of neighbouring outputs overlap, so the block loads each
+// column of the union of its outputs' stencils once per z block and feeds
+// every output whose stencil contains it: for 2 x 2 outputs, 32 union
+// columns instead of 4 x 21.
+
+// Stencil column of offset (ox, oy), or -1.
+__host__ __device__ constexpr int col_of(int ox, int oy)
+{
+    for (int c = 0; c < kCols; ++c)
+        if (col_at(c).ox == ox && col_at(c).oy == oy) return c;
+    return -1;
+}
+
+struct UCol { int ox, oy, d; };
+
+template <int TX, int TY>
+struct Blk
+{
+    static constexpr int NT = TX * TY;
+    // union columns relative to output (0, 0), row-major; d: the deepest
+    // z reach any output needs from it
+    static constexpr UCol find(int want, int *count)
+    {
+        int n = 0;
+        for (int oy = -2; oy <= TY + 1; ++oy)
+            for (int ox = -2; ox <= TX + 1; ++ox)
+            {
+                int d = 0;
+                for (int ty = 0; ty < TY; ++ty)
+                    for (int tx = 0; tx < TX; ++tx)
+                    {
+                        const int c = col_of(ox - tx, oy - ty);
+                        if (c >= 0 && col_at(c).d > d) d = col_at(c).d;
+                    }
+                if (!d) continue;
+                if (n == want) { if (count) *count = n + 1; return UCol{ox, oy, d}; }
+                ++n;
+            }
+        if (count) *count = n;
+        return UCol{0, 0, 0};
+    }
+    static constexpr int count() { int n = 0; find(-1, &n); return n; }
+    static constexpr int NU = count();
+    static constexpr UCol ucol(int u) { return find(u, nullptr); }
+};
+
+// Values of union column U for outputs iz0 .. iz0+K-1 (no x/y clamping: the
+// block lies two columns inside the lattice).
+template <typename T, int TX, int TY, int K, int U>
+__device__ __forceinline__ void blk_load(const T *__restrict__ g, const Lat &L, int bx, int by,
+                                         int iz0, int lin, size_t zstride, T (&v)[K + 4])
+{
+    constexpr UCol uc = Blk<TX, TY>::ucol(U);
+    constexpr int NV = K + 2 * uc.d;
+    const int z0 = iz0 - uc.d;
+    const __amdgpu_buffer_rsrc_t rg = brsrc(
+        g + ((size_t)(bx + uc.ox) + (size_t)L.nx * (by + uc.oy)) * L.S + (size_t)z0 * zstride);
+    const int pb = (int)(zstride * sizeof(T));
+    const int zmax = L.nz - 1 - z0;
+#pragma clang loop unroll(full)
+    for (int j = 0; j < NV; ++j) v[j] = bload_so<T>(rg, lin, (j < zmax ? j : zmax) * pb);
+}
+
+template <typename T, int TX, int TY, int K, int U, int... TI>
+__device__ __forceinline__ void blk_fma(const T (&cr)[kOff], const T (&v)[K + 4],
+                                        T (&acc)[TX * TY][K], std::integer_sequence<int, TI...>)
+{
+    constexpr UCol uc = Blk<TX, TY>::ucol(U);
+    auto one = [&](auto tic) {
+        constexpr int t = decltype(tic)::value;
+        constexpr int c = col_of(uc.ox - t % TX, uc.oy - t / TX);
+        if constexpr (c >= 0)
+        {
+            constexpr int dc = col_at(c).d, o0 = col_at(c).o0;
+#pragma clang loop unroll(full)
+            for (int k = 0; k < K; ++k)
+#pragma clang loop unroll(full)
+                for (int dz = -dc; dz <= dc; ++dz)
+                    acc[t][k] += cr[o0 + dz + dc] * v[k + dz + uc.d];
+        }
+    };
+    (one(std::integral_constant<int, TI>{}), ...);
+}
+
+template <typename T, int TX, int TY, int K, int U>
+__device__ __forceinline__ void blk_step(const T *__restrict__ g, const T (&cr)[kOff],
+                                         const Lat &L, int bx, int by, int iz0, int lin,
+                                         size_t zstride, T (&va)[K + 4], T (&vb)[K + 4],
+                                         T (&acc)[TX * TY][K])
+{
+    if constexpr (U + 1 < Blk<TX, TY>::NU)
+    {
+        if constexpr ((U & 1) == 0) blk_load<T, TX, TY, K, U + 1>(g, L, bx, by, iz0, lin, zstride, vb);
+        else blk_load<T, TX, TY, K, U + 1>(g, L, bx, by, iz0, lin, zstride, va);
+    }
+    if constexpr ((U & 1) == 0)
+        blk_fma<T, TX, TY, K, U>(cr, va, acc, std::make_integer_sequence<int, TX * TY>{});
+    else
+        blk_fma<T, TX, TY, K, U>(cr, vb, acc, std::make_integer_sequence<int, TX * TY>{});
+#pragma clang loop unroll(full)
+    for (int t = 0; t < TX * TY; ++t)
+#pragma clang loop unroll(full)
+        for (int k = 0; k < K; ++k) asm volatile("" : "+v"(acc[t][k])::"memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <typename T, int TX, int TY, int K, int... U>
+__device__ __forceinline__ void blk_columns(const T *__restrict__ g, const T (&cr)[kOff],
+                                            const Lat &L, int bx, int by, int iz0, int lin,
+                                            size_t zstride, T (&acc)[TX * TY][K],
+                                            std::integer_sequence<int, U...>)
+{
+    T va[K + 4], vb[K + 4];
+    blk_load<T, TX, TY, K, 0>(g, L, bx, by, iz0, lin, zstride, va);
+    (blk_step<T, TX, TY, K, U>(g, cr, L, bx, by, iz0, lin, zstride, va, vb, acc), ...);
+}
+
+// End-plane pair of a block: per stencil column, the two classes'
+// coefficients once for the block, then every output's column values.
+template <typename T, int TX, int TY, bool TOP, int C, int... TI>
+__device__ __forceinline__ void blk_pair_step(const T *__restrict__ g, __amdgpu_buffer_rsrc_t ra,
+                                              __amdgpu_buffer_rsrc_t rb, const Lat &L, int bx,
+                                              int by, int zlo, int lin, T (&a0)[TX * TY],
+                                              T (&a1)[TX * TY], std::integer_sequence<int, TI...>)
+{
+    constexpr Col k = col_at(C);
+    constexpr int lo = TOP ? 2 - k.d : 0, hi = TOP ? 3 : 1 + k.d;
+    T ca[5], cb[5], gv[TX * TY][4];
+#pragma clang loop unroll(full)
+    for (int j = 0; j <= 2 * k.d; ++j)
+    {
+        ca[j] = bload_so<T>(ra, lin, (k.o0 + j) * kWave * (int)sizeof(T));
+        cb[j] = bload_so<T>(rb, lin, (k.o0 + j) * kWave * (int)sizeof(T));
+    }
+    const int pb = L.nx * L.ny * L.S * (int)sizeof(T);
+    auto load = [&](auto tic) {
+        constexpr int t = decltype(tic)::value;
+        const __amdgpu_buffer_rsrc_t rg = brsrc(
+            g + ((size_t)(bx + t % TX + k.ox) + (size_t)L.nx * (by + t / TX + k.oy + (size_t)L.ny * zlo)) * L.S);
+#pragma clang loop unroll(full)
+        for (int j = lo; j <= hi; ++j) gv[t][j] = bload_so<T>(rg, lin, j * pb);
+    };
+    (load(std::integral_constant<int, TI>{}), ...);
+#pragma clang loop unroll(full)
+    for (int t = 0; t < TX * TY; ++t)
+#pragma clang loop unroll(full)
+        for (int j = 0; j <= 2 * k.d; ++j)
+        {
+            a0[t] += ca[j] * gv[t][pair_idx<TOP>(0, j - k.d)];
+            a1[t] += cb[j] * gv[t][pair_idx<TOP>(1, j - k.d)];
+        }
+#pragma clang loop unroll(full)
+    for (int t = 0; t < TX * TY; ++t) asm volatile("" : "+v"(a0[t]), "+v"(a1[t])::"memory");
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+template <typename T, int TX, int TY, bool TOP, int... C>
+__device__ __forceinline__ void blk_pair(const T *__restrict__ g, const T *__restrict__ cfa,
+                                         const T *__restrict__ cfb, const Lat &L, int bx, int by,
+                                         int lin, T (&a0)[TX * TY], T (&a1)[TX * TY],
+                                         std::integer_sequence<int, C...>)
+{
+    const int zlo = TOP ? L.nz - 4 : 0;
+    const __amdgpu_buffer_rsrc_t ra = brsrc(cfa), rb = brsrc(cfb);
+#pragma clang loop unroll(full)
+    for (int t = 0; t < TX * TY; ++t) { a0[t] = T(0); a1[t] = T(0); }
+    (blk_pair_step<T, TX, TY, TOP, C>(g, ra, rb, L, bx, by, zlo, lin, a0, a1,
+                                      std::make_integer_sequence<int, TX * TY>{}), ...);
+}
+
+// One wave = a TX x TY block of interior columns x one 64-node chunk.
+// Requires (nx - 4) % TX == 0 and (ny - 4) % TY == 0; the edge columns are
+// left to k_apply (edges only).
+template <typename T, int TX, int TY, int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_apply_blk(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
+{
+    const int nbx = (L.nx - 4) / TX, nby = (L.ny - 4) / TY, nblk = nbx * nby;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    const int voff = lane * (int)sizeof(T);
+    if (wv >= nblk * L.nchunk) return;
+    const int chunk = wv / nblk, blk = wv % nblk;             // chunk-major, as k_apply
+    const int bx = 2 + (blk % nbx) * TX, by = 2 + (blk / nbx) * TY;
+    const size_t plane = (size_t)L.nx * L.ny;
+    const size_t c0 = (size_t)chunk * kWave;
+    const T *gw = g + c0;
+    T *gtw = gt + c0;
+    auto cls_rows = [&](int cz) {
+        return coef + ((size_t)(2 + 5 * (2 + 5 * cz)) * L.nchunk + chunk) * kOff * kWave;
+    };
+    auto store = [&](int t, int iz, T v) {
+        bstore(gtw + ((size_t)(bx + t % TX) + (size_t)L.nx * (by + t / TX) + plane * iz) * L.S, voff, v);
+    };
+    auto end_pair = [&](auto top) {
+        constexpr bool TOP = decltype(top)::value;
+        const int iz = TOP ? L.nz - 2 : 0;
+        T a0[TX * TY], a1[TX * TY];
+        blk_pair<T, TX, TY, TOP>(gw, cls_rows(category(iz, L.nz)), cls_rows(category(iz + 1, L.nz)),
+                                 L, bx, by, voff, a0, a1, std::make_integer_sequence<int, kCols>{});
+#pragma unroll
+        for (int t = 0; t < TX * TY; ++t) { store(t, iz, a0[t]); store(t, iz + 1, a1[t]); }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+    end_pair(std::false_type{});
+    end_pair(std::true_type{});
+
+    T cr[kOff];
+    {
+        const __amdgpu_buffer_rsrc_t rc = brsrc(cls_rows(2));
+#pragma unroll
+        for (int o = 0; o < kOff; ++o) cr[o] = bload_so<T>(rc, voff, o * kWave * (int)sizeof(T));
+    }
+    for (int iz0 = 2; iz0 <= L.nz - 3; iz0 += K)
+    {
+        T acc[TX * TY][K];
+#pragma unroll
+        for (int t = 0; t < TX * TY; ++t)
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc[t][k] = T(0);
+        blk_columns<T, TX, TY, K>(gw, cr, L, bx, by, iz0, voff, plane * L.S, acc,
+                                  std::make_integer_sequence<int, Blk<TX, TY>::NU>{});
+#pragma unroll
+        for (int t = 0; t < TX * TY; ++t)
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (iz0 + k <= L.nz - 3) store(t, iz0 + k, acc[t][k]);
     }
 }
 
@@ -1024,23 +1274,46 @@ bool is_fp32(const ab_ctx *ctx) { return ctx->cfg.precision == AB_PREC_MIXED; }
 // K outputs per z block (values reused across K + 4 planes), W the waves per
 // SIMD the register allocation targets. Every shape does the same products
 // in the same order per output, so the results do not depend on the choice.
-struct ApplyShape { int K, W; };
-constexpr ApplyShape kShapes32[] = {{9, 3}, {8, 3}, {7, 3}, {6, 3}, {12, 3}, {10, 3}, {5, 3}, {4, 3}};
-constexpr ApplyShape kShapes64[] = {{9, 2}, {8, 2}, {7, 2}, {6, 2}};
+struct ApplyShape { int K, W, TX, TY; };   // TX x TY > 1: interior blocks + edge columns
+constexpr ApplyShape kShapes32[] = {{9, 3, 1, 1}, {8, 3, 1, 1}, {7, 3, 1, 1}, {6, 3, 1, 1},
+                                    {12, 3, 1, 1}, {10, 3, 1, 1}, {5, 3, 1, 1}, {4, 3, 1, 1},
+                                    {8, 2, 2, 2}, {6, 2, 2, 2}, {4, 2, 2, 2}, {12, 2, 2, 2}};
+constexpr ApplyShape kShapes64[] = {{9, 2, 1, 1}, {8, 2, 1, 1}, {7, 2, 1, 1}, {6, 2, 1, 1}};
 constexpr int kMaxBlockPlanes = 12 + 4;
 
 template <typename T>
-constexpr int nshapes() { return sizeof(T) == 4 ? 8 : 4; }
+constexpr int nshapes() { return sizeof(T) == 4 ? 12 : 4; }
 
 template <typename T>
 ApplyShape shape_at(int i) { return sizeof(T) == 4 ? kShapes32[i] : kShapes64[i]; }
 
+// Launch `shape` over the lattice; a block shape runs the edge columns with
+// the single-column kernel (K = 8) and the interior with k_apply_blk.
 template <typename T>
-void launch_shape(int i, const Lat &L, const T *g, T *gt, const T *cf, const double *scal,
-                  int nb, hipStream_t s)
+void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *scal,
+                  hipStream_t s)
 {
+    auto waves = [&](int ncol) { return (ncol * L.nchunk + 3) / 4; };
+    const int all = L.nx * L.ny, inner = (L.nx - 4) * (L.ny - 4);
     if constexpr (sizeof(T) == 4)
     {
+        const ApplyShape sh = kShapes32[i];
+        if (sh.TX > 1)
+        {
+            L.edges = 1;
+            k_apply<T, 8, 3><<<waves(all - inner), 256, 0, s>>>(L, g, gt, cf, scal);
+            L.edges = 0;
+            const int nblk = inner / (sh.TX * sh.TY);
+            switch (i)
+            {
+            case 8: k_apply_blk<T, 2, 2, 8><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
+            case 9: k_apply_blk<T, 2, 2, 6><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
+            case 10: k_apply_blk<T, 2, 2, 4><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
+            default: k_apply_blk<T, 2, 2, 12><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
+            }
+            return;
+        }
+        const int nb = waves(all);
         switch (i)
         {
         case 0: k_apply<T, 9, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
@@ -1055,6 +1328,7 @@ void launch_shape(int i, const Lat &L, const T *g, T *gt, const T *cf, const dou
     }
     else
     {
+        const int nb = waves(all);
         switch (i)
         {
         case 0: k_apply<T, 9, 2><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
@@ -1063,6 +1337,14 @@ void launch_shape(int i, const Lat &L, const T *g, T *gt, const T *cf, const dou
         default: k_apply<T, 6, 2><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
         }
     }
+}
+
+// A shape applies to this lattice (blocks must tile the interior columns).
+template <typename T>
+bool shape_fits(int i, int nx, int ny)
+{
+    const ApplyShape sh = shape_at<T>(i);
+    return sh.TX == 1 || ((nx - 4) % sh.TX == 0 && (ny - 4) % sh.TY == 0 && nx > 4 && ny > 4);
 }
 
 // Default shape before tuning: the z block that divides the interior run.
@@ -1076,17 +1358,16 @@ int default_shape(int nz)
 template <typename T>
 int apply_launch(ab_ctx *ctx, const Lat &L, int shape)
 {
-    const int nwaves = ctx->cfg.nx * ctx->cfg.ny * L.nchunk;
-    const int nb = (nwaves + 3) / 4;
     // buffer offsets within a z block must stay below 2^31
     if ((double)kMaxBlockPlanes * ctx->cfg.nx * ctx->cfg.ny * (double)ctx->S * sizeof(T) >= 2147483648.0)
     {
         ctx->err = "table transport: a z block of the lattice exceeds 2 GiB";
         return AB_ERR_UNSUPPORTED;
     }
-    if (shape < 0 || shape >= nshapes<T>()) shape = default_shape<T>(ctx->cfg.nz);
+    if (shape < 0 || shape >= nshapes<T>() || !shape_fits<T>(shape, L.nx, L.ny))
+        shape = default_shape<T>(ctx->cfg.nz);
     launch_shape<T>(shape, L, (const T *)ctx->gs, (T *)ctx->gts, (const T *)ctx->coef, ctx->scal,
-                    nb, ctx->stream);
+                    ctx->stream);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 
@@ -1305,7 +1586,7 @@ int table_step(ab_ctx *ctx, double dt)
         int rc = table_prepare(ctx, dt);
         if (rc) return rc;
     }
-    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave};
+    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave, 0};
     ab::DevCloud c = ctx->view();
     hipStream_t s = ctx->stream;
     auto tic = [&]() { prof_begin(ctx); };
@@ -1357,7 +1638,7 @@ int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32],
         int rc = table_prepare(ctx, dt);
         if (rc) return rc;
     }
-    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave};
+    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave, 0};
     const bool f32 = is_fp32(ctx);
     const int ns = f32 ? nshapes<float>() : nshapes<double>();
     hipEvent_t e0, e1;
@@ -1368,6 +1649,13 @@ int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32],
     int rc = AB_OK;
     for (int i = 0; i < ns && !rc; ++i)
     {
+        const bool fits = f32 ? shape_fits<float>(i, L.nx, L.ny) : shape_fits<double>(i, L.nx, L.ny);
+        if (!fits)
+        {
+            if (ms && i < cap) ms[i] = -1.0f;
+            if (names && i < cap) snprintf(names[i], 32, "n/a");
+            continue;
+        }
         float tmin = 0.0f;
         for (int r = 0; r <= reps && !rc; ++r)           // r == 0 warms up
         {
@@ -1384,7 +1672,8 @@ int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32],
         if (names && i < cap)
         {
             const ApplyShape sh = f32 ? shape_at<float>(i) : shape_at<double>(i);
-            snprintf(names[i], 32, "K%d/W%d", sh.K, sh.W);
+            if (sh.TX > 1) snprintf(names[i], 32, "B%dx%d/K%d", sh.TX, sh.TY, sh.K);
+            else snprintf(names[i], 32, "K%d/W%d", sh.K, sh.W);
         }
         if (best < 0 || tmin < tbest) { best = i; tbest = tmin; }
     }
@@ -1401,6 +1690,9 @@ int table_set_shape(ab_ctx *ctx, int shape)
 {
     const int ns = is_fp32(ctx) ? nshapes<float>() : nshapes<double>();
     if (shape < -1 || shape >= ns) { ctx->err = "transport shape index out of range"; return AB_ERR_ARG; }
+    const bool fits = shape < 0 || (is_fp32(ctx) ? shape_fits<float>(shape, ctx->cfg.nx, ctx->cfg.ny)
+                                                 : shape_fits<double>(shape, ctx->cfg.nx, ctx->cfg.ny));
+    if (!fits) { ctx->err = "transport shape does not tile this lattice"; return AB_ERR_UNSUPPORTED; }
     ctx->apply_shape = shape;
     return AB_OK;
 }

@@ -150,7 +150,9 @@ def main():
     mix.mlp_enable(True)
     train_s = time.perf_counter() - t0
     shapes, chosen = mix.autotune(dt, reps=3)
-    log(f"[bench] transport shapes (ms): {shapes} -> {chosen}")
+    shape_ms = {name: ms for _, name, ms in shapes}
+    chosen_name = [name for i, name, _ in shapes if i == chosen][0]
+    log(f"[bench] transport shapes (ms): {shape_ms} -> {chosen_name}")
     log(f"[bench] MLP: {args.train_iters} Adam steps on {rec * N} rows in {train_s:.2f} s, "
         f"loss {loss0:.3e} -> {loss1:.3e}")
     if args.counters:
@@ -181,7 +183,7 @@ def main():
     # ---- the same without the warm start (plain Newton) --------------------
     pln = A.Solver(cloud, precision=A.PREC_MIXED, collision=DISC)
     pln.steps(dt, args.warmup)
-    pln.set_transport_shape(list(shapes).index(chosen))
+    pln.set_transport_shape(chosen)
     pln.synchronize()
     t0 = time.perf_counter()
     pln.steps(dt, args.steps)
@@ -247,7 +249,7 @@ def main():
         "newton_passes_per_solve": passes,
         "mlp": {"train_seconds": train_s, "adam_steps": args.train_iters, "batch": 4096,
                 "rows": rec * N, "loss_first": loss0, "loss_last": loss1},
-        "autotune": {"transport_shapes_ms": shapes, "chosen": chosen},
+        "autotune": {"transport_shapes_ms": shape_ms, "chosen": chosen_name},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,

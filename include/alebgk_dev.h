@@ -186,14 +186,16 @@ int ab_synchronize(ab_ctx *ctx);
 
 /* ---- Autotuner of the transport launch shape -----------------------------
  * No counterpart in the reference. Times every launch shape of the table
- * pipelines' transport contraction (z block size K, target waves per SIMD W)
- * on the context's current state, `reps` launches each after a warm-up (the
- * transport writes only its scratch output, the state is not advanced), and
- * keeps the fastest for the following steps. The results of a step do not
- * depend on the shape. names/ms (cap entries each, may be NULL) receive
- * every candidate's name ("K9/W3") and best time in ms; *n the number of
- * candidates, *chosen the index kept. ab_set_transport_shape forces an index
- * (-1: the default for the lattice). */
+ * pipelines' transport contraction on the context's current state, `reps`
+ * launches each after a warm-up (the transport writes only its scratch
+ * output, the state is not advanced), and keeps the fastest for the
+ * following steps. Shapes: one lattice column per wave with z blocks of K
+ * outputs ("K9/W3": bit-identical results for every K), or TX x TY blocks of
+ * interior columns per wave ("B2x2/K8": the same sum in another order, equal
+ * to FP32 rounding). names/ms (cap entries each, may be NULL) receive every
+ * candidate's name and best time in ms, "n/a" and -1 for a shape that does
+ * not tile the lattice; *n the number of candidates, *chosen the index kept.
+ * ab_set_transport_shape forces an index (-1: the default for the lattice). */
 int ab_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32], float *ms,
                 int *n, int *chosen);
 int ab_set_transport_shape(ab_ctx *ctx, int shape);

@@ -58,20 +58,28 @@ def test_graph_refuses_profiling_context():
 
 
 def test_autotune_keeps_results_and_picks_a_shape():
-    """Every transport launch shape gives bit-identical steps; the autotuner
-    times them all and keeps one."""
+    """Single-column transport shapes give bit-identical steps (the same
+    products in the same order per output); block shapes visit an output's
+    stencil columns in another order, so they agree to FP32 rounding. The
+    autotuner times every shape that fits and keeps one."""
     _say("[autotune] cavity3d 12^3 nv8 mixed")
     case, cl, ocl, neigh, nn = setup("cavity3d", 12, 8, order=2, precision=A.PREC_MIXED)
     dt = 1e-11
     base = A.Solver(cl, precision=A.PREC_MIXED)
     table, chosen = base.autotune(dt, reps=2)
     _say(f"  {table} -> {chosen}")
-    assert chosen in table and len(table) >= 4 and all(t > 0 for t in table.values())
+    assert chosen in [i for i, _, _ in table] and len(table) >= 8
+    assert all(ms > 0 for _, _, ms in table)
+    base.set_transport_shape(0)
     base.steps(dt, 3)
     ref = base.g()
-    for shape in range(len(table)):
+    for i, name, _ in table:
         s = A.Solver(cl, precision=A.PREC_MIXED)
-        s.set_transport_shape(shape)
+        s.set_transport_shape(i)
         s.steps(dt, 3)
-        assert np.array_equal(s.g(), ref), shape
+        g = s.g()
+        if name.startswith("B"):
+            assert np.abs(g - ref).max() <= 1e-6 * np.abs(ref).max(), name
+        else:
+            assert np.array_equal(g, ref), name
         s.close()
