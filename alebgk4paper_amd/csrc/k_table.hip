@@ -219,7 +219,7 @@ __global__ void k_build(DevCloud c, const int *__restrict__ rep, const int *__re
 }
 
 // ---- transport apply ---------------------------------------------------------
-struct Lat { int nx, ny, nz, S, nchunk, edges; };   // edges: only the columns outside [2, n-3]^2
+struct Lat { int nx, ny, nz, S, nchunk; };
 
 // Addresses: every pointer below is wave-uniform and only the node index
 // varies across lanes. Loads and stores go through raw buffer instructions
@@ -415,27 +415,11 @@ __device__ __forceinline__ void apply_columns(const T *__restrict__ g, const T (
     (col_step<T, K, CC>(g, cr, L, ix, iy, iz0, lin, zstride, va, vb, acc), ...);
 }
 
-template <typename T, int K, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
-void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef,
-             const double *__restrict__ scal)
+// The lattice column (ix, iy) of a wave in an "all columns" or "edge
+// columns only" enumeration (edges: ix or iy in {0, 1, n-2, n-1}).
+__device__ __forceinline__ void column_of(const Lat &L, int e, bool edges, int &ix, int &iy)
 {
-    const int nwave_blk = blockDim.x / kWave;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    // wave-uniform: keep it (and every address part derived from it) in SGPRs
-    const int wv = __builtin_amdgcn_readfirstlane(b * nwave_blk + threadIdx.x / kWave);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int voff = lane * (int)sizeof(T);          // the lane's byte offset
-    // all columns, or only the edge columns (ix or iy in {0, 1, n-2, n-1})
-    // when k_apply_blk takes the interior ones
-    const int ncol = L.edges ? L.nx * L.ny - (L.nx - 4) * (L.ny - 4) : L.nx * L.ny;
-    if (wv >= ncol * L.nchunk) return;
-    // chunk-major: the waves of one XCD (a contiguous range after the remap)
-    // share one or two 64-node chunks, so the chunk's coefficient rows and
-    // the neighbouring columns' values stay in that XCD's L2
-    const int chunk = wv / ncol, e = wv % ncol;
-    int ix, iy;
-    if (!L.edges) { ix = e % L.nx; iy = e / L.nx; }
+    if (!edges) { ix = e % L.nx; iy = e / L.nx; }
     else if (e < 4 * L.nx)
     {
         const int r = e / L.nx;                  // rows 0, 1, ny-2, ny-1
@@ -448,8 +432,16 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
         iy = 2 + q / 4;
         ix = (q % 4) < 2 ? q % 4 : L.nx - 4 + q % 4;
     }
+}
+
+// One wave = lattice column (ix, iy) x one 64-node chunk: the end-plane
+// pairs, then the interior march in z blocks of K outputs.
+template <typename T, int K>
+__device__ __forceinline__ void column_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
+                                            const T *__restrict__ coef, int chunk, int ix, int iy,
+                                            int voff)
+{
     const int col = ix + L.nx * iy;
-    (void)scal;
     const int cx = category(ix, L.nx), cy = category(iy, L.ny);
     const size_t plane = (size_t)L.nx * L.ny;
     // this wave's chunk of nodes: offset every base pointer once
@@ -494,6 +486,26 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
             if (iz0 + k <= L.nz - 3)
                 bstore(gtw + ((size_t)col + plane * (iz0 + k)) * L.S, voff, acc[k]);
     }
+}
+
+template <typename T, int K, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W)))
+void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef,
+             const double *__restrict__ scal)
+{
+    (void)scal;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    // wave-uniform: keep it (and every address part derived from it) in SGPRs
+    const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
+    const int voff = (threadIdx.x & (kWave - 1)) * (int)sizeof(T);   // the lane's byte offset
+    const int ncol = L.nx * L.ny;
+    if (wv >= ncol * L.nchunk) return;
+    // chunk-major: the waves of one XCD (a contiguous range after the remap)
+    // share one or two 64-node chunks, so the chunk's coefficient rows and
+    // the neighbouring columns' values stay in that XCD's L2
+    int ix, iy;
+    column_of(L, wv % ncol, false, ix, iy);
+    column_wave<T, K>(L, g, gt, coef, wv / ncol, ix, iy, voff);
 }
 
 // ---- blocks of interior columns ------------------------------------------
@@ -669,20 +681,14 @@ __device__ __forceinline__ void blk_pair(const T *__restrict__ g, const T *__res
                                       std::make_integer_sequence<int, TX * TY>{}), ...);
 }
 
-// One wave = a TX x TY block of interior columns x one 64-node chunk.
-// Requires (nx - 4) % TX == 0 and (ny - 4) % TY == 0; the edge columns are
-// left to k_apply (edges only).
+// One wave = a TX x TY block of interior columns (block index blk, row-major
+// over the interior) x one 64-node chunk. Requires (nx - 4) % TX == 0 and
+// (ny - 4) % TY == 0.
 template <typename T, int TX, int TY, int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-void k_apply_blk(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
+__device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
+                                           const T *__restrict__ coef, int chunk, int blk, int voff)
 {
-    const int nbx = (L.nx - 4) / TX, nby = (L.ny - 4) / TY, nblk = nbx * nby;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int voff = lane * (int)sizeof(T);
-    if (wv >= nblk * L.nchunk) return;
-    const int chunk = wv / nblk, blk = wv % nblk;             // chunk-major, as k_apply
+    const int nbx = (L.nx - 4) / TX;
     const int bx = 2 + (blk % nbx) * TX, by = 2 + (blk / nbx) * TY;
     const size_t plane = (size_t)L.nx * L.ny;
     const size_t c0 = (size_t)chunk * kWave;
@@ -727,6 +733,32 @@ void k_apply_blk(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
 #pragma unroll
             for (int k = 0; k < K; ++k)
                 if (iz0 + k <= L.nz - 3) store(t, iz0 + k, acc[t][k]);
+    }
+}
+
+// The whole transport in one launch: per chunk, first the interior blocks
+// (TX x TY columns per wave), then the edge columns one per wave (z blocks of
+// KE). One launch keeps the edge waves from running as a second, short and
+// latency-bound grid after the blocks.
+template <typename T, int TX, int TY, int K, int KE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
+{
+    const int nblk = ((L.nx - 4) / TX) * ((L.ny - 4) / TY);
+    const int nedge = L.nx * L.ny - (L.nx - 4) * (L.ny - 4);
+    const int per = nblk + nedge;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
+    const int voff = (threadIdx.x & (kWave - 1)) * (int)sizeof(T);
+    if (wv >= per * L.nchunk) return;
+    const int chunk = wv / per, r = wv % per;               // chunk-major
+    if (r < nblk)
+        block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r, voff);
+    else
+    {
+        int ix, iy;
+        column_of(L, r - nblk, true, ix, iy);
+        column_wave<T, KE>(L, g, gt, coef, chunk, ix, iy, voff);
     }
 }
 
@@ -1287,8 +1319,8 @@ constexpr int nshapes() { return sizeof(T) == 4 ? 12 : 4; }
 template <typename T>
 ApplyShape shape_at(int i) { return sizeof(T) == 4 ? kShapes32[i] : kShapes64[i]; }
 
-// Launch `shape` over the lattice; a block shape runs the edge columns with
-// the single-column kernel (K = 8) and the interior with k_apply_blk.
+// Launch `shape` over the lattice (a block shape: k_apply_mix, blocks and the
+// edge columns in one grid).
 template <typename T>
 void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *scal,
                   hipStream_t s)
@@ -1300,16 +1332,13 @@ void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *sc
         const ApplyShape sh = kShapes32[i];
         if (sh.TX > 1)
         {
-            L.edges = 1;
-            k_apply<T, 8, 3><<<waves(all - inner), 256, 0, s>>>(L, g, gt, cf, scal);
-            L.edges = 0;
-            const int nblk = inner / (sh.TX * sh.TY);
+            const int nw = (inner / (sh.TX * sh.TY) + all - inner);
             switch (i)
             {
-            case 8: k_apply_blk<T, 2, 2, 8><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
-            case 9: k_apply_blk<T, 2, 2, 6><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
-            case 10: k_apply_blk<T, 2, 2, 4><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
-            default: k_apply_blk<T, 2, 2, 12><<<waves(nblk), 256, 0, s>>>(L, g, gt, cf); break;
+            case 8: k_apply_mix<T, 2, 2, 8, 8><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 9: k_apply_mix<T, 2, 2, 6, 8><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 10: k_apply_mix<T, 2, 2, 4, 8><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            default: k_apply_mix<T, 2, 2, 12, 12><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
             }
             return;
         }
@@ -1586,7 +1615,7 @@ int table_step(ab_ctx *ctx, double dt)
         int rc = table_prepare(ctx, dt);
         if (rc) return rc;
     }
-    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave, 0};
+    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave};
     ab::DevCloud c = ctx->view();
     hipStream_t s = ctx->stream;
     auto tic = [&]() { prof_begin(ctx); };
@@ -1638,7 +1667,7 @@ int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32],
         int rc = table_prepare(ctx, dt);
         if (rc) return rc;
     }
-    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave, 0};
+    Lat L{ctx->cfg.nx, ctx->cfg.ny, ctx->cfg.nz, ctx->S, ctx->S / kWave};
     const bool f32 = is_fp32(ctx);
     const int ns = f32 ? nshapes<float>() : nshapes<double>();
     hipEvent_t e0, e1;
