@@ -319,7 +319,7 @@ int ab_create(const ab_config *cfg, ab_ctx **out)
     if (!cfg || !out) return AB_ERR_ARG;
     *out = nullptr;
     if ((cfg->dim != 2 && cfg->dim != 3) || cfg->n_particles <= 0 || cfg->nv < 2 ||
-        cfg->nv > 48 || !(cfg->vmax > cfg->vmin) || !(cfg->R > 0) || !(cfg->tau > 0) ||
+        cfg->nv > ab::kMaxNv || !(cfg->vmax > cfg->vmin) || !(cfg->R > 0) || !(cfg->tau > 0) ||
         !(cfg->radius > 0) || (cfg->mls_order != 1 && cfg->mls_order != 2) ||
         cfg->precision < AB_PREC_FP64 || cfg->precision > AB_PREC_FP64_TABLE ||
         (cfg->collision != AB_COLLISION_CONTINUOUS && cfg->collision != AB_COLLISION_DISCRETE))

@@ -11,6 +11,7 @@ namespace ab {
 
 constexpr int kWave = 64;          // CDNA wavefront
 constexpr int kBlock = 256;        // 4 waves per workgroup for the per-particle kernels
+constexpr int kMaxNv = 48;         // velocity nodes per axis (ab_create refuses more)
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8). Give every XCD a contiguous range of logical block ids so the
@@ -38,6 +39,16 @@ __device__ __forceinline__ double uniform(double x)
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Orders a wave's LDS writes before its later LDS reads of other lanes'
+// entries (a wave-private table; no workgroup barrier, so waves that left
+// the kernel early do not matter).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <typename T>

@@ -58,6 +58,48 @@ __device__ __forceinline__ void dm_accum(double S[15], double e, double x, doubl
     S[14] += eq * q;
 }
 
+// The same 15 sums over the tensor grid xi_i x xi_j x xi_k without visiting
+// its Nv^3 nodes. exp(a . psi) = e^a0 E_x(xi_i) E_y(xi_j) E_z(xi_k) with
+// E_x(t) = exp(a1 t + a4 t^2) (and a2, a3 for y, z), so every sum of
+// e x^p y^q z^r is e^a0 X_p Y_q Z_r, X_p = sum_i E_x(xi_i) xi_i^p. One
+// particle per thread: 3 Nv exponentials instead of Nv^3.
+__device__ __forceinline__ void dm_grid_sums(const double a[5], int Nv, double vlo, double dv,
+                                             double ics, double dv3, double S[15])
+{
+    double X[5] = {0, 0, 0, 0, 0}, Y[5] = {0, 0, 0, 0, 0}, Z[5] = {0, 0, 0, 0, 0};
+    for (int l = 0; l < Nv; ++l)
+    {
+        const double xi = vnode(vlo, dv, l) * ics, t2 = a[4] * xi * xi;
+        double ex = exp(a[1] * xi + t2), ey = exp(a[2] * xi + t2), ez = exp(a[3] * xi + t2);
+#pragma unroll
+        for (int p = 0; p < 5; ++p)
+        {
+            X[p] += ex; Y[p] += ey; Z[p] += ez;
+            ex *= xi; ey *= xi; ez *= xi;
+        }
+    }
+    const double e0 = exp(a[0]) * dv3;
+    const double x0 = e0 * X[0];
+    const double y0z0 = Y[0] * Z[0], y1z0 = Y[1] * Z[0], y0z1 = Y[0] * Z[1];
+    const double y2z0 = Y[2] * Z[0], y0z2 = Y[0] * Z[2], y1z1 = Y[1] * Z[1];
+    S[0] = x0 * y0z0;
+    S[1] = e0 * X[1] * y0z0;
+    S[2] = x0 * y1z0;
+    S[3] = x0 * y0z1;
+    S[4] = e0 * (X[2] * y0z0 + X[0] * (y2z0 + y0z2));
+    S[5] = e0 * X[2] * y0z0;
+    S[6] = e0 * X[1] * y1z0;
+    S[7] = e0 * X[1] * y0z1;
+    S[8] = e0 * (X[3] * y0z0 + X[1] * (y2z0 + y0z2));
+    S[9] = x0 * y2z0;
+    S[10] = x0 * y1z1;
+    S[11] = e0 * (X[2] * y1z0 + X[0] * (Y[3] * Z[0] + Y[1] * Z[2]));
+    S[12] = x0 * y0z2;
+    S[13] = e0 * (X[2] * y0z1 + X[0] * (Y[2] * Z[1] + Y[0] * Z[3]));
+    S[14] = e0 * (X[4] * y0z0 + X[0] * (Y[4] * Z[0] + Y[0] * Z[4] + 2.0 * Y[2] * Z[2])
+                  + 2.0 * X[2] * (y2z0 + y0z2));
+}
+
 __device__ __forceinline__ int dm_uidx(int p, int q)
 {
     // packed upper-triangle index of (min, max)
@@ -69,7 +111,7 @@ __device__ __forceinline__ int dm_uidx(int p, int q)
 // sums). Returns false when a pivot is not positive.
 __device__ __forceinline__ bool dm_solve5(const double S[15], const double F[5], double d[5])
 {
-    double L[5][5];
+    double L[5][5], inv[5];
     double y[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i)
@@ -84,8 +126,9 @@ __device__ __forceinline__ bool dm_solve5(const double S[15], const double F[5],
             {
                 if (!(s > 0.0)) return false;
                 L[i][i] = sqrt(s);
+                inv[i] = 1.0 / L[i][i];
             }
-            else L[i][j] = s / L[j][j];
+            else L[i][j] = s * inv[j];
         }
     }
 #pragma unroll
@@ -94,7 +137,7 @@ __device__ __forceinline__ bool dm_solve5(const double S[15], const double F[5],
         double s = F[i];
 #pragma unroll
         for (int k = 0; k < i; ++k) s -= L[i][k] * y[k];
-        y[i] = s / L[i][i];
+        y[i] = s * inv[i];
     }
 #pragma unroll
     for (int i = 4; i >= 0; --i)
@@ -102,7 +145,7 @@ __device__ __forceinline__ bool dm_solve5(const double S[15], const double F[5],
         double s = y[i];
 #pragma unroll
         for (int k = i + 1; k < 5; ++k) s -= L[k][i] * d[k];
-        d[i] = s / L[i][i];
+        d[i] = s * inv[i];
     }
     return true;
 }
@@ -142,6 +185,42 @@ __device__ __forceinline__ void mlp_wave_forward(const unsigned short *__restric
         const float y = wave_sum(bf16_to_float(wb[kMlpW3 + o * kMlpHid + lane]) * h2) + pf[kMlpB3 + o];
         out[o] = y * pf[kMlpYs + o] + pf[kMlpYm + o];
     }
+}
+
+// MLP forward for one sample per lane (a wave evaluates 64 samples; the
+// weights are read at wave-uniform addresses, through scalar loads), BF16
+// weights, FP32 biases, normalisation and accumulation. De-normalised outputs.
+__device__ __forceinline__ void mlp_lane_forward(const unsigned short *__restrict__ wb,
+                                                 const float *__restrict__ pf,
+                                                 const float x[kMlpIn], float out[kMlpOut])
+{
+    float xn[kMlpIn];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) xn[i] = (x[i] - pf[kMlpXm + i]) / pf[kMlpXs + i];
+    float h1[kMlpHid];
+#pragma unroll
+    for (int j = 0; j < kMlpHid; ++j)
+    {
+        float z = pf[kMlpB1 + j];
+#pragma unroll
+        for (int i = 0; i < kMlpIn; ++i) z += bf16_to_float(wb[kMlpW1 + j * kMlpIn + i]) * xn[i];
+        h1[j] = tanhf(z);
+    }
+    float y[kMlpOut];
+#pragma unroll
+    for (int o = 0; o < kMlpOut; ++o) y[o] = pf[kMlpB3 + o];
+    for (int j = 0; j < kMlpHid; ++j)
+    {
+        float z = pf[kMlpB2 + j];
+        const unsigned short *w2 = wb + kMlpW2 + j * kMlpHid;
+#pragma unroll
+        for (int k = 0; k < kMlpHid; ++k) z += bf16_to_float(w2[k]) * h1[k];
+        const float h2 = tanhf(z);
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o) y[o] += bf16_to_float(wb[kMlpW3 + o * kMlpHid + j]) * h2;
+    }
+#pragma unroll
+    for (int o = 0; o < kMlpOut; ++o) out[o] = y[o] * pf[kMlpYs + o] + pf[kMlpYm + o];
 }
 
 }  // namespace ab

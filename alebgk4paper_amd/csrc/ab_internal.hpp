@@ -118,6 +118,7 @@ struct ab_ctx
     double m0rho = 1.0, m0T = 270.0;
     double *part = nullptr;        // per-particle partials of the table pipeline
     double *red = nullptr;         // block sums of the partials (2 per reduction block)
+    double *pst = nullptr;         // per-particle state between the discrete-collision kernels
     int *bad = nullptr;            // stencil check result
     int apply_shape = -1;          // transport launch shape (-1: default for nz)
 

@@ -15,7 +15,7 @@
 // includes exactly these; self included for the gated nodes), grouped in 21
 // (ox, oy) columns of 5 or 3 z-offsets.
 //
-// Step (four kernels, no host synchronisation):
+// Step (four kernels, six with the discrete collision; no host synchronisation):
 //   k_apply      dt = sum_o C d  (stored deviation, below) one wave = 64 nodes of a
 //                lattice column (ix, iy); marches along z in blocks of K
 //                outputs with the class coefficients in registers and each
@@ -23,7 +23,10 @@
 //   k_post       one wave per particle: positivity (fallback to the
 //                pre-transport value), NAD, then the diffuse wall (boundary)
 //                or moments + BGK collision (interior), written back in place
-//                of g unscaled; FP64 partial sums per particle.
+//                of g unscaled; FP64 partial sums per particle. With the
+//                discrete collision it stops after the moments, k_dm_solve
+//                (one thread per particle) finds the discrete Maxwellian and
+//                k_collide (one wave per particle) applies the collision.
 //   k_reduce     mass sums -> the two global rescale factors, folded into one
 //                lazy factor the stored g carries into the next step.
 //   k_finalize   per-particle rho, u, T of the step's end.
@@ -35,6 +38,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <utility>
 #include <vector>
 
@@ -266,6 +270,18 @@ __device__ __forceinline__ void bstore(T *row, int voff, T v)
     {
         typedef unsigned u2 __attribute__((ext_vector_type(2)));
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, 0, 0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore_so(__amdgpu_buffer_rsrc_t r, int voff, int soff, T v)
+{
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+    else
+    {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, voff, soff, 0);
     }
 }
 
@@ -787,45 +803,81 @@ struct Ref
     double rho, T;          // reference state of M0
 };
 
-// Walks one lane's nodes k = lane, lane + 64, ... carrying (iv, jv, kv) of
-// lin = iv + Nv (jv + Nv kv) incrementally (no integer division per node).
-struct NodeWalk
+// A particle's row visited pair by pair: the lane takes the (iv, jv) pairs
+// p = iv + Nv jv = lane, lane + 64, ... of the velocity plane and, for each,
+// the Nv nodes k = p + P kv (P = Nv^2) in order. v_x, v_y (and anything that
+// depends only on them) are then fixed per pair and v_z is the same on every
+// lane, so a moment sum_k psi(v) d costs three accumulators per node
+// (sum d, sum kv d, sum kv^2 d), combined with v_x, v_y once per pair.
+// Accesses are buffer operations with the lane's pair in the vector offset
+// and the node's plane kv P in the scalar offset (no per-node address
+// arithmetic). A pair's loads are issued in batches of kB; a batch past the
+// last plane reloads the last plane (in bounds, no branch around the loads)
+// and skips the work.
+//   begin(iv, jv, vx, vy)   node(p, kv, t, m0)   end()
+template <typename T>
+struct Row
 {
-    int iv, jv, kv, a, b, c, Nv;
-    __device__ NodeWalk(int lane, int nv) : Nv(nv)
+    __amdgpu_buffer_rsrc_t rg;    // stored row (pre-transport values; written last)
+    __amdgpu_buffer_rsrc_t rt;    // transported row
+    __amdgpu_buffer_rsrc_t rm;    // M0
+    int lane, Nv;
+    double vlo, dv;
+    __device__ Row(T *g, T *t, const double *m0, int ln, int nv, double lo, double d)
+        : rg(brsrc(g)), rt(brsrc(t)), rm(brsrc(m0)), lane(ln), Nv(nv), vlo(lo), dv(d) {}
+    __device__ __forceinline__ int soff(int kv, int bytes) const { return kv * Nv * Nv * bytes; }
+    __device__ __forceinline__ T g(int p, int kv) const
+    { return bload_so<T>(rg, p * (int)sizeof(T), soff(kv, sizeof(T))); }
+    __device__ __forceinline__ void set_g(int p, int kv, T v) const
+    { bstore_so<T>(rg, p * (int)sizeof(T), soff(kv, sizeof(T)), v); }
+    __device__ __forceinline__ void set_t(int p, int kv, T v) const
+    { bstore_so<T>(rt, p * (int)sizeof(T), soff(kv, sizeof(T)), v); }
+    template <typename FB, typename FN, typename FE>
+    __device__ __forceinline__ void pairs(FB &&begin, FN &&node, FE &&end) const
     {
-        iv = lane % nv; jv = (lane / nv) % nv; kv = lane / (nv * nv);
-        a = kWave % nv; b = (kWave / nv) % nv; c = kWave / (nv * nv);
-        // opaque start: each pass recomputes its velocities instead of the
-        // compiler keeping every pass's values live across the others
-        asm volatile("" : "+v"(iv), "+v"(jv), "+v"(kv));
-    }
-    __device__ __forceinline__ void step()
-    {
-        iv += a; jv += b; kv += c;
-        if (iv >= Nv) { iv -= Nv; ++jv; }
-        if (jv >= Nv) { jv -= Nv; ++kv; }
+        constexpr int kB = 5;
+        const int P = Nv * Nv;
+        for (int p = lane; p < P; p += kWave)
+        {
+            const int jv = p / Nv, iv = p - jv * Nv;
+            begin(iv, jv, vnode(vlo, dv, iv), vnode(vlo, dv, jv));
+            for (int k0 = 0; k0 < Nv; k0 += kB)
+            {
+                T tv[kB];
+                double mv[kB];
+#pragma unroll
+                for (int j = 0; j < kB; ++j)
+                {
+                    const int kv = min(k0 + j, Nv - 1);
+                    tv[j] = bload_so<T>(rt, p * (int)sizeof(T), soff(kv, sizeof(T)));
+                    mv[j] = bload_so<double>(rm, p * 8, soff(kv, 8));
+                }
+#pragma unroll
+                for (int j = 0; j < kB; ++j)
+                    if (k0 + j < Nv) node(p, k0 + j, (double)tv[j], mv[j]);
+            }
+            end();
+        }
     }
 };
 
 // Boundary particle (one wave): positivity, then the diffuse wall in FP64
 // (kWallBC3D, CudaBackend.cu:956-1030), written back as a deviation.
-// Streams the row from memory (14 % of the particles at 40^3).
 template <typename T>
-__device__ __forceinline__ void post_boundary(const DevCloud &c, int i, int lane, T *__restrict__ gp,
-                                              const T *__restrict__ tp, const Ref &ref, double Lo,
-                                              double R, double out[kPart])
+__device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Row<T> &row,
+                                              const Ref &ref, double Lo, double R, double *tab,
+                                              double out[kPart])
 {
-    const double *m0 = ref.m0;
-    const int nodes = c.nodes, Nv = c.Nv;
+    const int lane = row.lane, Nv = row.Nv;
     const double dv3 = c.dv * c.dv * c.dv, vlo = c.vlo, dv = c.dv;
+    auto nop = [] {};
+    auto nob = [](int, int, double, double) {};
     double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
-    for (int k = lane; k < nodes; k += kWave)
-    {
-        const double v = m0[k] + (double)tp[k];
+    row.pairs(nob, [&](int p, int kv, double t, double m) {
+        const double v = m + t;
         if (v >= 0.0) mPos += v;
-        else { mNeg += v; mFb += m0[k] + (double)gp[k]; }
-    }
+        else { mNeg += v; mFb += m + (double)row.g(p, kv); }
+    }, nop);
     mPos = wave_sum(mPos); mNeg = wave_sum(mNeg); mFb = wave_sum(mFb);
     const bool fires = (Lo * mNeg < -1e-30 && (mPos + mNeg) > 0);
     double sc = 1.0;
@@ -834,32 +886,43 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, int lane
         const double replaced = mPos + mFb;
         sc = (Lo * replaced > 1e-30) ? (mPos + mNeg) / replaced : 0.0;
     }
-    auto limited = [&](int k) -> double {
-        const double dg = (double)tp[k];
-        if (!fires) return dg;
-        const double sel = (m0[k] + dg < 0.0) ? (double)gp[k] : dg;
-        return sc * sel + (sc - 1.0) * m0[k];
+    auto limited = [&](int p, int kv, double t, double m) -> double {
+        if (!fires) return t;
+        const double sel = (m + t < 0.0) ? (double)row.g(p, kv) : t;
+        return sc * sel + (sc - 1.0) * m;
     };
     const double nx = c.wnx[i], ny = c.wny[i], nz = c.wnz[i];
     const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = c.wuz[i], Tw = c.wt[i];
     const double RT2w = 2.0 * R * Tw;
-    const double norm = pow(AB_PI * RT2w, 1.5);
-    double m0s = 0, intM = 0, intP = 0;
-    NodeWalk w(lane, Nv);
-    for (int k = lane; k < nodes; k += kWave, w.step())
+    const double inorm = 1.0 / pow(AB_PI * RT2w, 1.5);
+    // the wall Maxwellian exp(-|v - U_w|^2 / 2RT_w) / norm: the v_z factor
+    // tabulated by the wave (lane l < Nv: node l), the (v_x, v_y) factor per pair
+    if (lane < Nv)
     {
-        const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
-        const double d = limited(k);
-        m0s += d;
-        const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
-        if (cn < 0.0) intM += cn * (m0[k] + d) * dv3;
-        else if (cn > 0.0)
-        {
-            const double Mw = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
-                                    + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm;
-            intP += cn * Mw * dv3;
-        }
+        const double t = vnode(vlo, dv, lane);
+        tab[lane] = exp(-((t - Uwz) * (t - Uwz)) / RT2w);
     }
+    wave_lds_sync();
+    // c_n = ((v_x - U_x) n_x + (v_y - U_y) n_y) + (v_z - U_z) n_z, summed in
+    // that order without contraction: the pair's part first, as written
+    double cnxy, mxy;
+    auto pair_wall = [&](int, int, double vx, double vy) {
+#pragma clang fp contract(off)
+        cnxy = (vx - Uwx) * nx + (vy - Uwy) * ny;
+        mxy = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)) / RT2w) * inorm;
+    };
+    auto cn_of = [&](double vz) -> double {
+#pragma clang fp contract(off)
+        return cnxy + (vz - Uwz) * nz;
+    };
+    double m0s = 0, intM = 0, intP = 0;
+    row.pairs(pair_wall, [&](int p, int kv, double t, double m) {
+        const double d = limited(p, kv, t, m);
+        m0s += d;
+        const double cn = cn_of(vnode(vlo, dv, kv));
+        if (cn < 0.0) intM += cn * (m + d) * dv3;
+        else if (cn > 0.0) intP += cn * (mxy * tab[kv]) * dv3;
+    }, nop);
     m0s = wave_sum(m0s); intM = wave_sum(intM); intP = wave_sum(intP);
     // intM is in L-units; the reference's fallback rho_w = 1 is in true units
     const double rhoWL = (intP > 1e-30) ? (-intM / intP) : 1.0 / Lo;
@@ -867,95 +930,75 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, int lane
     out[2] = Lo * rhoWL;
     const T Lt = T(Lo);
     double h0 = 0;
-    NodeWalk w2(lane, Nv);
-    for (int k = lane; k < nodes; k += kWave, w2.step())
-    {
-        const double vx = vnode(vlo, dv, w2.iv), vy = vnode(vlo, dv, w2.jv), vz = vnode(vlo, dv, w2.kv);
-        const double cn = wall_cn3(vx, vy, vz, Uwx, Uwy, Uwz, nx, ny, nz);
-        double dh;
-        if (cn > 0.0)
-            dh = rhoWL * exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)
-                               + (vz - Uwz) * (vz - Uwz)) / RT2w) / norm - m0[k];
-        else dh = limited(k);
+    row.pairs(pair_wall, [&](int p, int kv, double t, double m) {
+        const double cn = cn_of(vnode(vlo, dv, kv));
+        const double dh = (cn > 0.0) ? rhoWL * (mxy * tab[kv]) - m : limited(p, kv, t, m);
         const T dhT = T(dh);
-        gp[k] = Lt * dhT + T((Lo - 1.0) * m0[k]);
+        row.set_g(p, kv, Lt * dhT + T((Lo - 1.0) * m));
         h0 += (double)dhT;
-    }
+    }, nop);
     out[1] = Lo * (ref.mom[0] + wave_sum(h0) * dv3);
     out[6] = fires ? 1.0 : 0.0;
 }
 
-// One wave per particle. Interior particles: positivity, NAD, moments, the
-// BGK collision (continuous target, or the discrete Maxwellian when DM) and
-// the partial sums. Each pass streams the row (the re-reads hit L2; holding
-// the row in registers measured slower: the occupancy it costs is worth more
-// than the loads it saves). The raw moments are summed in the positivity
-// pass and redone only when the repair fires.
-template <typename T, bool DM>
-__global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *__restrict__ g,
-                                              T *__restrict__ gt, Ref ref,
-                                              const double *__restrict__ rho,
-                                              const double *__restrict__ scal, double dt,
-                                              double tau, double R, double *__restrict__ part)
+// An interior particle after positivity and NAD (L-units).
+struct Committed
 {
-    const int i = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (i >= c.N) return;
-    const double Lo = scal[S_LAZY];
-    T *gp = g + (size_t)i * L.S;
-    T *tp = gt + (size_t)i * L.S;
-    double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (c.bnd[i])
-    {
-        post_boundary<T>(c, i, lane, gp, tp, ref, Lo, R, out);
-        if (lane < kPart) part[(size_t)i * kPart + lane] = out[lane];
-        return;
-    }
-    const double *m0 = ref.m0;
-    const int nodes = c.nodes, Nv = c.Nv;
+    double sc2;               // NAD factor
+    double flag;              // limiter bits
+    double mr, px, py, pz, E; // committed moments: sc2 (rho, rho u, sum |v|^2 g)
+    double ux, uy, uz, T;     // velocity and temperature (ratios: unaffected by sc2)
+    bool okm;                 // mass positive: u and T defined
+};
+
+// Positivity (fallback to the pre-transport value, then the rescale to the
+// transported mass), the moments, and NAD against the neighbours' current
+// densities. The raw moments are summed in the positivity pass and redone
+// only when the repair fires (it then rewrites the transported row).
+template <typename T>
+__device__ __forceinline__ Committed commit_interior(const DevCloud &c, int i, const Row<T> &row,
+                                                     const Ref &ref, const double *rho, double Lo,
+                                                     double R)
+{
+    const int lane = row.lane;
     const double dv3 = c.dv * c.dv * c.dv;
-    const double vlo = c.vlo, dv = c.dv;
-
-    // the lane's nodes k = lane, lane + 64, ...
-    auto each = [&](auto &&f) {
-        NodeWalk w(lane, Nv);
-        for (int k = lane; k < nodes; k += kWave) { f(k, w); w.step(); }
-    };
-
-    // pass 1: positivity sums and the raw moments of M0 + d (L-units)
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};      // mPos mNeg mFb r0..r4
-    auto moments_of = [&](const NodeWalk &w, double d) {
-        const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
-        s[3] += d; s[4] += vx * d; s[5] += vy * d; s[6] += vz * d;
-        s[7] += (vx * vx + vy * vy + vz * vz) * d;
+    // per pair: A = sum d, K1 = sum kv d, K2 = sum kv^2 d, then the pair's
+    // share of the moments (v_z = vlo + kv dv)
+    const double vlo = c.vlo, dv = c.dv;
+    double A = 0, K1 = 0, K2 = 0, px = 0, py = 0;
+    auto begin = [&](int, int, double vx, double vy) { A = K1 = K2 = 0.0; px = vx; py = vy; };
+    auto end = [&]() {
+        s[3] += A; s[4] += px * A; s[5] += py * A; s[6] += vlo * A + dv * K1;
+        s[7] += (px * px + py * py + vlo * vlo) * A + dv * (2.0 * vlo * K1 + dv * K2);
     };
-    each([&](int k, const NodeWalk &w) {
-        const double m = m0[k], d = (double)tp[k], v = m + d;
+    auto accum = [&](int kv, double d) {
+        const double q = (double)kv;
+        A += d; K1 += q * d; K2 += (q * q) * d;
+    };
+    row.pairs(begin, [&](int p, int kv, double d, double m) {
+        const double v = m + d;
         if (v >= 0.0) s[0] += v;
-        else { s[1] += v; s[2] += m + (double)gp[k]; }
-        moments_of(w, d);
-    });
+        else { s[1] += v; s[2] += m + (double)row.g(p, kv); }
+        accum(kv, d);
+    }, end);
 #pragma unroll
     for (int q = 0; q < 8; ++q) s[q] = uniform(wave_sum(s[q]));
-    const bool fires = (Lo * s[1] < -1e-30 && (s[0] + s[1]) > 0);
-    double flag = 0.0;
-    if (fires)
+    Committed r;
+    r.flag = 0.0;
+    if (Lo * s[1] < -1e-30 && (s[0] + s[1]) > 0)
     {
-        // negatives fall back to the pre-transport value, then the rescale
-        // to the transported mass; the limited row replaces the transported
-        // one and its moments are summed again
-        flag = 1.0;
+        r.flag = 1.0;
         const double replaced = s[0] + s[2];
         const double sc = (Lo * replaced > 1e-30) ? (s[0] + s[1]) / replaced : 0.0;
 #pragma unroll
         for (int q = 3; q < 8; ++q) s[q] = 0.0;
-        each([&](int k, const NodeWalk &w) {
-            const double dg = (double)tp[k], m = m0[k];
-            const double sel = (m + dg < 0.0) ? (double)gp[k] : dg;
+        row.pairs(begin, [&](int p, int kv, double dg, double m) {
+            const double sel = (m + dg < 0.0) ? (double)row.g(p, kv) : dg;
             const T lim = T(sc * sel + (sc - 1.0) * m);
-            tp[k] = lim;
-            moments_of(w, (double)lim);
-        });
+            row.set_t(p, kv, lim);
+            accum(kv, (double)lim);
+        }, end);
 #pragma unroll
         for (int q = 3; q < 8; ++q) s[q] = uniform(wave_sum(s[q]));
     }
@@ -964,7 +1007,6 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
     const double pyL = ref.mom[2] + s[5] * dv3;
     const double pzL = ref.mom[3] + s[6] * dv3;
     const double EL = ref.mom[4] + s[7] * dv3;
-    // NAD against the neighbours' current densities (true units)
     const int nn = c.nneigh[i];
     double rmin = 1e300, rmax = -1e300;
     for (int k = lane; k < nn; k += kWave)
@@ -982,196 +1024,255 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, DmArgs dm, Lat L, T *_
         bool nad = false;
         if (rho_new > hi && rho_new > 1e-30) { sc2 = hi / rho_new; nad = true; }
         else if (rho_new < lo && rho_new > 1e-30) { sc2 = lo / rho_new; nad = true; }
-        if (nad) flag += 2.0;
+        if (nad) r.flag += 2.0;
     }
-    out[0] = sc2 * rho_new;                                    // committed mass
-    // moments of the committed field (L-units); u and T are ratios
-    const double mr = sc2 * rhoL;
-    double mux = 0, muy = 0, muz = 0, mT = ref.T;
-    const bool okm = Lo * mr > 1e-30;
-    if (okm)
+    r.sc2 = sc2;
+    r.mr = sc2 * rhoL;
+    r.px = sc2 * pxL; r.py = sc2 * pyL; r.pz = sc2 * pzL; r.E = sc2 * EL;
+    r.ux = 0.0; r.uy = 0.0; r.uz = 0.0; r.T = ref.T;
+    r.okm = Lo * r.mr > 1e-30;
+    if (r.okm)
     {
-        mux = pxL / rhoL; muy = pyL / rhoL; muz = pzL / rhoL;
-        const double u2 = mux * mux + muy * muy + muz * muz;
+        r.ux = pxL / rhoL; r.uy = pyL / rhoL; r.uz = pzL / rhoL;
+        const double u2 = r.ux * r.ux + r.uy * r.uy + r.uz * r.uz;
         const double t1 = (EL / rhoL - u2) / (3.0 * R);
-        mT = t1 > 1e-6 ? t1 : 1e-6;
+        r.T = t1 > 1e-6 ? t1 : 1e-6;
     }
-    // collision: h = coeff (g'' + lam M)  =>  dh = coeff (d'' + lam (M - M0))
-    // with M - M0 = M0 expm1(D),
-    // D = ln(rho/rho_ref) - 1.5 ln(T/T_ref) - |u|^2/2RT + |v|^2 (1/2RT_ref - 1/2RT) + v.u/RT
-    const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
-    double A0 = okm ? log(mr / ref.rho) - 1.5 * log(mT / ref.T)
-                          - (mux * mux + muy * muy + muz * muz) / (2.0 * R * mT)
-                    : 0.0;
-    double Bq = 1.0 / (2.0 * R * ref.T) - 1.0 / (2.0 * R * mT);
-    double Cx = mux / (R * mT), Cy = muy / (R * mT), Cz = muz / (R * mT);
-    const T cT = T(coeff), lamT = T(lam), Lt = T(Lo);
-    double h[5] = {0, 0, 0, 0, 0};
-    bool written = false;        // the discrete collision already wrote g (FP64 pass)
+    return r;
+}
+
+// Collision target relative to M0 as an exponent D(v) = A0 + Bq |v|^2 + C . v
+// (M = M0 exp(D)); the continuous Maxwellian of the committed moments:
+// D = ln(rho/rho_ref) - 1.5 ln(T/T_ref) - |u|^2/2RT + |v|^2 (1/2RT_ref - 1/2RT) + v.u/RT
+__device__ __forceinline__ void continuous_target(const Committed &m, const Ref &ref, double R,
+                                                  double D[5])
+{
+    D[0] = m.okm ? log(m.mr / ref.rho) - 1.5 * log(m.T / ref.T)
+                       - (m.ux * m.ux + m.uy * m.uy + m.uz * m.uz) / (2.0 * R * m.T)
+                 : 0.0;
+    D[1] = 1.0 / (2.0 * R * ref.T) - 1.0 / (2.0 * R * m.T);
+    D[2] = m.ux / (R * m.T); D[3] = m.uy / (R * m.T); D[4] = m.uz / (R * m.T);
+}
+
+// BGK collision of one row: h = coeff (g'' + lam M)  =>  dh = coeff (d'' + lam (M - M0)),
+// d'' = sc2 d + (sc2 - 1) M0 the committed deviation, M - M0 = M0 (exp(D) - 1)
+// in FP64 (target 0 when !okm). exp(D) = G_xy(v_x, v_y) G_z(v_z) with
+// G_xy = exp(A0 + Bq (v_x^2 + v_y^2) + Cx v_x + Cy v_y) once per pair and
+// G_z(t) = exp(Bq t^2 + Cz t) tabulated by the wave in LDS (lane l < Nv: node
+// l): no exponential per node. Writes the stored row (folding the pending
+// rescale Lo into it) and returns the grid moments of dh (wave-uniform).
+template <typename T>
+__device__ __forceinline__ void collide_row(const Row<T> &row, double *tab, double sc2, bool okm,
+                                            const double D[5], double lam, double Lo, double h[5])
+{
+    const int lane = row.lane;
+    const double coeff = 1.0 / (1.0 + lam);
+    if (lane < row.Nv)
+    {
+        const double t = vnode(row.vlo, row.dv, lane);
+        tab[lane] = exp(D[1] * t * t + D[4] * t);
+    }
+    wave_lds_sync();
+    const T Lt = T(Lo);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) h[q] = 0.0;
+    // per pair: A = sum dh, K1 = sum kv dh, K2 = sum kv^2 dh (v_z = vlo + kv dv)
+    const double vlo = row.vlo, dv = row.dv;
+    double A = 0, K1 = 0, K2 = 0, px = 0, py = 0, Exy = 0;
+    row.pairs([&](int, int, double vx, double vy) {
+        A = K1 = K2 = 0.0; px = vx; py = vy;
+        Exy = okm ? exp(D[0] + D[1] * (vx * vx + vy * vy) + D[2] * vx + D[3] * vy) : 0.0;
+    }, [&](int p, int kv, double t, double m) {
+        const double d2 = sc2 * t + (sc2 - 1.0) * m;
+        const double E = Exy * tab[kv];
+        const T dh = T(coeff * (d2 + lam * (m * (E - 1.0))));
+        row.set_g(p, kv, Lt * dh + T((Lo - 1.0) * m));
+        const double hd = (double)dh, q = (double)kv;
+        A += hd; K1 += q * hd; K2 += (q * q) * hd;
+    }, [&]() {
+        h[0] += A; h[1] += px * A; h[2] += py * A; h[3] += vlo * A + dv * K1;
+        h[4] += (px * px + py * py + vlo * vlo) * A + dv * (2.0 * vlo * K1 + dv * K2);
+    });
+#pragma unroll
+    for (int q = 0; q < 5; ++q) h[q] = wave_sum(h[q]);
+}
+
+// Per-particle state between the discrete-Maxwellian kernels, structure of
+// arrays (slot q of particle i at q N + i).
+enum PostSlot { P_SC2, P_OKM, P_UX, P_UY, P_UZ, P_T, P_MS, P_D = P_MS + 5, P_COUNT = P_D + 5 };
+
+// One wave per particle. Boundary particles: positivity and the diffuse wall.
+// Interior particles: positivity, NAD and moments; then (continuous
+// collision) the BGK collision in the same kernel, or (DM) the committed
+// state is handed to k_dm_solve and k_collide. Partials: [0] committed mass,
+// [1..5] moments after the collision, [6] limiter bits.
+template <typename T, bool DM>
+__global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__ g,
+                                              T *__restrict__ gt, Ref ref,
+                                              const double *__restrict__ rho,
+                                              const double *__restrict__ scal, double dt,
+                                              double tau, double R, double *__restrict__ part,
+                                              double *__restrict__ pst)
+{
+    // the wave's particle, in a scalar register (the row's buffer descriptors are uniform)
+    const int i = __builtin_amdgcn_readfirstlane(
+        (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (i >= c.N) return;
+    const double Lo = scal[S_LAZY];
+    const Row<T> row(g + (size_t)i * L.S, gt + (size_t)i * L.S, ref.m0, lane, c.Nv, c.vlo, c.dv);
+    // the wave's table of the v_z factor of a separable exponential
+    __shared__ double axis_tab[kBlock / kWave][kMaxNv];
+    double *tab = axis_tab[threadIdx.x / kWave];
+    double *pp = part + (size_t)i * kPart;
+    if (c.bnd[i])
+    {
+        double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
+        post_boundary<T>(c, i, row, ref, Lo, R, tab, out);
+        if (lane < kPart - 1) pp[lane] = out[lane];
+        return;
+    }
+    const Committed m = commit_interior<T>(c, i, row, ref, rho, Lo, R);
+    double D[5];
+    continuous_target(m, ref, R, D);
     if constexpr (DM)
     {
-        if (okm)
+        if (lane == 0)
         {
-            // discrete Maxwellian with the grid moments of the committed
-            // field (L-units): Newton in FP64
-            const double cs = dm.cs, ics = 1.0 / cs;
-            double ac[5], a[5], aref[5];
-            dm_guess(mr, mux, muy, muz, mT, R, cs, ac);
-            dm_guess(ref.rho, 0.0, 0.0, 0.0, ref.T, R, cs, aref);
-            const float feat[kMlpIn] = {float(mux * ics), float(muy * ics), float(muz * ics),
-                                        float(R * mT * ics * ics), float(dv * ics)};
-            float da[kMlpOut] = {0, 0, 0, 0, 0};
-            if (dm.mlp) mlp_wave_forward(dm.wb, dm.pf, feat, da);
+            const size_t N = c.N;
+            const double ms[5] = {m.mr, m.px, m.py, m.pz, m.E};
+            pst[P_SC2 * N + i] = m.sc2;
+            pst[P_OKM * N + i] = m.okm ? 1.0 : 0.0;
+            pst[P_UX * N + i] = m.ux; pst[P_UY * N + i] = m.uy; pst[P_UZ * N + i] = m.uz;
+            pst[P_T * N + i] = m.T;
 #pragma unroll
-            for (int q = 0; q < 5; ++q)
-            {
-                ac[q] = uniform(ac[q]);
-                aref[q] = uniform(aref[q]);
-                a[q] = uniform(ac[q] + (double)da[q]);
-            }
-            const double ms[5] = {mr, sc2 * pxL * ics, sc2 * pyL * ics, sc2 * pzL * ics,
-                                  sc2 * EL * ics * ics};
-            auto residual = [&](const double (&S)[5], double (&F)[5]) -> double {
-                double r = 0.0;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) { F[q] = S[q] - ms[q]; r = fmax(r, fabs(F[q])); }
-                return r / fabs(ms[0]);
-            };
-            // one grid pass: the residual and the Jacobian at a; a Newton
-            // update unless converged. Returns the residual before the update.
-            bool ok = true;
-            auto newton = [&]() -> double {
-                double S[15];
-#pragma unroll
-                for (int q = 0; q < 15; ++q) S[q] = 0.0;
-                each([&](int, const NodeWalk &w) {
-                    const double xx = vnode(vlo, dv, w.iv) * ics, yy = vnode(vlo, dv, w.jv) * ics;
-                    const double zz = vnode(vlo, dv, w.kv) * ics;
-                    const double e = exp(a[0] + a[1] * xx + a[2] * yy + a[3] * zz
-                                         + a[4] * (xx * xx + yy * yy + zz * zz)) * dv3;
-                    dm_accum(S, e, xx, yy, zz);
-                });
-#pragma unroll
-                for (int q = 0; q < 15; ++q) S[q] = uniform(wave_sum(S[q]));
-                double F[5];
-                const double r = residual(*reinterpret_cast<const double(*)[5]>(S), F);
-                if (r > dm.tol)
-                {
-                    double d[5];
-                    if (!dm_solve5(S, F, d)) ok = false;
-                    else
-#pragma unroll
-                        for (int q = 0; q < 5; ++q) a[q] = uniform(a[q] - d[q]);
-                }
-                return r;
-            };
-            // the collision towards M_d(a) = M0 exp((a - a_ref) . psi) in FP64,
-            // written to g, with the grid moments of M_d as the check
-            auto collide = [&]() -> double {
-                const double A1 = a[0] - aref[0], Bx = a[1] * ics, By = a[2] * ics, Bz = a[3] * ics;
-                const double B2 = (a[4] - aref[4]) * ics * ics;
-                double f[5] = {0, 0, 0, 0, 0};
-#pragma unroll
-                for (int q = 0; q < 5; ++q) h[q] = 0.0;
-                each([&](int k, const NodeWalk &w) {
-                    const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
-                    const double m = m0[k];
-                    const double v2 = vx * vx + vy * vy + vz * vz;
-                    const double d2 = sc2 * (double)tp[k] + (sc2 - 1.0) * m;
-                    const double Md = m * exp(A1 + B2 * v2 + Bx * vx + By * vy + Bz * vz);
-                    const T dh = T(coeff * (d2 + lam * (Md - m)));
-                    gp[k] = Lt * dh + T((Lo - 1.0) * m);
-                    const double hd = (double)dh;
-                    h[0] += hd; h[1] += vx * hd; h[2] += vy * hd; h[3] += vz * hd; h[4] += v2 * hd;
-                    f[0] += Md; f[1] += Md * vx; f[2] += Md * vy; f[3] += Md * vz; f[4] += Md * v2;
-                });
-#pragma unroll
-                for (int q = 0; q < 5; ++q) f[q] = uniform(wave_sum(f[q])) * dv3;
-                const double Fs[5] = {f[0], f[1] * ics, f[2] * ics, f[3] * ics, f[4] * ics * ics};
-                double F[5];
-                return residual(Fs, F);
-            };
-            int pass = 0;
-            double res = 0.0;
-            if (dm.mlp)
-            {
-                // warm start: one Newton update, verified by the collision
-                // pass itself (FP64); another update only when it fails
-                for (;;)
-                {
-                    ++pass;
-                    res = newton();
-                    if (!ok) break;
-                    ++pass;
-                    res = collide();
-                    written = true;
-                    if (res <= dm.tol || pass > 2 * dm.maxit) break;
-                }
-            }
-            else
-            {
-                // Newton until the residual passes, then the FP32 collision
-                for (;;)
-                {
-                    ++pass;
-                    res = newton();
-                    if (!ok || res <= dm.tol || pass > dm.maxit) break;
-                }
-            }
-            bool conv = ok && res <= dm.tol;
-#pragma unroll
-            for (int q = 0; q < 5; ++q) conv = conv && isfinite(a[q]);
-            if (!conv)
-            {
-#pragma unroll
-                for (int q = 0; q < 5; ++q) a[q] = ac[q];           // continuous fallback
-                written = false;
-            }
-            if (lane == 0)
-            {
-#pragma unroll
-                for (int q = 0; q < 5; ++q) dm.alpha[(size_t)i * 5 + q] = a[q];
-                dm.passes[i] = conv ? pass : 2 * dm.maxit + 2;
-                dm.res[i] = res;
-                if (dm.rec)
-                {
-                    float *row = dm.rec + (size_t)i * kMlpRow;
-#pragma unroll
-                    for (int q = 0; q < kMlpIn; ++q) row[q] = feat[q];
-#pragma unroll
-                    for (int q = 0; q < kMlpOut; ++q) row[kMlpIn + q] = float(a[q] - ac[q]);
-                    row[kMlpIn + kMlpOut] = conv ? 1.0f : 0.0f;
-                }
-            }
-            // D = (a - a_ref) . psi in unscaled velocities, a_ref the
-            // parameters of M0
-            A0 = a[0] - aref[0];
-            Cx = a[1] * ics; Cy = a[2] * ics; Cz = a[3] * ics;
-            Bq = (a[4] - aref[4]) * ics * ics;
+            for (int q = 0; q < 5; ++q) { pst[(P_MS + q) * N + i] = ms[q]; pst[(P_D + q) * N + i] = D[q]; }
+            pp[0] = Lo * m.mr;
+            pp[6] = m.flag;
         }
     }
-    if (!written)
+    else
+    {
+        double h[5];
+        collide_row<T>(row, tab, m.sc2, m.okm, D, dt / tau, Lo, h);
+        const double dv3 = c.dv * c.dv * c.dv;
+        double out = 0.0;
+        if (lane == 0) out = Lo * m.mr;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) if (lane == 1 + q) out = Lo * (ref.mom[q] + h[q] * dv3);
+        if (lane == 6) out = m.flag;
+        if (lane < kPart - 1) pp[lane] = out;
+    }
+}
+
+// Discrete-Maxwellian Newton, one thread per interior particle (64 particles
+// share each instruction of the 5x5 FP64 solves): the continuous Maxwellian's
+// parameters, plus the MLP's prediction of the difference when enabled, then
+// Newton on the separable grid sums until the relative residual passes. A
+// converged solution replaces the collision target exponent.
+__global__ __launch_bounds__(256) void k_dm_solve(int N, const unsigned char *__restrict__ bnd,
+                                                  DmArgs dm, Ref ref, double R, int Nv,
+                                                  double vlo, double dv,
+                                                  double *__restrict__ pst)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || bnd[i] || pst[P_OKM * (size_t)N + i] == 0.0) return;
+    const size_t n = N;
+    const double cs = dm.cs, ics = 1.0 / cs, dv3 = dv * dv * dv;
+    const double ux = pst[P_UX * n + i], uy = pst[P_UY * n + i], uz = pst[P_UZ * n + i];
+    const double T = pst[P_T * n + i];
+    double ms[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) ms[q] = pst[(P_MS + q) * n + i];
+    double ac[5], aref[5], a[5];
+    dm_guess(ms[0], ux, uy, uz, T, R, cs, ac);
+    dm_guess(ref.rho, 0.0, 0.0, 0.0, ref.T, R, cs, aref);
+    const float feat[kMlpIn] = {float(ux * ics), float(uy * ics), float(uz * ics),
+                                float(R * T * ics * ics), float(dv * ics)};
+    float da[kMlpOut] = {0, 0, 0, 0, 0};
+    if (dm.mlp) mlp_lane_forward(dm.wb, dm.pf, feat, da);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) a[q] = ac[q] + (double)da[q];
+    // the moments in the scaled basis psi = (1, v/cs, |v|^2/cs^2)
+    ms[1] *= ics; ms[2] *= ics; ms[3] *= ics; ms[4] *= ics * ics;
+    bool ok = true;
+    int pass = 0;
+    double res = 0.0;
+    for (;;)
+    {
+        ++pass;
+        double S[15], F[5];
+        dm_grid_sums(a, Nv, vlo, dv, ics, dv3, S);
+        res = 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) { F[q] = S[q] - ms[q]; res = fmax(res, fabs(F[q])); }
+        res /= fabs(ms[0]);
+        if (res <= dm.tol || pass > dm.maxit) break;
+        double d[5];
+        if (!dm_solve5(S, F, d)) { ok = false; break; }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) a[q] -= d[q];
+    }
+    bool conv = ok && res <= dm.tol;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) conv = conv && isfinite(a[q]);
+    if (!conv)
     {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) h[q] = 0.0;
-        each([&](int k, const NodeWalk &w) {
-            const double vx = vnode(vlo, dv, w.iv), vy = vnode(vlo, dv, w.jv), vz = vnode(vlo, dv, w.kv);
-            const double m = m0[k];
-            const double v2 = vx * vx + vy * vy + vz * vz;
-            const double d2 = sc2 * (double)tp[k] + (sc2 - 1.0) * m;
-            const double D = A0 + Bq * v2 + Cx * vx + Cy * vy + Cz * vz;
-            const T mm = okm ? T(m) * expm1(T(D)) : T(-m);
-            const T dh = cT * (T(d2) + lamT * mm);
-            // fold the pending rescale of this step's input into the stored value
-            gp[k] = Lt * dh + T((Lo - 1.0) * m);
-            const double hd = (double)dh;
-            h[0] += hd; h[1] += vx * hd; h[2] += vy * hd; h[3] += vz * hd; h[4] += v2 * hd;
-        });
+        for (int q = 0; q < 5; ++q) a[q] = ac[q];               // continuous target kept
+    }
+    else
+    {
+        // D = (a - a_ref) . psi in unscaled velocities, a_ref the parameters of M0
+        pst[(P_D + 0) * n + i] = a[0] - aref[0];
+        pst[(P_D + 1) * n + i] = (a[4] - aref[4]) * ics * ics;
+        pst[(P_D + 2) * n + i] = a[1] * ics;
+        pst[(P_D + 3) * n + i] = a[2] * ics;
+        pst[(P_D + 4) * n + i] = a[3] * ics;
     }
 #pragma unroll
-    for (int q = 0; q < 5; ++q) out[1 + q] = Lo * (ref.mom[q] + wave_sum(h[q]) * dv3);
-    out[6] = flag;
-    if (lane < kPart) part[(size_t)i * kPart + lane] = out[lane];
+    for (int q = 0; q < 5; ++q) dm.alpha[(size_t)i * 5 + q] = a[q];
+    dm.passes[i] = conv ? pass : 2 * dm.maxit + 2;
+    dm.res[i] = res;
+    if (dm.rec)
+    {
+        float *rw = dm.rec + (size_t)i * kMlpRow;
+#pragma unroll
+        for (int q = 0; q < kMlpIn; ++q) rw[q] = feat[q];
+#pragma unroll
+        for (int q = 0; q < kMlpOut; ++q) rw[kMlpIn + q] = float(a[q] - ac[q]);
+        rw[kMlpIn + kMlpOut] = conv ? 1.0f : 0.0f;
+    }
+}
+
+// The collision of the interior particles towards the target k_dm_solve left
+// (one wave per particle); partials [1..5].
+template <typename T>
+__global__ __launch_bounds__(256) void k_collide(DevCloud c, Lat L, T *__restrict__ g,
+                                                 const T *__restrict__ gt, Ref ref,
+                                                 const double *__restrict__ scal, double dt,
+                                                 double tau, const double *__restrict__ pst,
+                                                 double *__restrict__ part)
+{
+    const int i = __builtin_amdgcn_readfirstlane(
+        (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (i >= c.N || c.bnd[i]) return;
+    const double Lo = scal[S_LAZY];
+    const size_t N = c.N;
+    const Row<T> row(g + (size_t)i * L.S, const_cast<T *>(gt) + (size_t)i * L.S, ref.m0, lane,
+                     c.Nv, c.vlo, c.dv);
+    __shared__ double axis_tab[kBlock / kWave][kMaxNv];
+    double *tab = axis_tab[threadIdx.x / kWave];
+    double D[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) D[q] = pst[(P_D + q) * N + i];
+    double h[5];
+    collide_row<T>(row, tab, pst[P_SC2 * N + i], pst[P_OKM * N + i] != 0.0, D, dt / tau, Lo, h);
+    const double dv3 = c.dv * c.dv * c.dv;
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        if (lane == q) part[(size_t)i * kPart + 1 + q] = Lo * (ref.mom[q] + h[q] * dv3);
 }
 
 // Global sums of the partials -> the two rescale factors of cudaStep
@@ -1405,10 +1506,10 @@ int apply_launch(ab_ctx *ctx, const Lat &L, int shape)
 void table_free(ab_ctx *ctx)
 {
     void *ptrs[] = {ctx->gs, ctx->gts, ctx->coef, ctx->cls, ctx->lat, ctx->part, ctx->bad,
-                    ctx->m0, ctx->red};
+                    ctx->m0, ctx->red, ctx->pst};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     ctx->gs = ctx->gts = ctx->coef = nullptr;
-    ctx->m0 = ctx->red = nullptr;
+    ctx->m0 = ctx->red = ctx->pst = nullptr;
     ctx->cls = ctx->lat = ctx->bad = nullptr;
     ctx->part = nullptr;
 }
@@ -1497,7 +1598,9 @@ int table_setup(ab_ctx *ctx)
               && hipMalloc((void **)&ctx->cls, sizeof(int) * kNcls) == hipSuccess
               && hipMalloc((void **)&ctx->part, sizeof(double) * kPart * (size_t)N) == hipSuccess
               && hipMalloc((void **)&ctx->m0, sizeof(double) * ctx->S) == hipSuccess
-              && hipMalloc((void **)&ctx->red, sizeof(double) * 2 * kRedBlocks) == hipSuccess;
+              && hipMalloc((void **)&ctx->red, sizeof(double) * 2 * kRedBlocks) == hipSuccess
+              && (ctx->cfg.collision != AB_COLLISION_DISCRETE
+                  || hipMalloc((void **)&ctx->pst, sizeof(double) * P_COUNT * (size_t)N) == hipSuccess);
         if (!a) { table_free(ctx); ctx->err = "table setup: device allocation failed"; return AB_ERR_NOMEM; }
         if (hipMemsetAsync(ctx->gts, 0, gsz, s) != hipSuccess)
         { ctx->err = "table setup: memset failed"; return AB_ERR_HIP; }
@@ -1633,18 +1736,25 @@ int table_step(ab_ctx *ctx, double dt)
     toc(ST_TRANSPORT);
     tic();
     const DmArgs dm = dm_args(ctx);
-    if (is_fp32(ctx))
-    {
-        auto k = dm.on ? k_post<float, true> : k_post<float, false>;
-        k<<<nbp, 256, 0, s>>>(c, dm, L, (float *)ctx->gs, (float *)ctx->gts, ref, ctx->rho,
-                              ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R, ctx->part);
-    }
-    else
-    {
-        auto k = dm.on ? k_post<double, true> : k_post<double, false>;
-        k<<<nbp, 256, 0, s>>>(c, dm, L, (double *)ctx->gs, (double *)ctx->gts, ref, ctx->rho,
-                              ctx->scal, dt, ctx->cfg.tau, ctx->cfg.R, ctx->part);
-    }
+    const int nbs = (ctx->N + 255) / 256;
+    auto post = [&](auto *gs) {
+        using T = std::remove_pointer_t<decltype(gs)>;
+        T *gts = (T *)ctx->gts;
+        if (!dm.on)
+        {
+            k_post<T, false><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->rho, ctx->scal, dt,
+                                                 ctx->cfg.tau, ctx->cfg.R, ctx->part, nullptr);
+            return;
+        }
+        k_post<T, true><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->rho, ctx->scal, dt,
+                                            ctx->cfg.tau, ctx->cfg.R, ctx->part, ctx->pst);
+        k_dm_solve<<<nbs, 256, 0, s>>>(ctx->N, c.bnd, dm, ref, ctx->cfg.R, ctx->Nv, ctx->vlo,
+                                       ctx->dv, ctx->pst);
+        k_collide<T><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->scal, dt, ctx->cfg.tau,
+                                         ctx->pst, ctx->part);
+    };
+    if (is_fp32(ctx)) post((float *)ctx->gs);
+    else post((double *)ctx->gs);
     toc(ST_WALL_COLLIDE);
     dm_step_done(ctx);
     tic();

@@ -30,12 +30,17 @@ def _say(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("nx,nv,order", [(8, 6, 2), (7, 8, 1)])
-def test_dm_fp64_step_matches_oracle(nx, nv, order):
-    _say(f"[dm] fp64 step vs oracle cavity3d nx={nx} nv={nv} order={order}")
+@pytest.mark.parametrize("nx,nv,order,prec", [(8, 6, 2, A.PREC_FP64), (7, 8, 1, A.PREC_FP64),
+                                              (8, 6, 2, A.PREC_FP64_TABLE),
+                                              (9, 7, 1, A.PREC_FP64_TABLE)])
+def test_dm_fp64_step_matches_oracle(nx, nv, order, prec):
+    """The direct FP64 path sums the Newton Jacobian node by node; the FP64
+    table path from the separable 1-D sums and the collision from per-axis
+    factor tables. Both against the oracle's node-by-node FP64."""
+    _say(f"[dm] fp64 step vs oracle cavity3d nx={nx} nv={nv} order={order} precision={prec}")
     case, cl, ocl, neigh, nn = setup("cavity3d", nx, nv, order=order)
     dt = 0.5 * (1e-6 / (nx - 1)) / 1303.0
-    sol = A.Solver(cl, precision=A.PREC_FP64, collision=DISC)
+    sol = A.Solver(cl, precision=prec, collision=DISC)
     st = O.State(ocl, cl.g, cl.mom)
     for _ in range(3):
         stats = sol.step(dt, stats=True)
