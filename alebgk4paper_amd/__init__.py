@@ -114,7 +114,7 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_body_loads", "ab_set_wall_state", "ab_synchronize", "ab_profile",
                "ab_profile_read", "ab_stage_name", "ab_dm_download", "ab_dm_record",
                "ab_dm_rows", "ab_mlp_init", "ab_mlp_train", "ab_mlp_enable", "ab_mlp_get",
-               "ab_mlp_set", "ab_mlp_forward", "ab_mlp_loss_grad", "ab_graph_create",
+               "ab_mlp_set", "ab_mlp_forward", "ab_mlp_infer", "ab_mlp_loss_grad", "ab_graph_create",
                "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy", "ab_autotune",
                "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels"]
 MLP_TRAINABLE = 4869
@@ -409,6 +409,15 @@ class Solver:
         y = np.zeros((x.shape[0], 5), np.float32)
         self._check(self._lib.ab_mlp_forward(self._ctx, _ptr(x, _fp), x.shape[0], _ptr(y, _fp)),
                     "ab_mlp_forward")
+        return y
+
+    def mlp_infer(self, x):
+        """The warm start's inference as the step runs it (BF16 weights on
+        MFMA, BF16 layer-1 activations, FP32 accumulation)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], 5), np.float32)
+        self._check(self._lib.ab_mlp_infer(self._ctx, _ptr(x, _fp), x.shape[0], _ptr(y, _fp)),
+                    "ab_mlp_infer")
         return y
 
     def mlp_loss_grad(self, x, t, w):

@@ -187,40 +187,111 @@ __device__ __forceinline__ void mlp_wave_forward(const unsigned short *__restric
     }
 }
 
-// MLP forward for one sample per lane (a wave evaluates 64 samples; the
-// weights are read at wave-uniform addresses, through scalar loads), BF16
-// weights, FP32 biases, normalisation and accumulation. De-normalised outputs.
-__device__ __forceinline__ void mlp_lane_forward(const unsigned short *__restrict__ wb,
-                                                 const float *__restrict__ pf,
-                                                 const float x[kMlpIn], float out[kMlpOut])
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// tanh(x) = 1 - 2 / (1 + e^{2x}) with the hardware base-2 exponential and
+// reciprocal (about 1e-7 absolute; the warm start needs no more). Saturates
+// to +-1 for large |x| (e^{2x} -> inf or 0).
+__device__ __forceinline__ float tanh_fast(float x)
 {
-    float xn[kMlpIn];
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);     // 2 log2(e)
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+}
+
+// MLP forward for the wave's 64 samples (lane l holds sample l; every lane
+// of the wave must call it), BF16 weights on MFMA with FP32 accumulation.
+// Layer 2 is computed transposed, Z2^T (n x p) = W2 (n x k) . H1^T (k x p),
+// with v_mfma_f32_32x32x16_bf16 (A = W2 rows straight from memory, B = the
+// layer-1 activations rounded to BF16, produced by each lane in the B layout:
+// column p = l % 32 (+ 32), k = 16 ks + 8 (l / 32) + e). The result layout
+// leaves each lane half of the 64 hidden units of its two columns, so layer 3
+// is a lane-local dot product plus one exchange with lane l ^ 32.
+__device__ __forceinline__ void mlp_wave_mfma(const unsigned short *__restrict__ wb,
+                                              const float *__restrict__ pf,
+                                              const float x[kMlpIn], float out[kMlpOut])
+{
+    const int lane = threadIdx.x & (kWave - 1), col = lane & 31, half = lane >> 5;
+    // normalised inputs of the lane's columns p = col (pt 0) and col + 32 (pt 1)
+    float xn[2][kMlpIn];
 #pragma unroll
-    for (int i = 0; i < kMlpIn; ++i) xn[i] = (x[i] - pf[kMlpXm + i]) / pf[kMlpXs + i];
-    float h1[kMlpHid];
-#pragma unroll
-    for (int j = 0; j < kMlpHid; ++j)
+    for (int i = 0; i < kMlpIn; ++i)
     {
-        float z = pf[kMlpB1 + j];
-#pragma unroll
-        for (int i = 0; i < kMlpIn; ++i) z += bf16_to_float(wb[kMlpW1 + j * kMlpIn + i]) * xn[i];
-        h1[j] = tanhf(z);
+        const float v = (x[i] - pf[kMlpXm + i]) / pf[kMlpXs + i];
+        const float o = __shfl_xor(v, 32, kWave);
+        xn[0][i] = half ? o : v;
+        xn[1][i] = half ? v : o;
     }
-    float y[kMlpOut];
+    // layer 1: H1^T in the B layout, rounded to BF16
+    bf16x8 B[2][4];
 #pragma unroll
-    for (int o = 0; o < kMlpOut; ++o) y[o] = pf[kMlpB3 + o];
-    for (int j = 0; j < kMlpHid; ++j)
-    {
-        float z = pf[kMlpB2 + j];
-        const unsigned short *w2 = wb + kMlpW2 + j * kMlpHid;
+    for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-        for (int k = 0; k < kMlpHid; ++k) z += bf16_to_float(w2[k]) * h1[k];
-        const float h2 = tanhf(z);
+        for (int e = 0; e < 8; ++e)
+        {
+            const int k = 16 * ks + 8 * half + e;
+            float w[kMlpIn];
 #pragma unroll
-        for (int o = 0; o < kMlpOut; ++o) y[o] += bf16_to_float(wb[kMlpW3 + o * kMlpHid + j]) * h2;
-    }
+            for (int i = 0; i < kMlpIn; ++i) w[i] = bf16_to_float(wb[kMlpW1 + k * kMlpIn + i]);
+            const float b = pf[kMlpB1 + k];
 #pragma unroll
-    for (int o = 0; o < kMlpOut; ++o) out[o] = y[o] * pf[kMlpYs + o] + pf[kMlpYm + o];
+            for (int pt = 0; pt < 2; ++pt)
+            {
+                float z = b;
+#pragma unroll
+                for (int i = 0; i < kMlpIn; ++i) z += w[i] * xn[pt][i];
+                B[pt][ks][e] = (__bf16)tanh_fast(z);
+            }
+        }
+    // layer 2: acc[nt][pt] = W2 rows 32 nt .. 32 nt + 31 times the columns of pt
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt) acc[nt][pt] = f32x16{};
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+        {
+            const bf16x8 A = *reinterpret_cast<const bf16x8 *>(
+                wb + kMlpW2 + (32 * nt + col) * kMlpHid + 16 * ks + 8 * half);
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+                acc[nt][pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B[pt][ks], acc[nt][pt], 0, 0, 0);
+        }
+    // h2 = tanh(z2 + b2); the lane holds rows n = 32 nt + 8 (r / 4) + 4 half + r % 4
+    float y[2][kMlpOut];
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o) y[pt][o] = 0.0f;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+        {
+            const int n = 32 * nt + 8 * (r >> 2) + 4 * half + (r & 3);
+            const float b = pf[kMlpB2 + n];
+            float w[kMlpOut];
+#pragma unroll
+            for (int o = 0; o < kMlpOut; ++o) w[o] = bf16_to_float(wb[kMlpW3 + o * kMlpHid + n]);
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt)
+            {
+                const float h = tanh_fast(acc[nt][pt][r] + b);
+#pragma unroll
+                for (int o = 0; o < kMlpOut; ++o) y[pt][o] += w[o] * h;
+            }
+        }
+    // the other 32 hidden units of each column are in lane l ^ 32
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o) y[pt][o] += __shfl_xor(y[pt][o], 32, kWave);
+#pragma unroll
+    for (int o = 0; o < kMlpOut; ++o)
+        out[o] = ((half ? y[1][o] : y[0][o]) + pf[kMlpB3 + o]) * pf[kMlpYs + o] + pf[kMlpYm + o];
 }
 
 }  // namespace ab

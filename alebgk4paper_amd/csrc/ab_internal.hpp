@@ -114,6 +114,8 @@ struct ab_ctx
     int ncls = 0;
     double coef_dt = -1.0;         // dt the tables were built for
     double *m0 = nullptr;          // equilibrium the stored deviation is taken from, S
+    double *m0x = nullptr;         // its 1-D factors (kMaxNv); M0 = ((m0c e_i) e_j) e_k
+    double m0c = 0.0;
     double m0mom[5] = {0};         // its moments: sum M0 dv^3, sum v M0 dv^3, sum |v|^2 M0 dv^3
     double m0rho = 1.0, m0T = 270.0;
     double *part = nullptr;        // per-particle partials of the table pipeline

@@ -255,6 +255,24 @@ __global__ __launch_bounds__(128) void k_mlp_forward(const float *__restrict__ P
     for (int o = 0; o < kMlpOut; ++o) y[(size_t)b * kMlpOut + o] = out[o] * P[kMlpYs + o] + P[kMlpYm + o];
 }
 
+// The warm start's inference as the solver runs it: BF16 weights on MFMA,
+// FP32 accumulation, one wave per 64 rows (rows past n compute on zeros and
+// are not written).
+__global__ __launch_bounds__(256) void k_mlp_infer(const unsigned short *__restrict__ wb,
+                                                   const float *__restrict__ pf,
+                                                   const float *__restrict__ x, int n,
+                                                   float *__restrict__ y)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    float xi[kMlpIn], out[kMlpOut];
+#pragma unroll
+    for (int i = 0; i < kMlpIn; ++i) xi[i] = b < n ? x[(size_t)b * kMlpIn + i] : 0.0f;
+    mlp_wave_mfma(wb, pf, xi, out);
+    if (b < n)
+#pragma unroll
+        for (int o = 0; o < kMlpOut; ++o) y[(size_t)b * kMlpOut + o] = out[o];
+}
+
 #define MLPCHK(ctx, expr)                                                        \
     do {                                                                         \
         hipError_t e_ = (expr);                                                  \
@@ -471,6 +489,22 @@ int ab_mlp_forward(ab_ctx *c, const float *x, int n, float *y)
     float *ydev = c->mlp_rows + (size_t)n * kMlpRow;
     MLPCHK(c, hipMemcpyAsync(c->mlp_rows, rows.data(), sizeof(float) * rows.size(), hipMemcpyHostToDevice, s));
     k_mlp_forward<<<(n + 127) / 128, 128, 0, s>>>(c->mlp_p, c->mlp_rows, n, ydev);
+    MLPCHK(c, hipMemcpyAsync(y, ydev, sizeof(float) * (size_t)n * kMlpOut, hipMemcpyDeviceToHost, s));
+    MLPCHK(c, hipStreamSynchronize(s));
+    return AB_OK;
+}
+
+int ab_mlp_infer(ab_ctx *c, const float *x, int n, float *y)
+{
+    if (!c || !x || !y || n < 0) return AB_ERR_ARG;
+    if (!c->mlp_ready) { c->err = "MLP not initialised"; return AB_ERR_STATE; }
+    if (n == 0) return AB_OK;
+    int rc;
+    if ((rc = mlp_rows(c, n))) return rc;
+    hipStream_t s = c->stream;
+    float *ydev = c->mlp_rows + (size_t)n * kMlpRow;
+    MLPCHK(c, hipMemcpyAsync(c->mlp_rows, x, sizeof(float) * (size_t)n * kMlpIn, hipMemcpyHostToDevice, s));
+    k_mlp_infer<<<(n + 255) / 256, 256, 0, s>>>(c->mlp_wb, c->mlp_p, c->mlp_rows, n, ydev);
     MLPCHK(c, hipMemcpyAsync(y, ydev, sizeof(float) * (size_t)n * kMlpOut, hipMemcpyDeviceToHost, s));
     MLPCHK(c, hipStreamSynchronize(s));
     return AB_OK;

@@ -799,6 +799,8 @@ constexpr int kPart = 8;
 struct Ref
 {
     const double *m0;       // M0 per node, S entries
+    const double *m0x;      // its 1-D factors e_l = exp(-v_l^2 / 2RT), Nv entries:
+    double m0c;             //   M0(iv, jv, kv) = ((m0c e_iv) e_jv) e_kv exactly
     double mom[5];          // sum M0 dv^3, sum v_a M0 dv^3, sum |v|^2 M0 dv^3
     double rho, T;          // reference state of M0
 };
@@ -814,17 +816,27 @@ struct Ref
 // arithmetic). A pair's loads are issued in batches of kB; a batch past the
 // last plane reloads the last plane (in bounds, no branch around the loads)
 // and skips the work.
+// M0 of a node is the product of the pair's factor (m0c e_iv) e_jv and the
+// plane's e_kv (from the wave's LDS copy of the factors, a broadcast read),
+// rounded as the table of M0 was built.
 //   begin(iv, jv, vx, vy)   node(p, kv, t, m0)   end()
 template <typename T>
 struct Row
 {
     __amdgpu_buffer_rsrc_t rg;    // stored row (pre-transport values; written last)
     __amdgpu_buffer_rsrc_t rt;    // transported row
-    __amdgpu_buffer_rsrc_t rm;    // M0
+    const double *m0x;            // 1-D factors of M0 (global)
+    const double *ez;             // the same in the wave's LDS table (Nv entries)
+    double m0c;
     int lane, Nv;
     double vlo, dv;
-    __device__ Row(T *g, T *t, const double *m0, int ln, int nv, double lo, double d)
-        : rg(brsrc(g)), rt(brsrc(t)), rm(brsrc(m0)), lane(ln), Nv(nv), vlo(lo), dv(d) {}
+    __device__ Row(T *g, T *t, const Ref &ref, double *lds, int ln, int nv, double lo, double d)
+        : rg(brsrc(g)), rt(brsrc(t)), m0x(ref.m0x), ez(lds), m0c(ref.m0c), lane(ln), Nv(nv),
+          vlo(lo), dv(d)
+    {
+        if (ln < nv) lds[ln] = ref.m0x[ln];
+        wave_lds_sync();
+    }
     __device__ __forceinline__ int soff(int kv, int bytes) const { return kv * Nv * Nv * bytes; }
     __device__ __forceinline__ T g(int p, int kv) const
     { return bload_so<T>(rg, p * (int)sizeof(T), soff(kv, sizeof(T))); }
@@ -837,24 +849,27 @@ struct Row
     {
         constexpr int kB = 5;
         const int P = Nv * Nv;
-        for (int p = lane; p < P; p += kWave)
+        // a uniform loop over the chunks of 64 pairs (the plane indices and the
+        // M0 factor loads stay scalar); lanes past the last pair sit out
+        for (int p0 = 0; p0 < P; p0 += kWave)
         {
+            const int p = p0 + lane;
+            if (p >= P) break;
             const int jv = p / Nv, iv = p - jv * Nv;
             begin(iv, jv, vnode(vlo, dv, iv), vnode(vlo, dv, jv));
+            const double mxy = (m0c * m0x[iv]) * m0x[jv];
             for (int k0 = 0; k0 < Nv; k0 += kB)
             {
                 T tv[kB];
-                double mv[kB];
 #pragma unroll
                 for (int j = 0; j < kB; ++j)
                 {
                     const int kv = min(k0 + j, Nv - 1);
                     tv[j] = bload_so<T>(rt, p * (int)sizeof(T), soff(kv, sizeof(T)));
-                    mv[j] = bload_so<double>(rm, p * 8, soff(kv, 8));
                 }
 #pragma unroll
                 for (int j = 0; j < kB; ++j)
-                    if (k0 + j < Nv) node(p, k0 + j, (double)tv[j], mv[j]);
+                    if (k0 + j < Nv) node(p, k0 + j, (double)tv[j], mxy * ez[k0 + j]);
             }
             end();
         }
@@ -1114,16 +1129,20 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
                                               double tau, double R, double *__restrict__ part,
                                               double *__restrict__ pst)
 {
-    // the wave's particle, in a scalar register (the row's buffer descriptors are uniform)
-    const int i = __builtin_amdgcn_readfirstlane(
-        (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave);
+    // the wave's particle, in a scalar register (the row's buffer descriptors
+    // are uniform). Consecutive blocks go to different XCDs (no remap): a
+    // wave reads only its own row, and the boundary faces, whose waves cost
+    // several interior ones, are then spread over all eight XCDs.
+    const int i = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
     const int lane = threadIdx.x & (kWave - 1);
     if (i >= c.N) return;
     const double Lo = scal[S_LAZY];
-    const Row<T> row(g + (size_t)i * L.S, gt + (size_t)i * L.S, ref.m0, lane, c.Nv, c.vlo, c.dv);
-    // the wave's table of the v_z factor of a separable exponential
-    __shared__ double axis_tab[kBlock / kWave][kMaxNv];
-    double *tab = axis_tab[threadIdx.x / kWave];
+    // the wave's LDS tables: M0's 1-D factors, then the v_z factor of a
+    // separable exponential
+    __shared__ double axis_tab[kBlock / kWave][2 * kMaxNv];
+    double *tab = axis_tab[threadIdx.x / kWave] + kMaxNv;
+    const Row<T> row(g + (size_t)i * L.S, gt + (size_t)i * L.S, ref, axis_tab[threadIdx.x / kWave],
+                     lane, c.Nv, c.vlo, c.dv);
     double *pp = part + (size_t)i * kPart;
     if (c.bnd[i])
     {
@@ -1167,30 +1186,36 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
 
 // Discrete-Maxwellian Newton, one thread per interior particle (64 particles
 // share each instruction of the 5x5 FP64 solves): the continuous Maxwellian's
-// parameters, plus the MLP's prediction of the difference when enabled, then
-// Newton on the separable grid sums until the relative residual passes. A
-// converged solution replaces the collision target exponent.
+// parameters, plus the MLP's prediction of the difference when enabled (the
+// wave's 64 particles on MFMA; every lane takes part), then Newton on the
+// separable grid sums until the relative residual passes. A converged
+// solution replaces the collision target exponent.
 __global__ __launch_bounds__(256) void k_dm_solve(int N, const unsigned char *__restrict__ bnd,
                                                   DmArgs dm, Ref ref, double R, int Nv,
                                                   double vlo, double dv,
                                                   double *__restrict__ pst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || bnd[i] || pst[P_OKM * (size_t)N + i] == 0.0) return;
     const size_t n = N;
+    const bool act = i < N && !bnd[i] && pst[P_OKM * n + i] != 0.0;
     const double cs = dm.cs, ics = 1.0 / cs, dv3 = dv * dv * dv;
-    const double ux = pst[P_UX * n + i], uy = pst[P_UY * n + i], uz = pst[P_UZ * n + i];
-    const double T = pst[P_T * n + i];
-    double ms[5];
+    double ux = 0.0, uy = 0.0, uz = 0.0, T = ref.T;
+    double ms[5] = {ref.rho, 0.0, 0.0, 0.0, 0.0};
+    if (act)
+    {
+        ux = pst[P_UX * n + i]; uy = pst[P_UY * n + i]; uz = pst[P_UZ * n + i];
+        T = pst[P_T * n + i];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) ms[q] = pst[(P_MS + q) * n + i];
-    double ac[5], aref[5], a[5];
-    dm_guess(ms[0], ux, uy, uz, T, R, cs, ac);
-    dm_guess(ref.rho, 0.0, 0.0, 0.0, ref.T, R, cs, aref);
+        for (int q = 0; q < 5; ++q) ms[q] = pst[(P_MS + q) * n + i];
+    }
     const float feat[kMlpIn] = {float(ux * ics), float(uy * ics), float(uz * ics),
                                 float(R * T * ics * ics), float(dv * ics)};
     float da[kMlpOut] = {0, 0, 0, 0, 0};
-    if (dm.mlp) mlp_lane_forward(dm.wb, dm.pf, feat, da);
+    if (dm.mlp) mlp_wave_mfma(dm.wb, dm.pf, feat, da);
+    if (!act) return;
+    double ac[5], aref[5], a[5];
+    dm_guess(ms[0], ux, uy, uz, T, R, cs, ac);
+    dm_guess(ref.rho, 0.0, 0.0, 0.0, ref.T, R, cs, aref);
 #pragma unroll
     for (int q = 0; q < 5; ++q) a[q] = ac[q] + (double)da[q];
     // the moments in the scaled basis psi = (1, v/cs, |v|^2/cs^2)
@@ -1260,10 +1285,10 @@ __global__ __launch_bounds__(256) void k_collide(DevCloud c, Lat L, T *__restric
     if (i >= c.N || c.bnd[i]) return;
     const double Lo = scal[S_LAZY];
     const size_t N = c.N;
-    const Row<T> row(g + (size_t)i * L.S, const_cast<T *>(gt) + (size_t)i * L.S, ref.m0, lane,
-                     c.Nv, c.vlo, c.dv);
-    __shared__ double axis_tab[kBlock / kWave][kMaxNv];
-    double *tab = axis_tab[threadIdx.x / kWave];
+    __shared__ double axis_tab[kBlock / kWave][2 * kMaxNv];
+    double *tab = axis_tab[threadIdx.x / kWave] + kMaxNv;
+    const Row<T> row(g + (size_t)i * L.S, const_cast<T *>(gt) + (size_t)i * L.S, ref,
+                     axis_tab[threadIdx.x / kWave], lane, c.Nv, c.vlo, c.dv);
     double D[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) D[q] = pst[(P_D + q) * N + i];
@@ -1506,10 +1531,10 @@ int apply_launch(ab_ctx *ctx, const Lat &L, int shape)
 void table_free(ab_ctx *ctx)
 {
     void *ptrs[] = {ctx->gs, ctx->gts, ctx->coef, ctx->cls, ctx->lat, ctx->part, ctx->bad,
-                    ctx->m0, ctx->red, ctx->pst};
+                    ctx->m0, ctx->m0x, ctx->red, ctx->pst};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     ctx->gs = ctx->gts = ctx->coef = nullptr;
-    ctx->m0 = ctx->red = ctx->pst = nullptr;
+    ctx->m0 = ctx->m0x = ctx->red = ctx->pst = nullptr;
     ctx->cls = ctx->lat = ctx->bad = nullptr;
     ctx->part = nullptr;
 }
@@ -1598,6 +1623,7 @@ int table_setup(ab_ctx *ctx)
               && hipMalloc((void **)&ctx->cls, sizeof(int) * kNcls) == hipSuccess
               && hipMalloc((void **)&ctx->part, sizeof(double) * kPart * (size_t)N) == hipSuccess
               && hipMalloc((void **)&ctx->m0, sizeof(double) * ctx->S) == hipSuccess
+              && hipMalloc((void **)&ctx->m0x, sizeof(double) * kMaxNv) == hipSuccess
               && hipMalloc((void **)&ctx->red, sizeof(double) * 2 * kRedBlocks) == hipSuccess
               && (ctx->cfg.collision != AB_COLLISION_DISCRETE
                   || hipMalloc((void **)&ctx->pst, sizeof(double) * P_COUNT * (size_t)N) == hipSuccess);
@@ -1617,14 +1643,23 @@ int table_setup(ab_ctx *ctx)
         const double R = cfg.R, twoRT = 2.0 * R * ctx->m0T;
         const double norm = std::pow(AB_PI * twoRT, 1.5);
         const double dv3 = ctx->dv * ctx->dv * ctx->dv;
-        std::vector<double> m0(ctx->S, 0.0);
+        // M0 as the product of its 1-D factors, in the order the kernels
+        // recompute it: ((m0c e_iv) e_jv) e_kv
+        std::vector<double> m0(ctx->S, 0.0), ex(kMaxNv, 0.0);
+        for (int l = 0; l < Nv; ++l)
+        {
+            const double v = vnode(ctx->vlo, ctx->dv, l);
+            ex[l] = std::exp(-(v * v) / twoRT);
+        }
+        ctx->m0c = ctx->m0rho / norm;
         for (int q = 0; q < 5; ++q) ctx->m0mom[q] = 0.0;
         for (int k = 0; k < ctx->nodes; ++k)
         {
-            const double vx = vnode(ctx->vlo, ctx->dv, k % Nv);
-            const double vy = vnode(ctx->vlo, ctx->dv, (k / Nv) % Nv);
-            const double vz = vnode(ctx->vlo, ctx->dv, k / (Nv * Nv));
-            m0[k] = ctx->m0rho / norm * std::exp(-(vx * vx + vy * vy + vz * vz) / twoRT);
+            const int iv = k % Nv, jv = (k / Nv) % Nv, kv = k / (Nv * Nv);
+            const double vx = vnode(ctx->vlo, ctx->dv, iv);
+            const double vy = vnode(ctx->vlo, ctx->dv, jv);
+            const double vz = vnode(ctx->vlo, ctx->dv, kv);
+            m0[k] = ((ctx->m0c * ex[iv]) * ex[jv]) * ex[kv];
             ctx->m0mom[0] += m0[k] * dv3;
             ctx->m0mom[1] += vx * m0[k] * dv3;
             ctx->m0mom[2] += vy * m0[k] * dv3;
@@ -1632,7 +1667,10 @@ int table_setup(ab_ctx *ctx)
             ctx->m0mom[4] += (vx * vx + vy * vy + vz * vz) * m0[k] * dv3;
         }
         if (hipMemcpyAsync(ctx->m0, m0.data(), sizeof(double) * ctx->S, hipMemcpyHostToDevice, s)
-                != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+                != hipSuccess
+            || hipMemcpyAsync(ctx->m0x, ex.data(), sizeof(double) * kMaxNv, hipMemcpyHostToDevice, s)
+                != hipSuccess
+            || hipStreamSynchronize(s) != hipSuccess)
         { ctx->err = "table setup: host-to-device copy failed"; return AB_ERR_HIP; }
     }
     if (hipMemcpyAsync(ctx->lat, lat.data(), 3 * sizeof(int) * (size_t)N, hipMemcpyHostToDevice, s) != hipSuccess
@@ -1726,6 +1764,8 @@ int table_step(ab_ctx *ctx, double dt)
     const int nbp = (ctx->N * kWave + 255) / 256;
     Ref ref;
     ref.m0 = ctx->m0;
+    ref.m0x = ctx->m0x;
+    ref.m0c = ctx->m0c;
     for (int q = 0; q < 5; ++q) ref.mom[q] = ctx->m0mom[q];
     ref.rho = ctx->m0rho;
     ref.T = ctx->m0T;

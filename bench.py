@@ -161,24 +161,28 @@ def main():
         A.copy_calibration(256 << 20, 2)
         log("[bench] counters pass done")
         return
-    mix.profile(True)
     t0 = time.perf_counter()
     mix.steps(dt, args.steps)
     mix.synchronize()
     t1 = time.perf_counter()
-    prof = mix.profile_read()
-    mix.profile(False)
     sec = t1 - t0
     value = updates * args.steps / sec
     log(f"[bench] mixed+DM+MLP: {sec * 1e3 / args.steps:.4f} ms/step, {value:.4e} updates/s")
-    log(f"[bench] stage times (ms total, launches): {prof}")
+    m_mix = mix.moments()                 # after warmup + steps steps: the accuracy check
+    # stage times from HIP events around each stage, in a run of their own
+    # (the events add their own time to the step)
+    nprof = max(1, min(args.steps, 50))
+    mix.profile(True)
+    mix.steps(dt, nprof)
+    mix.synchronize()
+    prof = mix.profile_read()
+    mix.profile(False)
+    log(f"[bench] stage times over {nprof} steps (ms total, launches): {prof}")
     tr_ms, tr_n = prof["transport"]
     tr_avg_s = tr_ms / max(tr_n, 1) * 1e-3
     alg_bytes = updates * 2 * 4           # read d once + write gt once, FP32
     achieved = alg_bytes / tr_avg_s / 1e9
-    m_mix = mix.moments()
     st_l = mix.step(dt, stats=True)       # one more step: Newton passes with the warm start
-    m_mix_n1 = mix.moments()
 
     # ---- the same without the warm start (plain Newton) --------------------
     pln = A.Solver(cloud, precision=A.PREC_MIXED, collision=DISC)

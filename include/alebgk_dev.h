@@ -261,6 +261,9 @@ int ab_mlp_get(ab_ctx *ctx, float *params);
 int ab_mlp_set(ab_ctx *ctx, const float *params);
 /* Forward pass with the FP32 parameters: x n x 5 -> y n x 5 (de-normalised). */
 int ab_mlp_forward(ab_ctx *ctx, const float *x, int n, float *y);
+/* The warm start's own inference, as the step runs it: BF16 weights on
+ * MFMA, layer-1 activations rounded to BF16, FP32 accumulation. */
+int ab_mlp_infer(ab_ctx *ctx, const float *x, int n, float *y);
 /* Loss and its gradient (AB_MLP_TRAINABLE floats) over rows x (n x 5),
  * targets t (n x 5, de-normalised), weights w (n), FP32 parameters. */
 int ab_mlp_loss_grad(ab_ctx *ctx, const float *x, const float *t, const float *w, int n,

@@ -48,6 +48,27 @@ def forward(p, x):
     return _fwd(q, x)[3] * q["ys"] + q["ym"]
 
 
+def bf16(a):
+    """Round FP32 values to BF16 (round to nearest, ties to even), as float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward_bf16(p, x):
+    """The warm start's inference as the solver runs it: W1, W2, W3 rounded to
+    BF16 (from the FP32 parameters), the layer-1 activations rounded to BF16
+    before layer 2, everything else in FP64 here (FP32 on the device)."""
+    q = unpack(np.asarray(p, np.float32))
+    for k in ("W1", "W2", "W3"):
+        q[k] = bf16(q[k])
+    xn = (np.asarray(x, np.float32).astype(np.float64) - q["xm"]) / q["xs"]
+    h1 = bf16(np.tanh(xn @ q["W1"].T + q["b1"]))
+    h2 = np.tanh(h1 @ q["W2"].T + q["b2"])
+    y = h2 @ q["W3"].T + q["b3"]
+    return y * q["ys"] + q["ym"]
+
+
 def loss(p, x, t, w):
     q = unpack(p)
     y = _fwd(q, x)[3]

@@ -101,6 +101,30 @@ def test_mlp_forward_and_gradient_match_oracle():
         assert np.abs(g[a:b] - g_ref[a:b]).max() <= 1e-4 * np.abs(g_ref[a:b]).max() + 1e-12, (a, b)
 
 
+def test_mlp_mfma_inference_matches_bf16_oracle():
+    """The warm start's inference (BF16 weights on MFMA, BF16 layer-1
+    activations, FP32 accumulation, one wave per 64 rows) against the oracle
+    emulating the same roundings. A layout mistake in the MFMA operands gives
+    errors of the outputs' own size; what remains here is FP32 accumulation
+    order and the rare layer-1 activation that rounds to the other BF16
+    neighbour. 300 rows: four full waves and a ragged fifth."""
+    _say("[mlp] MFMA inference vs BF16-emulating oracle")
+    sol = _mlp_solver()
+    rng = np.random.default_rng(11)
+    p = M.random_params(23).astype(np.float32)
+    sol.mlp_set(p)
+    x = rng.normal(size=(300, 5)).astype(np.float32)
+    y = sol.mlp_infer(x)
+    y_ref = M.forward_bf16(p, x)
+    scale = np.abs(y_ref).max()
+    err = np.abs(y - y_ref) / scale
+    _say(f"  max {err.max():.2e} median {np.median(err):.2e}")
+    assert np.median(err) <= 1e-5
+    assert err.max() <= 5e-3
+    # and it is the BF16 network, not the FP32 one
+    assert np.abs(sol.mlp_forward(x) - y).max() > 1e-5 * scale
+
+
 def test_learned_warm_start_gain():
     """Train the MLP on rows the plain discrete solve generates, then run the
     same steps with and without the warm start: every solve still converges

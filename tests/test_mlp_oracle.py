@@ -47,3 +47,28 @@ def test_zero_weight_rows_do_not_count():
     t2 = t.copy(); t2[5:] += 100.0
     L2, g2 = M.loss_grad(p, x, t2, w)
     assert L == L2 and np.array_equal(g, g2)
+
+
+def test_bf16_rounding_known_values():
+    """Round to nearest, ties to even, on the 8-bit significand."""
+    one = 1.0
+    cases = [(one, one), (one + 2.0 ** -9, one),                 # tie -> even (down)
+             (one + 3 * 2.0 ** -9, one + 2.0 ** -7),              # tie -> even (up)
+             (one + 2.0 ** -8 + 2.0 ** -20, one + 2.0 ** -7),     # above the tie
+             (-3.0, -3.0), (0.0, 0.0), (2.0 ** -130, 2.0 ** -130)]
+    for x, want in cases:
+        assert M.bf16(np.float32(x)) == want, (x, M.bf16(np.float32(x)), want)
+
+
+def test_bf16_forward_close_to_fp32_forward():
+    """The BF16 inference differs from the FP32 network by the rounding of the
+    weights and of the layer-1 activations only: a few 1e-3 of the outputs'
+    spread, not more (a misplaced rounding of a bias or of layer 2 would show
+    up as a larger error; no rounding at all as zero)."""
+    rng = np.random.default_rng(5)
+    p = M.random_params(5)
+    x = rng.normal(size=(256, M.NIN)).astype(np.float32)
+    y = M.forward(p, x)
+    yb = M.forward_bf16(p, x)
+    d = np.abs(yb - y).max() / np.abs(y).max()
+    assert 1e-5 < d < 3e-2, d
