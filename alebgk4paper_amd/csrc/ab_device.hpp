@@ -41,6 +41,42 @@ __device__ __forceinline__ double uniform(double x)
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// Sums eight values over the wave with 10 exchanges instead of 48. At each
+// of the first three stages a lane keeps half of its values and adds its
+// partner's copy of that half (its lane bit picks the half), then a plain
+// butterfly over lane bits 2..0; value q ends in lanes 8q .. 8q + 7 and is
+// read from lane 8q into scalar registers. Fixed order: deterministic.
+__device__ __forceinline__ void wave_sum8(double (&v)[8])
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    double a[4], b[2];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+    {
+        const double keep = b5 ? v[j + 4] : v[j], send = b5 ? v[j] : v[j + 4];
+        a[j] = keep + __shfl_xor(send, 32, kWave);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+    {
+        const double keep = b4 ? a[j + 2] : a[j], send = b4 ? a[j] : a[j + 2];
+        b[j] = keep + __shfl_xor(send, 16, kWave);
+    }
+    double c = (b3 ? b[1] : b[0]) + __shfl_xor(b3 ? b[0] : b[1], 8, kWave);
+    c += __shfl_xor(c, 4, kWave);
+    c += __shfl_xor(c, 2, kWave);
+    c += __shfl_xor(c, 1, kWave);
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, c);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+    {
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, 8 * q);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), 8 * q);
+        v[q] = __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
 // Orders a wave's LDS writes before its later LDS reads of other lanes'
 // entries (a wave-private table; no workgroup barrier, so waves that left
 // the kernel early do not matter).

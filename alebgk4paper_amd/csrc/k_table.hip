@@ -877,7 +877,14 @@ struct Row
 };
 
 // Boundary particle (one wave): positivity, then the diffuse wall in FP64
-// (kWallBC3D, CudaBackend.cu:956-1030), written back as a deviation.
+// (kWallBC3D, CudaBackend.cu:956-1030), written back as a deviation. Two
+// passes over the row. The wall's sums need the limited row, which depends
+// on the positivity outcome only through the rescale factor sc and linearly:
+// with sel = the fallback value where m + t < 0, else t, the limited row is
+// d = sc (m + sel) - m when the repair fires and t when not. The first pass
+// therefore sums both variants alongside the positivity sums, and the
+// outcome picks one: sum d = sc S_sel - S_m or S_t, and the incoming flux
+// sum_{c_n<0} c_n (m + d) = sc I_sel or I_t.
 template <typename T>
 __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Row<T> &row,
                                               const Ref &ref, double Lo, double R, double *tab,
@@ -886,26 +893,6 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     const int lane = row.lane, Nv = row.Nv;
     const double dv3 = c.dv * c.dv * c.dv, vlo = c.vlo, dv = c.dv;
     auto nop = [] {};
-    auto nob = [](int, int, double, double) {};
-    double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
-    row.pairs(nob, [&](int p, int kv, double t, double m) {
-        const double v = m + t;
-        if (v >= 0.0) mPos += v;
-        else { mNeg += v; mFb += m + (double)row.g(p, kv); }
-    }, nop);
-    mPos = wave_sum(mPos); mNeg = wave_sum(mNeg); mFb = wave_sum(mFb);
-    const bool fires = (Lo * mNeg < -1e-30 && (mPos + mNeg) > 0);
-    double sc = 1.0;
-    if (fires)
-    {
-        const double replaced = mPos + mFb;
-        sc = (Lo * replaced > 1e-30) ? (mPos + mNeg) / replaced : 0.0;
-    }
-    auto limited = [&](int p, int kv, double t, double m) -> double {
-        if (!fires) return t;
-        const double sel = (m + t < 0.0) ? (double)row.g(p, kv) : t;
-        return sc * sel + (sc - 1.0) * m;
-    };
     const double nx = c.wnx[i], ny = c.wny[i], nz = c.wnz[i];
     const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = c.wuz[i], Tw = c.wt[i];
     const double RT2w = 2.0 * R * Tw;
@@ -926,28 +913,58 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
         cnxy = (vx - Uwx) * nx + (vy - Uwy) * ny;
         mxy = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)) / RT2w) * inorm;
     };
-    auto cn_of = [&](double vz) -> double {
+    auto cn_of = [&](int kv) -> double {
 #pragma clang fp contract(off)
-        return cnxy + (vz - Uwz) * nz;
+        return cnxy + (vnode(vlo, dv, kv) - Uwz) * nz;
     };
-    double m0s = 0, intM = 0, intP = 0;
+    // pass 1: positivity sums; S_t, S_sel, S_m; I_t, I_sel; the outgoing flux
+    double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
+    double St = 0.0, Ssel = 0.0, Sm = 0.0, It = 0.0, Isel = 0.0, intP = 0.0;
     row.pairs(pair_wall, [&](int p, int kv, double t, double m) {
-        const double d = limited(p, kv, t, m);
-        m0s += d;
-        const double cn = cn_of(vnode(vlo, dv, kv));
-        if (cn < 0.0) intM += cn * (m + d) * dv3;
-        else if (cn > 0.0) intP += cn * (mxy * tab[kv]) * dv3;
+        const double v = m + t;
+        double vs = v;                                  // m + sel
+        if (v >= 0.0) mPos += v;
+        else
+        {
+            vs = m + (double)row.g(p, kv);
+            mNeg += v;
+            mFb += vs;
+        }
+        St += t; Ssel += vs; Sm += m;
+        const double cn = cn_of(kv);
+        if (cn < 0.0) { It += cn * v; Isel += cn * vs; }
+        else if (cn > 0.0) intP += cn * (mxy * tab[kv]);
     }, nop);
-    m0s = wave_sum(m0s); intM = wave_sum(intM); intP = wave_sum(intP);
+    double r[9] = {mPos, mNeg, mFb, St, Ssel, Sm, It, Isel, intP};
+    wave_sum8(*reinterpret_cast<double(*)[8]>(r));
+    r[8] = uniform(wave_sum(r[8]));
+    mPos = r[0]; mNeg = r[1]; mFb = r[2];
+    const bool fires = (Lo * mNeg < -1e-30 && (mPos + mNeg) > 0);
+    double sc = 1.0;
+    if (fires)
+    {
+        const double replaced = mPos + mFb;
+        sc = (Lo * replaced > 1e-30) ? (mPos + mNeg) / replaced : 0.0;
+    }
+    const double m0s = fires ? sc * r[4] - r[5] : r[3];
+    const double intM = (fires ? sc * r[7] : r[6]) * dv3;
+    intP = r[8] * dv3;
     // intM is in L-units; the reference's fallback rho_w = 1 is in true units
     const double rhoWL = (intP > 1e-30) ? (-intM / intP) : 1.0 / Lo;
     out[0] = Lo * (ref.mom[0] + m0s * dv3);
     out[2] = Lo * rhoWL;
+    // pass 2: the wall's re-emission where c_n > 0, the limited row elsewhere
     const T Lt = T(Lo);
     double h0 = 0;
     row.pairs(pair_wall, [&](int p, int kv, double t, double m) {
-        const double cn = cn_of(vnode(vlo, dv, kv));
-        const double dh = (cn > 0.0) ? rhoWL * (mxy * tab[kv]) - m : limited(p, kv, t, m);
+        double dh;
+        if (cn_of(kv) > 0.0) dh = rhoWL * (mxy * tab[kv]) - m;
+        else if (!fires) dh = t;
+        else
+        {
+            const double sel = (m + t < 0.0) ? (double)row.g(p, kv) : t;
+            dh = sc * sel + (sc - 1.0) * m;
+        }
         const T dhT = T(dh);
         row.set_g(p, kv, Lt * dhT + T((Lo - 1.0) * m));
         h0 += (double)dhT;
@@ -956,7 +973,16 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     out[6] = fires ? 1.0 : 0.0;
 }
 
-// An interior particle after positivity and NAD (L-units).
+// An interior particle after positivity and NAD (L-units): the NAD factor,
+// the limiter bits and the moments before the NAD factor.
+struct Raw
+{
+    double sc2, flag;
+    double rho, px, py, pz, E;    // sum (1, v, |v|^2) (M0 + d) dv^3
+};
+
+// The committed state derived from Raw (scalar arithmetic of one particle:
+// a wave evaluates it uniformly, or one thread per particle).
 struct Committed
 {
     double sc2;               // NAD factor
@@ -971,9 +997,8 @@ struct Committed
 // densities. The raw moments are summed in the positivity pass and redone
 // only when the repair fires (it then rewrites the transported row).
 template <typename T>
-__device__ __forceinline__ Committed commit_interior(const DevCloud &c, int i, const Row<T> &row,
-                                                     const Ref &ref, const double *rho, double Lo,
-                                                     double R)
+__device__ __forceinline__ Raw commit_interior(const DevCloud &c, int i, const Row<T> &row,
+                                               const Ref &ref, const double *rho, double Lo)
 {
     const int lane = row.lane;
     const double dv3 = c.dv * c.dv * c.dv;
@@ -997,9 +1022,8 @@ __device__ __forceinline__ Committed commit_interior(const DevCloud &c, int i, c
         else { s[1] += v; s[2] += m + (double)row.g(p, kv); }
         accum(kv, d);
     }, end);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) s[q] = uniform(wave_sum(s[q]));
-    Committed r;
+    wave_sum8(s);
+    Raw r;
     r.flag = 0.0;
     if (Lo * s[1] < -1e-30 && (s[0] + s[1]) > 0)
     {
@@ -1014,8 +1038,8 @@ __device__ __forceinline__ Committed commit_interior(const DevCloud &c, int i, c
             row.set_t(p, kv, lim);
             accum(kv, (double)lim);
         }, end);
-#pragma unroll
-        for (int q = 3; q < 8; ++q) s[q] = uniform(wave_sum(s[q]));
+        s[0] = s[1] = s[2] = 0.0;
+        wave_sum8(s);
     }
     const double rhoL = ref.mom[0] + s[3] * dv3;
     const double pxL = ref.mom[1] + s[4] * dv3;
@@ -1042,15 +1066,26 @@ __device__ __forceinline__ Committed commit_interior(const DevCloud &c, int i, c
         if (nad) r.flag += 2.0;
     }
     r.sc2 = sc2;
-    r.mr = sc2 * rhoL;
-    r.px = sc2 * pxL; r.py = sc2 * pyL; r.pz = sc2 * pzL; r.E = sc2 * EL;
+    r.rho = rhoL; r.px = pxL; r.py = pyL; r.pz = pzL; r.E = EL;
+    return r;
+}
+
+__device__ __forceinline__ Committed finish_commit(const Raw &w, const Ref &ref, double Lo,
+                                                  double R)
+{
+    Committed r;
+    r.sc2 = w.sc2;
+    r.flag = w.flag;
+    r.mr = w.sc2 * w.rho;
+    r.px = w.sc2 * w.px; r.py = w.sc2 * w.py; r.pz = w.sc2 * w.pz; r.E = w.sc2 * w.E;
     r.ux = 0.0; r.uy = 0.0; r.uz = 0.0; r.T = ref.T;
     r.okm = Lo * r.mr > 1e-30;
     if (r.okm)
     {
-        r.ux = pxL / rhoL; r.uy = pyL / rhoL; r.uz = pzL / rhoL;
+        const double ir = 1.0 / w.rho;
+        r.ux = w.px * ir; r.uy = w.py * ir; r.uz = w.pz * ir;
         const double u2 = r.ux * r.ux + r.uy * r.uy + r.uz * r.uz;
-        const double t1 = (EL / rhoL - u2) / (3.0 * R);
+        const double t1 = (w.E * ir - u2) / (3.0 * R);
         r.T = t1 > 1e-6 ? t1 : 1e-6;
     }
     return r;
@@ -1108,19 +1143,24 @@ __device__ __forceinline__ void collide_row(const Row<T> &row, double *tab, doub
         h[0] += A; h[1] += px * A; h[2] += py * A; h[3] += vlo * A + dv * K1;
         h[4] += (px * px + py * py + vlo * vlo) * A + dv * (2.0 * vlo * K1 + dv * K2);
     });
+    double t[8] = {h[0], h[1], h[2], h[3], h[4], 0.0, 0.0, 0.0};
+    wave_sum8(t);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) h[q] = wave_sum(h[q]);
+    for (int q = 0; q < 5; ++q) h[q] = t[q];
 }
 
 // Per-particle state between the discrete-Maxwellian kernels, structure of
 // arrays (slot q of particle i at q N + i).
-enum PostSlot { P_SC2, P_OKM, P_UX, P_UY, P_UZ, P_T, P_MS, P_D = P_MS + 5, P_COUNT = P_D + 5 };
+// P_RAW: the moments before the NAD factor (Raw), P_D: the collision target
+// exponent (continuous, or discrete where Newton converged).
+enum PostSlot { P_SC2, P_OKM, P_RAW, P_D = P_RAW + 5, P_COUNT = P_D + 5 };
 
 // One wave per particle. Boundary particles: positivity and the diffuse wall.
 // Interior particles: positivity, NAD and moments; then (continuous
-// collision) the BGK collision in the same kernel, or (DM) the committed
-// state is handed to k_dm_solve and k_collide. Partials: [0] committed mass,
-// [1..5] moments after the collision, [6] limiter bits.
+// collision) the BGK collision in the same kernel, or (DM) the raw moments
+// and the NAD factor are handed to k_dm_solve, which derives the committed
+// state one thread per particle, and k_collide. Partials: [0] committed
+// mass, [1..5] moments after the collision, [6] limiter bits.
 template <typename T, bool DM>
 __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__ g,
                                               T *__restrict__ gt, Ref ref,
@@ -1151,27 +1191,25 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
         if (lane < kPart - 1) pp[lane] = out[lane];
         return;
     }
-    const Committed m = commit_interior<T>(c, i, row, ref, rho, Lo, R);
-    double D[5];
-    continuous_target(m, ref, R, D);
+    const Raw w = commit_interior<T>(c, i, row, ref, rho, Lo);
     if constexpr (DM)
     {
         if (lane == 0)
         {
             const size_t N = c.N;
-            const double ms[5] = {m.mr, m.px, m.py, m.pz, m.E};
-            pst[P_SC2 * N + i] = m.sc2;
-            pst[P_OKM * N + i] = m.okm ? 1.0 : 0.0;
-            pst[P_UX * N + i] = m.ux; pst[P_UY * N + i] = m.uy; pst[P_UZ * N + i] = m.uz;
-            pst[P_T * N + i] = m.T;
+            const double raw[5] = {w.rho, w.px, w.py, w.pz, w.E};
+            pst[P_SC2 * N + i] = w.sc2;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) { pst[(P_MS + q) * N + i] = ms[q]; pst[(P_D + q) * N + i] = D[q]; }
-            pp[0] = Lo * m.mr;
-            pp[6] = m.flag;
+            for (int q = 0; q < 5; ++q) pst[(P_RAW + q) * N + i] = raw[q];
+            pp[0] = Lo * (w.sc2 * w.rho);
+            pp[6] = w.flag;
         }
     }
     else
     {
+        const Committed m = finish_commit(w, ref, Lo, R);
+        double D[5];
+        continuous_target(m, ref, R, D);
         double h[5];
         collide_row<T>(row, tab, m.sc2, m.okm, D, dt / tau, Lo, h);
         const double dv3 = c.dv * c.dv * c.dv;
@@ -1191,23 +1229,35 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
 // separable grid sums until the relative residual passes. A converged
 // solution replaces the collision target exponent.
 __global__ __launch_bounds__(256) void k_dm_solve(int N, const unsigned char *__restrict__ bnd,
-                                                  DmArgs dm, Ref ref, double R, int Nv,
-                                                  double vlo, double dv,
+                                                  DmArgs dm, Ref ref, const double *__restrict__ scal,
+                                                  double R, int Nv, double vlo, double dv,
                                                   double *__restrict__ pst)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const size_t n = N;
-    const bool act = i < N && !bnd[i] && pst[P_OKM * n + i] != 0.0;
-    const double cs = dm.cs, ics = 1.0 / cs, dv3 = dv * dv * dv;
-    double ux = 0.0, uy = 0.0, uz = 0.0, T = ref.T;
-    double ms[5] = {ref.rho, 0.0, 0.0, 0.0, 0.0};
-    if (act)
+    const bool inner = i < N && !bnd[i];
+    // the committed state and the continuous target (one thread per particle)
+    Raw w{1.0, 0.0, ref.rho, 0.0, 0.0, 0.0, 0.0};
+    if (inner)
     {
-        ux = pst[P_UX * n + i]; uy = pst[P_UY * n + i]; uz = pst[P_UZ * n + i];
-        T = pst[P_T * n + i];
-#pragma unroll
-        for (int q = 0; q < 5; ++q) ms[q] = pst[(P_MS + q) * n + i];
+        w.sc2 = pst[P_SC2 * n + i];
+        w.rho = pst[(P_RAW + 0) * n + i]; w.px = pst[(P_RAW + 1) * n + i];
+        w.py = pst[(P_RAW + 2) * n + i]; w.pz = pst[(P_RAW + 3) * n + i];
+        w.E = pst[(P_RAW + 4) * n + i];
     }
+    const Committed m = finish_commit(w, ref, scal[S_LAZY], R);
+    double D[5];
+    continuous_target(m, ref, R, D);
+    if (inner)
+    {
+        pst[P_OKM * n + i] = m.okm ? 1.0 : 0.0;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) pst[(P_D + q) * n + i] = D[q];
+    }
+    const bool act = inner && m.okm;
+    const double cs = dm.cs, ics = 1.0 / cs, dv3 = dv * dv * dv;
+    const double ux = m.ux, uy = m.uy, uz = m.uz, T = m.T;
+    double ms[5] = {m.mr, m.px, m.py, m.pz, m.E};
     const float feat[kMlpIn] = {float(ux * ics), float(uy * ics), float(uz * ics),
                                 float(R * T * ics * ics), float(dv * ics)};
     float da[kMlpOut] = {0, 0, 0, 0, 0};
@@ -1788,8 +1838,8 @@ int table_step(ab_ctx *ctx, double dt)
         }
         k_post<T, true><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->rho, ctx->scal, dt,
                                             ctx->cfg.tau, ctx->cfg.R, ctx->part, ctx->pst);
-        k_dm_solve<<<nbs, 256, 0, s>>>(ctx->N, c.bnd, dm, ref, ctx->cfg.R, ctx->Nv, ctx->vlo,
-                                       ctx->dv, ctx->pst);
+        k_dm_solve<<<nbs, 256, 0, s>>>(ctx->N, c.bnd, dm, ref, ctx->scal, ctx->cfg.R, ctx->Nv,
+                                       ctx->vlo, ctx->dv, ctx->pst);
         k_collide<T><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->scal, dt, ctx->cfg.tau,
                                          ctx->pst, ctx->part);
     };
