@@ -573,21 +573,36 @@ struct Blk
     static constexpr UCol ucol(int u) { return find(u, nullptr); }
 };
 
+// Addressing of a block's z block: one buffer descriptor at the lowest union
+// column (bx - 2, by - 2) and plane iz0 - 2; the lane's node at the vector
+// offset, and in the scalar offset union column U (cofs(U), cheap to
+// recompute) plus plane iz0 - 2 + zz (zof[zz], clamped to the top plane,
+// per z block): no descriptor per column, no per-load vector arithmetic.
+template <typename T, int K>
+struct BlkAddr
+{
+    __amdgpu_buffer_rsrc_t r;
+    int lin;                  // the lane's byte offset
+    int col, row;             // bytes between neighbouring columns in x and in y
+    int zof[K + 4];
+    template <int U, int TX, int TY>
+    __device__ __forceinline__ int cofs() const
+    {
+        constexpr UCol uc = Blk<TX, TY>::ucol(U);
+        return (uc.ox + 2) * col + (uc.oy + 2) * row;
+    }
+};
+
 // Values of union column U for outputs iz0 .. iz0+K-1 (no x/y clamping: the
 // block lies two columns inside the lattice).
 template <typename T, int TX, int TY, int K, int U>
-__device__ __forceinline__ void blk_load(const T *__restrict__ g, const Lat &L, int bx, int by,
-                                         int iz0, int lin, size_t zstride, T (&v)[K + 4])
+__device__ __forceinline__ void blk_load(const BlkAddr<T, K> &a, T (&v)[K + 4])
 {
     constexpr UCol uc = Blk<TX, TY>::ucol(U);
     constexpr int NV = K + 2 * uc.d;
-    const int z0 = iz0 - uc.d;
-    const __amdgpu_buffer_rsrc_t rg = brsrc(
-        g + ((size_t)(bx + uc.ox) + (size_t)L.nx * (by + uc.oy)) * L.S + (size_t)z0 * zstride);
-    const int pb = (int)(zstride * sizeof(T));
-    const int zmax = L.nz - 1 - z0;
+    const int so = a.template cofs<U, TX, TY>();
 #pragma clang loop unroll(full)
-    for (int j = 0; j < NV; ++j) v[j] = bload_so<T>(rg, lin, (j < zmax ? j : zmax) * pb);
+    for (int j = 0; j < NV; ++j) v[j] = bload_so<T>(a.r, a.lin, so + a.zof[j + 2 - uc.d]);
 }
 
 template <typename T, int TX, int TY, int K, int U, int... TI>
@@ -612,15 +627,13 @@ __device__ __forceinline__ void blk_fma(const T (&cr)[kOff], const T (&v)[K + 4]
 }
 
 template <typename T, int TX, int TY, int K, int U>
-__device__ __forceinline__ void blk_step(const T *__restrict__ g, const T (&cr)[kOff],
-                                         const Lat &L, int bx, int by, int iz0, int lin,
-                                         size_t zstride, T (&va)[K + 4], T (&vb)[K + 4],
-                                         T (&acc)[TX * TY][K])
+__device__ __forceinline__ void blk_step(const BlkAddr<T, K> &a, const T (&cr)[kOff],
+                                         T (&va)[K + 4], T (&vb)[K + 4], T (&acc)[TX * TY][K])
 {
     if constexpr (U + 1 < Blk<TX, TY>::NU)
     {
-        if constexpr ((U & 1) == 0) blk_load<T, TX, TY, K, U + 1>(g, L, bx, by, iz0, lin, zstride, vb);
-        else blk_load<T, TX, TY, K, U + 1>(g, L, bx, by, iz0, lin, zstride, va);
+        if constexpr ((U & 1) == 0) blk_load<T, TX, TY, K, U + 1>(a, vb);
+        else blk_load<T, TX, TY, K, U + 1>(a, va);
     }
     if constexpr ((U & 1) == 0)
         blk_fma<T, TX, TY, K, U>(cr, va, acc, std::make_integer_sequence<int, TX * TY>{});
@@ -639,9 +652,18 @@ __device__ __forceinline__ void blk_columns(const T *__restrict__ g, const T (&c
                                             size_t zstride, T (&acc)[TX * TY][K],
                                             std::integer_sequence<int, U...>)
 {
+    BlkAddr<T, K> a;
+    const int zlo = iz0 - 2;
+    a.r = brsrc(g + ((size_t)(bx - 2) + (size_t)L.nx * (by - 2)) * L.S + (size_t)zlo * zstride);
+    a.lin = lin;
+    a.col = L.S * (int)sizeof(T);
+    a.row = L.nx * L.S * (int)sizeof(T);
+    const int pb = (int)(zstride * sizeof(T)), ztop = L.nz - 1 - zlo;
+#pragma clang loop unroll(full)
+    for (int zz = 0; zz < K + 4; ++zz) a.zof[zz] = (zz < ztop ? zz : ztop) * pb;
     T va[K + 4], vb[K + 4];
-    blk_load<T, TX, TY, K, 0>(g, L, bx, by, iz0, lin, zstride, va);
-    (blk_step<T, TX, TY, K, U>(g, cr, L, bx, by, iz0, lin, zstride, va, vb, acc), ...);
+    blk_load<T, TX, TY, K, 0>(a, va);
+    (blk_step<T, TX, TY, K, U>(a, cr, va, vb, acc), ...);
 }
 
 // End-plane pair of a block: per stencil column, the two classes'
@@ -702,7 +724,8 @@ __device__ __forceinline__ void blk_pair(const T *__restrict__ g, const T *__res
 // (ny - 4) % TY == 0.
 template <typename T, int TX, int TY, int K>
 __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
-                                           const T *__restrict__ coef, int chunk, int blk, int voff)
+                                           const T *__restrict__ coef, int chunk, int blk, int slab,
+                                           int voff)
 {
     const int nbx = (L.nx - 4) / TX;
     const int bx = 2 + (blk % nbx) * TX, by = 2 + (blk / nbx) * TY;
@@ -726,8 +749,12 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
         for (int t = 0; t < TX * TY; ++t) { store(t, iz, a0[t]); store(t, iz + 1, a1[t]); }
         __builtin_amdgcn_sched_barrier(0);
     };
-    end_pair(std::false_type{});
-    end_pair(std::true_type{});
+    if (slab < 0)
+    {
+        end_pair(std::false_type{});
+        end_pair(std::true_type{});
+        if (slab == -1) return;
+    }
 
     T cr[kOff];
     {
@@ -735,7 +762,8 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
 #pragma unroll
         for (int o = 0; o < kOff; ++o) cr[o] = bload_so<T>(rc, voff, o * kWave * (int)sizeof(T));
     }
-    for (int iz0 = 2; iz0 <= L.nz - 3; iz0 += K)
+    const int zb = slab < 0 ? 2 : 2 + slab * K, ze = slab < 0 ? L.nz - 3 : min(zb + K - 1, L.nz - 3);
+    for (int iz0 = zb; iz0 <= ze; iz0 += K)
     {
         T acc[TX * TY][K];
 #pragma unroll
@@ -756,24 +784,37 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
 // (TX x TY columns per wave), then the edge columns one per wave (z blocks of
 // KE). One launch keeps the edge waves from running as a second, short and
 // latency-bound grid after the blocks.
-template <typename T, int TX, int TY, int K, int KE>
+//
+// SLAB: a block wave covers one z block of K planes instead of the whole
+// column, and the waves of a chunk run slab by slab (first every block's end
+// planes, then every block's first z block, ...), so the waves in flight on
+// an XCD share one slab of the chunk's columns in L2 instead of spanning all
+// of its planes.
+template <int K>
+__host__ __device__ constexpr int zblocks(int nz) { return (nz - 4 + K - 1) / K; }
+
+template <typename T, int TX, int TY, int K, int KE, bool SLAB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
 {
     const int nblk = ((L.nx - 4) / TX) * ((L.ny - 4) / TY);
     const int nedge = L.nx * L.ny - (L.nx - 4) * (L.ny - 4);
-    const int per = nblk + nedge;
+    const int nitem = SLAB ? nblk * (zblocks<K>(L.nz) + 1) : nblk;
+    const int per = nitem + nedge;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
     const int voff = (threadIdx.x & (kWave - 1)) * (int)sizeof(T);
     if (wv >= per * L.nchunk) return;
     const int chunk = wv / per, r = wv % per;               // chunk-major
-    if (r < nblk)
-        block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r, voff);
+    if (r < nitem)
+    {
+        if constexpr (SLAB) block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r % nblk, r / nblk - 1, voff);
+        else block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r, -2, voff);
+    }
     else
     {
         int ix, iy;
-        column_of(L, r - nblk, true, ix, iy);
+        column_of(L, r - nitem, true, ix, iy);
         column_wave<T, KE>(L, g, gt, coef, chunk, ix, iy, voff);
     }
 }
@@ -1482,15 +1523,19 @@ bool is_fp32(const ab_ctx *ctx) { return ctx->cfg.precision == AB_PREC_MIXED; }
 // K outputs per z block (values reused across K + 4 planes), W the waves per
 // SIMD the register allocation targets. Every shape does the same products
 // in the same order per output, so the results do not depend on the choice.
-struct ApplyShape { int K, W, TX, TY; };   // TX x TY > 1: interior blocks + edge columns
-constexpr ApplyShape kShapes32[] = {{9, 3, 1, 1}, {8, 3, 1, 1}, {7, 3, 1, 1}, {6, 3, 1, 1},
-                                    {12, 3, 1, 1}, {10, 3, 1, 1}, {5, 3, 1, 1}, {4, 3, 1, 1},
-                                    {8, 2, 2, 2}, {6, 2, 2, 2}, {4, 2, 2, 2}, {12, 2, 2, 2}};
-constexpr ApplyShape kShapes64[] = {{9, 2, 1, 1}, {8, 2, 1, 1}, {7, 2, 1, 1}, {6, 2, 1, 1}};
+// TX x TY > 1: interior blocks + edge columns; S: z-slab order of the blocks
+struct ApplyShape { int K, W, TX, TY, S; };
+constexpr ApplyShape kShapes32[] = {{9, 3, 1, 1, 0}, {8, 3, 1, 1, 0}, {7, 3, 1, 1, 0},
+                                    {6, 3, 1, 1, 0}, {12, 3, 1, 1, 0}, {10, 3, 1, 1, 0},
+                                    {5, 3, 1, 1, 0}, {4, 3, 1, 1, 0}, {8, 2, 2, 2, 0},
+                                    {6, 2, 2, 2, 0}, {4, 2, 2, 2, 0}, {12, 2, 2, 2, 0},
+                                    {12, 2, 2, 2, 1}, {8, 2, 2, 2, 1}, {6, 2, 2, 2, 1}};
+constexpr ApplyShape kShapes64[] = {{9, 2, 1, 1, 0}, {8, 2, 1, 1, 0}, {7, 2, 1, 1, 0},
+                                    {6, 2, 1, 1, 0}};
 constexpr int kMaxBlockPlanes = 12 + 4;
 
 template <typename T>
-constexpr int nshapes() { return sizeof(T) == 4 ? 12 : 4; }
+constexpr int nshapes() { return sizeof(T) == 4 ? 15 : 4; }
 
 template <typename T>
 ApplyShape shape_at(int i) { return sizeof(T) == 4 ? kShapes32[i] : kShapes64[i]; }
@@ -1508,13 +1553,17 @@ void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *sc
         const ApplyShape sh = kShapes32[i];
         if (sh.TX > 1)
         {
-            const int nw = (inner / (sh.TX * sh.TY) + all - inner);
+            const int nblk = inner / (sh.TX * sh.TY);
+            const int nw = (sh.S ? nblk * ((L.nz - 4 + sh.K - 1) / sh.K + 1) : nblk) + all - inner;
             switch (i)
             {
-            case 8: k_apply_mix<T, 2, 2, 8, 8><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 9: k_apply_mix<T, 2, 2, 6, 8><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 10: k_apply_mix<T, 2, 2, 4, 8><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            default: k_apply_mix<T, 2, 2, 12, 12><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 8: k_apply_mix<T, 2, 2, 8, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 9: k_apply_mix<T, 2, 2, 6, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 10: k_apply_mix<T, 2, 2, 4, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 12: k_apply_mix<T, 2, 2, 12, 12, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 13: k_apply_mix<T, 2, 2, 8, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 14: k_apply_mix<T, 2, 2, 6, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            default: k_apply_mix<T, 2, 2, 12, 12, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
             }
             return;
         }
@@ -1901,7 +1950,7 @@ int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32],
         if (names && i < cap)
         {
             const ApplyShape sh = f32 ? shape_at<float>(i) : shape_at<double>(i);
-            if (sh.TX > 1) snprintf(names[i], 32, "B%dx%d/K%d", sh.TX, sh.TY, sh.K);
+            if (sh.TX > 1) snprintf(names[i], 32, "B%dx%d/K%d%s", sh.TX, sh.TY, sh.K, sh.S ? "/S" : "");
             else snprintf(names[i], 32, "K%d/W%d", sh.K, sh.W);
         }
         if (best < 0 || tmin < tbest) { best = i; tbest = tmin; }

@@ -257,7 +257,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "k_apply (MLS transport contraction)",
+                     "kernel": "k_apply_mix (MLS transport contraction, autotuned shape)",
                      "kernel_avg_ms": tr_avg_s * 1e3,
                      "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
