@@ -1042,6 +1042,144 @@ double orc_step_dm(const orc_cloud *c, orc_state *s, double dt, double tau,
 }
 
 /* ======================================================================== */
+/* The reference's CPU step with its MOOD limiter                           */
+/* ======================================================================== */
+
+/* The positivity repair inside the CPU transport (MLSTransport/MLSTransport.cpp
+ * :189-200 in 3-D, :287-307 in 2-D): negative values to zero, the rest scaled
+ * so that the mass is kept; in 2-D on the g1 layer, with g2 following g1.
+ * Returns 1 when it fired. */
+static int positivity_zero(const orc_cloud *c, double *p)
+{
+    const int n = (c->dim == 3) ? c->gstride : c->Nv * c->Nv;
+    double mPos = 0.0, mNeg = 0.0;
+    for (int k = 0; k < n; k++) {
+        if (p[k] >= 0.0) mPos += p[k];
+        else mNeg += p[k];
+    }
+    if (!(mNeg < -1.0e-30 && mPos > 1.0e-30 && (mPos + mNeg) > 0.0)) return 0;
+    const double sc = (mPos + mNeg) / mPos;
+    for (int k = 0; k < n; k++) {
+        if (c->dim == 3) p[k] = (p[k] < 0.0) ? 0.0 : p[k] * sc;
+        else if (p[k] < 0.0) { p[k] = 0.0; p[k + n] = 0.0; }
+        else { p[k] *= sc; p[k + n] *= sc; }
+    }
+    return 1;
+}
+
+/* checkTransportPAD (MLSTransport/MLSTransport.cpp:324-379) for one interior
+ * particle: the transported row is inadmissible when its mass is not finite
+ * or not positive, or when its density leaves the neighbours' current range
+ * widened by 0.1 of the range (at least 1e-12) plus 1e-9. */
+static int transport_flag(const orc_cloud *c, int i, const double *p, const double *rho)
+{
+    const int n = (c->dim == 3) ? c->Nv * c->Nv * c->Nv : c->Nv * c->Nv;
+    const double dvN = (c->dim == 3) ? c->dv * c->dv * c->dv : c->dv * c->dv;
+    double m = 0.0;
+    for (int k = 0; k < n; k++) {
+        if (!isfinite(p[k])) return 1;
+        m += p[k];
+    }
+    if (m <= 0.0) return 1;
+    const double rn = m * dvN;
+    double lo = INFINITY, hi = -INFINITY;
+    for (int t = 0; t < c->nneigh[i]; t++) {
+        const double q = rho[c->neigh[(size_t)i * ORC_MAX_NEIGH + t]];
+        lo = fmin(lo, q);
+        hi = fmax(hi, q);
+    }
+    if (!(isfinite(lo) && isfinite(hi))) return 0;
+    const double slack = 0.1 * fmax(hi - lo, 1.0e-12) + 1.0e-9;
+    return (rn < lo - slack || rn > hi + slack);
+}
+
+/* padCheck (MLSTransport/MLSBasis.hpp:208-222) of an interior particle. */
+static int pad_fails(const orc_cloud *c, const double *p, double rho, double T)
+{
+    if (rho < 1.0e-10 || T < 1.0e-6) return 1;
+    const int n = (c->dim == 3) ? c->Nv * c->Nv * c->Nv : c->Nv * c->Nv;
+    for (int k = 0; k < n; k++)
+        if (p[k] < -1.0e-14) return 1;
+    return 0;
+}
+
+static void moments_of(const orc_cloud *c, orc_state *s, double R)
+{
+    if (c->dim == 3) orc_moments3d(c, s->g, R, s->rho, s->ux, s->uy, s->uz, s->T);
+    else orc_moments2d(c, s->g, R, s->rho, s->ux, s->uy, s->T);
+}
+
+static void collide(const orc_cloud *c, orc_state *s, double dt, double tau, double R)
+{
+    if (c->dim == 3) orc_collision3d(c, s->g, dt, tau, R, s->rho, s->ux, s->uy, s->uz, s->T);
+    else orc_collision2d(c, s->g, dt, tau, R, s->rho, s->ux, s->uy, s->T);
+}
+
+/* The time loop body of runSolver with the MOOD limiter (Solver/Solver.cpp
+ * :641-693): transport with the positivity repair; checkTransportPAD, and
+ * the flagged particles transported again with the linear basis; commit; one
+ * mass rescale; wall, moments, collision, moments; moodCheck
+ * (MomentCalculations/Conservation.cpp:16-38) and, if any interior particle
+ * fails padCheck, a second collision of every interior particle and the
+ * moments again. The MLS solve is this oracle's (the GPU path's Cholesky);
+ * the CPU code solves by Gauss elimination, equal to round-off on these
+ * stencils. limflag: +1 positivity repair, +2 transported again, +4 failed
+ * the post-collision check. */
+double orc_step_mood(const orc_cloud *c, orc_state *s, double dt, double tau, double R,
+                     double h, int mlsOrder, double tol)
+{
+    const int N = c->N;
+    const size_t gs = (size_t)c->gstride;
+    for (int i = 0; i < N; i++) {
+        s->limflag[i] = 0.0;
+        if (s->mlsflag) s->mlsflag[i] = 0;
+        if (s->mlsovr) s->mlsovr[i] = 0.0;
+    }
+    const double massOld = orc_sum_mass(c, s->g);
+    if (c->dim == 3) orc_transport3d(c, s->g, s->gt, dt, h, mlsOrder, tol, 0, NULL, NULL);
+    else orc_transport2d(c, s->g, s->gt, dt, h, mlsOrder);
+    for (int i = 0; i < N; i++)
+        if (positivity_zero(c, s->gt + (size_t)i * gs)) s->limflag[i] = 1.0;
+    int nf = 0;
+    for (int i = 0; i < N; i++)
+        if (!c->boundary[i] && transport_flag(c, i, s->gt + (size_t)i * gs, s->rho)) {
+            s->limflag[i] += 2.0;
+            nf++;
+        }
+    if (nf > 0) {
+        double *gl = (double *)malloc(sizeof(double) * gs * (size_t)N);
+        if (c->dim == 3) orc_transport3d(c, s->g, gl, dt, h, 1, tol, 0, NULL, NULL);
+        else orc_transport2d(c, s->g, gl, dt, h, 1);
+        for (int i = 0; i < N; i++) {
+            if (!((int)s->limflag[i] & 2)) continue;
+            double *p = s->gt + (size_t)i * gs;
+            memcpy(p, gl + (size_t)i * gs, sizeof(double) * gs);
+            if (positivity_zero(c, p) && !((int)s->limflag[i] & 1)) s->limflag[i] += 1.0;
+        }
+        free(gl);
+    }
+    memcpy(s->g, s->gt, sizeof(double) * gs * (size_t)N);          /* mlsTransportCommit */
+    const double massNew = orc_sum_mass(c, s->g);
+    if (massNew > 1e-30) orc_scale(c, s->g, massOld / massNew);
+    if (c->dim == 3) orc_wallbc3d(c, s->g, R, s->rho, s->ux, s->uy, s->uz, s->T);
+    else orc_wallbc2d(c, s->g, R, s->rho, s->ux, s->uy, s->T);
+    moments_of(c, s, R);
+    collide(c, s, dt, tau, R);
+    moments_of(c, s, R);
+    int fail = 0;
+    for (int i = 0; i < N; i++)
+        if (!c->boundary[i] && pad_fails(c, s->g + (size_t)i * gs, s->rho[i], s->T[i])) {
+            s->limflag[i] += 4.0;
+            fail = 1;
+        }
+    if (fail) {
+        collide(c, s, dt, tau, R);
+        moments_of(c, s, R);
+    }
+    return massOld;
+}
+
+/* ======================================================================== */
 /* Body loads                                                                */
 /* ======================================================================== */
 

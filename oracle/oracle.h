@@ -150,6 +150,14 @@ double orc_step_dm(const orc_cloud *c, orc_state *s, double dt, double tau,
                    double R, double h, int mlsOrder, double tol, int nofix,
                    double dm_tol, int dm_maxit);
 
+/* The reference's CPU step with its default limiter, MOOD (Solver/Solver.cpp
+ * :641-693): positivity to zero, checkTransportPAD with a linear
+ * re-transport of the flagged particles, one mass rescale, and a second
+ * collision when moodCheck fails. Returns massOld; limflag +1 positivity,
+ * +2 transported again, +4 failed the post-collision check. */
+double orc_step_mood(const orc_cloud *c, orc_state *s, double dt, double tau, double R,
+                     double h, int mlsOrder, double tol);
+
 /* Surface-stress loads of one body, computeBodyLoads
  * (RigidBody/BodyDynamics.cpp:17-97). out6 = Fx,Fy,Fz,Tx,Ty,Tz. */
 void orc_body_loads(const orc_cloud *c, const double *g, const int *bodyid,

@@ -11,6 +11,8 @@
 //   ref_harness cube    <outdir> <Nx> <Nv> <scale> cavity3d-cube body loads
 //   ref_harness cpustep <Nx> <Nv> <order> <nsteps> time the reference CPU step
 //   ref_harness ckpt    <outdir> <case> <Nx> <Nv>   a restart file + its fields
+//   ref_harness mood    <outdir> <case> <Nx> <Nv> <order> <nsteps> <a> <b>
+//                                                  CPU steps with the MOOD limiter
 //   ref_harness ckptread <file> <outdir> <case> <Nx> <Nv>  restore a file, dump fields
 //
 // Functions called (reference file:line):
@@ -27,6 +29,8 @@
 //   bgkCollision             MomentCalculations/Collision.cpp:70
 //   applyBoundaryBC          BoundaryConditions/DiffuseWall.cpp:19
 //   sumMass / scaleG         MomentCalculations/Conservation.cpp:48,92
+//   checkTransportPAD, mlsTransportCommit  MLSTransport/MLSTransport.cpp:324, 384
+//   moodCheck                MomentCalculations/Conservation.cpp:16
 //   computeBodyLoads (static) RigidBody/BodyDynamics.cpp:17 (TU included below)
 //   ckpt::save / ckpt::restore Core/Checkpoint.hpp:78, 161
 // =============================================================================
@@ -42,6 +46,7 @@
 #include "Checkpoint.hpp"
 
 #include <chrono>
+#include <random>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -501,6 +506,101 @@ int modeCkptRead(const std::string &file, const std::string &dir, const std::str
     return 0;
 }
 
+// The CPU time loop of runSolver with its default limiter, MOOD
+// (Solver/Solver.cpp:641-693), for nsteps fixed steps from a start that
+// makes the limiters act: two opposing streams, the Maxwellian of the case's
+// density and wall temperature moving at u_x = +a left of the domain's
+// middle and -a right of it (the density piles up where they meet, a new
+// extremum for the NAD check), with every value scaled by (1 + b u), u
+// uniform in [-1, 1] (mt19937_64, seed 20261202); the moments are
+// recomputed. Writes the start (g0, mom0), the end (g1, mom1), the particles
+// the transport check flagged at each step (flags, nsteps x N) and whether
+// the post-collision check failed (pad).
+int modeMood(const std::string &dir, const std::string &cname, int Nx, int Nv, int order,
+             int nsteps, double a, double b)
+{
+    Setup S;
+    build(S, caseOf(cname), Nx, Nv, 1.0);
+    S.sp.mlsOrder = order;
+    auto &P = S.P;
+    const SimParameters &sp = S.sp;
+    const double dt = sp.dt, R = S.gc.R;
+    std::mt19937_64 rng(20261202);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    const double xmid = 0.5 * (S.dom.xleft + S.dom.xright);
+    for (auto &p : P)
+    {
+        const double ux = (p.x < xmid) ? a : -a, rho = S.BC.rho, T = S.BC.Tw;
+        const int Nv = p.Nv_local, Nv2 = Nv * Nv;
+        for (int kv = 0; kv < (p.dim == 3 ? Nv : 1); ++kv)
+            for (int jv = 0; jv < Nv; ++jv)
+                for (int iv = 0; iv < Nv; ++iv)
+                {
+                    const double vx = p.vlo + iv * p.dv_local, vy = p.vlo + jv * p.dv_local;
+                    const double vz = p.vlo + kv * p.dv_local;
+                    if (p.dim == 3)
+                        p.g[iv + Nv * (jv + Nv * kv)] = maxwellian3D(vx, vy, vz, ux, 0.0, 0.0, T, rho, R);
+                    else
+                    {
+                        const double g1 = maxwellianG1(vx, vy, ux, 0.0, T, rho, R);
+                        p.g[iv + Nv * jv] = g1;
+                        p.g[iv + Nv * jv + Nv2] = 0.5 * R * T * g1;
+                    }
+                }
+        for (auto &v : p.g) v *= 1.0 + b * U(rng);
+    }
+    computeMoments(P, R);
+    const size_t N = P.size();
+    auto moments = [&]() {
+        std::vector<double> m(N * 5);
+        for (size_t i = 0; i < N; ++i)
+        {
+            double q[5] = {P[i].rho, P[i].ux, P[i].uy, P[i].uz, P[i].T};
+            std::memcpy(&m[5 * i], q, sizeof q);
+        }
+        return m;
+    };
+    writeMeta(dir, S, dt);
+    const size_t gs = P[0].g.size();
+    npyF8(dir + "/g0.npy", gatherG(P, false), {N, gs});
+    npyF8(dir + "/mom0.npy", moments(), {N, 5});
+    std::vector<uint8_t> flags;
+    std::vector<uint8_t> pad;
+    for (int st = 0; st < nsteps; ++st)
+    {
+        double massOld = sumMass(P);
+        mlsTransportCompute(P, dt, S.cp, S.dom, S.BC, sp.mlsOrder);
+        int nf = checkTransportPAD(P);
+        for (const auto &p : P) flags.push_back(p.moodFlag ? 1 : 0);
+        if (nf > 0) mlsTransportCompute(P, dt, S.cp, S.dom, S.BC, 1, true);
+        mlsTransportCommit(P);
+        double massNew = sumMass(P);
+        if (massNew > 1.0e-30) scaleG(P, massOld / massNew);
+        applyGravity(P, sp.gravityY, dt);
+        applyBoundaryBC(P, R, S.dom, S.BC);
+        computeMoments(P, R);
+        bgkCollision(P, dt, sp.tao, R, sp.tauVariable, sp.d);
+        computeMoments(P, R);
+        const bool fail = moodCheck(P);
+        pad.push_back(fail ? 1 : 0);
+        if (fail)
+        {
+            bgkCollision(P, dt, sp.tao, R, sp.tauVariable, sp.d);
+            computeMoments(P, R);
+        }
+        std::printf("step %d: %d flagged, post-collision check %s\n", st + 1, nf,
+                    fail ? "failed" : "passed");
+    }
+    npyF8(dir + "/g1.npy", gatherG(P, false), {N, gs});
+    npyF8(dir + "/mom1.npy", moments(), {N, 5});
+    npyU1(dir + "/flags.npy", flags, {(size_t)nsteps, N});
+    npyU1(dir + "/pad.npy", pad, {(size_t)nsteps});
+    std::vector<double> par{(double)order, (double)nsteps, a, b};
+    npyF8(dir + "/params.npy", par, {4});
+    std::printf("wrote %s: N=%zu\n", dir.c_str(), N);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 1; }
@@ -516,6 +616,10 @@ int main(int argc, char **argv)
                            std::atoi(argv[5]));
     if (mode == "ckpt" && argc == 6)
         return modeCkpt(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]));
+    if (mode == "mood" && argc == 10)
+        return modeMood(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
+                        std::atoi(argv[6]), std::atoi(argv[7]), std::atof(argv[8]),
+                        std::atof(argv[9]));
     if (mode == "ckptread" && argc == 7)
         return modeCkptRead(argv[2], argv[3], argv[4], std::atoi(argv[5]), std::atoi(argv[6]));
     std::fprintf(stderr, "bad arguments\n");

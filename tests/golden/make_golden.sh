@@ -17,6 +17,9 @@ H="$ROOT/oracle/_ref/ref_harness"
 # restart files written by the reference's ckpt::save, with the fields they hold
 "$H" ckpt "$HERE/ckpt_cavity3d_n6_v5" cavity3d 6 5
 "$H" ckpt "$HERE/ckpt_cube_n8_v4" cube 8 4
+# three CPU steps with the MOOD limiter from opposing streams (u_x = +-300 m/s)
+"$H" mood "$HERE/mood_cavity3d_n7_v6" cavity3d 7 6 2 3 300 0
+"$H" mood "$HERE/mood_cavity2d_n12_v8" cavity2d 12 8 2 3 300 0
 rm -f "$HERE"/ckpt_*/neigh_flat.npy "$HERE"/ckpt_*/nneigh.npy
 ( cd "$HERE" && find . -name '*.npy' -o -name 'meta.json' -o -name '*.ckpt' | sort | xargs sha256sum > MANIFEST.sha256 )
 echo "fixtures written under $HERE"

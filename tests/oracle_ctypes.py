@@ -69,6 +69,7 @@ def lib():
         L.orc_sum_mass.restype = C.c_double
         L.orc_step.restype = C.c_double
         L.orc_step_dm.restype = C.c_double
+        L.orc_step_mood.restype = C.c_double
         L.orc_dm_scale.restype = C.c_double
         L.orc_dm_scale.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_maxwellian3d.restype = C.c_double
@@ -288,6 +289,11 @@ def step_dm(cloud, state, dt, tau, R, h, order, tol=1e-12, nofix=0, dm_tol=1e-12
     return lib().orc_step_dm(cloud.ref, C.byref(state._s), C.c_double(dt), C.c_double(tau),
                              C.c_double(R), C.c_double(h), C.c_int(order), C.c_double(tol),
                              C.c_int(nofix), C.c_double(dm_tol), C.c_int(dm_maxit))
+
+
+def step_mood(cloud, state, dt, tau, R, h, order, tol=1e-12):
+    return lib().orc_step_mood(cloud.ref, C.byref(state._s), C.c_double(dt), C.c_double(tau),
+                               C.c_double(R), C.c_double(h), C.c_int(order), C.c_double(tol))
 
 
 def dm_scale(Nv, vlo, dv):
