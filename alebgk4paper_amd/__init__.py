@@ -48,7 +48,8 @@ class ABConfig(C.Structure):
                 ("xleft", C.c_double), ("ybottom", C.c_double), ("zfront", C.c_double),
                 ("hbox", C.c_double), ("vbox", C.c_double), ("wbox", C.c_double),
                 ("nbhbox", C.c_int), ("nbvbox", C.c_int), ("nbwbox", C.c_int),
-                ("collision", C.c_int), ("dm_tol", C.c_double), ("dm_maxit", C.c_int)]
+                ("collision", C.c_int), ("dm_tol", C.c_double), ("dm_maxit", C.c_int),
+                ("limiter", C.c_int)]
 
 
 class ABCloudView(C.Structure):
@@ -66,7 +67,7 @@ class ABStepStats(C.Structure):
                 ("n_mls_extrapolate", C.c_int), ("mls_ovr_sum", C.c_double),
                 ("mls_ovr_max", C.c_double), ("dm_particles", C.c_int),
                 ("dm_unconverged", C.c_int), ("dm_passes", C.c_longlong),
-                ("dm_res_max", C.c_double)]
+                ("dm_res_max", C.c_double), ("n_pad_fail", C.c_int)]
 
 
 class ALHCase(C.Structure):
@@ -121,6 +122,8 @@ MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
 COLLISION_DISCRETE = 1
+LIMITER_GPU = 0          # the reference's GPU step (cudaStep)
+LIMITER_MOOD = 1         # its CPU step with the MOOD limiter (runSolver)
 HOST_EXPORTS = ["alh_configure", "alh_lattice_size", "alh_build_lattice", "alh_ckpt_info_read",
                 "alh_ckpt_read", "alh_ckpt_write"]
 
@@ -229,7 +232,7 @@ class Solver:
     """One device context (ab_ctx): the cloud resident on the GPU."""
 
     def __init__(self, cloud: Cloud, precision=None, mls_order=None, no_mass_fix=False,
-                 mls_clamp=False, collision=None, dm_tol=None, dm_maxit=None):
+                 mls_clamp=False, collision=None, dm_tol=None, dm_maxit=None, limiter=None):
         self._lib = hip_lib()
         cfg = ABConfig()
         C.memmove(C.byref(cfg), C.byref(cloud.cfg), C.sizeof(ABConfig))
@@ -246,6 +249,8 @@ class Solver:
             cfg.dm_tol = float(dm_tol)
         if dm_maxit is not None:
             cfg.dm_maxit = int(dm_maxit)
+        if limiter is not None:
+            cfg.limiter = int(limiter)
         self.cfg = cfg
         self.cloud = cloud
         self.N, self.dim = cloud.N, cfg.dim

@@ -48,9 +48,11 @@ void free_cloud(ab_ctx *c)
                     c->wt, c->mvx, c->mvy, c->mvz, c->rho, c->ux, c->uy, c->uz, c->T,
                     c->neigh, c->nneigh, c->ilist, c->blist, c->g, c->gt, c->limflag,
                     c->mlsflag, c->mlsovr, c->pmass, c->vox, c->vcount, c->vstart, c->vlist,
-                    c->bmom, c->bidx, c->bwu, c->dm_alpha, c->dm_pass, c->dm_res, c->dm_rec};
+                    c->bmom, c->bidx, c->bwu, c->dm_alpha, c->dm_pass, c->dm_res, c->dm_rec,
+                    c->mood, c->moodfail};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     c->dm_alpha = c->dm_res = nullptr; c->dm_pass = nullptr; c->dm_rec = nullptr;
+    c->mood = c->moodfail = nullptr;
     c->dm_rec_slots = c->dm_rec_next = c->dm_rec_filled = c->dm_rec_left = 0;
     c->x = c->y = c->z = nullptr; c->bnd = nullptr;
     c->wnx = c->wny = c->wnz = c->wux = c->wuy = c->wuz = c->wt = nullptr;
@@ -79,6 +81,7 @@ int alloc_cloud(ab_ctx *c)
     if ((rc = dalloc(c, &c->ilist, N))) return rc;
     if ((rc = dalloc(c, &c->blist, N))) return rc;
     if ((rc = dalloc(c, &c->mlsflag, N))) return rc;
+    if ((rc = dalloc(c, &c->mood, N)) || (rc = dalloc(c, &c->moodfail, 1))) return rc;
     if ((rc = dalloc(c, &c->g, N * c->gstride))) return rc;
     if (!table_mode(c) && (rc = dalloc(c, &c->gt, N * c->gstride))) return rc;
     if (c->cfg.collision == AB_COLLISION_DISCRETE)
@@ -126,8 +129,10 @@ int zero_step_flags(ab_ctx *c)
     k_fill_int<<<nb, 256, 0, c->stream>>>(c->mlsflag, c->N, 0);
     k_fill_dbl<<<nb, 256, 0, c->stream>>>(c->mlsovr, c->N, 0.0);
     k_fill_dbl<<<nb, 256, 0, c->stream>>>(c->limflag, c->N, 0.0);
+    k_fill_int<<<1, 256, 0, c->stream>>>(c->moodfail, 1, 0);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
+
 
 struct Timer
 {
@@ -185,6 +190,56 @@ int fp64_step(ab_ctx *c, double dt)
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 
+// The reference's CPU step with its MOOD limiter (Solver/Solver.cpp:641-693):
+// transport with the positivity repair; the transport check and the flagged
+// particles' linear re-transport; commit; one mass rescale (M_old / M_new);
+// wall, moments, collision, moments; the post-collision check and, if any
+// interior particle fails it, a second collision of every interior particle
+// and the moments again. No host synchronisation: the re-transport and the
+// second collision are launched always and act only where flagged.
+int fp64_step_mood(ab_ctx *c, double dt)
+{
+    DevCloud v = c->view();
+    hipStream_t s = c->stream;
+    Timer tm(c);
+    int rc;
+    tm.tic();
+    if (!c->mass_valid) launch_mass_partials(v, c->g, c->pmass, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_OLD, 0, s);
+    if ((rc = zero_step_flags(c))) return rc;
+    tm.toc(ST_REDUCE);
+    tm.tic();
+    if ((rc = fp64_transport(c, dt))) return rc;
+    tm.toc(ST_TRANSPORT);
+    tm.tic();
+    launch_mood_check(v, c->gt, c->rho, c->mood, c->limflag, s);
+    const double tol = c->cfg.mls_tol > 0 ? c->cfg.mls_tol : 1e-12;
+    launch_transport_fp64(v, c->g, c->gt, dt, c->cfg.radius, 1, tol, c->cfg.mls_clamp,
+                          c->mlsflag, c->mlsovr, c->ilist, c->n_int, false, s, c->mood);
+    launch_mood_commit(v, c->g, c->gt, c->mood, c->limflag, c->pmass, s);
+    tm.toc(ST_LIMIT_COMMIT);
+    tm.tic();
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s);
+    tm.toc(ST_REDUCE);
+    tm.tic();
+    launch_wall_collide(v, dm_args(c), c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux,
+                        c->uy, c->uz, c->T, c->pmass, s);
+    tm.toc(ST_WALL_COLLIDE);
+    tm.tic();
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, 1, s);
+    tm.toc(ST_REDUCE);
+    tm.tic();
+    launch_final(v, c->g, c->scal, c->cfg.R, c->rho, c->ux, c->uy, c->uz, c->T, c->pmass, s);
+    launch_pad_check(v, c->g, c->rho, c->T, c->limflag, c->moodfail, s);
+    launch_wall_collide(v, dm_args(c), c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux,
+                        c->uy, c->uz, c->T, c->pmass, s, c->moodfail);
+    launch_final(v, c->g, c->scal, c->cfg.R, c->rho, c->ux, c->uy, c->uz, c->T, c->pmass, s,
+                 c->moodfail);
+    tm.toc(ST_FINAL);
+    c->mass_valid = true;
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
 int do_step(ab_ctx *c, double dt)
 {
     if (!c->uploaded) { c->err = "ab_step before ab_upload"; return AB_ERR_STATE; }
@@ -195,6 +250,7 @@ int do_step(ab_ctx *c, double dt)
         if (rc && c->err.empty()) c->err = "table step failed";
         return rc;
     }
+    if (c->cfg.limiter == AB_LIMITER_MOOD) return fp64_step_mood(c, dt);
     return fp64_step(c, dt);
 }
 
@@ -325,6 +381,10 @@ int ab_create(const ab_config *cfg, ab_ctx **out)
         (cfg->collision != AB_COLLISION_CONTINUOUS && cfg->collision != AB_COLLISION_DISCRETE))
         return AB_ERR_ARG;
     if (cfg->collision == AB_COLLISION_DISCRETE && cfg->dim != 3) return AB_ERR_UNSUPPORTED;
+    if (cfg->limiter != AB_LIMITER_GPU && cfg->limiter != AB_LIMITER_MOOD) return AB_ERR_ARG;
+    if (cfg->limiter == AB_LIMITER_MOOD &&
+        (cfg->precision != AB_PREC_FP64 || cfg->collision != AB_COLLISION_CONTINUOUS))
+        return AB_ERR_UNSUPPORTED;
     ab_ctx *c = new (std::nothrow) ab_ctx;
     if (!c) return AB_ERR_NOMEM;
     c->cfg = *cfg;
@@ -506,6 +566,7 @@ int ab_step(ab_ctx *c, double dt, ab_step_stats *st)
         int f = (int)(lim[i] + 0.5);
         if (f & 1) st->n_positivity++;
         if (f & 2) st->n_nad++;
+        if (f & 4) st->n_pad_fail++;
     }
     if (table_mode(c))
     {

@@ -95,6 +95,8 @@ struct ab_ctx
     double *g = nullptr, *gt = nullptr;
     double *limflag = nullptr;
     int *mlsflag = nullptr;
+    int *mood = nullptr;           // N: flagged by the MOOD transport check (AB_LIMITER_MOOD)
+    int *moodfail = nullptr;       // 1: some particle failed the post-collision check
     double *mlsovr = nullptr;
     double *pmass = nullptr;       // per-particle partial sums
     double *scal = nullptr;        // ab::ScalarSlot
@@ -186,18 +188,31 @@ void prof_flush(ab_ctx *c);
 void launch_transport_fp64(const DevCloud &c, const double *g, double *gt, double dt,
                            double h, int order, double tol, int clamp, int *mlsflag,
                            double *mlsovr, const int *list, int nlist, bool boundary,
-                           hipStream_t s);
+                           hipStream_t s, const int *only = nullptr);
+// The reference's CPU-step limiter (MOOD): positivity to zero and the
+// transport check (flags), the commit after the flagged particles'
+// re-transport, and the post-collision admissibility check.
+void launch_mood_check(const DevCloud &c, double *gt, const double *rho, int *mood,
+                       double *limflag, hipStream_t s);
+void launch_mood_commit(const DevCloud &c, double *g, double *gt, const int *mood,
+                        double *limflag, double *pmass, hipStream_t s);
+void launch_pad_check(const DevCloud &c, const double *g, const double *rho, const double *T,
+                      double *limflag, int *fail, hipStream_t s);
 void launch_mass_partials(const DevCloud &c, const double *g, double *pmass, hipStream_t s);
 void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int nofix,
                         hipStream_t s);
 void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
                          double *limflag, double *pmass, hipStream_t s);
+// guard (may be null): when given, the launch acts only if *guard is set,
+// and the wall-collide launch then collides the interior particles again
+// without rescaling (the MOOD limiter's second collision).
 void launch_wall_collide(const DevCloud &c, const DmArgs &dm, double *g, const double *scal, double dt,
                          double tau, double R, double *rho, double *ux, double *uy,
-                         double *uz, double *T, double *pmass, hipStream_t s);
+                         double *uz, double *T, double *pmass, hipStream_t s,
+                         const int *guard = nullptr);
 void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
                   double *ux, double *uy, double *uz, double *T, double *pmass,
-                  hipStream_t s);
+                  hipStream_t s, const int *guard = nullptr);
 
 // ---- k_neigh.hip -------------------------------------------------------------
 int launch_neighbours(const DevCloud &c, const ab_config &cfg, int *vox, int *vcount,

@@ -39,13 +39,14 @@ __global__ __launch_bounds__(kBlock) void k_transport3d(DevCloud c, const double
                                                         double h, double tol, int clamp,
                                                         int *mlsflag, double *mlsovr,
                                                         const int *__restrict__ list,
-                                                        int nchunk)
+                                                        int nchunk, const int *__restrict__ only)
 {
     __shared__ double sx[AB_MAX_NEIGH], sy[AB_MAX_NEIGH], sz[AB_MAX_NEIGH];
     __shared__ int sj[AB_MAX_NEIGH];
     __shared__ double sred[2 * (kBlock / kWave)];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int i = list[b / nchunk];
+    if (only && !only[i]) return;            // whole block: one particle
     const int lin = (b % nchunk) * kBlock + threadIdx.x;
     const int nn = c.nneigh[i];
     for (int t = threadIdx.x; t < nn; t += kBlock)
@@ -196,12 +197,13 @@ template <int P, bool BND>
 __global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double *__restrict__ g,
                                                         double *__restrict__ gt, double dt,
                                                         double h, const int *__restrict__ list,
-                                                        int nchunk)
+                                                        int nchunk, const int *__restrict__ only)
 {
     __shared__ double sx[AB_MAX_NEIGH], sy[AB_MAX_NEIGH];
     __shared__ int sj[AB_MAX_NEIGH];
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int i = list[b / nchunk];
+    if (only && !only[i]) return;            // whole block: one particle
     const int lin = (b % nchunk) * kBlock + threadIdx.x;
     const int nn = c.nneigh[i];
     for (int t = threadIdx.x; t < nn; t += kBlock)
@@ -483,12 +485,14 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                                                          double dt, double tau, double R,
                                                          double *rho, double *ux, double *uy,
                                                          double *uz, double *T,
-                                                         double *__restrict__ pmass)
+                                                         double *__restrict__ pmass,
+                                                         const int *__restrict__ guard)
 {
     __shared__ double sh[15 * (kBlock / kWave)];
     const int i = xcd_remap(blockIdx.x, gridDim.x);
+    if (guard && (!*guard || c.bnd[i])) return;        // whole block: one particle
     const double s1 = scal[S_SCALE1];
-    const bool doScale = scal[S_MASS_NEW] > 1e-30;
+    const bool doScale = !guard && scal[S_MASS_NEW] > 1e-30;
     const size_t gs = (size_t)c.gstride;
     double *gp = g + (size_t)i * gs;
     const int nodes = c.nodes, Nv = c.Nv;
@@ -708,10 +712,12 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
                                                   const double *__restrict__ scal, double R,
                                                   double *rho, double *ux, double *uy,
                                                   double *uz, double *T,
-                                                  double *__restrict__ pmass)
+                                                  double *__restrict__ pmass,
+                                                  const int *__restrict__ guard)
 {
     __shared__ double sh[6 * (kBlock / kWave)];
     const int i = xcd_remap(blockIdx.x, gridDim.x);
+    if (guard && !*guard) return;
     const double s2 = scal[S_SCALE2];
     const bool doScale = (s2 != 1.0);
     const size_t gs = (size_t)c.gstride;
@@ -764,11 +770,150 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
     }
 }
 
+// ---- the reference's CPU-step limiter (MOOD) ---------------------------------
+
+// The positivity repair inside the CPU transport (MLSTransport/MLSTransport.cpp
+// :189-200 in 3-D, :287-307 in 2-D) on one particle's row: negatives to
+// zero, the rest scaled to keep the mass (2-D: on g1, g2 following). Returns
+// whether it fired (every thread).
+template <int DIM>
+__device__ __forceinline__ bool positivity_zero(const DevCloud &c, double *p, double *sh)
+{
+    const int n = c.nodes;
+    double r[2] = {0.0, 0.0};                  // mPos, mNeg
+    for (int k = threadIdx.x; k < n; k += kBlock)
+    {
+        const double v = p[k];
+        if (v >= 0.0) r[0] += v; else r[1] += v;
+    }
+    block_sum<2>(r, sh);
+    if (!(r[1] < -1e-30 && r[0] > 1e-30 && (r[0] + r[1]) > 0.0)) return false;
+    const double sc = (r[0] + r[1]) / r[0];
+    for (int k = threadIdx.x; k < n; k += kBlock)
+    {
+        const bool neg = p[k] < 0.0;
+        p[k] = neg ? 0.0 : p[k] * sc;
+        if constexpr (DIM == 2) p[k + n] = neg ? 0.0 : p[k + n] * sc;
+    }
+    __syncthreads();
+    return true;
+}
+
+// Positivity on every particle's transported row, then checkTransportPAD
+// (MLSTransport/MLSTransport.cpp:324-379) on the interior ones: flagged when
+// the row's mass is not finite or not positive, or its density leaves the
+// neighbours' current range widened by 0.1 of the range (at least 1e-12) plus
+// 1e-9. mood[i] = flagged; limflag: +1 positivity, +2 flagged.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_mood_check(DevCloud c, double *__restrict__ gt,
+                                                       const double *__restrict__ rho,
+                                                       int *__restrict__ mood,
+                                                       double *__restrict__ limflag)
+{
+    __shared__ double sh[2 * (kBlock / kWave)];
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    double *p = gt + (size_t)i * c.gstride;
+    const bool fired = positivity_zero<DIM>(c, p, sh);
+    bool flag = false;
+    if (!c.bnd[i])
+    {
+        double m[1] = {0.0};
+        for (int k = threadIdx.x; k < c.nodes; k += kBlock) m[0] += p[k];
+        block_sum<1>(m, sh);
+        const double dvN = (DIM == 3) ? c.dv * c.dv * c.dv : c.dv * c.dv;
+        flag = !isfinite(m[0]) || m[0] <= 0.0;     // a non-finite value makes the sum non-finite
+        const int nn = c.nneigh[i];
+        double lo = 1e300, hi = -1e300;
+        for (int k = threadIdx.x; k < nn; k += kBlock)
+        {
+            const double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
+            lo = fmin(lo, q); hi = fmax(hi, q);
+        }
+        block_minmax(lo, hi, sh);
+        if (!flag && nn > 0)
+        {
+            const double rn = m[0] * dvN;
+            const double slack = 0.1 * fmax(hi - lo, 1e-12) + 1e-9;
+            flag = (rn < lo - slack || rn > hi + slack);
+        }
+    }
+    if (threadIdx.x == 0)
+    {
+        mood[i] = flag ? 1 : 0;
+        limflag[i] = (fired ? 1.0 : 0.0) + (flag ? 2.0 : 0.0);
+    }
+}
+
+// The flagged particles' rows came from the linear re-transport: positivity
+// once more on those (the repair runs inside every transport pass); then the
+// commit gt -> g for every particle and its mass partial.
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_mood_commit(DevCloud c, double *__restrict__ g,
+                                                        double *__restrict__ gt,
+                                                        const int *__restrict__ mood,
+                                                        double *__restrict__ limflag,
+                                                        double *__restrict__ pmass)
+{
+    __shared__ double sh[2 * (kBlock / kWave)];
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    const size_t gs = (size_t)c.gstride;
+    double *tp = gt + (size_t)i * gs;
+    double *gp = g + (size_t)i * gs;
+    if (mood[i])
+    {
+        const bool fired = positivity_zero<DIM>(c, tp, sh);
+        if (fired && threadIdx.x == 0 && !((int)limflag[i] & 1)) limflag[i] += 1.0;
+    }
+    const double dvd = (DIM == 3) ? c.dv * c.dv * c.dv : c.dv * c.dv;
+    double m[1] = {0.0};
+    for (int k = threadIdx.x; k < (int)gs; k += kBlock)
+    {
+        const double v = tp[k];
+        gp[k] = v;
+        if (k < c.nodes) m[0] += v * dvd;
+    }
+    block_sum<1>(m, sh);
+    if (threadIdx.x == 0) pmass[i] = m[0];
+}
+
+// moodCheck (MomentCalculations/Conservation.cpp:16-38) with padCheck
+// (MLSTransport/MLSBasis.hpp:208-222) on the interior particles after the
+// collision: rho < 1e-10, T < 1e-6 or a mass-carrying value below -1e-14.
+// limflag +4 and *fail set for a failing particle.
+__global__ __launch_bounds__(kBlock) void k_pad_check(DevCloud c, const double *__restrict__ g,
+                                                      const double *__restrict__ rho,
+                                                      const double *__restrict__ T,
+                                                      double *__restrict__ limflag,
+                                                      int *__restrict__ fail)
+{
+    __shared__ int sbad[kBlock / kWave];
+    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    if (c.bnd[i]) return;
+    const double *p = g + (size_t)i * c.gstride;
+    int bad = 0;
+    for (int k = threadIdx.x; k < c.nodes; k += kBlock) bad |= (p[k] < -1.0e-14) ? 1 : 0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) sbad[threadIdx.x / kWave] = bad;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int b = (rho[i] < 1.0e-10 || T[i] < 1.0e-6) ? 1 : 0;
+        for (int q = 0; q < kBlock / kWave; ++q) b |= sbad[q];
+        if (b)
+        {
+            limflag[i] += 4.0;
+            atomicOr(fail, 1);
+        }
+    }
+}
+
 }  // namespace
 
 void launch_transport_fp64(const DevCloud &c, const double *g, double *gt, double dt, double h,
                            int order, double tol, int clamp, int *mlsflag, double *mlsovr,
-                           const int *list, int nlist, bool boundary, hipStream_t s)
+                           const int *list, int nlist, bool boundary, hipStream_t s,
+                           const int *only)
 {
     if (nlist <= 0) return;
     const int nchunk = (c.nodes + kBlock - 1) / kBlock;
@@ -777,22 +922,22 @@ void launch_transport_fp64(const DevCloud &c, const double *g, double *gt, doubl
     {
         if (boundary)
             k_transport3d<4, true><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, tol, clamp, mlsflag,
-                                                         mlsovr, list, nchunk);
+                                                         mlsovr, list, nchunk, only);
         else if (order >= 2)
             k_transport3d<10, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, tol, clamp, mlsflag,
-                                                           mlsovr, list, nchunk);
+                                                           mlsovr, list, nchunk, only);
         else
             k_transport3d<4, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, tol, clamp, mlsflag,
-                                                          mlsovr, list, nchunk);
+                                                          mlsovr, list, nchunk, only);
     }
     else
     {
         if (boundary)
-            k_transport2d<3, true><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk);
+            k_transport2d<3, true><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk, only);
         else if (order >= 2)
-            k_transport2d<6, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk);
+            k_transport2d<6, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk, only);
         else
-            k_transport2d<3, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk);
+            k_transport2d<3, false><<<nb, kBlock, 0, s>>>(c, g, gt, dt, h, list, nchunk, only);
     }
 }
 
@@ -816,23 +961,44 @@ void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const d
 
 void launch_wall_collide(const DevCloud &c, const DmArgs &dm, double *g, const double *scal,
                          double dt, double tau, double R, double *rho, double *ux, double *uy,
-                         double *uz, double *T, double *pmass, hipStream_t s)
+                         double *uz, double *T, double *pmass, hipStream_t s, const int *guard)
 {
     if (c.dim == 3)
         k_wall_collide<3><<<c.N, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
-                                                 pmass);
+                                                 pmass, guard);
     else
         k_wall_collide<2><<<c.N, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
-                                                 pmass);
+                                                 pmass, guard);
 }
 
 void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
-                  double *ux, double *uy, double *uz, double *T, double *pmass, hipStream_t s)
+                  double *ux, double *uy, double *uz, double *T, double *pmass, hipStream_t s,
+                  const int *guard)
 {
     if (c.dim == 3)
-        k_final<3><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass);
+        k_final<3><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
     else
-        k_final<2><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass);
+        k_final<2><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
+}
+
+void launch_mood_check(const DevCloud &c, double *gt, const double *rho, int *mood,
+                       double *limflag, hipStream_t s)
+{
+    if (c.dim == 3) k_mood_check<3><<<c.N, kBlock, 0, s>>>(c, gt, rho, mood, limflag);
+    else k_mood_check<2><<<c.N, kBlock, 0, s>>>(c, gt, rho, mood, limflag);
+}
+
+void launch_mood_commit(const DevCloud &c, double *g, double *gt, const int *mood,
+                        double *limflag, double *pmass, hipStream_t s)
+{
+    if (c.dim == 3) k_mood_commit<3><<<c.N, kBlock, 0, s>>>(c, g, gt, mood, limflag, pmass);
+    else k_mood_commit<2><<<c.N, kBlock, 0, s>>>(c, g, gt, mood, limflag, pmass);
+}
+
+void launch_pad_check(const DevCloud &c, const double *g, const double *rho, const double *T,
+                      double *limflag, int *fail, hipStream_t s)
+{
+    k_pad_check<<<c.N, kBlock, 0, s>>>(c, g, rho, T, limflag, fail);
 }
 
 }  // namespace ab

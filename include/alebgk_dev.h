@@ -80,10 +80,21 @@ typedef struct ab_config {
     int collision;
     double dm_tol;
     int dm_maxit;
+    /* Limiter semantics. AB_LIMITER_GPU is the reference's GPU step
+     * (cudaStep: positivity falls back to the pre-transport value, NAD
+     * rescales into the neighbours' range, two mass rescales). AB_LIMITER_MOOD
+     * is its CPU step with the default limiter, MOOD (runSolver,
+     * Solver/Solver.cpp:641-693: positivity to zero, checkTransportPAD with a
+     * linear re-transport of the flagged particles, one mass rescale, a
+     * second collision when moodCheck fails); FP64 with the continuous
+     * collision only. */
+    int limiter;
 } ab_config;
 
 #define AB_COLLISION_CONTINUOUS 0
 #define AB_COLLISION_DISCRETE 1
+#define AB_LIMITER_GPU 0
+#define AB_LIMITER_MOOD 1
 
 /* Borrowed host view of a cloud (the SoA fields cudaUploadCloud gathers,
  * Solver/CudaBackend.cu:1223-1381). Any of mvx/mvy/mvz may be NULL (zero
@@ -121,6 +132,11 @@ typedef struct ab_step_stats {
     int dm_unconverged;
     long long dm_passes;
     double dm_res_max;
+    /* AB_LIMITER_MOOD: n_positivity counts the positivity repairs, n_nad the
+     * particles the transport check sent to the linear re-transport, and
+     * n_pad_fail those that failed the post-collision check (any one makes
+     * the step collide every interior particle again). */
+    int n_pad_fail;
 } ab_step_stats;
 
 /* Lifecycle. Replaces the file-static DeviceCloud (CudaBackend.cu:115) with
