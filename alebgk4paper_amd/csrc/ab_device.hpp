@@ -77,6 +77,16 @@ __device__ __forceinline__ void wave_sum8(double (&v)[8])
     }
 }
 
+// Lane l's value of v in scalar registers (l wave-uniform): a per-wave table
+// held one entry per lane is read without memory.
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, l);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Orders a wave's LDS writes before its later LDS reads of other lanes'
 // entries (a wave-private table; no workgroup barrier, so waves that left
 // the kernel early do not matter).

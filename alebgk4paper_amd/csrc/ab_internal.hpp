@@ -116,7 +116,9 @@ struct ab_ctx
     int ncls = 0;
     double coef_dt = -1.0;         // dt the tables were built for
     double *m0 = nullptr;          // equilibrium the stored deviation is taken from, S
-    double *m0x = nullptr;         // its 1-D factors (kMaxNv); M0 = ((m0c e_i) e_j) e_k
+    double *m0x = nullptr;         // its 1-D factors e_l (kMaxNv); M0 = ((m0c e_i) e_j) e_k
+    double *m0xy = nullptr;        // its pair factors (m0c e_i) e_j (kMaxNv^2)
+    double m0sum = 0.0;            // sum of M0 over the nodes
     double m0c = 0.0;
     double m0mom[5] = {0};         // its moments: sum M0 dv^3, sum v M0 dv^3, sum |v|^2 M0 dv^3
     double m0rho = 1.0, m0T = 270.0;

@@ -840,8 +840,10 @@ constexpr int kPart = 8;
 struct Ref
 {
     const double *m0;       // M0 per node, S entries
+    const double *m0xy;     // its (v_x, v_y) factor (m0c e_iv) e_jv per pair p = iv + Nv jv
     const double *m0x;      // its 1-D factors e_l = exp(-v_l^2 / 2RT), Nv entries:
     double m0c;             //   M0(iv, jv, kv) = ((m0c e_iv) e_jv) e_kv exactly
+    double m0sum;           // sum over the nodes of M0
     double mom[5];          // sum M0 dv^3, sum v_a M0 dv^3, sum |v|^2 M0 dv^3
     double rho, T;          // reference state of M0
 };
@@ -854,66 +856,126 @@ struct Ref
 // (sum d, sum kv d, sum kv^2 d), combined with v_x, v_y once per pair.
 // Accesses are buffer operations with the lane's pair in the vector offset
 // and the node's plane kv P in the scalar offset (no per-node address
-// arithmetic). A pair's loads are issued in batches of kB; a batch past the
-// last plane reloads the last plane (in bounds, no branch around the loads)
-// and skips the work.
-// M0 of a node is the product of the pair's factor (m0c e_iv) e_jv and the
-// plane's e_kv (from the wave's LDS copy of the factors, a broadcast read),
-// rounded as the table of M0 was built.
-//   begin(iv, jv, vx, vy)   node(p, kv, t, m0)   end()
+// arithmetic).
+// The pass is a sequence of items (64 pairs x kB planes). The loads of item
+// n + 1 are issued before item n is processed, so a wave keeps two items in
+// flight (for Nv <= 10 the whole row); the first item's loads are issued
+// before `setup`, whose own memory latency (per-particle constants, tables)
+// then overlaps them. Lanes past the last pair and planes past the last
+// reload the last valid ones (in bounds, no branch around the loads) and
+// skip the work.
+// M0 of a node is the pair's factor (m0c e_iv) e_jv (a table built with M0)
+// times the plane's e_kv, rounded as the table of M0 was built. Per-wave
+// tables over v_z (e_kv here, the callers' separable factors) are held one
+// entry per lane (lane l: node l) and read into scalar registers with
+// lane_value: no memory access per node.
+//   setup()   begin(iv, jv, vx, vy)   node(p, kv, t, m0)   end()
 template <typename T>
 struct Row
 {
+    static constexpr int kB = 10;
     __amdgpu_buffer_rsrc_t rg;    // stored row (pre-transport values; written last)
     __amdgpu_buffer_rsrc_t rt;    // transported row
-    const double *m0x;            // 1-D factors of M0 (global)
-    const double *ez;             // the same in the wave's LDS table (Nv entries)
-    double m0c;
-    int lane, Nv;
+    const double *m0xy;
+    double ezl;                   // lane l: e_l (0 past the last node)
+    int lane, Nv, P;
     double vlo, dv;
-    __device__ Row(T *g, T *t, const Ref &ref, double *lds, int ln, int nv, double lo, double d)
-        : rg(brsrc(g)), rt(brsrc(t)), m0x(ref.m0x), ez(lds), m0c(ref.m0c), lane(ln), Nv(nv),
-          vlo(lo), dv(d)
+    __device__ Row(T *g, T *t, const Ref &r, int ln, int nv, double lo, double d)
+        : rg(brsrc(g)), rt(brsrc(t)), m0xy(r.m0xy), ezl(ln < nv ? r.m0x[ln] : 0.0), lane(ln),
+          Nv(nv), P(nv * nv), vlo(lo), dv(d)
     {
-        if (ln < nv) lds[ln] = ref.m0x[ln];
-        wave_lds_sync();
     }
-    __device__ __forceinline__ int soff(int kv, int bytes) const { return kv * Nv * Nv * bytes; }
+    // (the plane offset is wave-uniform; readfirstlane keeps it in a scalar
+    // register where the compiler would otherwise loop over the lanes)
+    __device__ __forceinline__ int soff(int kv, int bytes) const
+    { return __builtin_amdgcn_readfirstlane(kv * P * bytes); }
     __device__ __forceinline__ T g(int p, int kv) const
     { return bload_so<T>(rg, p * (int)sizeof(T), soff(kv, sizeof(T))); }
     __device__ __forceinline__ void set_g(int p, int kv, T v) const
     { bstore_so<T>(rg, p * (int)sizeof(T), soff(kv, sizeof(T)), v); }
     __device__ __forceinline__ void set_t(int p, int kv, T v) const
     { bstore_so<T>(rt, p * (int)sizeof(T), soff(kv, sizeof(T)), v); }
-    template <typename FB, typename FN, typename FE>
-    __device__ __forceinline__ void pairs(FB &&begin, FN &&node, FE &&end) const
+    // one item (chunk ch of 64 pairs, planes k0 .. k0 + kB - 1): the
+    // transported values and the pair's M0 factor
+    struct Item
     {
-        constexpr int kB = 5;
-        const int P = Nv * Nv;
-        // a uniform loop over the chunks of 64 pairs (the plane indices and the
-        // M0 factor loads stay scalar); lanes past the last pair sit out
-        for (int p0 = 0; p0 < P; p0 += kWave)
+        T v[kB];
+        double mxy;
+    };
+    // (ch and k0 are wave-uniform; readfirstlane states it to the compiler,
+    // which otherwise can lose track of it in the callers' control flow and
+    // wrap each buffer access in a loop over the lanes' scalar offsets)
+    __device__ __forceinline__ void load(int ch, int k0, Item &it) const
+    {
+        ch = __builtin_amdgcn_readfirstlane(ch);
+        k0 = __builtin_amdgcn_readfirstlane(k0);
+        const int p = min(ch * kWave + lane, P - 1);
+        it.mxy = m0xy[p];
+#pragma unroll
+        for (int j = 0; j < kB; ++j)
+            it.v[j] = bload_so<T>(rt, p * (int)sizeof(T), soff(min(k0 + j, Nv - 1), sizeof(T)));
+    }
+    template <typename FB, typename FN, typename FE>
+    __device__ __forceinline__ void process(int ch, int k0, const Item &it, FB &begin, FN &node,
+                                            FE &end) const
+    {
+        ch = __builtin_amdgcn_readfirstlane(ch);
+        k0 = __builtin_amdgcn_readfirstlane(k0);
+        const int p = ch * kWave + lane;
+        if (p < P)
         {
-            const int p = p0 + lane;
-            if (p >= P) break;
-            const int jv = p / Nv, iv = p - jv * Nv;
-            begin(iv, jv, vnode(vlo, dv, iv), vnode(vlo, dv, jv));
-            const double mxy = (m0c * m0x[iv]) * m0x[jv];
-            for (int k0 = 0; k0 < Nv; k0 += kB)
+            if (k0 == 0)
             {
-                T tv[kB];
-#pragma unroll
-                for (int j = 0; j < kB; ++j)
-                {
-                    const int kv = min(k0 + j, Nv - 1);
-                    tv[j] = bload_so<T>(rt, p * (int)sizeof(T), soff(kv, sizeof(T)));
-                }
-#pragma unroll
-                for (int j = 0; j < kB; ++j)
-                    if (k0 + j < Nv) node(p, k0 + j, (double)tv[j], mxy * ez[k0 + j]);
+                const int jv = p / Nv, iv = p - jv * Nv;
+                begin(iv, jv, vnode(vlo, dv, iv), vnode(vlo, dv, jv));
             }
-            end();
+#pragma unroll
+            for (int j = 0; j < kB; ++j)
+                if (k0 + j < Nv)
+                    node(p, k0 + j, (double)it.v[j], it.mxy * lane_value(ezl, k0 + j));
+            if (k0 + kB >= Nv) end();
         }
+        // every value of the item is complete here, also on the paths that
+        // skipped it (otherwise the wait for it lands on a later reuse of its
+        // register, which waits for every load issued since)
+#pragma unroll
+        for (int j = 0; j < kB; ++j) asm volatile("" ::"v"(it.v[j]));
+        asm volatile("" ::"v"(it.mxy));
+    }
+    __device__ __forceinline__ void advance(int &ch, int &k0) const
+    {
+        k0 += kB;
+        if (k0 >= Nv) { k0 = 0; ++ch; }
+    }
+    // Two item buffers in alternating roles (the loop is unrolled by two):
+    // rotating one buffer into the other would wait for its loads. A round
+    // issues the loads of its second item, then processes its first, then
+    // issues the loads of the next round's first item and processes its
+    // second (the waits of that item also cover the next round's loads,
+    // as the last round issues none). Up to Nv = 10 a pass is one round.
+    template <typename FS, typename FB, typename FN, typename FE>
+    __device__ __forceinline__ void pairs(FS &&setup, FB &&begin, FN &&node, FE &&end) const
+    {
+        const int nkb = (Nv + kB - 1) / kB;
+        const int n = ((P + kWave - 1) / kWave) * nkb;
+        Item a, b;
+        load(0, 0, a);
+        setup();
+        int ch = 0, k0 = 0;
+        int it = 0;
+        for (; it + 1 < n; it += 2)
+        {
+            int c1 = ch, k1 = k0;
+            advance(c1, k1);
+            load(c1, k1, b);
+            process(ch, k0, a, begin, node, end);
+            int c2 = c1, k2 = k1;
+            advance(c2, k2);
+            if (it + 2 < n) load(c2, k2, a);
+            process(c1, k1, b, begin, node, end);
+            ch = c2; k0 = k2;
+        }
+        if (it < n) process(ch, k0, a, begin, node, end);
     }
 };
 
@@ -928,7 +990,7 @@ struct Row
 // sum_{c_n<0} c_n (m + d) = sc I_sel or I_t.
 template <typename T>
 __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Row<T> &row,
-                                              const Ref &ref, double Lo, double R, double *tab,
+                                              const Ref &ref, double Lo, double R,
                                               double out[kPart])
 {
     const int lane = row.lane, Nv = row.Nv;
@@ -936,16 +998,20 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     auto nop = [] {};
     const double nx = c.wnx[i], ny = c.wny[i], nz = c.wnz[i];
     const double Uwx = c.wux[i], Uwy = c.wuy[i], Uwz = c.wuz[i], Tw = c.wt[i];
-    const double RT2w = 2.0 * R * Tw;
-    const double inorm = 1.0 / pow(AB_PI * RT2w, 1.5);
+    double RT2w = 0.0, inorm = 0.0;
     // the wall Maxwellian exp(-|v - U_w|^2 / 2RT_w) / norm: the v_z factor
-    // tabulated by the wave (lane l < Nv: node l), the (v_x, v_y) factor per pair
-    if (lane < Nv)
-    {
-        const double t = vnode(vlo, dv, lane);
-        tab[lane] = exp(-((t - Uwz) * (t - Uwz)) / RT2w);
-    }
-    wave_lds_sync();
+    // tabulated by the wave (lane l < Nv: node l) while the row's first loads
+    // are in flight, the (v_x, v_y) factor per pair
+    double wz = 0.0;                                   // lane l: node l's v_z factor
+    auto wall_tab = [&] {
+        RT2w = 2.0 * R * Tw;
+        inorm = 1.0 / pow(AB_PI * RT2w, 1.5);
+        if (lane < Nv)
+        {
+            const double t = vnode(vlo, dv, lane);
+            wz = exp(-((t - Uwz) * (t - Uwz)) / RT2w);
+        }
+    };
     // c_n = ((v_x - U_x) n_x + (v_y - U_y) n_y) + (v_z - U_z) n_z, summed in
     // that order without contraction: the pair's part first, as written
     double cnxy, mxy;
@@ -961,20 +1027,20 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     // pass 1: positivity sums; S_t, S_sel, S_m; I_t, I_sel; the outgoing flux
     double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
     double St = 0.0, Ssel = 0.0, Sm = 0.0, It = 0.0, Isel = 0.0, intP = 0.0;
-    row.pairs(pair_wall, [&](int p, int kv, double t, double m) {
+    row.pairs(wall_tab, pair_wall, [&](int p, int kv, double t, double m) {
+        // (selected values, not a selected accumulator: the latter becomes a
+        // private-memory array indexed per node)
         const double v = m + t;
+        const bool neg = v < 0.0;
         double vs = v;                                  // m + sel
-        if (v >= 0.0) mPos += v;
-        else
-        {
-            vs = m + (double)row.g(p, kv);
-            mNeg += v;
-            mFb += vs;
-        }
+        if (neg) vs = m + (double)row.g(p, kv);
+        mPos += neg ? 0.0 : v;
+        mNeg += neg ? v : 0.0;
+        mFb += neg ? vs : 0.0;
         St += t; Ssel += vs; Sm += m;
         const double cn = cn_of(kv);
         if (cn < 0.0) { It += cn * v; Isel += cn * vs; }
-        else if (cn > 0.0) intP += cn * (mxy * tab[kv]);
+        else if (cn > 0.0) intP += cn * (mxy * lane_value(wz, kv));
     }, nop);
     double r[9] = {mPos, mNeg, mFb, St, Ssel, Sm, It, Isel, intP};
     wave_sum8(*reinterpret_cast<double(*)[8]>(r));
@@ -997,9 +1063,9 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     // pass 2: the wall's re-emission where c_n > 0, the limited row elsewhere
     const T Lt = T(Lo);
     double h0 = 0;
-    row.pairs(pair_wall, [&](int p, int kv, double t, double m) {
+    row.pairs(nop, pair_wall, [&](int p, int kv, double t, double m) {
         double dh;
-        if (cn_of(kv) > 0.0) dh = rhoWL * (mxy * tab[kv]) - m;
+        if (cn_of(kv) > 0.0) dh = rhoWL * (mxy * lane_value(wz, kv)) - m;
         else if (!fires) dh = t;
         else
         {
@@ -1036,13 +1102,27 @@ struct Committed
 // Positivity (fallback to the pre-transport value, then the rescale to the
 // transported mass), the moments, and NAD against the neighbours' current
 // densities. The raw moments are summed in the positivity pass and redone
-// only when the repair fires (it then rewrites the transported row).
+// only when the repair fires (it then rewrites the transported row). The
+// pass sums the negative nodes only (mNeg, and their fallback values mFb);
+// the positive part is the row's total sum M0 + sum d less mNeg.
 template <typename T>
 __device__ __forceinline__ Raw commit_interior(const DevCloud &c, int i, const Row<T> &row,
                                                const Ref &ref, const double *rho, double Lo)
 {
     const int lane = row.lane;
     const double dv3 = c.dv * c.dv * c.dv;
+    auto nop = [] {};
+    // NAD bounds: the neighbours' current densities, two per lane at most
+    // (lanes past the list repeat its last entry). The list is read before
+    // the row's loads are issued and the densities while they are in
+    // flight; min and max are taken after the pass.
+    static_assert(AB_MAX_NEIGH <= 2 * kWave, "two neighbours per lane");
+    const int nn = c.nneigh[i];
+    const int *nb = c.neigh + (size_t)i * AB_MAX_NEIGH;
+    int j0 = 0, j1 = 0;
+    if (nn > 0) { j0 = nb[min(lane, nn - 1)]; j1 = nb[min(lane + kWave, nn - 1)]; }
+    double r0 = 0.0, r1 = 0.0;
+    auto gather = [&] { if (nn > 0) { r0 = rho[j0]; r1 = rho[j1]; } };
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};      // mPos mNeg mFb r0..r4
     // per pair: A = sum d, K1 = sum kv d, K2 = sum kv^2 d, then the pair's
     // share of the moments (v_z = vlo + kv dv)
@@ -1057,13 +1137,13 @@ __device__ __forceinline__ Raw commit_interior(const DevCloud &c, int i, const R
         const double q = (double)kv;
         A += d; K1 += q * d; K2 += (q * q) * d;
     };
-    row.pairs(begin, [&](int p, int kv, double d, double m) {
+    row.pairs(gather, begin, [&](int p, int kv, double d, double m) {
         const double v = m + d;
-        if (v >= 0.0) s[0] += v;
-        else { s[1] += v; s[2] += m + (double)row.g(p, kv); }
+        if (v < 0.0) { s[1] += v; s[2] += m + (double)row.g(p, kv); }
         accum(kv, d);
     }, end);
     wave_sum8(s);
+    s[0] = (ref.m0sum + s[3]) - s[1];
     Raw r;
     r.flag = 0.0;
     if (Lo * s[1] < -1e-30 && (s[0] + s[1]) > 0)
@@ -1073,7 +1153,7 @@ __device__ __forceinline__ Raw commit_interior(const DevCloud &c, int i, const R
         const double sc = (Lo * replaced > 1e-30) ? (s[0] + s[1]) / replaced : 0.0;
 #pragma unroll
         for (int q = 3; q < 8; ++q) s[q] = 0.0;
-        row.pairs(begin, [&](int p, int kv, double dg, double m) {
+        row.pairs(nop, begin, [&](int p, int kv, double dg, double m) {
             const double sel = (m + dg < 0.0) ? (double)row.g(p, kv) : dg;
             const T lim = T(sc * sel + (sc - 1.0) * m);
             row.set_t(p, kv, lim);
@@ -1087,14 +1167,12 @@ __device__ __forceinline__ Raw commit_interior(const DevCloud &c, int i, const R
     const double pyL = ref.mom[2] + s[5] * dv3;
     const double pzL = ref.mom[3] + s[6] * dv3;
     const double EL = ref.mom[4] + s[7] * dv3;
-    const int nn = c.nneigh[i];
     double rmin = 1e300, rmax = -1e300;
-    for (int k = lane; k < nn; k += kWave)
+    if (nn > 0)
     {
-        const double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
-        rmin = fmin(rmin, q); rmax = fmax(rmax, q);
+        rmin = uniform(wave_min(fmin(r0, r1)));
+        rmax = uniform(wave_max(fmax(r0, r1)));
     }
-    rmin = uniform(wave_min(rmin)); rmax = uniform(wave_max(rmax));
     const double rho_new = Lo * rhoL;
     double sc2 = 1.0;
     if (!(rmin > rmax))
@@ -1153,29 +1231,32 @@ __device__ __forceinline__ void continuous_target(const Committed &m, const Ref 
 // l): no exponential per node. Writes the stored row (folding the pending
 // rescale Lo into it) and returns the grid moments of dh (wave-uniform).
 template <typename T>
-__device__ __forceinline__ void collide_row(const Row<T> &row, double *tab, double sc2, bool okm,
+__device__ __forceinline__ void collide_row(const Row<T> &row, double sc2, bool okm,
                                             const double D[5], double lam, double Lo, double h[5])
 {
     const int lane = row.lane;
     const double coeff = 1.0 / (1.0 + lam);
-    if (lane < row.Nv)
-    {
-        const double t = vnode(row.vlo, row.dv, lane);
-        tab[lane] = exp(D[1] * t * t + D[4] * t);
-    }
-    wave_lds_sync();
+    // G_z (lane l: node l), tabulated while the row's first loads are in flight
+    double gz = 0.0;
+    auto gz_tab = [&] {
+        if (lane < row.Nv)
+        {
+            const double t = vnode(row.vlo, row.dv, lane);
+            gz = exp(D[1] * t * t + D[4] * t);
+        }
+    };
     const T Lt = T(Lo);
 #pragma unroll
     for (int q = 0; q < 5; ++q) h[q] = 0.0;
     // per pair: A = sum dh, K1 = sum kv dh, K2 = sum kv^2 dh (v_z = vlo + kv dv)
     const double vlo = row.vlo, dv = row.dv;
     double A = 0, K1 = 0, K2 = 0, px = 0, py = 0, Exy = 0;
-    row.pairs([&](int, int, double vx, double vy) {
+    row.pairs(gz_tab, [&](int, int, double vx, double vy) {
         A = K1 = K2 = 0.0; px = vx; py = vy;
         Exy = okm ? exp(D[0] + D[1] * (vx * vx + vy * vy) + D[2] * vx + D[3] * vy) : 0.0;
     }, [&](int p, int kv, double t, double m) {
         const double d2 = sc2 * t + (sc2 - 1.0) * m;
-        const double E = Exy * tab[kv];
+        const double E = Exy * lane_value(gz, kv);
         const T dh = T(coeff * (d2 + lam * (m * (E - 1.0))));
         row.set_g(p, kv, Lt * dh + T((Lo - 1.0) * m));
         const double hd = (double)dh, q = (double)kv;
@@ -1218,17 +1299,12 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
     const int lane = threadIdx.x & (kWave - 1);
     if (i >= c.N) return;
     const double Lo = scal[S_LAZY];
-    // the wave's LDS tables: M0's 1-D factors, then the v_z factor of a
-    // separable exponential
-    __shared__ double axis_tab[kBlock / kWave][2 * kMaxNv];
-    double *tab = axis_tab[threadIdx.x / kWave] + kMaxNv;
-    const Row<T> row(g + (size_t)i * L.S, gt + (size_t)i * L.S, ref, axis_tab[threadIdx.x / kWave],
-                     lane, c.Nv, c.vlo, c.dv);
+    const Row<T> row(g + (size_t)i * L.S, gt + (size_t)i * L.S, ref, lane, c.Nv, c.vlo, c.dv);
     double *pp = part + (size_t)i * kPart;
     if (c.bnd[i])
     {
         double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
-        post_boundary<T>(c, i, row, ref, Lo, R, tab, out);
+        post_boundary<T>(c, i, row, ref, Lo, R, out);
         if (lane < kPart - 1) pp[lane] = out[lane];
         return;
     }
@@ -1252,7 +1328,7 @@ __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__
         double D[5];
         continuous_target(m, ref, R, D);
         double h[5];
-        collide_row<T>(row, tab, m.sc2, m.okm, D, dt / tau, Lo, h);
+        collide_row<T>(row, m.sc2, m.okm, D, dt / tau, Lo, h);
         const double dv3 = c.dv * c.dv * c.dv;
         double out = 0.0;
         if (lane == 0) out = Lo * m.mr;
@@ -1376,15 +1452,13 @@ __global__ __launch_bounds__(256) void k_collide(DevCloud c, Lat L, T *__restric
     if (i >= c.N || c.bnd[i]) return;
     const double Lo = scal[S_LAZY];
     const size_t N = c.N;
-    __shared__ double axis_tab[kBlock / kWave][2 * kMaxNv];
-    double *tab = axis_tab[threadIdx.x / kWave] + kMaxNv;
-    const Row<T> row(g + (size_t)i * L.S, const_cast<T *>(gt) + (size_t)i * L.S, ref,
-                     axis_tab[threadIdx.x / kWave], lane, c.Nv, c.vlo, c.dv);
+    const Row<T> row(g + (size_t)i * L.S, const_cast<T *>(gt) + (size_t)i * L.S, ref, lane,
+                     c.Nv, c.vlo, c.dv);
     double D[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) D[q] = pst[(P_D + q) * N + i];
     double h[5];
-    collide_row<T>(row, tab, pst[P_SC2 * N + i], pst[P_OKM * N + i] != 0.0, D, dt / tau, Lo, h);
+    collide_row<T>(row, pst[P_SC2 * N + i], pst[P_OKM * N + i] != 0.0, D, dt / tau, Lo, h);
     const double dv3 = c.dv * c.dv * c.dv;
 #pragma unroll
     for (int q = 0; q < 5; ++q)
@@ -1630,10 +1704,10 @@ int apply_launch(ab_ctx *ctx, const Lat &L, int shape)
 void table_free(ab_ctx *ctx)
 {
     void *ptrs[] = {ctx->gs, ctx->gts, ctx->coef, ctx->cls, ctx->lat, ctx->part, ctx->bad,
-                    ctx->m0, ctx->m0x, ctx->red, ctx->pst};
+                    ctx->m0, ctx->m0x, ctx->m0xy, ctx->red, ctx->pst};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     ctx->gs = ctx->gts = ctx->coef = nullptr;
-    ctx->m0 = ctx->m0x = ctx->red = ctx->pst = nullptr;
+    ctx->m0 = ctx->m0x = ctx->m0xy = ctx->red = ctx->pst = nullptr;
     ctx->cls = ctx->lat = ctx->bad = nullptr;
     ctx->part = nullptr;
 }
@@ -1723,6 +1797,7 @@ int table_setup(ab_ctx *ctx)
               && hipMalloc((void **)&ctx->part, sizeof(double) * kPart * (size_t)N) == hipSuccess
               && hipMalloc((void **)&ctx->m0, sizeof(double) * ctx->S) == hipSuccess
               && hipMalloc((void **)&ctx->m0x, sizeof(double) * kMaxNv) == hipSuccess
+              && hipMalloc((void **)&ctx->m0xy, sizeof(double) * kMaxNv * kMaxNv) == hipSuccess
               && hipMalloc((void **)&ctx->red, sizeof(double) * 2 * kRedBlocks) == hipSuccess
               && (ctx->cfg.collision != AB_COLLISION_DISCRETE
                   || hipMalloc((void **)&ctx->pst, sizeof(double) * P_COUNT * (size_t)N) == hipSuccess);
@@ -1751,6 +1826,9 @@ int table_setup(ab_ctx *ctx)
             ex[l] = std::exp(-(v * v) / twoRT);
         }
         ctx->m0c = ctx->m0rho / norm;
+        std::vector<double> mxy((size_t)kMaxNv * kMaxNv, 0.0);
+        for (int p = 0; p < Nv * Nv; ++p) mxy[p] = (ctx->m0c * ex[p % Nv]) * ex[p / Nv];
+        ctx->m0sum = 0.0;
         for (int q = 0; q < 5; ++q) ctx->m0mom[q] = 0.0;
         for (int k = 0; k < ctx->nodes; ++k)
         {
@@ -1759,6 +1837,7 @@ int table_setup(ab_ctx *ctx)
             const double vy = vnode(ctx->vlo, ctx->dv, jv);
             const double vz = vnode(ctx->vlo, ctx->dv, kv);
             m0[k] = ((ctx->m0c * ex[iv]) * ex[jv]) * ex[kv];
+            ctx->m0sum += m0[k];
             ctx->m0mom[0] += m0[k] * dv3;
             ctx->m0mom[1] += vx * m0[k] * dv3;
             ctx->m0mom[2] += vy * m0[k] * dv3;
@@ -1768,6 +1847,8 @@ int table_setup(ab_ctx *ctx)
         if (hipMemcpyAsync(ctx->m0, m0.data(), sizeof(double) * ctx->S, hipMemcpyHostToDevice, s)
                 != hipSuccess
             || hipMemcpyAsync(ctx->m0x, ex.data(), sizeof(double) * kMaxNv, hipMemcpyHostToDevice, s)
+               != hipSuccess
+            || hipMemcpyAsync(ctx->m0xy, mxy.data(), sizeof(double) * mxy.size(), hipMemcpyHostToDevice, s)
                 != hipSuccess
             || hipStreamSynchronize(s) != hipSuccess)
         { ctx->err = "table setup: host-to-device copy failed"; return AB_ERR_HIP; }
@@ -1863,8 +1944,10 @@ int table_step(ab_ctx *ctx, double dt)
     const int nbp = (ctx->N * kWave + 255) / 256;
     Ref ref;
     ref.m0 = ctx->m0;
+    ref.m0xy = ctx->m0xy;
     ref.m0x = ctx->m0x;
     ref.m0c = ctx->m0c;
+    ref.m0sum = ctx->m0sum;
     for (int q = 0; q < 5; ++q) ref.mom[q] = ctx->m0mom[q];
     ref.rho = ctx->m0rho;
     ref.T = ctx->m0T;
