@@ -848,32 +848,34 @@ struct Ref
     double rho, T;          // reference state of M0
 };
 
-// A particle's row visited pair by pair: the lane takes the (iv, jv) pairs
-// p = iv + Nv jv = lane, lane + 64, ... of the velocity plane and, for each,
-// the Nv nodes k = p + P kv (P = Nv^2) in order. v_x, v_y (and anything that
-// depends only on them) are then fixed per pair and v_z is the same on every
-// lane, so a moment sum_k psi(v) d costs three accumulators per node
-// (sum d, sum kv d, sum kv^2 d), combined with v_x, v_y once per pair.
+// A particle's row visited pair by pair: the lane takes two (iv, jv) pairs
+// of the velocity plane at a time (slots s = 0, 1: p = 128 c + 64 s + lane
+// for chunk pair c) and, for each, the Nv nodes k = p + P kv (P = Nv^2).
+// v_x, v_y (and anything that depends only on them) are then fixed per pair
+// and v_z is the same on every lane, so a moment sum_k psi(v) d costs three
+// accumulators per node (sum d, sum kv d, sum kv^2 d), combined with v_x,
+// v_y once per pair; what depends on v_z alone is computed once per plane
+// for both slots (the `plane` callback).
 // Accesses are buffer operations with the lane's pair in the vector offset
 // and the node's plane kv P in the scalar offset (no per-node address
-// arithmetic).
-// The pass is a sequence of items (64 pairs x kB planes). The loads of item
-// n + 1 are issued before item n is processed, so a wave keeps two items in
-// flight (for Nv <= 10 the whole row); the first item's loads are issued
-// before `setup`, whose own memory latency (per-particle constants, tables)
-// then overlaps them. Lanes past the last pair and planes past the last
-// reload the last valid ones (in bounds, no branch around the loads) and
-// skip the work.
+// arithmetic). An item is kB planes of both slots: all its loads are in
+// flight together (for Nv <= 10 the whole row), the first item's issued
+// before `setup`, whose own memory latency (per-particle constants) then
+// overlaps them. A slot past the last pair gets an out-of-range offset
+// (its loads return 0, its stores are dropped) and an M0 factor of 0, so
+// it adds nothing to any sum; planes past the last reload the last one
+// and are skipped.
 // M0 of a node is the pair's factor (m0c e_iv) e_jv (a table built with M0)
 // times the plane's e_kv, rounded as the table of M0 was built. Per-wave
 // tables over v_z (e_kv here, the callers' separable factors) are held one
 // entry per lane (lane l: node l) and read into scalar registers with
 // lane_value: no memory access per node.
-//   setup()   begin(iv, jv, vx, vy)   node(p, kv, t, m0)   end()
+//   setup()  begin(s, iv, jv, vx, vy, ok)  plane(kv)  node(s, p, kv, t, m0)  end(s)
 template <typename T>
 struct Row
 {
     static constexpr int kB = 10;
+    static constexpr int kOut = (int)0x80000000;   // vector offset past any buffer
     __amdgpu_buffer_rsrc_t rg;    // stored row (pre-transport values; written last)
     __amdgpu_buffer_rsrc_t rt;    // transported row
     const double *m0xy;
@@ -887,95 +889,72 @@ struct Row
     }
     // (the plane offset is wave-uniform; readfirstlane keeps it in a scalar
     // register where the compiler would otherwise loop over the lanes)
-    __device__ __forceinline__ int soff(int kv, int bytes) const
-    { return __builtin_amdgcn_readfirstlane(kv * P * bytes); }
-    __device__ __forceinline__ T g(int p, int kv) const
-    { return bload_so<T>(rg, p * (int)sizeof(T), soff(kv, sizeof(T))); }
+    __device__ __forceinline__ int soff(int kv) const
+    { return __builtin_amdgcn_readfirstlane(kv * P * (int)sizeof(T)); }
+    // the byte offset of pair p's value in a plane (kOut for a slot past the end)
+    __device__ __forceinline__ int voff(int p) const
+    { return p < P ? p * (int)sizeof(T) : kOut; }
+    __device__ __forceinline__ T g(int p, int kv) const { return bload_so<T>(rg, voff(p), soff(kv)); }
     __device__ __forceinline__ void set_g(int p, int kv, T v) const
-    { bstore_so<T>(rg, p * (int)sizeof(T), soff(kv, sizeof(T)), v); }
+    { bstore_so<T>(rg, voff(p), soff(kv), v); }
     __device__ __forceinline__ void set_t(int p, int kv, T v) const
-    { bstore_so<T>(rt, p * (int)sizeof(T), soff(kv, sizeof(T)), v); }
-    // one item (chunk ch of 64 pairs, planes k0 .. k0 + kB - 1): the
-    // transported values and the pair's M0 factor
-    struct Item
+    { bstore_so<T>(rt, voff(p), soff(kv), v); }
+    template <typename FS, typename FB, typename FP, typename FN, typename FE>
+    __device__ __forceinline__ void pairs(FS &&setup, FB &&begin, FP &&plane, FN &&node,
+                                          FE &&end) const
     {
-        T v[kB];
-        double mxy;
-    };
-    // (ch and k0 are wave-uniform; readfirstlane states it to the compiler,
-    // which otherwise can lose track of it in the callers' control flow and
-    // wrap each buffer access in a loop over the lanes' scalar offsets)
-    __device__ __forceinline__ void load(int ch, int k0, Item &it) const
-    {
-        ch = __builtin_amdgcn_readfirstlane(ch);
-        k0 = __builtin_amdgcn_readfirstlane(k0);
-        const int p = min(ch * kWave + lane, P - 1);
-        it.mxy = m0xy[p];
-#pragma unroll
-        for (int j = 0; j < kB; ++j)
-            it.v[j] = bload_so<T>(rt, p * (int)sizeof(T), soff(min(k0 + j, Nv - 1), sizeof(T)));
-    }
-    template <typename FB, typename FN, typename FE>
-    __device__ __forceinline__ void process(int ch, int k0, const Item &it, FB &begin, FN &node,
-                                            FE &end) const
-    {
-        ch = __builtin_amdgcn_readfirstlane(ch);
-        k0 = __builtin_amdgcn_readfirstlane(k0);
-        const int p = ch * kWave + lane;
-        if (p < P)
-        {
-            if (k0 == 0)
-            {
-                const int jv = p / Nv, iv = p - jv * Nv;
-                begin(iv, jv, vnode(vlo, dv, iv), vnode(vlo, dv, jv));
-            }
-#pragma unroll
-            for (int j = 0; j < kB; ++j)
-                if (k0 + j < Nv)
-                    node(p, k0 + j, (double)it.v[j], it.mxy * lane_value(ezl, k0 + j));
-            if (k0 + kB >= Nv) end();
-        }
-        // every value of the item is complete here, also on the paths that
-        // skipped it (otherwise the wait for it lands on a later reuse of its
-        // register, which waits for every load issued since)
-#pragma unroll
-        for (int j = 0; j < kB; ++j) asm volatile("" ::"v"(it.v[j]));
-        asm volatile("" ::"v"(it.mxy));
-    }
-    __device__ __forceinline__ void advance(int &ch, int &k0) const
-    {
-        k0 += kB;
-        if (k0 >= Nv) { k0 = 0; ++ch; }
-    }
-    // Two item buffers in alternating roles (the loop is unrolled by two):
-    // rotating one buffer into the other would wait for its loads. A round
-    // issues the loads of its second item, then processes its first, then
-    // issues the loads of the next round's first item and processes its
-    // second (the waits of that item also cover the next round's loads,
-    // as the last round issues none). Up to Nv = 10 a pass is one round.
-    template <typename FS, typename FB, typename FN, typename FE>
-    __device__ __forceinline__ void pairs(FS &&setup, FB &&begin, FN &&node, FE &&end) const
-    {
+        const int ncp = (P + 2 * kWave - 1) / (2 * kWave);
         const int nkb = (Nv + kB - 1) / kB;
-        const int n = ((P + kWave - 1) / kWave) * nkb;
-        Item a, b;
-        load(0, 0, a);
-        setup();
-        int ch = 0, k0 = 0;
-        int it = 0;
-        for (; it + 1 < n; it += 2)
+        for (int c = 0; c < ncp; ++c)
         {
-            int c1 = ch, k1 = k0;
-            advance(c1, k1);
-            load(c1, k1, b);
-            process(ch, k0, a, begin, node, end);
-            int c2 = c1, k2 = k1;
-            advance(c2, k2);
-            if (it + 2 < n) load(c2, k2, a);
-            process(c1, k1, b, begin, node, end);
-            ch = c2; k0 = k2;
+            int p[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) p[s] = (2 * c + s) * kWave + lane;
+            for (int b = 0; b < nkb; ++b)
+            {
+                const int k0 = __builtin_amdgcn_readfirstlane(b * kB);
+                T v[2][kB];
+                double mxy[2];
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                {
+                    mxy[s] = p[s] < P ? m0xy[p[s]] : 0.0;
+#pragma unroll
+                    for (int j = 0; j < kB; ++j)
+                        v[s][j] = bload_so<T>(rt, voff(p[s]), soff(min(k0 + j, Nv - 1)));
+                }
+                if (c == 0 && b == 0) setup();
+                if (b == 0)
+                {
+#pragma unroll
+                    for (int s = 0; s < 2; ++s)
+                    {
+                        const int q = min(p[s], P - 1), jv = q / Nv, iv = q - jv * Nv;
+                        begin(s, iv, jv, vnode(vlo, dv, iv), vnode(vlo, dv, jv), p[s] < P);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kB; ++j)
+                {
+                    const int kv = k0 + j;
+                    if (kv < Nv)
+                    {
+                        const double ez = lane_value(ezl, kv);
+                        plane(kv);
+#pragma unroll
+                        for (int s = 0; s < 2; ++s) node(s, p[s], kv, (double)v[s][j], mxy[s] * ez);
+                    }
+                    // planes one after the other: hoisting the later planes'
+                    // conversions would hold them all in registers
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (k0 + kB >= Nv)
+                {
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) end(s);
+                }
+            }
         }
-        if (it < n) process(ch, k0, a, begin, node, end);
     }
 };
 
@@ -1001,7 +980,8 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     double RT2w = 0.0, inorm = 0.0;
     // the wall Maxwellian exp(-|v - U_w|^2 / 2RT_w) / norm: the v_z factor
     // tabulated by the wave (lane l < Nv: node l) while the row's first loads
-    // are in flight, the (v_x, v_y) factor per pair
+    // are in flight, the (v_x, v_y) factor per pair (0 for a slot past the
+    // last pair)
     double wz = 0.0;                                   // lane l: node l's v_z factor
     auto wall_tab = [&] {
         RT2w = 2.0 * R * Tw;
@@ -1014,20 +994,25 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     };
     // c_n = ((v_x - U_x) n_x + (v_y - U_y) n_y) + (v_z - U_z) n_z, summed in
     // that order without contraction: the pair's part first, as written
-    double cnxy, mxy;
-    auto pair_wall = [&](int, int, double vx, double vy) {
+    double cnxy[2], wxy[2], cz = 0.0, wzk = 0.0;
+    auto pair_wall = [&](int s, int, int, double vx, double vy, bool ok) {
 #pragma clang fp contract(off)
-        cnxy = (vx - Uwx) * nx + (vy - Uwy) * ny;
-        mxy = exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)) / RT2w) * inorm;
+        cnxy[s] = (vx - Uwx) * nx + (vy - Uwy) * ny;
+        wxy[s] = ok ? exp(-((vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy)) / RT2w) * inorm
+                    : 0.0;
     };
-    auto cn_of = [&](int kv) -> double {
+    auto plane_wall = [&](int kv) {
+        cz = (vnode(vlo, dv, kv) - Uwz) * nz;
+        wzk = lane_value(wz, kv);
+    };
+    auto cn_of = [&](int s) -> double {
 #pragma clang fp contract(off)
-        return cnxy + (vnode(vlo, dv, kv) - Uwz) * nz;
+        return cnxy[s] + cz;
     };
     // pass 1: positivity sums; S_t, S_sel, S_m; I_t, I_sel; the outgoing flux
     double mPos = 0.0, mNeg = 0.0, mFb = 0.0;
     double St = 0.0, Ssel = 0.0, Sm = 0.0, It = 0.0, Isel = 0.0, intP = 0.0;
-    row.pairs(wall_tab, pair_wall, [&](int p, int kv, double t, double m) {
+    row.pairs(wall_tab, pair_wall, plane_wall, [&](int s, int p, int kv, double t, double m) {
         // (selected values, not a selected accumulator: the latter becomes a
         // private-memory array indexed per node)
         const double v = m + t;
@@ -1038,10 +1023,10 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
         mNeg += neg ? v : 0.0;
         mFb += neg ? vs : 0.0;
         St += t; Ssel += vs; Sm += m;
-        const double cn = cn_of(kv);
+        const double cn = cn_of(s);
         if (cn < 0.0) { It += cn * v; Isel += cn * vs; }
-        else if (cn > 0.0) intP += cn * (mxy * lane_value(wz, kv));
-    }, nop);
+        else if (cn > 0.0) intP += cn * (wxy[s] * wzk);
+    }, [](int) {});
     double r[9] = {mPos, mNeg, mFb, St, Ssel, Sm, It, Isel, intP};
     wave_sum8(*reinterpret_cast<double(*)[8]>(r));
     r[8] = uniform(wave_sum(r[8]));
@@ -1063,9 +1048,9 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
     // pass 2: the wall's re-emission where c_n > 0, the limited row elsewhere
     const T Lt = T(Lo);
     double h0 = 0;
-    row.pairs(nop, pair_wall, [&](int p, int kv, double t, double m) {
+    row.pairs(nop, pair_wall, plane_wall, [&](int s, int p, int kv, double t, double m) {
         double dh;
-        if (cn_of(kv) > 0.0) dh = rhoWL * (mxy * lane_value(wz, kv)) - m;
+        if (cn_of(s) > 0.0) dh = rhoWL * (wxy[s] * wzk) - m;
         else if (!fires) dh = t;
         else
         {
@@ -1075,7 +1060,7 @@ __device__ __forceinline__ void post_boundary(const DevCloud &c, int i, const Ro
         const T dhT = T(dh);
         row.set_g(p, kv, Lt * dhT + T((Lo - 1.0) * m));
         h0 += (double)dhT;
-    }, nop);
+    }, [](int) {});
     out[1] = Lo * (ref.mom[0] + wave_sum(h0) * dv3);
     out[6] = fires ? 1.0 : 0.0;
 }
@@ -1127,20 +1112,22 @@ __device__ __forceinline__ Raw commit_interior(const DevCloud &c, int i, const R
     // per pair: A = sum d, K1 = sum kv d, K2 = sum kv^2 d, then the pair's
     // share of the moments (v_z = vlo + kv dv)
     const double vlo = c.vlo, dv = c.dv;
-    double A = 0, K1 = 0, K2 = 0, px = 0, py = 0;
-    auto begin = [&](int, int, double vx, double vy) { A = K1 = K2 = 0.0; px = vx; py = vy; };
-    auto end = [&]() {
-        s[3] += A; s[4] += px * A; s[5] += py * A; s[6] += vlo * A + dv * K1;
-        s[7] += (px * px + py * py + vlo * vlo) * A + dv * (2.0 * vlo * K1 + dv * K2);
+    double A[2], K1[2], K2[2], px[2], py[2], q = 0.0, q2 = 0.0;
+    auto begin = [&](int sl, int, int, double vx, double vy, bool) {
+        A[sl] = K1[sl] = K2[sl] = 0.0; px[sl] = vx; py[sl] = vy;
     };
-    auto accum = [&](int kv, double d) {
-        const double q = (double)kv;
-        A += d; K1 += q * d; K2 += (q * q) * d;
+    auto plane = [&](int kv) { q = (double)kv; q2 = q * q; };
+    auto end = [&](int sl) {
+        s[3] += A[sl]; s[4] += px[sl] * A[sl]; s[5] += py[sl] * A[sl];
+        s[6] += vlo * A[sl] + dv * K1[sl];
+        s[7] += (px[sl] * px[sl] + py[sl] * py[sl] + vlo * vlo) * A[sl]
+              + dv * (2.0 * vlo * K1[sl] + dv * K2[sl]);
     };
-    row.pairs(gather, begin, [&](int p, int kv, double d, double m) {
+    auto accum = [&](int sl, double d) { A[sl] += d; K1[sl] += q * d; K2[sl] += q2 * d; };
+    row.pairs(gather, begin, plane, [&](int sl, int p, int kv, double d, double m) {
         const double v = m + d;
         if (v < 0.0) { s[1] += v; s[2] += m + (double)row.g(p, kv); }
-        accum(kv, d);
+        accum(sl, d);
     }, end);
     wave_sum8(s);
     s[0] = (ref.m0sum + s[3]) - s[1];
@@ -1152,12 +1139,12 @@ __device__ __forceinline__ Raw commit_interior(const DevCloud &c, int i, const R
         const double replaced = s[0] + s[2];
         const double sc = (Lo * replaced > 1e-30) ? (s[0] + s[1]) / replaced : 0.0;
 #pragma unroll
-        for (int q = 3; q < 8; ++q) s[q] = 0.0;
-        row.pairs(nop, begin, [&](int p, int kv, double dg, double m) {
+        for (int k = 3; k < 8; ++k) s[k] = 0.0;
+        row.pairs(nop, begin, plane, [&](int sl, int p, int kv, double dg, double m) {
             const double sel = (m + dg < 0.0) ? (double)row.g(p, kv) : dg;
             const T lim = T(sc * sel + (sc - 1.0) * m);
             row.set_t(p, kv, lim);
-            accum(kv, (double)lim);
+            accum(sl, (double)lim);
         }, end);
         s[0] = s[1] = s[2] = 0.0;
         wave_sum8(s);
@@ -1227,15 +1214,18 @@ __device__ __forceinline__ void continuous_target(const Committed &m, const Ref 
 // d'' = sc2 d + (sc2 - 1) M0 the committed deviation, M - M0 = M0 (exp(D) - 1)
 // in FP64 (target 0 when !okm). exp(D) = G_xy(v_x, v_y) G_z(v_z) with
 // G_xy = exp(A0 + Bq (v_x^2 + v_y^2) + Cx v_x + Cy v_y) once per pair and
-// G_z(t) = exp(Bq t^2 + Cz t) tabulated by the wave in LDS (lane l < Nv: node
-// l): no exponential per node. Writes the stored row (folding the pending
-// rescale Lo into it) and returns the grid moments of dh (wave-uniform).
+// G_z(t) = exp(Bq t^2 + Cz t) tabulated by the wave (lane l < Nv: node l):
+// no exponential per node. Per node the update is evaluated as
+// dh = c1 d + M0 (c2 + c3 (E - 1)) with c1 = coeff sc2, c2 = coeff (sc2 - 1),
+// c3 = coeff lam. Writes the stored row (folding the pending rescale Lo into
+// it) and returns the grid moments of dh (wave-uniform).
 template <typename T>
 __device__ __forceinline__ void collide_row(const Row<T> &row, double sc2, bool okm,
                                             const double D[5], double lam, double Lo, double h[5])
 {
     const int lane = row.lane;
     const double coeff = 1.0 / (1.0 + lam);
+    const double c1 = coeff * sc2, c2 = coeff * (sc2 - 1.0), c3 = coeff * lam;
     // G_z (lane l: node l), tabulated while the row's first loads are in flight
     double gz = 0.0;
     auto gz_tab = [&] {
@@ -1246,29 +1236,34 @@ __device__ __forceinline__ void collide_row(const Row<T> &row, double sc2, bool 
         }
     };
     const T Lt = T(Lo);
+    const double Lo1 = Lo - 1.0;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) h[q] = 0.0;
+    for (int k = 0; k < 5; ++k) h[k] = 0.0;
     // per pair: A = sum dh, K1 = sum kv dh, K2 = sum kv^2 dh (v_z = vlo + kv dv)
     const double vlo = row.vlo, dv = row.dv;
-    double A = 0, K1 = 0, K2 = 0, px = 0, py = 0, Exy = 0;
-    row.pairs(gz_tab, [&](int, int, double vx, double vy) {
-        A = K1 = K2 = 0.0; px = vx; py = vy;
-        Exy = okm ? exp(D[0] + D[1] * (vx * vx + vy * vy) + D[2] * vx + D[3] * vy) : 0.0;
-    }, [&](int p, int kv, double t, double m) {
-        const double d2 = sc2 * t + (sc2 - 1.0) * m;
-        const double E = Exy * lane_value(gz, kv);
-        const T dh = T(coeff * (d2 + lam * (m * (E - 1.0))));
-        row.set_g(p, kv, Lt * dh + T((Lo - 1.0) * m));
-        const double hd = (double)dh, q = (double)kv;
-        A += hd; K1 += q * hd; K2 += (q * q) * hd;
-    }, [&]() {
-        h[0] += A; h[1] += px * A; h[2] += py * A; h[3] += vlo * A + dv * K1;
-        h[4] += (px * px + py * py + vlo * vlo) * A + dv * (2.0 * vlo * K1 + dv * K2);
+    double A[2], K1[2], K2[2], px[2], py[2], Exy[2], gzk = 0.0, q = 0.0, q2 = 0.0;
+    row.pairs(gz_tab, [&](int s, int, int, double vx, double vy, bool) {
+        A[s] = K1[s] = K2[s] = 0.0; px[s] = vx; py[s] = vy;
+        Exy[s] = okm ? exp(D[0] + D[1] * (vx * vx + vy * vy) + D[2] * vx + D[3] * vy) : 0.0;
+    }, [&](int kv) {
+        gzk = lane_value(gz, kv);
+        q = (double)kv; q2 = q * q;
+    }, [&](int s, int p, int kv, double t, double m) {
+        const double Em1 = fma(Exy[s], gzk, -1.0);
+        const T dh = T(fma(c1, t, m * fma(c3, Em1, c2)));
+        row.set_g(p, kv, Lt * dh + T(Lo1 * m));
+        const double hd = (double)dh;
+        A[s] += hd; K1[s] += q * hd; K2[s] += q2 * hd;
+    }, [&](int s) {
+        h[0] += A[s]; h[1] += px[s] * A[s]; h[2] += py[s] * A[s];
+        h[3] += vlo * A[s] + dv * K1[s];
+        h[4] += (px[s] * px[s] + py[s] * py[s] + vlo * vlo) * A[s]
+              + dv * (2.0 * vlo * K1[s] + dv * K2[s]);
     });
     double t[8] = {h[0], h[1], h[2], h[3], h[4], 0.0, 0.0, 0.0};
     wave_sum8(t);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) h[q] = t[q];
+    for (int k = 0; k < 5; ++k) h[k] = t[k];
 }
 
 // Per-particle state between the discrete-Maxwellian kernels, structure of
@@ -1277,37 +1272,57 @@ __device__ __forceinline__ void collide_row(const Row<T> &row, double sc2, bool 
 // exponent (continuous, or discrete where Newton converged).
 enum PostSlot { P_SC2, P_OKM, P_RAW, P_D = P_RAW + 5, P_COUNT = P_D + 5 };
 
-// One wave per particle. Boundary particles: positivity and the diffuse wall.
-// Interior particles: positivity, NAD and moments; then (continuous
-// collision) the BGK collision in the same kernel, or (DM) the raw moments
-// and the NAD factor are handed to k_dm_solve, which derives the committed
-// state one thread per particle, and k_collide. Partials: [0] committed
-// mass, [1..5] moments after the collision, [6] limiter bits.
+// The particle of a wave of a kernel over a particle list, in a scalar
+// register (the row's buffer descriptors are uniform); -1 past the list.
+// Consecutive blocks go to different XCDs (no remap): a wave reads only its
+// own row.
+__device__ __forceinline__ int list_particle(const int *__restrict__ list, int n)
+{
+    const int w = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    return w < n ? __builtin_amdgcn_readfirstlane(list[w]) : -1;
+}
+
+// One wave per boundary particle (the boundary list): positivity and the
+// diffuse wall. Partials: [0] committed mass, [1] mass after the wall, [2]
+// the wall density, [6] limiter bits.
+template <typename T>
+__global__ __launch_bounds__(256) void k_wall(DevCloud c, Lat L, T *__restrict__ g,
+                                              T *__restrict__ gt, Ref ref,
+                                              const double *__restrict__ scal, double R,
+                                              double *__restrict__ part,
+                                              const int *__restrict__ blist, int nb)
+{
+    const int i = list_particle(blist, nb);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (i < 0) return;
+    const double Lo = scal[S_LAZY];
+    const Row<T> row(g + (size_t)i * L.S, gt + (size_t)i * L.S, ref, lane, c.Nv, c.vlo, c.dv);
+    double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
+    post_boundary<T>(c, i, row, ref, Lo, R, out);
+    if (lane < kPart - 1) part[(size_t)i * kPart + lane] = out[lane];
+}
+
+// One wave per interior particle (the interior list): positivity, NAD and
+// moments; then (continuous collision) the BGK collision in the same kernel,
+// or (DM) the raw moments and the NAD factor are handed to k_dm_solve, which
+// derives the committed state one thread per particle, and k_collide.
+// Partials: [0] committed mass, [1..5] moments after the collision, [6]
+// limiter bits.
 template <typename T, bool DM>
 __global__ __launch_bounds__(256) void k_post(DevCloud c, Lat L, T *__restrict__ g,
                                               T *__restrict__ gt, Ref ref,
                                               const double *__restrict__ rho,
                                               const double *__restrict__ scal, double dt,
                                               double tau, double R, double *__restrict__ part,
-                                              double *__restrict__ pst)
+                                              double *__restrict__ pst,
+                                              const int *__restrict__ ilist, int ni)
 {
-    // the wave's particle, in a scalar register (the row's buffer descriptors
-    // are uniform). Consecutive blocks go to different XCDs (no remap): a
-    // wave reads only its own row, and the boundary faces, whose waves cost
-    // several interior ones, are then spread over all eight XCDs.
-    const int i = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    const int i = list_particle(ilist, ni);
     const int lane = threadIdx.x & (kWave - 1);
-    if (i >= c.N) return;
+    if (i < 0) return;
     const double Lo = scal[S_LAZY];
     const Row<T> row(g + (size_t)i * L.S, gt + (size_t)i * L.S, ref, lane, c.Nv, c.vlo, c.dv);
     double *pp = part + (size_t)i * kPart;
-    if (c.bnd[i])
-    {
-        double out[kPart] = {0, 0, 0, 0, 0, 0, 0, 0};
-        post_boundary<T>(c, i, row, ref, Lo, R, out);
-        if (lane < kPart - 1) pp[lane] = out[lane];
-        return;
-    }
     const Raw w = commit_interior<T>(c, i, row, ref, rho, Lo);
     if constexpr (DM)
     {
@@ -1444,12 +1459,12 @@ __global__ __launch_bounds__(256) void k_collide(DevCloud c, Lat L, T *__restric
                                                  const T *__restrict__ gt, Ref ref,
                                                  const double *__restrict__ scal, double dt,
                                                  double tau, const double *__restrict__ pst,
-                                                 double *__restrict__ part)
+                                                 double *__restrict__ part,
+                                                 const int *__restrict__ ilist, int ni)
 {
-    const int i = __builtin_amdgcn_readfirstlane(
-        (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave);
+    const int i = list_particle(ilist, ni);
     const int lane = threadIdx.x & (kWave - 1);
-    if (i >= c.N || c.bnd[i]) return;
+    if (i < 0) return;
     const double Lo = scal[S_LAZY];
     const size_t N = c.N;
     const Row<T> row(g + (size_t)i * L.S, const_cast<T *>(gt) + (size_t)i * L.S, ref, lane,
@@ -1941,7 +1956,6 @@ int table_step(ab_ctx *ctx, double dt)
     hipStream_t s = ctx->stream;
     auto tic = [&]() { prof_begin(ctx); };
     auto toc = [&](int st) { prof_end(ctx, st); };
-    const int nbp = (ctx->N * kWave + 255) / 256;
     Ref ref;
     ref.m0 = ctx->m0;
     ref.m0xy = ctx->m0xy;
@@ -1959,21 +1973,30 @@ int table_step(ab_ctx *ctx, double dt)
     tic();
     const DmArgs dm = dm_args(ctx);
     const int nbs = (ctx->N + 255) / 256;
+    // one wave per particle of each list (four per block)
+    const int ni = ctx->n_int, nb = ctx->n_bnd;
+    const int nbi = (ni + 3) / 4, nbb = (nb + 3) / 4;
     auto post = [&](auto *gs) {
         using T = std::remove_pointer_t<decltype(gs)>;
         T *gts = (T *)ctx->gts;
+        if (nb)
+            k_wall<T><<<nbb, 256, 0, s>>>(c, L, gs, gts, ref, ctx->scal, ctx->cfg.R, ctx->part,
+                                          ctx->blist, nb);
+        if (!ni) return;
         if (!dm.on)
         {
-            k_post<T, false><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->rho, ctx->scal, dt,
-                                                 ctx->cfg.tau, ctx->cfg.R, ctx->part, nullptr);
+            k_post<T, false><<<nbi, 256, 0, s>>>(c, L, gs, gts, ref, ctx->rho, ctx->scal, dt,
+                                                 ctx->cfg.tau, ctx->cfg.R, ctx->part, nullptr,
+                                                 ctx->ilist, ni);
             return;
         }
-        k_post<T, true><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->rho, ctx->scal, dt,
-                                            ctx->cfg.tau, ctx->cfg.R, ctx->part, ctx->pst);
+        k_post<T, true><<<nbi, 256, 0, s>>>(c, L, gs, gts, ref, ctx->rho, ctx->scal, dt,
+                                            ctx->cfg.tau, ctx->cfg.R, ctx->part, ctx->pst,
+                                            ctx->ilist, ni);
         k_dm_solve<<<nbs, 256, 0, s>>>(ctx->N, c.bnd, dm, ref, ctx->scal, ctx->cfg.R, ctx->Nv,
                                        ctx->vlo, ctx->dv, ctx->pst);
-        k_collide<T><<<nbp, 256, 0, s>>>(c, L, gs, gts, ref, ctx->scal, dt, ctx->cfg.tau,
-                                         ctx->pst, ctx->part);
+        k_collide<T><<<nbi, 256, 0, s>>>(c, L, gs, gts, ref, ctx->scal, dt, ctx->cfg.tau,
+                                         ctx->pst, ctx->part, ctx->ilist, ni);
     };
     if (is_fp32(ctx)) post((float *)ctx->gs);
     else post((double *)ctx->gs);
