@@ -39,6 +39,27 @@ _up = C.POINTER(C.c_ubyte)
 _fp = C.POINTER(C.c_float)
 
 
+BODY_SQUARE, BODY_CIRCLE, BODY_SPHERE, BODY_CUBE = 1, 2, 3, 4
+
+
+class ABBody(C.Structure):
+    _fields_ = [("shape", C.c_int), ("c", C.c_double * 3), ("size", C.c_double),
+                ("o", C.c_double * 2), ("rm", C.c_double * 9)]
+
+
+def body(shape, centre, size, o=(1.0, 0.0), rm=(1, 0, 0, 0, 1, 0, 0, 0, 1)):
+    """A rigid body for cloud management (RigidBody/Geometry.hpp): shape
+    BODY_SQUARE (side size, orientation o), BODY_CIRCLE (radius), BODY_SPHERE
+    (radius), BODY_CUBE (side size, body-to-world rotation rm, row-major)."""
+    b = ABBody()
+    b.shape = int(shape)
+    b.c[:] = [float(v) for v in centre] + [0.0] * (3 - len(centre))
+    b.size = float(size)
+    b.o[:] = [float(v) for v in o]
+    b.rm[:] = [float(v) for v in rm]
+    return b
+
+
 class ABConfig(C.Structure):
     _fields_ = [("dim", C.c_int), ("n_particles", C.c_int), ("nv", C.c_int),
                 ("vmin", C.c_double), ("vmax", C.c_double), ("R", C.c_double),
@@ -117,7 +138,8 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_dm_rows", "ab_mlp_init", "ab_mlp_train", "ab_mlp_enable", "ab_mlp_get",
                "ab_mlp_set", "ab_mlp_forward", "ab_mlp_infer", "ab_mlp_loss_grad", "ab_graph_create",
                "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy", "ab_autotune",
-               "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels"]
+               "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels",
+               "ab_cloud_manage", "ab_cloud_source_map", "ab_download_particles"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
@@ -465,6 +487,43 @@ class Solver:
         m = C.c_double()
         self._check(self._lib.ab_total_mass(self._ctx, C.byref(m)), "ab_total_mass")
         return m.value
+
+    def particles(self):
+        """Positions, boundary flags and wall state of the device cloud
+        (ab_download_particles): dict x, y, z, boundary, wall (N x 7)."""
+        x, y, z = np.zeros(self.N), np.zeros(self.N), np.zeros(self.N)
+        b = np.zeros(self.N, np.uint8)
+        w = np.zeros((self.N, 7))
+        self._check(self._lib.ab_download_particles(self._ctx, _ptr(x, _dp), _ptr(y, _dp),
+                                                    _ptr(z, _dp), _ptr(b, _up), _ptr(w, _dp)),
+                    "ab_download_particles")
+        return dict(x=x, y=y, z=z, boundary=b, wall=w)
+
+    def cloud_manage(self, bodies, body_id, dx, min_dist, dom):
+        """manageRemove, mergeCloseParticles and manageRefill with the
+        neighbour rebuilds between them (RigidBody/CloudManagement.cpp:17-275,
+        Solver/Solver.cpp:379-398), on the device (ab_cloud_manage). bodies:
+        ABBody values (see body()); body_id: Particle::bodyId of the current
+        cloud; dom: (xleft, xright, ybottom, ytop, zfront, zback). Returns
+        (src, counts): for each particle of the new cloud its index before
+        the call (-1 for a refilled one), and (removed, merged, added)."""
+        nb = len(bodies)
+        arr = (ABBody * max(nb, 1))(*bodies)
+        bid = np.ascontiguousarray(body_id, dtype=np.int32)
+        if len(bid) != self.N:
+            raise ValueError("body_id needs one entry per particle")
+        d6 = np.ascontiguousarray(dom, dtype=np.float64)
+        n_new = C.c_int()
+        counts = (C.c_int * 3)()
+        self._check(self._lib.ab_cloud_manage(self._ctx, C.c_int(nb), arr, _ptr(bid, _ip),
+                                              C.c_double(dx), C.c_double(min_dist),
+                                              _ptr(d6, _dp), C.byref(n_new), counts),
+                    "ab_cloud_manage")
+        self.N = n_new.value
+        src = np.zeros(self.N, np.int32)
+        self._check(self._lib.ab_cloud_source_map(self._ctx, _ptr(src, _ip)),
+                    "ab_cloud_source_map")
+        return src, tuple(counts)
 
     def body_loads(self, surf_idx, surf_body, centres, areas, wu=None):
         si = np.ascontiguousarray(surf_idx, dtype=np.int32)

@@ -275,6 +275,101 @@ int graph_prepare(ab_ctx *c, double dt)
 
 int graph_step(ab_ctx *c, double dt) { return do_step(c, dt); }
 
+template <typename T>
+__global__ void k_gather(const T *__restrict__ src, T *__restrict__ dst,
+                         const int *__restrict__ map, int n, int stride)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * stride) return;
+    const int i = (int)(e / stride), k = (int)(e - (size_t)i * stride);
+    const int j = map[i];
+    dst[e] = j >= 0 ? src[(size_t)j * stride + k] : T(0);
+}
+
+// Re-lays the cloud out for n particles: particle i takes the state of old
+// particle src[i] (src: n entries on the device), or zero where src[i] < 0
+// (a particle the caller fills in). The per-step scratch starts at zero;
+// the neighbour-search scratch and the training-row ring, sized by N, are
+// dropped. Neighbour lists and the interior/boundary lists are left to the
+// caller.
+int relayout_cloud(ab_ctx *c, const int *src, int n)
+{
+    hipStream_t s = c->stream;
+    double **carried[] = {&c->x, &c->y, &c->z, &c->wnx, &c->wny, &c->wnz, &c->wux, &c->wuy,
+                          &c->wuz, &c->wt, &c->mvx, &c->mvy, &c->mvz, &c->rho, &c->ux, &c->uy,
+                          &c->uz, &c->T};
+    constexpr int nc = sizeof(carried) / sizeof(carried[0]);
+    double *od[nc];
+    for (int k = 0; k < nc; ++k) { od[k] = *carried[k]; *carried[k] = nullptr; }
+    double *og = c->g, *ogt = c->gt, *olim = c->limflag, *oovr = c->mlsovr, *opm = c->pmass;
+    double *oalpha = c->dm_alpha, *ores = c->dm_res;
+    int *opass = c->dm_pass, *omls = c->mlsflag, *omood = c->mood, *omf = c->moodfail;
+    int *onb = c->neigh, *onn = c->nneigh, *oil = c->ilist, *obl = c->blist;
+    unsigned char *obnd = c->bnd;
+    c->g = c->gt = c->limflag = c->mlsovr = c->pmass = c->dm_alpha = c->dm_res = nullptr;
+    c->dm_pass = c->mlsflag = c->mood = c->moodfail = nullptr;
+    c->neigh = c->nneigh = c->ilist = c->blist = nullptr;
+    c->bnd = nullptr;
+    const int oldN = c->N;
+    c->N = n;
+    c->cfg.n_particles = n;
+    int rc = alloc_cloud(c);
+    auto gather = [&](auto *from, auto *to, int stride) {
+        const size_t tot = (size_t)n * stride;
+        if (!from || !to || !tot) return;
+        k_gather<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(from, to, src, n, stride);
+    };
+    if (!rc)
+    {
+        for (int k = 0; k < nc; ++k) gather(od[k], *carried[k], 1);
+        gather(obnd, c->bnd, 1);
+        gather(og, c->g, c->gstride);
+        gather(oalpha, c->dm_alpha, 5);
+        gather(ores, c->dm_res, 1);
+        gather(opass, c->dm_pass, 1);
+        const int nb = (n + 255) / 256;
+        if (n)
+        {
+            k_fill_dbl<<<nb, 256, 0, s>>>(c->limflag, n, 0.0);
+            k_fill_dbl<<<nb, 256, 0, s>>>(c->mlsovr, n, 0.0);
+            k_fill_dbl<<<nb, 256, 0, s>>>(c->pmass, n, 0.0);
+            k_fill_int<<<nb, 256, 0, s>>>(c->mlsflag, n, 0);
+            k_fill_int<<<nb, 256, 0, s>>>(c->mood, n, 0);
+            k_fill_int<<<nb, 256, 0, s>>>(c->nneigh, n, 0);
+        }
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        { c->err = "cloud re-layout: gather failed"; rc = AB_ERR_HIP; }
+    }
+    void *old[] = {og, ogt, olim, oovr, opm, oalpha, ores, opass, omls, omood, omf,
+                   onb, onn, oil, obl, obnd, c->vox, c->vlist, c->dm_rec};
+    for (void *p : old) if (p) (void)hipFree(p);
+    for (int k = 0; k < nc; ++k) if (od[k]) (void)hipFree(od[k]);
+    c->vox = c->vlist = nullptr;
+    c->dm_rec = nullptr;
+    c->dm_rec_slots = c->dm_rec_next = c->dm_rec_filled = c->dm_rec_left = 0;
+    c->mass_valid = false;
+    if (rc) { free_cloud(c); c->uploaded = false; c->N = oldN; }
+    return rc;
+}
+
+// Rebuilds the interior and boundary lists from the device boundary flags.
+int rebuild_lists(ab_ctx *c)
+{
+    const int N = c->N;
+    std::vector<unsigned char> b(N);
+    if (N && hipMemcpyAsync(b.data(), c->bnd, N, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+    { c->err = "list rebuild: copy failed"; return AB_ERR_HIP; }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<int> il, bl;
+    il.reserve(N); bl.reserve(N);
+    for (int i = 0; i < N; ++i) (b[i] ? bl : il).push_back(i);
+    c->n_int = (int)il.size(); c->n_bnd = (int)bl.size();
+    if (c->n_int) HIPCHK(c, hipMemcpyAsync(c->ilist, il.data(), 4 * il.size(), hipMemcpyHostToDevice, c->stream));
+    if (c->n_bnd) HIPCHK(c, hipMemcpyAsync(c->blist, bl.data(), 4 * bl.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
 constexpr int kRecSlots = 16;     // steps the training-row ring holds
 
 __global__ void k_fill_f(float *p, size_t n, float v)
@@ -750,6 +845,32 @@ int ab_download_moments(ab_ctx *c, double *rho, double *ux, double *uy, double *
     for (int k = 0; k < 6; ++k)
         if (dst[k]) HIPCHK(c, hipMemcpyAsync(dst[k], src[k], n8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AB_OK;
+}
+
+int ab_download_particles(ab_ctx *c, double *x, double *y, double *z, unsigned char *boundary,
+                          double *wall)
+{
+    if (!c) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    const size_t N = (size_t)c->N, n8 = sizeof(double) * N;
+    double *dst[] = {x, y, z};
+    const double *src[] = {c->x, c->y, c->z};
+    for (int k = 0; k < 3; ++k)
+        if (dst[k]) HIPCHK(c, hipMemcpyAsync(dst[k], src[k], n8, hipMemcpyDeviceToHost, c->stream));
+    if (boundary) HIPCHK(c, hipMemcpyAsync(boundary, c->bnd, N, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> w;
+    if (wall)
+    {
+        w.resize(7 * N);
+        const double *ws[] = {c->wnx, c->wny, c->wnz, c->wux, c->wuy, c->wuz, c->wt};
+        for (int k = 0; k < 7; ++k)
+            HIPCHK(c, hipMemcpyAsync(w.data() + k * N, ws[k], n8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (wall)
+        for (size_t i = 0; i < N; ++i)
+            for (int k = 0; k < 7; ++k) wall[7 * i + k] = w[k * N + i];
     return AB_OK;
 }
 

@@ -147,6 +147,9 @@ struct ab_ctx
     int mlp_rows_cap = 0;
     long mlp_step = 0;             // Adam step count
     bool mlp_ready = false;        // parameters initialised
+
+    // cloud management (ab_cloud_manage): the last call's source map
+    std::vector<int> cm_src;
     bool mlp_on = false;           // warm start enabled
 
     // body loads scratch
@@ -237,6 +240,12 @@ DmArgs dm_args(ab_ctx *ctx);
 void dm_step_done(ab_ctx *ctx);
 void mlp_free(ab_ctx *ctx);   // k_mlp.hip
 void table_free(ab_ctx *ctx);
+
+// Cloud re-layout (ab_api.hip): particle i of the new cloud takes old
+// particle src[i] (device array), zero where src[i] < 0; the interior and
+// boundary lists from the boundary flags.
+int relayout_cloud(ab_ctx *ctx, const int *src, int n);
+int rebuild_lists(ab_ctx *ctx);
 
 // ---- k_body.hip --------------------------------------------------------------
 void launch_body_moments(const DevCloud &c, const double *g_fp64, const void *g_table,

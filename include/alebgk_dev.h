@@ -176,6 +176,11 @@ int ab_download_g(ab_ctx *ctx, double *g);
 int ab_download_gt(ab_ctx *ctx, double *gt);
 int ab_download_moments(ab_ctx *ctx, double *rho, double *ux, double *uy,
                         double *uz, double *t, double *limflag);
+/* The particle fields of cudaDownloadFull (CudaBackend.cu:2056) that cloud
+ * management changes: positions, the boundary flag and the wall state
+ * (wall: 7 per particle, wn xyz, wU xyz, wT). Any pointer may be NULL. */
+int ab_download_particles(ab_ctx *ctx, double *x, double *y, double *z,
+                          unsigned char *boundary, double *wall);
 
 /* cudaTotalMass (CudaBackend.cu:2018). */
 int ab_total_mass(ab_ctx *ctx, double *mass);
@@ -199,6 +204,44 @@ int ab_set_wall_state(ab_ctx *ctx, int n, const int *idx, const double *x,
                       const double *wu, const double *mv);
 
 int ab_synchronize(ab_ctx *ctx);
+
+/* ---- Cloud management around moving bodies --------------------------------
+ * manageRemove, mergeCloseParticles and manageRefill
+ * (RigidBody/CloudManagement.cpp:17-275) with the neighbour rebuilds the
+ * moving-body step puts between them (Solver/Solver.cpp:379-398), on the
+ * device: the cloud is re-laid out in place, with no download or upload of
+ * g (the reference's cudaDownloadFull / cudaUploadCloud round trip,
+ * Solver.cpp:373,401). AB_PREC_FP64 only (a moving body leaves the lattice).
+ *
+ * A body is one of the reference's rigid bodies (RigidBody/Geometry.hpp):
+ * shape AB_BODY_SQUARE (side size, orientation o), AB_BODY_CIRCLE (radius
+ * size), AB_BODY_SPHERE (radius size), AB_BODY_CUBE (side size, body-to-world
+ * rotation rm, row-major as Cube::Rm). body_id holds Particle::bodyId of the
+ * current cloud (N entries, -1 for none): the body surface particles are the
+ * boundary particles with body_id >= 0. dx is CalcParameters::dx, min_dist
+ * CalcParameters::minDist (<= 0: no merging), dom the box (xleft, xright,
+ * ybottom, ytop, zfront, zback); the MLS radius and the velocity grid are the
+ * configuration's. On return the cloud holds *n_new particles and counts (may
+ * be NULL) holds {removed, merged, added}. Particles keep their order,
+ * refilled ones are appended in site order, as in the reference.
+ * ab_cloud_source_map then gives, for each particle of the new cloud, the
+ * index it had before the call, or -1 for a refilled one (bodyId -1): src
+ * holds *n_new entries. */
+#define AB_BODY_SQUARE 1
+#define AB_BODY_CIRCLE 2
+#define AB_BODY_SPHERE 3
+#define AB_BODY_CUBE 4
+typedef struct ab_body {
+    int shape;
+    double c[3];
+    double size;
+    double o[2];
+    double rm[9];
+} ab_body;
+int ab_cloud_manage(ab_ctx *ctx, int nbodies, const ab_body *bodies, const int *body_id,
+                    double dx, double min_dist, const double dom[6], int *n_new,
+                    int counts[3]);
+int ab_cloud_source_map(ab_ctx *ctx, int *src);
 
 /* ---- Autotuner of the transport launch shape -----------------------------
  * No counterpart in the reference. Times every launch shape of the table

@@ -1241,3 +1241,229 @@ void orc_body_loads(const orc_cloud *c, const double *g, const int *bodyid,
     }
     out[0] = Fx; out[1] = Fy; out[2] = Fz; out[3] = Tx; out[4] = Ty; out[5] = Tz;
 }
+
+/* ---- Cloud management around moving bodies --------------------------------- */
+
+/* isInObject / isInObject3 (RigidBody/Geometry.hpp): Square :179-186 and
+ * Circle :233-237 test (x, y) only; Sphere :105-109 and Cube :287-293 take
+ * z = cz in 2-D (their isInObject). */
+int orc_in_body(const orc_body *b, double x, double y, double z, int d3)
+{
+    switch (b->shape)
+    {
+    case 1:
+    {
+        double xC = x - b->c[0], yC = y - b->c[1];
+        double D = b->o[0] * b->o[0] + b->o[1] * b->o[1];
+        double xp = (b->o[0] * xC + b->o[1] * yC) / D;
+        double yp = (b->o[1] * xC - b->o[0] * yC) / D;
+        return (fabs(xp) <= b->size / 2) && (fabs(yp) <= b->size / 2);
+    }
+    case 2:
+    {
+        double dx = x - b->c[0], dy = y - b->c[1];
+        return dx * dx + dy * dy <= b->size * b->size;
+    }
+    case 3:
+    {
+        double zz = d3 ? z : b->c[2];
+        double dx = x - b->c[0], dy = y - b->c[1], dz = zz - b->c[2];
+        return dx * dx + dy * dy + dz * dz <= b->size * b->size;
+    }
+    case 4:
+    {
+        double zz = d3 ? z : b->c[2];
+        double dx = x - b->c[0], dy = y - b->c[1], dz = zz - b->c[2];
+        const double *R = b->rm;
+        double bx = R[0] * dx + R[3] * dy + R[6] * dz;   /* R^T (world -> body) */
+        double by = R[1] * dx + R[4] * dy + R[7] * dz;
+        double bz = R[2] * dx + R[5] * dy + R[8] * dz;
+        return fabs(bx) <= b->size / 2 && fabs(by) <= b->size / 2 && fabs(bz) <= b->size / 2;
+    }
+    default:
+        return 0;
+    }
+}
+
+static int in_any_body(int nb, const orc_body *b, double x, double y, double z, int d3)
+{
+    for (int k = 0; k < nb; ++k)
+        if (orc_in_body(&b[k], x, y, z, d3)) return 1;
+    return 0;
+}
+
+/* |p - q|^2 as CloudManagement.cpp writes it (dz = 0 in 2-D) */
+static double cm_d2(double px, double py, double pz, double qx, double qy, double qz, int d3)
+{
+    double dx = px - qx, dy = py - qy;
+    double dz = d3 ? pz - qz : 0.0;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+/* manageRemove (RigidBody/CloudManagement.cpp:17-69) */
+int orc_cloud_remove(int dim, int N, const double *x, const double *y, const double *z,
+                     const unsigned char *bnd, const int *bodyid, int nb, const orc_body *b,
+                     double dx, unsigned char *keep)
+{
+    const int d3 = dim == 3;
+    const double hug2 = pow(0.4 * fabs(dx), 2);
+    int removed = 0;
+    for (int i = 0; i < N; ++i)
+    {
+        int drop = 0;
+        if (!bnd[i] && nb > 0)
+        {
+            drop = in_any_body(nb, b, x[i], y[i], z[i], d3);
+            for (int q = 0; q < N && !drop; ++q)      /* the surface points, in order */
+                if (bnd[q] && bodyid[q] >= 0)
+                    drop = cm_d2(x[i], y[i], z[i], x[q], y[q], z[q], d3) < hug2;
+        }
+        keep[i] = drop ? 0 : 1;
+        removed += drop;
+    }
+    return removed;
+}
+
+/* mergeCloseParticles (RigidBody/CloudManagement.cpp:80-119) */
+int orc_cloud_merge(int dim, int N, double *x, double *y, double *z, const unsigned char *bnd,
+                    const int *neigh, const int *nneigh, double min_dist, int gstride,
+                    double *g, unsigned char *dead)
+{
+    const double min2 = min_dist * min_dist;
+    memset(dead, 0, (size_t)N);
+    if (min2 <= 0.0) return 0;
+    int merged = 0;
+    for (int i = 0; i < N; ++i)
+    {
+        if (dead[i] || bnd[i]) continue;
+        for (int k = 0; k < nneigh[i]; ++k)
+        {
+            int j = neigh[(size_t)i * ORC_MAX_NEIGH + k];
+            if (j <= i || j >= N || dead[j]) continue;
+            if (bnd[j]) continue;
+            if (cm_d2(x[i], y[i], z[i], x[j], y[j], z[j], dim == 3) < min2)
+            {
+                x[i] = 0.5 * (x[i] + x[j]);
+                y[i] = 0.5 * (y[i] + y[j]);
+                if (dim == 3) z[i] = 0.5 * (z[i] + z[j]);
+                for (int l = 0; l < gstride; ++l)
+                    g[(size_t)i * gstride + l] =
+                        0.5 * (g[(size_t)i * gstride + l] + g[(size_t)j * gstride + l]);
+                dead[j] = 1;
+                merged++;
+                break;
+            }
+        }
+    }
+    return merged;
+}
+
+/* manageRefill (RigidBody/CloudManagement.cpp:121-275) */
+int orc_cloud_refill(int dim, int N, const double *x, const double *y, const double *z,
+                     const unsigned char *bnd, const int *bodyid, const double *mom, int nb,
+                     const orc_body *b, double dx, const double *dom6, double radius, int Nv,
+                     double vlo, double dv, double R, int gstride, int cap, double *out,
+                     double *gout)
+{
+    if (nb <= 0) return 0;
+    const int d3 = dim == 3;
+    const double dxl = fabs(dx);
+    const double hole2 = pow(0.8 * dxl, 2), clr2 = pow(0.6 * dxl, 2);
+    double bx0 = 1e300, bx1 = -1e300, by0 = 1e300, by1 = -1e300, bz0 = 1e300, bz1 = -1e300;
+    int any = 0;
+    for (int q = 0; q < N; ++q)
+    {
+        if (!bnd[q] || bodyid[q] < 0) continue;
+        bx0 = fmin(bx0, x[q]); bx1 = fmax(bx1, x[q]);
+        by0 = fmin(by0, y[q]); by1 = fmax(by1, y[q]);
+        if (d3) { bz0 = fmin(bz0, z[q]); bz1 = fmax(bz1, z[q]); }
+        any = 1;
+    }
+    if (!any) return 0;
+    const double margin = 3.5 * dxl;
+    bx0 -= margin; bx1 += margin; by0 -= margin; by1 += margin;
+    if (d3) { bz0 -= margin; bz1 += margin; }
+    const int nxs = (int)llround((dom6[1] - dom6[0]) / dxl);
+    const int nys = (int)llround((dom6[3] - dom6[2]) / dxl);
+    const int nzs = d3 ? (int)llround((dom6[5] - dom6[4]) / dxl) : 0;
+    int i0 = (int)ceil((bx0 - dom6[0]) / dxl), i1 = (int)floor((bx1 - dom6[0]) / dxl);
+    int j0 = (int)ceil((by0 - dom6[2]) / dxl), j1 = (int)floor((by1 - dom6[2]) / dxl);
+    int k0 = d3 ? (int)ceil((bz0 - dom6[4]) / dxl) : 0, k1 = d3 ? (int)floor((bz1 - dom6[4]) / dxl) : 0;
+    if (i0 < 1) i0 = 1;
+    if (i1 > nxs - 1) i1 = nxs - 1;
+    if (j0 < 1) j0 = 1;
+    if (j1 > nys - 1) j1 = nys - 1;
+    if (d3 && k0 < 1) k0 = 1;
+    if (d3 && k1 > nzs - 1) k1 = nzs - 1;
+    const double r2 = radius * radius;
+    int added = 0;
+    for (int vk = k0; vk <= k1; ++vk)
+        for (int vj = j0; vj <= j1; ++vj)
+            for (int vi = i0; vi <= i1; ++vi)
+            {
+                double xm = dom6[0] + vi * dxl, ym = dom6[2] + vj * dxl;
+                double zm = d3 ? dom6[4] + vk * dxl : 0.0;
+                /* the cloud, then the particles added so far (P grows) */
+                int hole = 1;
+                for (int q = 0; q < N + added && hole; ++q)
+                {
+                    double qx = q < N ? x[q] : out[q - N], qy = q < N ? y[q] : out[cap + q - N];
+                    double qz = q < N ? z[q] : out[2 * cap + q - N];
+                    if (cm_d2(qx, qy, qz, xm, ym, zm, d3) < hole2) hole = 0;
+                }
+                if (!hole) continue;
+                if (in_any_body(nb, b, xm, ym, zm, d3)) continue;
+                int hugs = 0;
+                for (int q = 0; q < N && !hugs; ++q)
+                    if (bnd[q] && bodyid[q] >= 0)
+                        hugs = cm_d2(x[q], y[q], z[q], xm, ym, zm, d3) < clr2;
+                if (hugs) continue;
+                double wsum = 0, rho = 0, ux = 0, uy = 0, uz = 0, T = 0;
+                for (int q = 0; q < N + added; ++q)
+                {
+                    const int a = q >= N, k = q - N;
+                    double qx = a ? out[k] : x[q], qy = a ? out[cap + k] : y[q];
+                    double qz = a ? out[2 * cap + k] : z[q];
+                    double d2 = cm_d2(qx, qy, qz, xm, ym, zm, d3);
+                    if (d2 >= r2) continue;
+                    double q2 = d2 / (radius * radius);     /* mlsWeightSq, MLSBasis.hpp:142 */
+                    double w = q2 >= 1.0 ? 0.0 : (1.0 - q2) * (1.0 - q2);
+                    wsum += w;
+                    rho += w * (a ? out[3 * cap + k] : mom[5 * q + 0]);
+                    ux += w * (a ? out[4 * cap + k] : mom[5 * q + 1]);
+                    uy += w * (a ? out[5 * cap + k] : mom[5 * q + 2]);
+                    uz += w * (a ? out[6 * cap + k] : mom[5 * q + 3]);
+                    T += w * (a ? out[7 * cap + k] : mom[5 * q + 4]);
+                }
+                if (wsum < 1e-30) continue;
+                rho /= wsum; ux /= wsum; uy /= wsum; uz /= wsum; T /= wsum;
+                if (added >= cap) return -1;
+                const int k = added;
+                out[k] = xm; out[cap + k] = ym; out[2 * cap + k] = zm;
+                out[3 * cap + k] = rho; out[4 * cap + k] = ux; out[5 * cap + k] = uy;
+                out[6 * cap + k] = d3 ? uz : 0.0; out[7 * cap + k] = T;
+                double *gr = gout + (size_t)k * gstride;
+                if (d3)
+                {
+                    for (int kv = 0; kv < Nv; ++kv)
+                        for (int jv = 0; jv < Nv; ++jv)
+                            for (int iv = 0; iv < Nv; ++iv)
+                                gr[iv + Nv * (jv + Nv * kv)] = orc_maxwellian3d(
+                                    vlo + iv * dv, vlo + jv * dv, vlo + kv * dv, ux, uy, uz, T,
+                                    rho, R);
+                }
+                else
+                {
+                    for (int jv = 0; jv < Nv; ++jv)
+                        for (int iv = 0; iv < Nv; ++iv)
+                        {
+                            double g1 = orc_maxwellian_g1(vlo + iv * dv, vlo + jv * dv, ux, uy, T,
+                                                          rho, R);
+                            gr[iv + Nv * jv] = g1;
+                            gr[iv + Nv * jv + Nv * Nv] = 0.5 * R * T * g1;
+                        }
+                }
+                added++;
+            }
+    return added;
+}

@@ -164,6 +164,44 @@ void orc_body_loads(const orc_cloud *c, const double *g, const int *bodyid,
                     int body, double cx, double cy, double cz, double area,
                     double *out6);
 
+/* ---- Cloud management around moving bodies ---------------------------------
+ * RigidBody/CloudManagement.cpp. A body (RigidBody/Geometry.hpp): shape 1
+ * square, 2 circle, 3 sphere, 4 cube; centre c, size (side or radius), the
+ * square's orientation o, the cube's body-to-world rotation rm (row-major). */
+typedef struct {
+    int shape;
+    double c[3];
+    double size;
+    double o[2];
+    double rm[9];
+} orc_body;
+
+/* isInObject / isInObject3 (Geometry.hpp:105-109, 179-186, 233-237, 287-293). */
+int orc_in_body(const orc_body *b, double x, double y, double z, int d3);
+
+/* manageRemove (:17-69): keep[i] = 0 for a removed particle; returns the
+ * number removed. */
+int orc_cloud_remove(int dim, int N, const double *x, const double *y, const double *z,
+                     const unsigned char *bnd, const int *bodyid, int nb, const orc_body *b,
+                     double dx, unsigned char *keep);
+
+/* mergeCloseParticles (:80-119) on the current neighbour lists: moves the
+ * merging particles (x, y, z in place) and averages their g; dead[j] = 1 for
+ * a particle merged away. Returns the number of merges. */
+int orc_cloud_merge(int dim, int N, double *x, double *y, double *z, const unsigned char *bnd,
+                    const int *neigh, const int *nneigh, double min_dist, int gstride,
+                    double *g, unsigned char *dead);
+
+/* manageRefill (:121-275): the particles it appends, at most cap. mom holds
+ * the cloud's rho ux uy uz T (5 per particle). out: 8 x cap values (x, y, z,
+ * rho, ux, uy, uz, T of each added particle, field-major), gout: cap rows of
+ * gstride. Returns the number added, -1 when cap is too small. */
+int orc_cloud_refill(int dim, int N, const double *x, const double *y, const double *z,
+                     const unsigned char *bnd, const int *bodyid, const double *mom, int nb,
+                     const orc_body *b, double dx, const double *dom6, double radius, int Nv,
+                     double vlo, double dv, double R, int gstride, int cap, double *out,
+                     double *gout);
+
 #ifdef __cplusplus
 }
 #endif

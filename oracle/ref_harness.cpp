@@ -14,6 +14,8 @@
 //   ref_harness mood    <outdir> <case> <Nx> <Nv> <order> <nsteps> <a> <b>
 //                                                  CPU steps with the MOOD limiter
 //   ref_harness ckptread <file> <outdir> <case> <Nx> <Nv>  restore a file, dump fields
+//   ref_harness cloud   <outdir> <case> <Nx> <Nv> <scale> <shift> <jitter>
+//                                                  cloud management after a body move
 //
 // Functions called (reference file:line):
 //   configureCase            TestCases/TestCases.cpp:113
@@ -33,6 +35,7 @@
 //   moodCheck                MomentCalculations/Conservation.cpp:16
 //   computeBodyLoads (static) RigidBody/BodyDynamics.cpp:17 (TU included below)
 //   ckpt::save / ckpt::restore Core/Checkpoint.hpp:78, 161
+//   manageRemove, mergeCloseParticles, manageRefill  RigidBody/CloudManagement.cpp:17, 80, 121
 // =============================================================================
 #include "Config.hpp"
 #include "Types.hpp"
@@ -44,6 +47,7 @@
 #include "Moments.hpp"
 #include "TestCases.hpp"
 #include "Checkpoint.hpp"
+#include "RigidBody.hpp"
 
 #include <chrono>
 #include <random>
@@ -460,6 +464,9 @@ void writeRecordFields(const std::string &dir, const Setup &S)
 TestCase caseOf(const std::string &name)
 {
     if (name == "cube") return TestCase::Cavity3D_Cube;
+    if (name == "sphere") return TestCase::Cavity3D_Sphere;
+    if (name == "circle") return TestCase::Cavity2D_Circle;
+    if (name == "square") return TestCase::Cavity2D_Square;
     if (name == "cavity2d") return TestCase::Cavity2D;
     return TestCase::Cavity3D;
 }
@@ -601,6 +608,101 @@ int modeMood(const std::string &dir, const std::string &cname, int Nx, int Nv, i
     return 0;
 }
 
+// Cloud management after a body displacement: the moving-body block of the
+// GPU step without the body dynamics (Solver/Solver.cpp:379-398) --
+// manageRemove, updateVoxelsAndNeighbours, mergeCloseParticles (+ rebuild),
+// manageRefill (+ rebuild). The interior particles are jittered first
+// (uniform, +-jitter dx per coordinate, mt19937_64 seed 20261203) so that
+// some pairs come closer than minDist; the body and its surface particles
+// are then translated by (shift, 0.6 shift, 0.3 shift) dx (z in 3-D only).
+// A cube body is also turned 0.3 rad about z (its Rm; the surface points
+// keep their positions: the tests only use the inside test).
+int modeCloud(const std::string &dir, const std::string &cname, int Nx, int Nv, double scale,
+              double shift, double jitter)
+{
+    Setup S;
+    build(S, caseOf(cname), Nx, Nv, scale);
+    perturb(S);
+    computeMoments(S.P, S.gc.R);
+    auto &P = S.P;
+    const bool d3 = S.cp.dim == 3;
+    const double dx = std::fabs(S.cp.dx);
+    std::mt19937_64 rng(20261203);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    for (auto &p : P)
+        if (!p.boundary)
+        {
+            p.x += jitter * dx * U(rng);
+            p.y += jitter * dx * U(rng);
+            if (d3) p.z += jitter * dx * U(rng);
+        }
+    RigidBody &b = *S.bodies[0];
+    const double sx = shift * dx, sy = 0.6 * shift * dx, sz = d3 ? 0.3 * shift * dx : 0.0;
+    b.cx += sx; b.cy += sy; b.cz += sz;
+    for (auto &p : P)
+        if (p.boundary && p.bodyId >= 0) { p.x += sx; p.y += sy; p.z += sz; }
+    // shape, centre, size, orientation, rotation (AB_BODY_* numbering)
+    std::vector<double> body(16, 0.0);
+    body[1] = b.cx; body[2] = b.cy; body[3] = b.cz; body[5] = b.ox; body[6] = b.oy;
+    body[7] = body[11] = body[15] = 1.0;
+    if (auto *q = dynamic_cast<Square *>(&b)) { body[0] = 1; body[4] = q->L; }
+    else if (auto *q = dynamic_cast<Circle *>(&b)) { body[0] = 2; body[4] = q->R; }
+    else if (auto *q = dynamic_cast<Sphere *>(&b)) { body[0] = 3; body[4] = q->R; }
+    else if (auto *q = dynamic_cast<Cube *>(&b))
+    {
+        body[0] = 4; body[4] = q->L;
+        const double a = 0.3, c = std::cos(a), s_ = std::sin(a);
+        const double Rm[9] = {c, -s_, 0.0, s_, c, 0.0, 0.0, 0.0, 1.0};
+        q->setOrientation(Rm);
+        for (int k = 0; k < 9; ++k) body[7 + k] = Rm[k];
+    }
+    auto dump = [&](const std::string &tag) {
+        const size_t N = P.size(), gs = P[0].g.size();
+        std::vector<double> x(N), y(N), z(N);
+        std::vector<uint8_t> bnd(N);
+        std::vector<int32_t> bid(N);
+        for (size_t i = 0; i < N; ++i)
+        {
+            x[i] = P[i].x; y[i] = P[i].y; z[i] = P[i].z;
+            bnd[i] = P[i].boundary ? 1 : 0; bid[i] = P[i].bodyId;
+        }
+        npyF8(dir + "/x_" + tag + ".npy", x, {N});
+        npyF8(dir + "/y_" + tag + ".npy", y, {N});
+        npyF8(dir + "/z_" + tag + ".npy", z, {N});
+        npyU1(dir + "/boundary_" + tag + ".npy", bnd, {N});
+        npyI4(dir + "/bodyid_" + tag + ".npy", bid, {N});
+        npyF8(dir + "/g_" + tag + ".npy", gatherG(P, false), {N, gs});
+        npyF8(dir + "/mom_" + tag + ".npy", moments(P), {N, 5});
+    };
+    writeMeta(dir, S, 0.0);
+    {
+        const size_t N = P.size();
+        std::vector<double> wall(N * 7);
+        for (size_t i = 0; i < N; ++i)
+        {
+            const double w[7] = {P[i].wnx, P[i].wny, P[i].wnz, P[i].wUx, P[i].wUy, P[i].wUz, P[i].wT};
+            std::memcpy(&wall[7 * i], w, sizeof w);
+        }
+        npyF8(dir + "/wall_before.npy", wall, {N, 7});
+    }
+    dump("before");
+    npyF8(dir + "/body.npy", body, {16});
+    const int removed = manageRemove(P, S.bodies, S.cp);
+    updateVoxelsAndNeighbours(P, S.cp, S.dom);
+    const int merged = mergeCloseParticles(P, S.cp);
+    if (merged > 0) updateVoxelsAndNeighbours(P, S.cp, S.dom);
+    const int added = manageRefill(P, S.bodies, S.sp, S.cp, S.dom, S.gc.R);
+    if (added > 0) updateVoxelsAndNeighbours(P, S.cp, S.dom);
+    dump("after");
+    std::vector<int32_t> cnt{removed, merged, added};
+    npyI4(dir + "/counts.npy", cnt, {3});
+    std::vector<double> par{scale, shift, jitter};
+    npyF8(dir + "/params.npy", par, {3});
+    std::printf("wrote %s: removed %d merged %d added %d, N %zu\n", dir.c_str(), removed, merged,
+                added, P.size());
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 1; }
@@ -620,6 +722,9 @@ int main(int argc, char **argv)
         return modeMood(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
                         std::atoi(argv[6]), std::atoi(argv[7]), std::atof(argv[8]),
                         std::atof(argv[9]));
+    if (mode == "cloud" && argc == 9)
+        return modeCloud(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
+                         std::atof(argv[6]), std::atof(argv[7]), std::atof(argv[8]));
     if (mode == "ckptread" && argc == 7)
         return modeCkptRead(argv[2], argv[3], argv[4], std::atoi(argv[5]), std::atoi(argv[6]));
     std::fprintf(stderr, "bad arguments\n");

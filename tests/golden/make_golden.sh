@@ -20,6 +20,12 @@ H="$ROOT/oracle/_ref/ref_harness"
 # three CPU steps with the MOOD limiter from opposing streams (u_x = +-300 m/s)
 "$H" mood "$HERE/mood_cavity3d_n7_v6" cavity3d 7 6 2 3 300 0
 "$H" mood "$HERE/mood_cavity2d_n12_v8" cavity2d 12 8 2 3 300 0
+# cloud management after a body move (remove, merge, refill; 3-D bodies
+# are removed and merged, the 2-D ones also refilled)
+"$H" cloud "$HERE/cloud_sphere_n12_v3" sphere 12 3 2.0 1.3 0.3
+"$H" cloud "$HERE/cloud_cube_n12_v3" cube 12 3 1.5 1.1 0.3
+"$H" cloud "$HERE/cloud_circle_n24_v6" circle 24 6 2.0 1.4 0.3
+"$H" cloud "$HERE/cloud_square_n24_v6" square 24 6 2.0 1.2 0.3
 rm -f "$HERE"/ckpt_*/neigh_flat.npy "$HERE"/ckpt_*/nneigh.npy
 ( cd "$HERE" && find . -name '*.npy' -o -name 'meta.json' -o -name '*.ckpt' | sort | xargs sha256sum > MANIFEST.sha256 )
 echo "fixtures written under $HERE"

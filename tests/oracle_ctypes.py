@@ -339,3 +339,90 @@ def body_loads(cloud, g, bodyid, body, centre, area):
                          C.c_double(centre[0]), C.c_double(centre[1]), C.c_double(centre[2]),
                          C.c_double(area), _ptr(out, _dp))
     return out
+
+
+# ---- cloud management around moving bodies ----------------------------------
+
+class OrcBody(C.Structure):
+    _fields_ = [("shape", C.c_int), ("c", C.c_double * 3), ("size", C.c_double),
+                ("o", C.c_double * 2), ("rm", C.c_double * 9)]
+
+
+def body_from_vector(b):
+    """An OrcBody from (shape, cx, cy, cz, size, ox, oy, rm[9])."""
+    o = OrcBody()
+    o.shape = int(b[0])
+    o.c[:] = [float(v) for v in b[1:4]]
+    o.size = float(b[4])
+    o.o[:] = [float(v) for v in b[5:7]]
+    o.rm[:] = [float(v) for v in b[7:16]]
+    return o
+
+
+def geom_from_meta(m):
+    g = OrcGeom()
+    g.dim, g.Nx, g.Ny, g.Nz = m["dim"], m["Nx"], m["Ny"], m["Nz"]
+    for k in ("xleft", "xright", "ybottom", "ytop", "zfront", "zback", "dx", "dy", "dz",
+              "radius", "minDist"):
+        setattr(g, k, m[k])
+    g.hBox, g.vBox, g.wBox = m["hBox"], m["vBox"], m["wBox"]
+    g.nbhBox, g.nbvBox, g.nbwBox, g.nvox = m["nbhBox"], m["nbvBox"], m["nbwBox"], m["nvox"]
+    return g
+
+
+def cloud_manage(m, x, y, z, boundary, bodyid, g, mom, bodies):
+    """The cloud management of the moving-body block as the reference
+    sequences it (Solver/Solver.cpp:379-398) on the oracle: remove,
+    neighbours, merge, refill. Returns the new cloud (x, y, z, boundary,
+    bodyid, g, mom), src (index before, -1 refilled) and the counts."""
+    L = lib()
+    d = int(m["dim"])
+    f8 = lambda a: np.ascontiguousarray(a, dtype=np.float64).copy()
+    x, y, z, g, mom = f8(x), f8(y), f8(z), f8(g), f8(mom)
+    b = np.ascontiguousarray(boundary, dtype=np.uint8).copy()
+    bid = np.ascontiguousarray(bodyid, dtype=np.int32).copy()
+    src = np.arange(len(x), dtype=np.int32)
+    nb = len(bodies)
+    barr = (OrcBody * max(nb, 1))(*bodies)
+    dx, geom = float(m["dx"]), geom_from_meta(m)
+    keep = np.zeros(len(x), np.uint8)
+    removed = L.orc_cloud_remove(d, len(x), _ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
+                                 _ptr(b, _up), _ptr(bid, _ip), nb, barr, C.c_double(dx),
+                                 _ptr(keep, _up))
+
+    def take(sel):
+        nonlocal x, y, z, b, bid, g, mom, src
+        x, y, z = [np.ascontiguousarray(a[sel]) for a in (x, y, z)]
+        b, bid, src = [np.ascontiguousarray(a[sel]) for a in (b, bid, src)]
+        g, mom = np.ascontiguousarray(g[sel]), np.ascontiguousarray(mom[sel])
+
+    take(keep == 1)
+    neigh, nn, _ = neighbours(geom, x, y, z)
+    dead = np.zeros(len(x), np.uint8)
+    merged = L.orc_cloud_merge(d, len(x), _ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
+                               _ptr(b, _up), _ptr(neigh, _ip), _ptr(nn, _ip),
+                               C.c_double(float(m["minDist"])), g.shape[1], _ptr(g, _dp),
+                               _ptr(dead, _up))
+    take(dead == 0)
+    cap = 4096
+    out = np.zeros(8 * cap)
+    gout = np.zeros((cap, g.shape[1]))
+    dom6 = np.array([m["xleft"], m["xright"], m["ybottom"], m["ytop"], m["zfront"],
+                     m["zback"]])
+    vlo, Nv = float(m["vmin"]), int(m["Nv"])
+    dv = (float(m["vmax"]) - vlo) / (Nv - 1)
+    added = L.orc_cloud_refill(d, len(x), _ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
+                               _ptr(b, _up), _ptr(bid, _ip), _ptr(mom, _dp), nb, barr,
+                               C.c_double(dx), _ptr(dom6, _dp), C.c_double(float(m["radius"])),
+                               Nv, C.c_double(vlo), C.c_double(dv), C.c_double(float(m["R"])),
+                               g.shape[1], cap, _ptr(out, _dp), _ptr(gout, _dp))
+    assert added >= 0
+    o = out.reshape(8, cap)[:, :added]
+    x, y, z = [np.concatenate([a, o[k]]) for k, a in enumerate((x, y, z))]
+    b = np.concatenate([b, np.zeros(added, np.uint8)])
+    bid = np.concatenate([bid, np.full(added, -1, np.int32)])
+    src = np.concatenate([src, np.full(added, -1, np.int32)])
+    g = np.concatenate([g, gout[:added]])
+    mom = np.concatenate([mom, o[3:8].T])
+    return dict(x=x, y=y, z=z, boundary=b, bodyid=bid, g=g, mom=mom, src=src,
+                counts=(removed, merged, added))
