@@ -585,22 +585,27 @@ struct BlkAddr
     int lin;                  // the lane's byte offset
     int col, row;             // bytes between neighbouring columns in x and in y
     int zof[K + 4];
-    template <int U, int TX, int TY>
+    int bx, by, nx, ny;       // block origin and lattice extent (edge blocks clamp)
+    template <int U, int TX, int TY, bool CLAMP = false>
     __device__ __forceinline__ int cofs() const
     {
         constexpr UCol uc = Blk<TX, TY>::ucol(U);
-        return (uc.ox + 2) * col + (uc.oy + 2) * row;
+        if constexpr (!CLAMP) return (uc.ox + 2) * col + (uc.oy + 2) * row;
+        // a union column outside the lattice reads the lattice's last column,
+        // as the stencil of an edge particle does
+        return (clampi(bx + uc.ox, 0, nx - 1) - bx + 2) * col +
+               (clampi(by + uc.oy, 0, ny - 1) - by + 2) * row;
     }
 };
 
 // Values of union column U for outputs iz0 .. iz0+K-1 (no x/y clamping: the
 // block lies two columns inside the lattice).
-template <typename T, int TX, int TY, int K, int U>
+template <typename T, int TX, int TY, int K, int U, bool CLAMP = false>
 __device__ __forceinline__ void blk_load(const BlkAddr<T, K> &a, T (&v)[K + 4])
 {
     constexpr UCol uc = Blk<TX, TY>::ucol(U);
     constexpr int NV = K + 2 * uc.d;
-    const int so = a.template cofs<U, TX, TY>();
+    const int so = a.template cofs<U, TX, TY, CLAMP>();
 #pragma clang loop unroll(full)
     for (int j = 0; j < NV; ++j) v[j] = bload_so<T>(a.r, a.lin, so + a.zof[j + 2 - uc.d]);
 }
@@ -626,14 +631,14 @@ __device__ __forceinline__ void blk_fma(const T (&cr)[kOff], const T (&v)[K + 4]
     (one(std::integral_constant<int, TI>{}), ...);
 }
 
-template <typename T, int TX, int TY, int K, int U>
+template <typename T, int TX, int TY, int K, bool CLAMP, int U>
 __device__ __forceinline__ void blk_step(const BlkAddr<T, K> &a, const T (&cr)[kOff],
                                          T (&va)[K + 4], T (&vb)[K + 4], T (&acc)[TX * TY][K])
 {
     if constexpr (U + 1 < Blk<TX, TY>::NU)
     {
-        if constexpr ((U & 1) == 0) blk_load<T, TX, TY, K, U + 1>(a, vb);
-        else blk_load<T, TX, TY, K, U + 1>(a, va);
+        if constexpr ((U & 1) == 0) blk_load<T, TX, TY, K, U + 1, CLAMP>(a, vb);
+        else blk_load<T, TX, TY, K, U + 1, CLAMP>(a, va);
     }
     if constexpr ((U & 1) == 0)
         blk_fma<T, TX, TY, K, U>(cr, va, acc, std::make_integer_sequence<int, TX * TY>{});
@@ -646,7 +651,7 @@ __device__ __forceinline__ void blk_step(const BlkAddr<T, K> &a, const T (&cr)[k
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <typename T, int TX, int TY, int K, int... U>
+template <typename T, int TX, int TY, int K, bool CLAMP, int... U>
 __device__ __forceinline__ void blk_columns(const T *__restrict__ g, const T (&cr)[kOff],
                                             const Lat &L, int bx, int by, int iz0, int lin,
                                             size_t zstride, T (&acc)[TX * TY][K],
@@ -654,21 +659,24 @@ __device__ __forceinline__ void blk_columns(const T *__restrict__ g, const T (&c
 {
     BlkAddr<T, K> a;
     const int zlo = iz0 - 2;
-    a.r = brsrc(g + ((size_t)(bx - 2) + (size_t)L.nx * (by - 2)) * L.S + (size_t)zlo * zstride);
+    // the descriptor's base (bx - 2, by - 2) may lie before the array for an
+    // edge block; every address it serves is inside (offsets are >= 0)
+    a.r = brsrc(g + ((ptrdiff_t)(bx - 2) + (ptrdiff_t)L.nx * (by - 2)) * L.S + (size_t)zlo * zstride);
     a.lin = lin;
     a.col = L.S * (int)sizeof(T);
     a.row = L.nx * L.S * (int)sizeof(T);
+    a.bx = bx; a.by = by; a.nx = L.nx; a.ny = L.ny;
     const int pb = (int)(zstride * sizeof(T)), ztop = L.nz - 1 - zlo;
 #pragma clang loop unroll(full)
     for (int zz = 0; zz < K + 4; ++zz) a.zof[zz] = (zz < ztop ? zz : ztop) * pb;
     T va[K + 4], vb[K + 4];
-    blk_load<T, TX, TY, K, 0>(a, va);
-    (blk_step<T, TX, TY, K, U>(a, cr, va, vb, acc), ...);
+    blk_load<T, TX, TY, K, 0, CLAMP>(a, va);
+    (blk_step<T, TX, TY, K, CLAMP, U>(a, cr, va, vb, acc), ...);
 }
 
 // End-plane pair of a block: per stencil column, the two classes'
 // coefficients once for the block, then every output's column values.
-template <typename T, int TX, int TY, bool TOP, int C, int... TI>
+template <typename T, int TX, int TY, bool TOP, bool CLAMP, int C, int... TI>
 __device__ __forceinline__ void blk_pair_step(const T *__restrict__ g, __amdgpu_buffer_rsrc_t ra,
                                               __amdgpu_buffer_rsrc_t rb, const Lat &L, int bx,
                                               int by, int zlo, int lin, T (&a0)[TX * TY],
@@ -686,8 +694,10 @@ __device__ __forceinline__ void blk_pair_step(const T *__restrict__ g, __amdgpu_
     const int pb = L.nx * L.ny * L.S * (int)sizeof(T);
     auto load = [&](auto tic) {
         constexpr int t = decltype(tic)::value;
-        const __amdgpu_buffer_rsrc_t rg = brsrc(
-            g + ((size_t)(bx + t % TX + k.ox) + (size_t)L.nx * (by + t / TX + k.oy + (size_t)L.ny * zlo)) * L.S);
+        int xx = bx + t % TX + k.ox, yy = by + t / TX + k.oy;
+        if constexpr (CLAMP) { xx = clampi(xx, 0, L.nx - 1); yy = clampi(yy, 0, L.ny - 1); }
+        const __amdgpu_buffer_rsrc_t rg =
+            brsrc(g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * zlo)) * L.S);
 #pragma clang loop unroll(full)
         for (int j = lo; j <= hi; ++j) gv[t][j] = bload_so<T>(rg, lin, j * pb);
     };
@@ -705,7 +715,7 @@ __device__ __forceinline__ void blk_pair_step(const T *__restrict__ g, __amdgpu_
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <typename T, int TX, int TY, bool TOP, int... C>
+template <typename T, int TX, int TY, bool TOP, bool CLAMP, int... C>
 __device__ __forceinline__ void blk_pair(const T *__restrict__ g, const T *__restrict__ cfa,
                                          const T *__restrict__ cfb, const Lat &L, int bx, int by,
                                          int lin, T (&a0)[TX * TY], T (&a1)[TX * TY],
@@ -715,26 +725,25 @@ __device__ __forceinline__ void blk_pair(const T *__restrict__ g, const T *__res
     const __amdgpu_buffer_rsrc_t ra = brsrc(cfa), rb = brsrc(cfb);
 #pragma clang loop unroll(full)
     for (int t = 0; t < TX * TY; ++t) { a0[t] = T(0); a1[t] = T(0); }
-    (blk_pair_step<T, TX, TY, TOP, C>(g, ra, rb, L, bx, by, zlo, lin, a0, a1,
-                                      std::make_integer_sequence<int, TX * TY>{}), ...);
+    (blk_pair_step<T, TX, TY, TOP, CLAMP, C>(g, ra, rb, L, bx, by, zlo, lin, a0, a1,
+                                             std::make_integer_sequence<int, TX * TY>{}), ...);
 }
 
-// One wave = a TX x TY block of interior columns (block index blk, row-major
-// over the interior) x one 64-node chunk. Requires (nx - 4) % TX == 0 and
-// (ny - 4) % TY == 0.
-template <typename T, int TX, int TY, int K>
-__device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
-                                           const T *__restrict__ coef, int chunk, int blk, int slab,
-                                           int voff)
+// One wave = a TX x TY block of columns with origin (bx, by), all of the
+// x/y class (cx, cy), x one 64-node chunk. Interior blocks (cx = cy = 2)
+// lie two columns inside the lattice; edge blocks (CLAMP) run along an edge
+// of the lattice, their stencil columns clamped to it.
+template <typename T, int TX, int TY, int K, bool CLAMP>
+__device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
+                                         const T *__restrict__ coef, int chunk, int bx, int by,
+                                         int cx, int cy, int slab, int voff)
 {
-    const int nbx = (L.nx - 4) / TX;
-    const int bx = 2 + (blk % nbx) * TX, by = 2 + (blk / nbx) * TY;
     const size_t plane = (size_t)L.nx * L.ny;
     const size_t c0 = (size_t)chunk * kWave;
     const T *gw = g + c0;
     T *gtw = gt + c0;
     auto cls_rows = [&](int cz) {
-        return coef + ((size_t)(2 + 5 * (2 + 5 * cz)) * L.nchunk + chunk) * kOff * kWave;
+        return coef + ((size_t)(cx + 5 * (cy + 5 * cz)) * L.nchunk + chunk) * kOff * kWave;
     };
     auto store = [&](int t, int iz, T v) {
         bstore(gtw + ((size_t)(bx + t % TX) + (size_t)L.nx * (by + t / TX) + plane * iz) * L.S, voff, v);
@@ -743,8 +752,9 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
         constexpr bool TOP = decltype(top)::value;
         const int iz = TOP ? L.nz - 2 : 0;
         T a0[TX * TY], a1[TX * TY];
-        blk_pair<T, TX, TY, TOP>(gw, cls_rows(category(iz, L.nz)), cls_rows(category(iz + 1, L.nz)),
-                                 L, bx, by, voff, a0, a1, std::make_integer_sequence<int, kCols>{});
+        blk_pair<T, TX, TY, TOP, CLAMP>(gw, cls_rows(category(iz, L.nz)),
+                                        cls_rows(category(iz + 1, L.nz)), L, bx, by, voff, a0, a1,
+                                        std::make_integer_sequence<int, kCols>{});
 #pragma unroll
         for (int t = 0; t < TX * TY; ++t) { store(t, iz, a0[t]); store(t, iz + 1, a1[t]); }
         __builtin_amdgcn_sched_barrier(0);
@@ -770,8 +780,8 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
         for (int t = 0; t < TX * TY; ++t)
 #pragma unroll
             for (int k = 0; k < K; ++k) acc[t][k] = T(0);
-        blk_columns<T, TX, TY, K>(gw, cr, L, bx, by, iz0, voff, plane * L.S, acc,
-                                  std::make_integer_sequence<int, Blk<TX, TY>::NU>{});
+        blk_columns<T, TX, TY, K, CLAMP>(gw, cr, L, bx, by, iz0, voff, plane * L.S, acc,
+                                         std::make_integer_sequence<int, Blk<TX, TY>::NU>{});
 #pragma unroll
         for (int t = 0; t < TX * TY; ++t)
 #pragma unroll
@@ -780,10 +790,25 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
     }
 }
 
+// Interior block blk (row-major over the interior). Requires (nx - 4) % TX
+// == 0 and (ny - 4) % TY == 0.
+template <typename T, int TX, int TY, int K>
+__device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
+                                           const T *__restrict__ coef, int chunk, int blk, int slab,
+                                           int voff)
+{
+    const int nbx = (L.nx - 4) / TX;
+    block_at<T, TX, TY, K, false>(L, g, gt, coef, chunk, 2 + (blk % nbx) * TX,
+                                  2 + (blk / nbx) * TY, 2, 2, slab, voff);
+}
+
 // The whole transport in one launch: per chunk, first the interior blocks
-// (TX x TY columns per wave), then the edge columns one per wave (z blocks of
-// KE). One launch keeps the edge waves from running as a second, short and
-// latency-bound grid after the blocks.
+// (TX x TY columns per wave), then the edge columns (z blocks of KE). One
+// launch keeps the edge waves from running as a second, short and
+// latency-bound grid after the blocks. Where the interior is a multiple of 4
+// columns wide, the edge columns along an edge share a class and run as
+// blocks too: 1 x 4 along the x edges (ix = 0, 1, nx-2, nx-1), 4 x 1 along
+// the y edges, and only the 16 corner columns one per wave.
 //
 // SLAB: a block wave covers one z block of K planes instead of the whole
 // column, and the waves of a chunk run slab by slab (first every block's end
@@ -793,14 +818,25 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
 template <int K>
 __host__ __device__ constexpr int zblocks(int nz) { return (nz - 4 + K - 1) / K; }
 
+__host__ __device__ inline bool edge_blocks(int nx, int ny)
+{
+    return ((nx - 4) & 3) == 0 && ((ny - 4) & 3) == 0;
+}
+
+// edge waves per chunk: blocks along the four x edges and four y edges plus
+// the corners, or one wave per edge column
+__host__ __device__ inline int edge_items(int nx, int ny)
+{
+    return edge_blocks(nx, ny) ? (ny - 4) + (nx - 4) + 16 : nx * ny - (nx - 4) * (ny - 4);
+}
+
 template <typename T, int TX, int TY, int K, int KE, bool SLAB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
 {
     const int nblk = ((L.nx - 4) / TX) * ((L.ny - 4) / TY);
-    const int nedge = L.nx * L.ny - (L.nx - 4) * (L.ny - 4);
     const int nitem = SLAB ? nblk * (zblocks<K>(L.nz) + 1) : nblk;
-    const int per = nitem + nedge;
+    const int per = nitem + edge_items(L.nx, L.ny);
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
     const int voff = (threadIdx.x & (kWave - 1)) * (int)sizeof(T);
@@ -810,13 +846,39 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
     {
         if constexpr (SLAB) block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r % nblk, r / nblk - 1, voff);
         else block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r, -2, voff);
+        return;
     }
-    else
+    int e = r - nitem;
+    // edge blocks march in z blocks of at most 8 (12 would raise the
+    // kernel's register count above three waves per SIMD)
+    constexpr int KB = KE > 8 ? 8 : KE;
+    if (edge_blocks(L.nx, L.ny))
     {
-        int ix, iy;
-        column_of(L, r - nitem, true, ix, iy);
+        const int qy = (L.ny - 4) / 4, qx = (L.nx - 4) / 4;
+        if (e < 4 * qy)                                      // x edges: 1 x 4 along y
+        {
+            const int side = e / qy, ix = side < 2 ? side : L.nx - 4 + side;
+            block_at<T, 1, 4, KB, true>(L, g, gt, coef, chunk, ix, 2 + 4 * (e % qy),
+                                        category(ix, L.nx), 2, -2, voff);
+            return;
+        }
+        e -= 4 * qy;
+        if (e < 4 * qx)                                      // y edges: 4 x 1 along x
+        {
+            const int side = e / qx, iy = side < 2 ? side : L.ny - 4 + side;
+            block_at<T, 4, 1, KB, true>(L, g, gt, coef, chunk, 2 + 4 * (e % qx), iy, 2,
+                                        category(iy, L.ny), -2, voff);
+            return;
+        }
+        e -= 4 * qx;                                         // the corners
+        const int a = e & 3, c = e >> 2;
+        const int ix = a < 2 ? a : L.nx - 4 + a, iy = c < 2 ? c : L.ny - 4 + c;
         column_wave<T, KE>(L, g, gt, coef, chunk, ix, iy, voff);
+        return;
     }
+    int ix, iy;
+    column_of(L, e, true, ix, iy);
+    column_wave<T, KE>(L, g, gt, coef, chunk, ix, iy, voff);
 }
 
 // ---- per-particle limiters, wall, moments, collision -------------------------
@@ -1643,7 +1705,8 @@ void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *sc
         if (sh.TX > 1)
         {
             const int nblk = inner / (sh.TX * sh.TY);
-            const int nw = (sh.S ? nblk * ((L.nz - 4 + sh.K - 1) / sh.K + 1) : nblk) + all - inner;
+            const int nw = (sh.S ? nblk * ((L.nz - 4 + sh.K - 1) / sh.K + 1) : nblk) +
+                           edge_items(L.nx, L.ny);
             switch (i)
             {
             case 8: k_apply_mix<T, 2, 2, 8, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
