@@ -28,7 +28,12 @@ namespace {
         }                                                                        \
     } while (0)
 
-bool table_mode(const ab_ctx *c) { return c->cfg.precision != AB_PREC_FP64; }
+// The 2-D cloud with AB_PREC_FP64_TABLE: the FP64 step with the table
+// transport (k_table2d.hip).
+bool table2d(const ab_ctx *c) { return c->dim == 2 && c->cfg.precision == AB_PREC_FP64_TABLE; }
+// The 3-D table pipelines (and the 2-D AB_PREC_MIXED request, which their
+// setup refuses).
+bool table_mode(const ab_ctx *c) { return c->cfg.precision != AB_PREC_FP64 && !table2d(c); }
 
 template <typename T>
 int dalloc(ab_ctx *ctx, T **p, size_t n)
@@ -64,6 +69,7 @@ void free_cloud(ab_ctx *c)
     c->vox = c->vcount = c->vstart = c->vlist = nullptr;
     c->bmom = nullptr; c->bidx = nullptr; c->bwu = nullptr; c->bmom_cap = 0;
     table_free(c);
+    table2d_free(c);
     c->coef_dt = -1.0;
 }
 
@@ -144,6 +150,7 @@ struct Timer
 
 int fp64_transport(ab_ctx *c, double dt)
 {
+    if (table2d(c)) return table2d_transport(c, dt);
     const double tol = c->cfg.mls_tol > 0 ? c->cfg.mls_tol : 1e-12;
     DevCloud v = c->view();
     launch_transport_fp64(v, c->g, c->gt, dt, c->cfg.radius, c->cfg.mls_order, tol,
@@ -268,6 +275,11 @@ int graph_prepare(ab_ctx *c, double dt)
     if (table_mode(c) && c->coef_dt != dt)
     {
         int rc = table_prepare(c, dt);
+        if (rc) return rc;
+    }
+    if (table2d(c) && c->t2_dt != dt)
+    {
+        int rc = table2d_prepare(c, dt);
         if (rc) return rc;
     }
     return AB_OK;
@@ -575,6 +587,7 @@ int ab_upload(ab_ctx *c, const ab_cloud_view *v)
         c->coef_dt = -1.0;
         if ((rc = table_upload_g(c, c->g))) { c->err = "table upload failed"; return rc; }
     }
+    if (table2d(c) && (rc = table2d_setup(c))) { c->uploaded = false; return rc; }
     HIPCHK(c, hipStreamSynchronize(s));
     return AB_OK;
 }
@@ -612,7 +625,18 @@ int ab_rebuild_neighbours(ab_ctx *c)
     { c->err = "neighbour rebuild launch failed"; return AB_ERR_HIP; }
     tm.toc(ST_NEIGH);
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return AB_OK;
+    // the table transports take their stencils from the lists: check the
+    // classes again and rebuild the coefficients at the next step
+    int rc = AB_OK;
+    if (table_mode(c) && c->gs)
+    {
+        rc = table_recheck(c);
+        c->coef_dt = -1.0;
+    }
+    else if (table2d(c) && c->t2_coef)
+        rc = table2d_setup(c);
+    if (rc) c->uploaded = false;
+    return rc;
 }
 
 int ab_download_neighbours(ab_ctx *c, int *neigh, int *nneigh)
@@ -1019,8 +1043,14 @@ int ab_set_wall_state(ab_ctx *c, int n, const int *idx, const double *x, const d
     (void)hipFree(di);
     (void)hipFree(ds);
     if (e != hipSuccess) { c->err = std::string("ab_set_wall_state: ") + hipGetErrorString(e); return AB_ERR_HIP; }
-    // moved surfaces change the MLS stencils of the table pipelines
-    if (table_mode(c) && (hx || mv)) c->coef_dt = -1.0;
+    // the table transports hold one stencil per lattice class: check the
+    // classes again, and rebuild the coefficients at the next step
+    if (table_mode(c) || table2d(c))
+    {
+        rc = table_mode(c) ? table_recheck(c) : table2d_setup(c);
+        c->coef_dt = -1.0;
+        if (rc) { c->uploaded = false; return rc; }
+    }
     return AB_OK;
 }
 

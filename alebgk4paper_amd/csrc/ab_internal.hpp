@@ -128,6 +128,12 @@ struct ab_ctx
     int *bad = nullptr;            // stencil check result
     int apply_shape = -1;          // transport launch shape (-1: default for nz)
 
+    // 2-D table transport (AB_PREC_FP64_TABLE with dim 2)
+    double *t2_coef = nullptr;     // [25 classes][21 offsets][Nv^2] MLS coefficients
+    int *t2_cls = nullptr;         // class of each particle, N
+    int *t2_rep = nullptr;         // representative particle of each class, 25
+    double t2_dt = -1.0;           // dt the coefficients were built for
+
     // discrete-Maxwellian collision (cfg.collision) and its MLP warm start
     double *dm_alpha = nullptr;    // N x 5 solution of the last step
     int *dm_pass = nullptr;        // N grid passes of the last step
@@ -233,6 +239,17 @@ int table_prepare(ab_ctx *ctx, double dt);   // coefficients for dt
 int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32], float *ms,
                    int *n, int *chosen);
 int table_set_shape(ab_ctx *ctx, int shape);
+// Re-runs the lattice class check (after a wall-state change).
+int table_recheck(ab_ctx *ctx);
+
+// ---- k_table2d.hip -------------------------------------------------------------
+// The 2-D table transport: classes and their check (after the upload and
+// every change of the cloud's geometry or wall state), gt from g for dt
+// (the coefficients built when dt changes), the coefficients for dt alone.
+int table2d_setup(ab_ctx *ctx);
+int table2d_transport(ab_ctx *ctx, double dt);
+int table2d_prepare(ab_ctx *ctx, double dt);
+void table2d_free(ab_ctx *ctx);
 
 // Discrete-Maxwellian settings of the next step (records training rows when
 // asked); defined in ab_api.hip.

@@ -1793,7 +1793,12 @@ void table_free(ab_ctx *ctx)
 // Validate that the cloud is the lattice the config names (every particle's
 // neighbour set is its class's stencil and its wall state is its class's),
 // then build the tables. Host-side check of the uploaded arrays.
-int table_setup(ab_ctx *ctx)
+// The lattice classes of the uploaded cloud and their check: every
+// particle has its class representative's stencil (neighbours at the same
+// lattice offsets and displacements, to 1e-12 of the support radius; the
+// list order may differ) and wall state. lat: (ix, iy, iz) per particle; rep:
+// the representative of each class.
+static int table_check(ab_ctx *ctx, std::vector<int> &lat, std::vector<int> &rep)
 {
     const ab_config &cfg = ctx->cfg;
     const int N = ctx->N;
@@ -1804,7 +1809,7 @@ int table_setup(ab_ctx *ctx)
     if (cfg.mls_clamp)
     { ctx->err = "mls_clamp is nonlinear in g; run it with AB_PREC_FP64"; return AB_ERR_UNSUPPORTED; }
 
-    std::vector<double> x(N), y(N), z(N), w(10 * (size_t)N), hr(N), hT(N);
+    std::vector<double> x(N), y(N), z(N), w(10 * (size_t)N);
     std::vector<unsigned char> b(N);
     std::vector<int> nb((size_t)N * AB_MAX_NEIGH), nn(N);
     hipStream_t s = ctx->stream;
@@ -1818,12 +1823,13 @@ int table_setup(ab_ctx *ctx)
     const double *wsrc[10] = {ctx->wnx, ctx->wny, ctx->wnz, ctx->wux, ctx->wuy, ctx->wuz,
                               ctx->wt, ctx->mvx, ctx->mvy, ctx->mvz};
     for (int k = 0; k < 10 && ok; ++k) ok = dl(&w[(size_t)k * N], wsrc[k], 8 * (size_t)N);
-    ok = ok && dl(hr.data(), ctx->rho, 8 * (size_t)N) && dl(hT.data(), ctx->T, 8 * (size_t)N);
     if (!ok || hipStreamSynchronize(s) != hipSuccess)
     { ctx->err = "table setup: device-to-host copy failed"; return AB_ERR_HIP; }
 
     const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
-    std::vector<int> lat(3 * (size_t)N), cls(N), rep(kNcls, -1);
+    lat.assign(3 * (size_t)N, 0);
+    rep.assign(kNcls, -1);
+    std::vector<int> cls(N);
     std::vector<unsigned long long> mlo(N, 0), mhi(N, 0);
     for (int p = 0; p < N; ++p)
     {
@@ -1848,18 +1854,50 @@ int table_setup(ab_ctx *ctx)
             if (o < 64) mlo[p] |= 1ull << o; else mhi[p] |= 1ull << (o - 64);
         }
     }
+    const double tol = 1e-12 * cfg.radius;
     for (int p = 0; p < N; ++p)
     {
         int r = rep[cls[p]];
         bool same = (mlo[p] == mlo[r]) && (mhi[p] == mhi[r]) && (nn[p] == nn[r]) && (b[p] == b[r]);
         for (int k = 0; k < 10 && same; ++k) same = (w[(size_t)k * N + p] == w[(size_t)k * N + r]);
+        // the neighbours at the same displacements (the representative has
+        // a neighbour at every lattice offset p has: the masks are equal)
+        for (int m = 0; m < nn[p] && same; ++m)
+        {
+            const int jp = nb[(size_t)p * AB_MAX_NEIGH + m], jr = r + (jp - p);
+            same = std::fabs((x[jp] - x[p]) - (x[jr] - x[r])) <= tol &&
+                   std::fabs((y[jp] - y[p]) - (y[jr] - y[r])) <= tol &&
+                   std::fabs((z[jp] - z[p]) - (z[jr] - z[r])) <= tol;
+        }
         if (!same)
         {
             ctx->err = "table setup: particle " + std::to_string(p) +
-                       " differs from its lattice class (stencil or wall state)";
+                       " differs from its lattice class (stencil, geometry or wall state)";
             return AB_ERR_UNSUPPORTED;
         }
     }
+    return AB_OK;
+}
+
+int table_recheck(ab_ctx *ctx)
+{
+    std::vector<int> lat, rep;
+    return table_check(ctx, lat, rep);
+}
+
+int table_setup(ab_ctx *ctx)
+{
+    const int N = ctx->N;
+    std::vector<int> lat, rep;
+    int rc = table_check(ctx, lat, rep);
+    if (rc) return rc;
+    std::vector<double> hr(N), hT(N);
+    hipStream_t s = ctx->stream;
+    if (hipMemcpyAsync(hr.data(), ctx->rho, 8 * (size_t)N, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(hT.data(), ctx->T, 8 * (size_t)N, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+    { ctx->err = "table setup: device-to-host copy failed"; return AB_ERR_HIP; }
+    const ab_config &cfg = ctx->cfg;
 
     const bool f32 = is_fp32(ctx);
     const size_t es = f32 ? sizeof(float) : sizeof(double);

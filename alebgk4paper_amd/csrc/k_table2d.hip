@@ -1,0 +1,337 @@
+// k_table2d.hip -- table-driven transport of the 2-D Chu path
+// (AB_PREC_FP64_TABLE with dim 2).
+//
+// The 2-D MLS transport (kTransportNode2D, Solver/CudaBackend.cu:199-300) is
+// linear in g: with the normal equations A s = b, b = sum_m w_m phi_m g_m,
+// solved by Gauss elimination with partial pivoting and gt = s[0], the
+// elimination's steps depend on A only, so s[0] = sum_k b_k s0(e_k) and
+//     gt[i, v] = sum_m C[i, m, v] g[m, v],  C = w_m sum_k phi_m[k] s0(e_k).
+// On a lattice every particle's stencil is one of 5 x 5 classes (per axis:
+// first, second, interior, second-to-last, last column), so the setup runs the
+// reference's assembly and elimination once per (class, velocity node) on the
+// class's representative particle, with the P unit vectors as right-hand
+// sides, and the step applies the 21-offset stencil (the lattice offsets with
+// ox^2 + oy^2 <= 6) to both layers g1 and g2. Gated boundary nodes
+// (c_n >= 0) and particles without a weighted neighbour keep their value, as
+// in the reference. The rest of the step is the FP64 formulation's.
+#include "ab_device.hpp"
+#include "ab_internal.hpp"
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+using namespace ab;
+
+namespace {
+
+constexpr int kOff2 = 21;
+constexpr int kNcls2 = 25;
+
+struct Off2 { int ox, oy; };
+
+__host__ __device__ constexpr Off2 off2_at(int o)
+{
+    int n = 0;
+    for (int oy = -2; oy <= 2; ++oy)
+        for (int ox = -2; ox <= 2; ++ox)
+            if (ox * ox + oy * oy <= 6)
+            {
+                if (n == o) return Off2{ox, oy};
+                ++n;
+            }
+    return Off2{0, 0};
+}
+
+__host__ __device__ constexpr int off2_index(int ox, int oy)
+{
+    int n = 0;
+    for (int y = -2; y <= 2; ++y)
+        for (int x = -2; x <= 2; ++x)
+            if (x * x + y * y <= 6)
+            {
+                if (x == ox && y == oy) return n;
+                ++n;
+            }
+    return -1;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__host__ __device__ inline int category2(int i, int n)
+{
+    return i == 0 ? 0 : i == 1 ? 1 : i == n - 2 ? 3 : i == n - 1 ? 4 : 2;
+}
+
+// One (class, node): the reference's assembly for the representative
+// particle i (its neighbour list, in order), then the pivoted elimination
+// with the P unit vectors as right-hand sides. out: kOff2 coefficients at
+// stride `stride`.
+template <int P, bool BND>
+__device__ void build2(const DevCloud &c, int i, int lin, double dt, double h, int nx,
+                       double *out, size_t stride)
+{
+    const double vx = vnode(c.vlo, c.dv, lin % c.Nv);
+    const double vy = vnode(c.vlo, c.dv, lin / c.Nv);
+    const int self = off2_index(0, 0);
+    if (BND && wall_cn2(vx, vy, c.wux[i], c.wuy[i], c.wnx[i], c.wny[i]) >= 0.0)
+    {
+        out[self * stride] = 1.0;
+        return;
+    }
+    const double xf = c.x[i] - (vx - c.mvx[i]) * dt;
+    const double yf = c.y[i] - (vy - c.mvy[i]) * dt;
+    const int nn = c.nneigh[i];
+    const int *nb = c.neigh + (size_t)i * AB_MAX_NEIGH;
+    auto weight_basis = [&](int j, double *basis) -> double {
+        const double dxn = c.x[j] - xf, dyn = c.y[j] - yf;
+        const double r = sqrt(dxn * dxn + dyn * dyn);
+        double w = 0.0;                                   // mlsWeight
+        if (r < h) { const double qq = r / h; const double t1 = 1.0 - qq * qq; w = t1 * t1; }
+        const double ssx = dxn / h, ssy = dyn / h;
+        basis[0] = 1.0; basis[1] = ssx; basis[2] = ssy;
+        if constexpr (P >= 6) { basis[3] = ssx * ssx; basis[4] = ssx * ssy; basis[5] = ssy * ssy; }
+        return w;
+    };
+    double A[P * P], B[P * P];                            // B: the right-hand sides, B[row][k]
+#pragma unroll
+    for (int q = 0; q < P * P; ++q) { A[q] = 0.0; B[q] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < P; ++k) B[k * P + k] = 1.0;
+    bool has = false;
+    for (int m = 0; m < nn; ++m)
+    {
+        double basis[P];
+        const double w = weight_basis(nb[m], basis);
+        if (w < 1e-30) continue;
+#pragma unroll
+        for (int p1 = 0; p1 < P; ++p1)
+#pragma unroll
+            for (int p2 = 0; p2 < P; ++p2) A[p1 * P + p2] += w * basis[p1] * basis[p2];
+        has = true;
+    }
+    if (!has)
+    {
+        out[self * stride] = 1.0;
+        return;
+    }
+#pragma unroll
+    for (int col = 0; col < P; ++col)
+    {
+        int piv = col;
+        double mx = fabs(A[col * P + col]);
+#pragma unroll
+        for (int row = col + 1; row < P; ++row)
+            if (fabs(A[row * P + col]) > mx) { mx = fabs(A[row * P + col]); piv = row; }
+#pragma unroll
+        for (int row = col + 1; row < P; ++row)
+            if (row == piv)
+            {
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                {
+                    double t1 = A[col * P + q]; A[col * P + q] = A[row * P + q]; A[row * P + q] = t1;
+                    t1 = B[col * P + q]; B[col * P + q] = B[row * P + q]; B[row * P + q] = t1;
+                }
+            }
+        const double d = A[col * P + col];
+        if (fabs(d) < 1e-30) continue;
+#pragma unroll
+        for (int row = col + 1; row < P; ++row)
+        {
+            const double f = A[row * P + col] / d;
+#pragma unroll
+            for (int q = col; q < P; ++q) A[row * P + q] -= f * A[col * P + q];
+#pragma unroll
+            for (int k = 0; k < P; ++k) B[row * P + k] -= f * B[col * P + k];
+        }
+    }
+    // back substitution, every right-hand side; only s[0] is kept
+    double s[P * P];
+#pragma unroll
+    for (int row = P - 1; row >= 0; --row)
+    {
+        const double d = A[row * P + row];
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+        {
+            double v = B[row * P + k];
+#pragma unroll
+            for (int q = row + 1; q < P; ++q) v -= A[row * P + q] * s[q * P + k];
+            s[row * P + k] = fabs(d) > 1e-30 ? v / d : 0.0;
+        }
+    }
+    const int ix = i % nx, iy = i / nx;
+    for (int m = 0; m < nn; ++m)
+    {
+        const int j = nb[m];
+        double basis[P];
+        const double w = weight_basis(j, basis);
+        if (w < 1e-30) continue;
+        double cm = 0.0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) cm += basis[k] * s[k];   // s[0 * P + k] = s0(e_k)
+        const int o = off2_index(j % nx - ix, j / nx - iy);
+        if (o >= 0) out[o * stride] += w * cm;
+    }
+}
+
+// Coefficients [class][offset][node].
+__global__ void k_build2(DevCloud c, const int *__restrict__ rep, double dt, double h, int order,
+                         int nx, double *__restrict__ coef)
+{
+    const int cls = blockIdx.y;
+    const int lin = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lin >= c.nodes) return;
+    const size_t stride = c.nodes;
+    double *out = coef + (size_t)cls * kOff2 * stride + lin;
+    for (int o = 0; o < kOff2; ++o) out[o * stride] = 0.0;
+    const int i = rep[cls];
+    if (i < 0) return;
+    if (c.bnd[i]) build2<3, true>(c, i, lin, dt, h, nx, out, stride);
+    else if (order >= 2) build2<6, false>(c, i, lin, dt, h, nx, out, stride);
+    else build2<3, false>(c, i, lin, dt, h, nx, out, stride);
+}
+
+// gt = the class stencil applied to both layers; one thread per (particle,
+// node), flat over N * Nv^2. Offsets outside the lattice have coefficient 0
+// and read a clamped (valid) neighbour.
+__global__ __launch_bounds__(256) void k_apply2(int N, int nx, int ny, int nodes, int gstride,
+                                                const int *__restrict__ cls,
+                                                const double *__restrict__ coef,
+                                                const double *__restrict__ g,
+                                                double *__restrict__ gt)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)N * nodes) return;
+    const int i = (int)(e / nodes), lin = (int)(e - (long)i * nodes);
+    const int ix = i % nx, iy = i / nx;
+    const double *cf = coef + (size_t)cls[i] * kOff2 * nodes + lin;
+    double a1 = 0.0, a2 = 0.0;
+#pragma unroll
+    for (int o = 0; o < kOff2; ++o)
+    {
+        const Off2 d = off2_at(o);
+        const int jx = clampi(ix + d.ox, 0, nx - 1), jy = clampi(iy + d.oy, 0, ny - 1);
+        const double *gj = g + ((size_t)jx + (size_t)nx * jy) * gstride + lin;
+        const double w = cf[(size_t)o * nodes];
+        a1 += w * gj[0];
+        a2 += w * gj[nodes];
+    }
+    gt[(size_t)i * gstride + lin] = a1;
+    gt[(size_t)i * gstride + lin + nodes] = a2;
+}
+
+}  // namespace
+
+namespace ab {
+
+void table2d_free(ab_ctx *c)
+{
+    if (c->t2_coef) (void)hipFree(c->t2_coef);
+    if (c->t2_cls) (void)hipFree(c->t2_cls);
+    if (c->t2_rep) (void)hipFree(c->t2_rep);
+    c->t2_coef = nullptr; c->t2_cls = nullptr; c->t2_rep = nullptr;
+    c->t2_dt = -1.0;
+}
+
+// Classes of the 2-D lattice cloud and their checks (every particle has its
+// class representative's stencil and wall state), once per upload.
+int table2d_setup(ab_ctx *c)
+{
+    const ab_config &cfg = c->cfg;
+    const int N = c->N, nx = cfg.nx, ny = cfg.ny;
+    table2d_free(c);
+    if (nx < 5 || ny < 5 || (long)nx * ny != N)
+    { c->err = "the 2-D table transport needs a lattice cloud: nx, ny >= 5 and N = nx*ny"; return AB_ERR_UNSUPPORTED; }
+    if (cfg.mls_clamp)
+    { c->err = "mls_clamp is nonlinear in g; run it with AB_PREC_FP64"; return AB_ERR_UNSUPPORTED; }
+    std::vector<unsigned char> b(N);
+    std::vector<int> nb((size_t)N * AB_MAX_NEIGH), nn(N);
+    std::vector<double> w(7 * (size_t)N), x(N), y(N);
+    hipStream_t s = c->stream;
+    auto dl = [&](void *dst, const void *src, size_t n) {
+        return hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s) == hipSuccess;
+    };
+    bool ok = dl(b.data(), c->bnd, N) && dl(nb.data(), c->neigh, 4 * (size_t)N * AB_MAX_NEIGH) &&
+              dl(nn.data(), c->nneigh, 4 * (size_t)N) && dl(x.data(), c->x, 8 * (size_t)N) &&
+              dl(y.data(), c->y, 8 * (size_t)N);
+    const double *ws[7] = {c->wnx, c->wny, c->wux, c->wuy, c->wt, c->mvx, c->mvy};
+    for (int k = 0; k < 7 && ok; ++k) ok = dl(&w[(size_t)k * N], ws[k], 8 * (size_t)N);
+    if (!ok || hipStreamSynchronize(s) != hipSuccess)
+    { c->err = "2-D table setup: device-to-host copy failed"; return AB_ERR_HIP; }
+    std::vector<int> cls(N), rep(kNcls2, -1);
+    std::vector<unsigned> mask(N, 0);
+    for (int p = 0; p < N; ++p)
+    {
+        const int ix = p % nx, iy = p / nx;
+        cls[p] = category2(ix, nx) + 5 * category2(iy, ny);
+        if (rep[cls[p]] < 0) rep[cls[p]] = p;
+        for (int m = 0; m < nn[p]; ++m)
+        {
+            const int j = nb[(size_t)p * AB_MAX_NEIGH + m];
+            const int o = off2_index(j % nx - ix, j / nx - iy);
+            if (o < 0)
+            {
+                c->err = "2-D table setup: particle " + std::to_string(p) +
+                         " has a neighbour outside the lattice stencil";
+                return AB_ERR_UNSUPPORTED;
+            }
+            mask[p] |= 1u << o;
+        }
+    }
+    const double tol = 1e-12 * cfg.radius;
+    for (int p = 0; p < N; ++p)
+    {
+        const int r = rep[cls[p]];
+        bool same = mask[p] == mask[r] && nn[p] == nn[r] && b[p] == b[r];
+        for (int k = 0; k < 7 && same; ++k) same = w[(size_t)k * N + p] == w[(size_t)k * N + r];
+        // the neighbours at the same displacements (the representative has
+        // a neighbour at every lattice offset p has: the masks are equal)
+        for (int m = 0; m < nn[p] && same; ++m)
+        {
+            const int jp = nb[(size_t)p * AB_MAX_NEIGH + m], jr = r + (jp - p);
+            same = std::fabs((x[jp] - x[p]) - (x[jr] - x[r])) <= tol &&
+                   std::fabs((y[jp] - y[p]) - (y[jr] - y[r])) <= tol;
+        }
+        if (!same)
+        {
+            c->err = "2-D table setup: particle " + std::to_string(p) +
+                     " differs from its lattice class (stencil, geometry or wall state)";
+            return AB_ERR_UNSUPPORTED;
+        }
+    }
+    if (hipMalloc((void **)&c->t2_coef, sizeof(double) * kNcls2 * kOff2 * (size_t)c->nodes) != hipSuccess ||
+        hipMalloc((void **)&c->t2_cls, sizeof(int) * (size_t)N) != hipSuccess ||
+        hipMalloc((void **)&c->t2_rep, sizeof(int) * kNcls2) != hipSuccess)
+    { table2d_free(c); c->err = "2-D table setup: device allocation failed"; return AB_ERR_NOMEM; }
+    if (hipMemcpyAsync(c->t2_cls, cls.data(), sizeof(int) * N, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(c->t2_rep, rep.data(), sizeof(int) * kNcls2, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+    { table2d_free(c); c->err = "2-D table setup: host-to-device copy failed"; return AB_ERR_HIP; }
+    return AB_OK;
+}
+
+int table2d_prepare(ab_ctx *c, double dt)
+{
+    if (!c->t2_coef) { c->err = "2-D table transport before its setup"; return AB_ERR_STATE; }
+    dim3 gb((c->nodes + 63) / 64, kNcls2);
+    k_build2<<<gb, 64, 0, c->stream>>>(c->view(), c->t2_rep, dt, c->cfg.radius, c->cfg.mls_order,
+                                       c->cfg.nx, c->t2_coef);
+    if (hipGetLastError() != hipSuccess) { c->err = "2-D table build launch failed"; return AB_ERR_HIP; }
+    c->t2_dt = dt;
+    return AB_OK;
+}
+
+// gt from g: the coefficients for dt (built when dt changes), then the apply.
+int table2d_transport(ab_ctx *c, double dt)
+{
+    int rc;
+    if (c->t2_dt != dt && (rc = table2d_prepare(c, dt))) return rc;
+    const long n = (long)c->N * c->nodes;
+    k_apply2<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+        c->N, c->cfg.nx, c->cfg.ny, c->nodes, c->gstride, c->t2_cls, c->t2_coef, c->g, c->gt);
+    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+}
+
+}  // namespace ab

@@ -153,10 +153,11 @@ def cloud_management(A, np):
 def components(A):
     """The other rows of the path, measured beside the headline: the neighbour
     rebuild on the 40^3 cloud, cloud management around a moved sphere in it,
-    the 2-D Chu FP64 step on the reference's own cavity2d case (Nx=31, Nv=16,
-    quadratic), and 16 independent cavity2d instances (Nx drawn from
+    the 2-D Chu step on the reference's own cavity2d case (Nx=31, Nv=16,
+    quadratic) with the direct FP64 transport and with the FP64 table
+    transport, and 16 independent cavity2d instances (Nx drawn from
     {31, 40, 50} with a fixed seed; plain cavities) replayed in one HIP graph
-    against the same instances stepped eagerly."""
+    against the same instances stepped eagerly, in both formulations."""
     import random
     import numpy as np
     out = {}
@@ -183,62 +184,66 @@ def components(A):
         log(f"[bench] neighbour rebuild 40^3: {ms:.3f} ms")
     except Exception as e:                          # reported, not hidden
         out["neighbour_rebuild_40cube"] = {"error": str(e)}
-    try:
-        case = A.configure("cavity2d", nx=31, nv=16, mls_order=2)
-        cl = A.build_lattice(case, precision=A.PREC_FP64, mls_order=2)
-        s = A.Solver(cl, precision=A.PREC_FP64)
-        s.steps(case.dt, 5)
-        s.synchronize()
-        n = 50
-        t0 = time.perf_counter()
-        s.steps(case.dt, n)
-        s.synchronize()
-        sec = (time.perf_counter() - t0) / n
-        upd = cl.N * 16 * 16
-        out["cavity2d_n31_nv16_fp64"] = {"ms_per_step": sec * 1e3, "updates_per_s": upd / sec,
-                                         "particles": cl.N, "velocity_nodes": 256}
-        s.close()
-        log(f"[bench] cavity2d 31^2 Nv16 FP64: {sec * 1e3:.3f} ms/step")
-    except Exception as e:
-        out["cavity2d_n31_nv16_fp64"] = {"error": str(e)}
-    try:
-        rng = random.Random(20261202)
-        sizes = [rng.choice([31, 40, 50]) for _ in range(16)]
-        sols, upd = [], 0
-        for nx in sizes:
-            case = A.configure("cavity2d", nx=nx, nv=16, mls_order=2)
+    for tag, prec in (("fp64", A.PREC_FP64), ("fp64_table", A.PREC_FP64_TABLE)):
+        key = f"cavity2d_n31_nv16_{tag}"
+        try:
+            case = A.configure("cavity2d", nx=31, nv=16, mls_order=2)
             cl = A.build_lattice(case, precision=A.PREC_FP64, mls_order=2)
-            sols.append(A.Solver(cl, precision=A.PREC_FP64))
-            upd += cl.N * 256
-        dt = A.configure("cavity2d", nx=31, nv=16, mls_order=2).dt
-        per = 10
-        graph = A.Graph(sols, dt, per)
-        graph.launch()
-        graph.synchronize()
-        reps = 5
-        t0 = time.perf_counter()
-        for _ in range(reps):
+            s = A.Solver(cl, precision=prec)
+            s.steps(case.dt, 5)
+            s.synchronize()
+            n = 50
+            t0 = time.perf_counter()
+            s.steps(case.dt, n)
+            s.synchronize()
+            sec = (time.perf_counter() - t0) / n
+            upd = cl.N * 16 * 16
+            out[key] = {"ms_per_step": sec * 1e3, "updates_per_s": upd / sec,
+                        "particles": cl.N, "velocity_nodes": 256}
+            s.close()
+            log(f"[bench] cavity2d 31^2 Nv16 {tag}: {sec * 1e3:.3f} ms/step")
+        except Exception as e:
+            out[key] = {"error": str(e)}
+    rng = random.Random(20261202)
+    sizes = [rng.choice([31, 40, 50]) for _ in range(16)]
+    for tag, prec in (("fp64", A.PREC_FP64), ("fp64_table", A.PREC_FP64_TABLE)):
+        key = f"batched_cavity2d_16_{tag}"
+        try:
+            sols, upd = [], 0
+            for nx in sizes:
+                case = A.configure("cavity2d", nx=nx, nv=16, mls_order=2)
+                cl = A.build_lattice(case, precision=A.PREC_FP64, mls_order=2)
+                sols.append(A.Solver(cl, precision=prec))
+                upd += cl.N * 256
+            dt = A.configure("cavity2d", nx=31, nv=16, mls_order=2).dt
+            per = 10
+            graph = A.Graph(sols, dt, per)
             graph.launch()
-        graph.synchronize()
-        sec_g = (time.perf_counter() - t0) / (reps * per)
-        graph.close()
-        t0 = time.perf_counter()
-        for _ in range(reps):
+            graph.synchronize()
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                graph.launch()
+            graph.synchronize()
+            sec_g = (time.perf_counter() - t0) / (reps * per)
+            graph.close()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                for sv in sols:
+                    sv.steps(dt, per)
             for sv in sols:
-                sv.steps(dt, per)
-        for sv in sols:
-            sv.synchronize()
-        sec_e = (time.perf_counter() - t0) / (reps * per)
-        out["batched_cavity2d_16"] = {"nx": sizes, "nv": 16, "graph_ms_per_step": sec_g * 1e3,
-                                      "eager_ms_per_step": sec_e * 1e3,
-                                      "graph_updates_per_s": upd / sec_g,
-                                      "eager_updates_per_s": upd / sec_e}
-        for sv in sols:
-            sv.close()
-        log(f"[bench] 16 cavity2d instances: graph {sec_g * 1e3:.3f} ms/step, "
-            f"eager {sec_e * 1e3:.3f} ms/step")
-    except Exception as e:
-        out["batched_cavity2d_16"] = {"error": str(e)}
+                sv.synchronize()
+            sec_e = (time.perf_counter() - t0) / (reps * per)
+            out[key] = {"nx": sizes, "nv": 16, "graph_ms_per_step": sec_g * 1e3,
+                        "eager_ms_per_step": sec_e * 1e3,
+                        "graph_updates_per_s": upd / sec_g,
+                        "eager_updates_per_s": upd / sec_e}
+            for sv in sols:
+                sv.close()
+            log(f"[bench] 16 cavity2d instances {tag}: graph {sec_g * 1e3:.3f} ms/step, "
+                f"eager {sec_e * 1e3:.3f} ms/step")
+        except Exception as e:
+            out[key] = {"error": str(e)}
     return out
 
 

@@ -1,0 +1,93 @@
+"""GPU parity of the 2-D table transport (AB_PREC_FP64_TABLE with dim 2):
+the 2-D Chu step with the MLS transport as per-class coefficient tables
+(Solver/CudaBackend.cu:199-300 factorised once per lattice class and dt).
+
+Tolerances (stated per check):
+  transport vs the oracle          1e-11 of each particle's largest |g|
+                                   (the same elimination, applied to unit
+                                   right-hand sides and summed per offset)
+  steps vs the FP64 GPU path       1e-10 relative on g, rho, u, T
+  mass                             1e-12 relative
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import alebgk4paper_amd as A
+import oracle_ctypes as O
+from helpers import rel_err, setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _dt(nx):
+    return 0.5 * (1e-6 / (nx - 1)) / 1303.0
+
+
+@pytest.mark.parametrize("nx,nv,order", [(10, 6, 1), (12, 8, 2), (31, 16, 2)])
+def test_table2d_transport_matches_oracle(nx, nv, order):
+    _say(f"[table2d transport] cavity2d nx={nx} nv={nv} order={order}")
+    case, cl, ocl, neigh, nn = setup("cavity2d", nx, nv, order=order)
+    dt = _dt(nx)
+    sol = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    sol.transport_only(dt)
+    gt = sol.gt()
+    want = O.transport(ocl, cl.g, dt, cl.cfg.radius, order)
+    scale = np.abs(cl.g).max(axis=1, keepdims=True)
+    err = (np.abs(gt - want) / scale).max()
+    assert err < 1e-11, err
+
+
+@pytest.mark.parametrize("nx,nv,order,nsteps", [(12, 8, 2, 5), (16, 8, 1, 5), (31, 16, 2, 20)])
+def test_table2d_steps_match_fp64(nx, nv, order, nsteps):
+    _say(f"[table2d steps] cavity2d nx={nx} nv={nv} order={order} steps={nsteps}")
+    case, cl, ocl, neigh, nn = setup("cavity2d", nx, nv, order=order)
+    dt = _dt(nx)
+    ref = A.Solver(cl, precision=A.PREC_FP64)
+    tab = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    ref.steps(dt, nsteps)
+    tab.steps(dt, nsteps)
+    assert rel_err(tab.g(), ref.g()) < 1e-10
+    mr, mt = ref.moments(), tab.moments()
+    for c in (0, 1, 2, 4):
+        assert rel_err(mt[:, c], mr[:, c], floor=1e-300) < 1e-10, c
+    assert abs(tab.total_mass() - ref.total_mass()) < 1e-12 * ref.total_mass()
+
+
+def test_table2d_dt_change_rebuilds():
+    """A new dt rebuilds the coefficients: two steps at different dt match
+    the FP64 path."""
+    _say("[table2d dt] cavity2d 12^2 nv8, two dt")
+    case, cl, ocl, neigh, nn = setup("cavity2d", 12, 8, order=2)
+    ref = A.Solver(cl, precision=A.PREC_FP64)
+    tab = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    for dt in (_dt(12), 0.5 * _dt(12)):
+        ref.step(dt)
+        tab.step(dt)
+    assert rel_err(tab.g(), ref.g()) < 1e-10
+
+
+def test_table2d_refuses_a_cloud_off_the_lattice():
+    """An interior particle moved off its lattice site has its own stencil:
+    the setup refuses the cloud instead of applying its class's table."""
+    _say("[table2d refusal] cavity2d 12^2 nv8, one particle moved")
+    case, cl, ocl, neigh, nn = setup("cavity2d", 12, 8, order=2)
+    p = 5 * 12 + 6
+    cl.x = cl.x.copy()
+    cl.x[p] += 1e-3 * (1e-6 / 11)
+    with pytest.raises(A.ALEBGKError, match="lattice class"):
+        A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    # the FP64 path takes the same cloud
+    A.Solver(cl, precision=A.PREC_FP64).close()
+
+
+def test_table2d_mixed_stays_refused():
+    _say("[table2d refusal] cavity2d AB_PREC_MIXED")
+    case, cl, ocl, neigh, nn = setup("cavity2d", 10, 6, order=2)
+    with pytest.raises(A.ALEBGKError):
+        A.Solver(cl, precision=A.PREC_MIXED)
