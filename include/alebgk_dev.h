@@ -43,8 +43,10 @@ extern "C" {
                                  transport contraction, collision and storage
                                  in FP32; conservation sums, wall scaling and
                                  body loads in FP64 */
-#define AB_PREC_FP64_TABLE 2  /* the MIXED pipeline with FP64 storage and
-                                 arithmetic */
+#define AB_PREC_FP64_TABLE 2  /* 3-D: the MIXED pipeline with FP64 storage
+                                 and arithmetic. 2-D: the FP64 step with
+                                 the MLS transport as per-class coefficient
+                                 tables (lattice clouds; MIXED is 3-D only) */
 
 typedef struct ab_ctx ab_ctx;
 
