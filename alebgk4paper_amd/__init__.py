@@ -139,7 +139,8 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_mlp_set", "ab_mlp_forward", "ab_mlp_infer", "ab_mlp_loss_grad", "ab_graph_create",
                "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy", "ab_autotune",
                "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels",
-               "ab_cloud_manage", "ab_cloud_source_map", "ab_download_particles"]
+               "ab_cloud_manage", "ab_cloud_source_map", "ab_download_particles",
+               "ab_batch_create", "ab_batch_scatter"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
@@ -624,6 +625,38 @@ def copy_calibration(nbytes=256 << 20, reps=2):
     rc = L.ab_copy_calibration(nbytes, reps)
     if rc:
         raise ALEBGKError(f"ab_copy_calibration: {ERRORS.get(rc, rc)}")
+
+
+class Batch(Solver):
+    """Several 2-D solvers' clouds in one context (ab_batch_create): one
+    launch per stage steps all of them, each exactly as it would alone.
+    Downloads (g(), moments(), ...) give the members' arrays concatenated in
+    order; ranges[k] is member k's particle range. scatter() copies g and
+    the moments back into the members, which must stay open meanwhile."""
+
+    def __init__(self, solvers):
+        self._lib = hip_lib()
+        self.members = list(solvers)
+        arr = (C.c_void_p * len(self.members))(*[m._ctx.value for m in self.members])
+        self._ctx = C.c_void_p()
+        rc = self._lib.ab_batch_create(arr, len(self.members), C.byref(self._ctx))
+        if rc:
+            msgs = "; ".join(m._lib.ab_last_error(m._ctx).decode() for m in self.members)
+            self._ctx = C.c_void_p()
+            raise ALEBGKError(f"ab_batch_create: {ERRORS.get(rc, rc)}: {msgs}")
+        self.cfg = self.members[0].cfg
+        self.cloud = None
+        self.N = sum(m.N for m in self.members)
+        self.dim = 2
+        self.gstride = self.members[0].gstride
+        ends = np.cumsum([0] + [m.N for m in self.members])
+        self.ranges = [(int(a), int(b)) for a, b in zip(ends[:-1], ends[1:])]
+
+    def upload(self, cloud):
+        raise ALEBGKError("a batch takes its clouds from its members")
+
+    def scatter(self):
+        self._check(self._lib.ab_batch_scatter(self._ctx), "ab_batch_scatter")
 
 
 class Graph:

@@ -54,8 +54,9 @@ void free_cloud(ab_ctx *c)
                     c->neigh, c->nneigh, c->ilist, c->blist, c->g, c->gt, c->limflag,
                     c->mlsflag, c->mlsovr, c->pmass, c->vox, c->vcount, c->vstart, c->vlist,
                     c->bmom, c->bidx, c->bwu, c->dm_alpha, c->dm_pass, c->dm_res, c->dm_rec,
-                    c->mood, c->moodfail};
+                    c->mood, c->moodfail, c->inst, c->ih, c->seg};
     for (void *p : ptrs) if (p) (void)hipFree(p);
+    c->inst = c->seg = nullptr; c->ih = nullptr;
     c->dm_alpha = c->dm_res = nullptr; c->dm_pass = nullptr; c->dm_rec = nullptr;
     c->mood = c->moodfail = nullptr;
     c->dm_rec_slots = c->dm_rec_next = c->dm_rec_filled = c->dm_rec_left = 0;
@@ -108,6 +109,16 @@ __global__ void k_fill_int(int *p, int n, int v)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+// Shifts the neighbour indices of n particles by add (a cloud moved into a
+// batch at particle add).
+__global__ void k_offset_neigh(int *neigh, const int *__restrict__ nneigh, int n, int add)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * AB_MAX_NEIGH) return;
+    const int i = (int)(e / AB_MAX_NEIGH), m = (int)(e % AB_MAX_NEIGH);
+    if (m < nneigh[i]) neigh[e] += add;
+}
+
 __global__ void k_fill_dbl(double *p, int n, double v)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -170,7 +181,7 @@ int fp64_step(ab_ctx *c, double dt)
     int rc;
     tm.tic();
     if (!c->mass_valid) launch_mass_partials(v, c->g, c->pmass, s);
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_OLD, 0, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_OLD, 0, s, c->seg, c->nbatch);
     if ((rc = zero_step_flags(c))) return rc;
     tm.toc(ST_REDUCE);
     tm.tic();
@@ -180,7 +191,7 @@ int fp64_step(ab_ctx *c, double dt)
     launch_limit_commit(v, c->g, c->gt, c->rho, c->limflag, c->pmass, s);
     tm.toc(ST_LIMIT_COMMIT);
     tm.tic();
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s, c->seg, c->nbatch);
     tm.toc(ST_REDUCE);
     tm.tic();
     launch_wall_collide(v, dm_args(c), c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux,
@@ -188,7 +199,7 @@ int fp64_step(ab_ctx *c, double dt)
     tm.toc(ST_WALL_COLLIDE);
     dm_step_done(c);
     tm.tic();
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, c->cfg.no_mass_fix, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, c->cfg.no_mass_fix, s, c->seg, c->nbatch);
     tm.toc(ST_REDUCE);
     tm.tic();
     launch_final(v, c->g, c->scal, c->cfg.R, c->rho, c->ux, c->uy, c->uz, c->T, c->pmass, s);
@@ -212,7 +223,7 @@ int fp64_step_mood(ab_ctx *c, double dt)
     int rc;
     tm.tic();
     if (!c->mass_valid) launch_mass_partials(v, c->g, c->pmass, s);
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_OLD, 0, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_OLD, 0, s, c->seg, c->nbatch);
     if ((rc = zero_step_flags(c))) return rc;
     tm.toc(ST_REDUCE);
     tm.tic();
@@ -226,14 +237,14 @@ int fp64_step_mood(ab_ctx *c, double dt)
     launch_mood_commit(v, c->g, c->gt, c->mood, c->limflag, c->pmass, s);
     tm.toc(ST_LIMIT_COMMIT);
     tm.tic();
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s, c->seg, c->nbatch);
     tm.toc(ST_REDUCE);
     tm.tic();
     launch_wall_collide(v, dm_args(c), c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux,
                         c->uy, c->uz, c->T, c->pmass, s);
     tm.toc(ST_WALL_COLLIDE);
     tm.tic();
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, 1, s);
+    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, 1, s, c->seg, c->nbatch);
     tm.toc(ST_REDUCE);
     tm.tic();
     launch_final(v, c->g, c->scal, c->cfg.R, c->rho, c->ux, c->uy, c->uz, c->T, c->pmass, s);
@@ -534,6 +545,7 @@ const char *ab_last_error(const ab_ctx *c) { return c ? c->err.c_str() : "null c
 int ab_upload(ab_ctx *c, const ab_cloud_view *v)
 {
     if (!c || !v) return AB_ERR_ARG;
+    if (c->nbatch) { c->err = "a batch takes its clouds from its members (ab_batch_create)"; return AB_ERR_STATE; }
     if (!v->x || !v->y || !v->z || !v->boundary || !v->wnx || !v->wny || !v->wnz || !v->wux ||
         !v->wuy || !v->wuz || !v->wt || !v->rho || !v->ux || !v->uy || !v->uz || !v->t || !v->g)
     { c->err = "ab_upload: a required array is NULL"; return AB_ERR_ARG; }
@@ -596,6 +608,7 @@ int ab_rebuild_neighbours(ab_ctx *c)
 {
     if (!c) return AB_ERR_ARG;
     if (!c->uploaded) { c->err = "ab_rebuild_neighbours before ab_upload"; return AB_ERR_STATE; }
+    if (c->nbatch) { c->err = "a batch keeps its members' neighbour lists; rebuild them in the members"; return AB_ERR_UNSUPPORTED; }
     const ab_config &f = c->cfg;
     const int nbw = (c->dim == 3) ? f.nbwbox : 1;
     if (f.nbhbox <= 0 || f.nbvbox <= 0 || nbw <= 0 || f.hbox == 0 || f.vbox == 0 ||
@@ -1051,6 +1064,112 @@ int ab_set_wall_state(ab_ctx *c, int n, const int *idx, const double *x, const d
         c->coef_dt = -1.0;
         if (rc) { c->uploaded = false; return rc; }
     }
+    return AB_OK;
+}
+
+int ab_batch_create(ab_ctx *const *m, int n, ab_ctx **out)
+{
+    if (!m || !out || n <= 0) return AB_ERR_ARG;
+    *out = nullptr;
+    for (int k = 0; k < n; ++k)
+    {
+        if (!m[k]) return AB_ERR_ARG;
+        if (!m[k]->uploaded || m[k]->nbatch) return AB_ERR_STATE;
+    }
+    const ab_config &f = m[0]->cfg;
+    long total = 0;
+    for (int k = 0; k < n; ++k)
+    {
+        const ab_config &g = m[k]->cfg;
+        if (g.dim != 2 || g.nv != f.nv || g.vmin != f.vmin || g.vmax != f.vmax || g.R != f.R ||
+            g.tau != f.tau || g.mls_order != f.mls_order || g.mls_tol != f.mls_tol ||
+            g.mls_clamp != f.mls_clamp || g.no_mass_fix != f.no_mass_fix ||
+            g.precision != f.precision || g.limiter != AB_LIMITER_GPU ||
+            g.collision != AB_COLLISION_CONTINUOUS)
+            return AB_ERR_UNSUPPORTED;
+        total += m[k]->N;
+    }
+    if (total > (1L << 30)) return AB_ERR_ARG;
+    ab_config cfg = f;
+    cfg.n_particles = (int)total;
+    cfg.nx = cfg.ny = 0;
+    cfg.nz = 1;
+    ab_ctx *b = nullptr;
+    int rc = ab_create(&cfg, &b);
+    if (rc) return rc;
+    auto fail = [&](int code) { ab_destroy(b); return code; };
+    b->N = (int)total;
+    if ((rc = alloc_cloud(b))) return fail(rc);
+    (void)hipFree(b->scal);
+    b->scal = nullptr;
+    if ((rc = dalloc(b, &b->scal, (size_t)n * S_COUNT)) || (rc = dalloc(b, &b->inst, (size_t)total)) ||
+        (rc = dalloc(b, &b->ih, (size_t)n)) || (rc = dalloc(b, &b->seg, (size_t)n + 1)))
+        return fail(rc);
+    hipStream_t s = b->stream;
+    b->bt_base.assign(1, 0);
+    for (int k = 0; k < n; ++k)
+    {
+        ab_ctx *a = m[k];
+        if (hipStreamSynchronize(a->stream) != hipSuccess) return fail(AB_ERR_HIP);
+        const size_t o = (size_t)b->bt_base[k], nk = (size_t)a->N;
+        const double *src[] = {a->x, a->y, a->z, a->wnx, a->wny, a->wnz, a->wux, a->wuy, a->wuz,
+                               a->wt, a->mvx, a->mvy, a->mvz, a->rho, a->ux, a->uy, a->uz, a->T};
+        double *dst[] = {b->x, b->y, b->z, b->wnx, b->wny, b->wnz, b->wux, b->wuy, b->wuz,
+                         b->wt, b->mvx, b->mvy, b->mvz, b->rho, b->ux, b->uy, b->uz, b->T};
+        bool ok = true;
+        for (int q = 0; q < 18 && ok; ++q)
+            ok = hipMemcpyAsync(dst[q] + o, src[q], 8 * nk, hipMemcpyDeviceToDevice, s) == hipSuccess;
+        ok = ok && hipMemcpyAsync(b->bnd + o, a->bnd, nk, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(b->g + o * b->gstride, a->g, 8 * nk * b->gstride,
+                            hipMemcpyDeviceToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(b->neigh + o * AB_MAX_NEIGH, a->neigh, 4 * nk * AB_MAX_NEIGH,
+                            hipMemcpyDeviceToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(b->nneigh + o, a->nneigh, 4 * nk, hipMemcpyDeviceToDevice, s) == hipSuccess;
+        if (!ok) { b->err = "ab_batch_create: device copy failed"; return fail(AB_ERR_HIP); }
+        const size_t ne = nk * AB_MAX_NEIGH;
+        k_offset_neigh<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(b->neigh + o * AB_MAX_NEIGH,
+                                                                    b->nneigh + o, (int)nk, (int)o);
+        k_fill_int<<<(unsigned)((nk + 255) / 256), 256, 0, s>>>(b->inst + o, (int)nk, k);
+        b->bt_base.push_back((int)(o + nk));
+        b->bt_nx.push_back(a->cfg.nx);
+        b->bt_ny.push_back(a->cfg.ny);
+        b->bt_h.push_back(a->cfg.radius);
+        b->members.push_back(a);
+    }
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(b->ih, b->bt_h.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(b->seg, b->bt_base.data(), 4 * ((size_t)n + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(b->scal, 0, sizeof(double) * S_COUNT * (size_t)n, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+    { b->err = "ab_batch_create: device copy failed"; return fail(AB_ERR_HIP); }
+    b->nbatch = n;
+    if ((rc = rebuild_lists(b))) return fail(rc);
+    b->uploaded = true;
+    b->mass_valid = false;
+    if (table2d(b) && (rc = table2d_setup(b))) return fail(rc);
+    *out = b;
+    return AB_OK;
+}
+
+int ab_batch_scatter(ab_ctx *b)
+{
+    if (!b) return AB_ERR_ARG;
+    if (!b->nbatch) { b->err = "ab_batch_scatter: not a batch"; return AB_ERR_STATE; }
+    HIPCHK(b, hipStreamSynchronize(b->stream));
+    for (int k = 0; k < b->nbatch; ++k)
+    {
+        ab_ctx *a = b->members[k];
+        const size_t o = (size_t)b->bt_base[k], nk = (size_t)a->N;
+        const double *src[] = {b->rho, b->ux, b->uy, b->uz, b->T};
+        double *dst[] = {a->rho, a->ux, a->uy, a->uz, a->T};
+        HIPCHK(b, hipStreamSynchronize(a->stream));
+        for (int q = 0; q < 5; ++q)
+            HIPCHK(b, hipMemcpyAsync(dst[q], src[q] + o, 8 * nk, hipMemcpyDeviceToDevice, b->stream));
+        HIPCHK(b, hipMemcpyAsync(a->g, b->g + o * b->gstride, 8 * nk * b->gstride,
+                                 hipMemcpyDeviceToDevice, b->stream));
+        a->mass_valid = false;
+    }
+    HIPCHK(b, hipStreamSynchronize(b->stream));
     return AB_OK;
 }
 

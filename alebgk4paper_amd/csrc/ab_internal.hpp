@@ -24,6 +24,11 @@ struct DevCloud
     const double *wnx, *wny, *wnz, *wux, *wuy, *wuz, *wt;
     const double *mvx, *mvy, *mvz;
     const int *neigh, *nneigh;
+    // several independent clouds in one context (ab_batch_create): the
+    // instance of each particle and each instance's MLS support radius;
+    // null for a single cloud
+    const int *inst;
+    const double *ih;
 };
 
 // Discrete-Maxwellian collision settings and outputs (3-D interior
@@ -129,9 +134,11 @@ struct ab_ctx
     int apply_shape = -1;          // transport launch shape (-1: default for nz)
 
     // 2-D table transport (AB_PREC_FP64_TABLE with dim 2)
-    double *t2_coef = nullptr;     // [25 classes][21 offsets][Nv^2] MLS coefficients
+    double *t2_coef = nullptr;     // [instances x 25 classes][21 offsets][Nv^2] MLS coefficients
     int *t2_cls = nullptr;         // class of each particle, N
-    int *t2_rep = nullptr;         // representative particle of each class, 25
+    int *t2_rep = nullptr;         // representative particle of each class
+    int *t2_lat = nullptr;         // (ix, iy, nx, ny) of each particle, 4*N
+    int t2_ncls = 0;
     double t2_dt = -1.0;           // dt the coefficients were built for
 
     // discrete-Maxwellian collision (cfg.collision) and its MLP warm start
@@ -153,6 +160,17 @@ struct ab_ctx
     int mlp_rows_cap = 0;
     long mlp_step = 0;             // Adam step count
     bool mlp_ready = false;        // parameters initialised
+
+    // a batch of independent 2-D clouds (ab_batch_create): per instance its
+    // first particle (nbatch + 1 entries), lattice size and support radius;
+    // empty for a single cloud
+    int nbatch = 0;
+    int *inst = nullptr;           // instance of each particle, N
+    double *ih = nullptr;          // support radius of each instance
+    int *seg = nullptr;            // first particle of each instance, nbatch + 1
+    std::vector<int> bt_base, bt_nx, bt_ny;
+    std::vector<double> bt_h;
+    std::vector<ab_ctx *> members;
 
     // cloud management (ab_cloud_manage): the last call's source map
     std::vector<int> cm_src;
@@ -184,6 +202,7 @@ struct ab_ctx
         c.wnx = wnx; c.wny = wny; c.wnz = wnz; c.wux = wux; c.wuy = wuy; c.wuz = wuz;
         c.wt = wt; c.mvx = mvx; c.mvy = mvy; c.mvz = mvz;
         c.neigh = neigh; c.nneigh = nneigh;
+        c.inst = inst; c.ih = ih;
         return c;
     }
 };
@@ -210,8 +229,10 @@ void launch_mood_commit(const DevCloud &c, double *g, double *gt, const int *moo
 void launch_pad_check(const DevCloud &c, const double *g, const double *rho, const double *T,
                       double *limflag, int *fail, hipStream_t s);
 void launch_mass_partials(const DevCloud &c, const double *g, double *pmass, hipStream_t s);
+// One sum per instance when seg is given (nseg instances, first particles
+// seg[0..nseg], scalars S_COUNT apart); else one over all N.
 void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int nofix,
-                        hipStream_t s);
+                        hipStream_t s, const int *seg = nullptr, int nseg = 1);
 void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
                          double *limflag, double *pmass, hipStream_t s);
 // guard (may be null): when given, the launch acts only if *guard is set,

@@ -534,6 +534,8 @@ extern "C" int ab_cloud_manage(ab_ctx *c, int nbodies, const ab_body *bodies, co
     if (!c->uploaded) { c->err = "ab_cloud_manage before ab_upload"; return AB_ERR_STATE; }
     if (c->cfg.precision != AB_PREC_FP64)
     { c->err = "ab_cloud_manage: the cloud changes, which the lattice tables cannot follow (AB_PREC_FP64 only)"; return AB_ERR_UNSUPPORTED; }
+    if (c->nbatch)
+    { c->err = "ab_cloud_manage: manage the members' clouds, then form the batch again"; return AB_ERR_UNSUPPORTED; }
     if (nbodies < 0 || nbodies > kMaxBodies || (nbodies && !bodies) || !body_id || !dom ||
         !(dx != 0.0) || !std::isfinite(dx))
     { c->err = "ab_cloud_manage: invalid argument"; return AB_ERR_ARG; }

@@ -206,6 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double
     if (only && !only[i]) return;            // whole block: one particle
     const int lin = (b % nchunk) * kBlock + threadIdx.x;
     const int nn = c.nneigh[i];
+    if (c.inst) h = c.ih[c.inst[i]];
     for (int t = threadIdx.x; t < nn; t += kBlock)
     {
         int j = c.neigh[(size_t)i * AB_MAX_NEIGH + t];
@@ -338,12 +339,22 @@ __global__ __launch_bounds__(kBlock) void k_mass_partials(DevCloud c, const doub
 // Fixed-order sum of the per-particle partials and the rescale factor
 // (sumMassHost + the scalar logic of cudaStep :1627, :1706-1710, :1773-1780),
 // kept on the device so no host synchronisation is needed.
+// With seg, block k sums instance k's particles seg[k] .. seg[k+1]-1 into
+// its scalars (S_COUNT apart) in the same order as a cloud of its own.
 __global__ __launch_bounds__(1024) void k_reduce_mass(int N, const double *__restrict__ pmass,
-                                                      double *scal, int mode, int nofix)
+                                                      double *scal, int mode, int nofix,
+                                                      const int *__restrict__ seg)
 {
     __shared__ double sh[1024 / kWave];
+    int lo = 0, hi = N;
+    if (seg)
+    {
+        lo = seg[blockIdx.x];
+        hi = seg[blockIdx.x + 1];
+        scal += (size_t)blockIdx.x * S_COUNT;
+    }
     double s = 0.0;
-    for (int i = threadIdx.x; i < N; i += 1024) s += pmass[i];
+    for (int i = lo + threadIdx.x; i < hi; i += 1024) s += pmass[i];
     s = wave_sum(s);
     if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = s;
     __syncthreads();
@@ -491,6 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
     __shared__ double sh[15 * (kBlock / kWave)];
     const int i = xcd_remap(blockIdx.x, gridDim.x);
     if (guard && (!*guard || c.bnd[i])) return;        // whole block: one particle
+    if (c.inst) scal += (size_t)c.inst[i] * S_COUNT;
     const double s1 = scal[S_SCALE1];
     const bool doScale = !guard && scal[S_MASS_NEW] > 1e-30;
     const size_t gs = (size_t)c.gstride;
@@ -718,6 +730,7 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
     __shared__ double sh[6 * (kBlock / kWave)];
     const int i = xcd_remap(blockIdx.x, gridDim.x);
     if (guard && !*guard) return;
+    if (c.inst) scal += (size_t)c.inst[i] * S_COUNT;
     const double s2 = scal[S_SCALE2];
     const bool doScale = (s2 != 1.0);
     const size_t gs = (size_t)c.gstride;
@@ -947,9 +960,9 @@ void launch_mass_partials(const DevCloud &c, const double *g, double *pmass, hip
 }
 
 void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int nofix,
-                        hipStream_t s)
+                        hipStream_t s, const int *seg, int nseg)
 {
-    k_reduce_mass<<<1, 1024, 0, s>>>(N, pmass, scal, mode, nofix);
+    k_reduce_mass<<<seg ? nseg : 1, 1024, 0, s>>>(N, pmass, scal, mode, nofix, seg);
 }
 
 void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,

@@ -68,8 +68,8 @@ __host__ __device__ inline int category2(int i, int n)
 // with the P unit vectors as right-hand sides. out: kOff2 coefficients at
 // stride `stride`.
 template <int P, bool BND>
-__device__ void build2(const DevCloud &c, int i, int lin, double dt, double h, int nx,
-                       double *out, size_t stride)
+__device__ void build2(const DevCloud &c, int i, int lin, double dt, double h,
+                       const int *__restrict__ lat, double *out, size_t stride)
 {
     const double vx = vnode(c.vlo, c.dv, lin % c.Nv);
     const double vy = vnode(c.vlo, c.dv, lin / c.Nv);
@@ -161,7 +161,6 @@ __device__ void build2(const DevCloud &c, int i, int lin, double dt, double h, i
             s[row * P + k] = fabs(d) > 1e-30 ? v / d : 0.0;
         }
     }
-    const int ix = i % nx, iy = i / nx;
     for (int m = 0; m < nn; ++m)
     {
         const int j = nb[m];
@@ -171,14 +170,15 @@ __device__ void build2(const DevCloud &c, int i, int lin, double dt, double h, i
         double cm = 0.0;
 #pragma unroll
         for (int k = 0; k < P; ++k) cm += basis[k] * s[k];   // s[0 * P + k] = s0(e_k)
-        const int o = off2_index(j % nx - ix, j / nx - iy);
+        const int o = off2_index(lat[4 * j] - lat[4 * i], lat[4 * j + 1] - lat[4 * i + 1]);
         if (o >= 0) out[o * stride] += w * cm;
     }
 }
 
-// Coefficients [class][offset][node].
-__global__ void k_build2(DevCloud c, const int *__restrict__ rep, double dt, double h, int order,
-                         int nx, double *__restrict__ coef)
+// Coefficients [class][offset][node]; classes of several instances use
+// their instance's support radius.
+__global__ void k_build2(DevCloud c, const int *__restrict__ rep, const int *__restrict__ lat,
+                         double dt, double h, int order, double *__restrict__ coef)
 {
     const int cls = blockIdx.y;
     const int lin = blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,15 +188,17 @@ __global__ void k_build2(DevCloud c, const int *__restrict__ rep, double dt, dou
     for (int o = 0; o < kOff2; ++o) out[o * stride] = 0.0;
     const int i = rep[cls];
     if (i < 0) return;
-    if (c.bnd[i]) build2<3, true>(c, i, lin, dt, h, nx, out, stride);
-    else if (order >= 2) build2<6, false>(c, i, lin, dt, h, nx, out, stride);
-    else build2<3, false>(c, i, lin, dt, h, nx, out, stride);
+    if (c.inst) h = c.ih[c.inst[i]];
+    if (c.bnd[i]) build2<3, true>(c, i, lin, dt, h, lat, out, stride);
+    else if (order >= 2) build2<6, false>(c, i, lin, dt, h, lat, out, stride);
+    else build2<3, false>(c, i, lin, dt, h, lat, out, stride);
 }
 
 // gt = the class stencil applied to both layers; one thread per (particle,
 // node), flat over N * Nv^2. Offsets outside the lattice have coefficient 0
 // and read a clamped (valid) neighbour.
-__global__ __launch_bounds__(256) void k_apply2(int N, int nx, int ny, int nodes, int gstride,
+__global__ __launch_bounds__(256) void k_apply2(int N, int nodes, int gstride,
+                                                const int *__restrict__ lat,
                                                 const int *__restrict__ cls,
                                                 const double *__restrict__ coef,
                                                 const double *__restrict__ g,
@@ -205,7 +207,9 @@ __global__ __launch_bounds__(256) void k_apply2(int N, int nx, int ny, int nodes
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long)N * nodes) return;
     const int i = (int)(e / nodes), lin = (int)(e - (long)i * nodes);
-    const int ix = i % nx, iy = i / nx;
+    const int4 l = reinterpret_cast<const int4 *>(lat)[i];   // ix, iy, nx, ny
+    const int ix = l.x, iy = l.y, nx = l.z, ny = l.w;
+    const size_t base = (size_t)i - ix - (size_t)nx * iy;
     const double *cf = coef + (size_t)cls[i] * kOff2 * nodes + lin;
     double a1 = 0.0, a2 = 0.0;
 #pragma unroll
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(256) void k_apply2(int N, int nx, int ny, int nodes
     {
         const Off2 d = off2_at(o);
         const int jx = clampi(ix + d.ox, 0, nx - 1), jy = clampi(iy + d.oy, 0, ny - 1);
-        const double *gj = g + ((size_t)jx + (size_t)nx * jy) * gstride + lin;
+        const double *gj = g + (base + jx + (size_t)nx * jy) * gstride + lin;
         const double w = cf[(size_t)o * nodes];
         a1 += w * gj[0];
         a2 += w * gj[nodes];
@@ -228,22 +232,30 @@ namespace ab {
 
 void table2d_free(ab_ctx *c)
 {
-    if (c->t2_coef) (void)hipFree(c->t2_coef);
-    if (c->t2_cls) (void)hipFree(c->t2_cls);
-    if (c->t2_rep) (void)hipFree(c->t2_rep);
-    c->t2_coef = nullptr; c->t2_cls = nullptr; c->t2_rep = nullptr;
+    void *ptrs[] = {c->t2_coef, c->t2_cls, c->t2_rep, c->t2_lat};
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    c->t2_coef = nullptr; c->t2_cls = nullptr; c->t2_rep = nullptr; c->t2_lat = nullptr;
+    c->t2_ncls = 0;
     c->t2_dt = -1.0;
 }
 
-// Classes of the 2-D lattice cloud and their checks (every particle has its
-// class representative's stencil and wall state), once per upload.
+// Classes of the 2-D lattice clouds (one, or each instance of a batch) and
+// their checks: every particle has its class representative's neighbour
+// offsets and displacements (to 1e-12 of the support radius; the list order
+// may differ) and wall state. Run after the upload and after every change of
+// the cloud's geometry, lists or wall state.
 int table2d_setup(ab_ctx *c)
 {
     const ab_config &cfg = c->cfg;
-    const int N = c->N, nx = cfg.nx, ny = cfg.ny;
+    const int N = c->N;
+    std::vector<int> base{0, N}, nxs{cfg.nx}, nys{cfg.ny};
+    std::vector<double> hs{cfg.radius};
+    if (c->nbatch > 0) { base = c->bt_base; nxs = c->bt_nx; nys = c->bt_ny; hs = c->bt_h; }
+    const int ninst = (int)nxs.size();
     table2d_free(c);
-    if (nx < 5 || ny < 5 || (long)nx * ny != N)
-    { c->err = "the 2-D table transport needs a lattice cloud: nx, ny >= 5 and N = nx*ny"; return AB_ERR_UNSUPPORTED; }
+    for (int k = 0; k < ninst; ++k)
+        if (nxs[k] < 5 || nys[k] < 5 || (long)nxs[k] * nys[k] != base[k + 1] - base[k])
+        { c->err = "the 2-D table transport needs lattice clouds: nx, ny >= 5 and N = nx*ny"; return AB_ERR_UNSUPPORTED; }
     if (cfg.mls_clamp)
     { c->err = "mls_clamp is nonlinear in g; run it with AB_PREC_FP64"; return AB_ERR_UNSUPPORTED; }
     std::vector<unsigned char> b(N);
@@ -260,53 +272,68 @@ int table2d_setup(ab_ctx *c)
     for (int k = 0; k < 7 && ok; ++k) ok = dl(&w[(size_t)k * N], ws[k], 8 * (size_t)N);
     if (!ok || hipStreamSynchronize(s) != hipSuccess)
     { c->err = "2-D table setup: device-to-host copy failed"; return AB_ERR_HIP; }
-    std::vector<int> cls(N), rep(kNcls2, -1);
+    const int ncls = kNcls2 * ninst;
+    std::vector<int> cls(N), rep(ncls, -1), lat(4 * (size_t)N);
     std::vector<unsigned> mask(N, 0);
-    for (int p = 0; p < N; ++p)
+    for (int k = 0; k < ninst; ++k)
     {
-        const int ix = p % nx, iy = p / nx;
-        cls[p] = category2(ix, nx) + 5 * category2(iy, ny);
-        if (rep[cls[p]] < 0) rep[cls[p]] = p;
-        for (int m = 0; m < nn[p]; ++m)
+        const int nx = nxs[k], ny = nys[k];
+        for (int p = base[k]; p < base[k + 1]; ++p)
         {
-            const int j = nb[(size_t)p * AB_MAX_NEIGH + m];
-            const int o = off2_index(j % nx - ix, j / nx - iy);
-            if (o < 0)
+            const int q = p - base[k], ix = q % nx, iy = q / nx;
+            lat[4 * p] = ix; lat[4 * p + 1] = iy; lat[4 * p + 2] = nx; lat[4 * p + 3] = ny;
+            cls[p] = kNcls2 * k + category2(ix, nx) + 5 * category2(iy, ny);
+            if (rep[cls[p]] < 0) rep[cls[p]] = p;
+        }
+    }
+    for (int k = 0; k < ninst; ++k)
+        for (int p = base[k]; p < base[k + 1]; ++p)
+            for (int m = 0; m < nn[p]; ++m)
+            {
+                const int j = nb[(size_t)p * AB_MAX_NEIGH + m];
+                const int o = (j < base[k] || j >= base[k + 1])
+                                  ? -1 : off2_index(lat[4 * j] - lat[4 * p], lat[4 * j + 1] - lat[4 * p + 1]);
+                if (o < 0)
+                {
+                    c->err = "2-D table setup: particle " + std::to_string(p) +
+                             " has a neighbour outside the lattice stencil";
+                    return AB_ERR_UNSUPPORTED;
+                }
+                mask[p] |= 1u << o;
+            }
+    for (int k = 0; k < ninst; ++k)
+    {
+        const double tol = 1e-12 * hs[k];
+        for (int p = base[k]; p < base[k + 1]; ++p)
+        {
+            const int r = rep[cls[p]];
+            bool same = mask[p] == mask[r] && nn[p] == nn[r] && b[p] == b[r];
+            for (int q = 0; q < 7 && same; ++q) same = w[(size_t)q * N + p] == w[(size_t)q * N + r];
+            // the neighbours at the same displacements (the representative
+            // has a neighbour at every lattice offset p has: the masks are equal)
+            for (int m = 0; m < nn[p] && same; ++m)
+            {
+                const int jp = nb[(size_t)p * AB_MAX_NEIGH + m], jr = r + (jp - p);
+                same = std::fabs((x[jp] - x[p]) - (x[jr] - x[r])) <= tol &&
+                       std::fabs((y[jp] - y[p]) - (y[jr] - y[r])) <= tol;
+            }
+            if (!same)
             {
                 c->err = "2-D table setup: particle " + std::to_string(p) +
-                         " has a neighbour outside the lattice stencil";
+                         " differs from its lattice class (stencil, geometry or wall state)";
                 return AB_ERR_UNSUPPORTED;
             }
-            mask[p] |= 1u << o;
         }
     }
-    const double tol = 1e-12 * cfg.radius;
-    for (int p = 0; p < N; ++p)
-    {
-        const int r = rep[cls[p]];
-        bool same = mask[p] == mask[r] && nn[p] == nn[r] && b[p] == b[r];
-        for (int k = 0; k < 7 && same; ++k) same = w[(size_t)k * N + p] == w[(size_t)k * N + r];
-        // the neighbours at the same displacements (the representative has
-        // a neighbour at every lattice offset p has: the masks are equal)
-        for (int m = 0; m < nn[p] && same; ++m)
-        {
-            const int jp = nb[(size_t)p * AB_MAX_NEIGH + m], jr = r + (jp - p);
-            same = std::fabs((x[jp] - x[p]) - (x[jr] - x[r])) <= tol &&
-                   std::fabs((y[jp] - y[p]) - (y[jr] - y[r])) <= tol;
-        }
-        if (!same)
-        {
-            c->err = "2-D table setup: particle " + std::to_string(p) +
-                     " differs from its lattice class (stencil, geometry or wall state)";
-            return AB_ERR_UNSUPPORTED;
-        }
-    }
-    if (hipMalloc((void **)&c->t2_coef, sizeof(double) * kNcls2 * kOff2 * (size_t)c->nodes) != hipSuccess ||
+    if (hipMalloc((void **)&c->t2_coef, sizeof(double) * ncls * kOff2 * (size_t)c->nodes) != hipSuccess ||
         hipMalloc((void **)&c->t2_cls, sizeof(int) * (size_t)N) != hipSuccess ||
-        hipMalloc((void **)&c->t2_rep, sizeof(int) * kNcls2) != hipSuccess)
+        hipMalloc((void **)&c->t2_rep, sizeof(int) * ncls) != hipSuccess ||
+        hipMalloc((void **)&c->t2_lat, sizeof(int) * 4 * (size_t)N) != hipSuccess)
     { table2d_free(c); c->err = "2-D table setup: device allocation failed"; return AB_ERR_NOMEM; }
+    c->t2_ncls = ncls;
     if (hipMemcpyAsync(c->t2_cls, cls.data(), sizeof(int) * N, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(c->t2_rep, rep.data(), sizeof(int) * kNcls2, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(c->t2_rep, rep.data(), sizeof(int) * ncls, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(c->t2_lat, lat.data(), sizeof(int) * 4 * (size_t)N, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
     { table2d_free(c); c->err = "2-D table setup: host-to-device copy failed"; return AB_ERR_HIP; }
     return AB_OK;
@@ -315,9 +342,9 @@ int table2d_setup(ab_ctx *c)
 int table2d_prepare(ab_ctx *c, double dt)
 {
     if (!c->t2_coef) { c->err = "2-D table transport before its setup"; return AB_ERR_STATE; }
-    dim3 gb((c->nodes + 63) / 64, kNcls2);
-    k_build2<<<gb, 64, 0, c->stream>>>(c->view(), c->t2_rep, dt, c->cfg.radius, c->cfg.mls_order,
-                                       c->cfg.nx, c->t2_coef);
+    dim3 gb((c->nodes + 63) / 64, c->t2_ncls);
+    k_build2<<<gb, 64, 0, c->stream>>>(c->view(), c->t2_rep, c->t2_lat, dt, c->cfg.radius,
+                                       c->cfg.mls_order, c->t2_coef);
     if (hipGetLastError() != hipSuccess) { c->err = "2-D table build launch failed"; return AB_ERR_HIP; }
     c->t2_dt = dt;
     return AB_OK;
@@ -330,7 +357,7 @@ int table2d_transport(ab_ctx *c, double dt)
     if (c->t2_dt != dt && (rc = table2d_prepare(c, dt))) return rc;
     const long n = (long)c->N * c->nodes;
     k_apply2<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
-        c->N, c->cfg.nx, c->cfg.ny, c->nodes, c->gstride, c->t2_cls, c->t2_coef, c->g, c->gt);
+        c->N, c->nodes, c->gstride, c->t2_lat, c->t2_cls, c->t2_coef, c->g, c->gt);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 

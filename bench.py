@@ -157,7 +157,9 @@ def components(A):
     quadratic) with the direct FP64 transport and with the FP64 table
     transport, and 16 independent cavity2d instances (Nx drawn from
     {31, 40, 50} with a fixed seed; plain cavities) replayed in one HIP graph
-    against the same instances stepped eagerly, in both formulations."""
+    against the same instances stepped eagerly, and as one batch context (one
+    launch per stage for all instances), eagerly and in a graph, in both
+    formulations."""
     import random
     import numpy as np
     out = {}
@@ -234,14 +236,36 @@ def components(A):
             for sv in sols:
                 sv.synchronize()
             sec_e = (time.perf_counter() - t0) / (reps * per)
+            # the same instances as one batch context: one launch per stage
+            batch = A.Batch(sols)
+            batch.steps(dt, per)
+            batch.synchronize()
+            t0 = time.perf_counter()
+            batch.steps(dt, reps * per)
+            batch.synchronize()
+            sec_b = (time.perf_counter() - t0) / (reps * per)
+            bgraph = A.Graph([batch], dt, per)
+            bgraph.launch()
+            bgraph.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                bgraph.launch()
+            bgraph.synchronize()
+            sec_bg = (time.perf_counter() - t0) / (reps * per)
+            bgraph.close()
+            batch.close()
             out[key] = {"nx": sizes, "nv": 16, "graph_ms_per_step": sec_g * 1e3,
                         "eager_ms_per_step": sec_e * 1e3,
                         "graph_updates_per_s": upd / sec_g,
-                        "eager_updates_per_s": upd / sec_e}
+                        "eager_updates_per_s": upd / sec_e,
+                        "batch_ms_per_step": sec_b * 1e3,
+                        "batch_graph_ms_per_step": sec_bg * 1e3,
+                        "batch_graph_updates_per_s": upd / sec_bg}
             for sv in sols:
                 sv.close()
             log(f"[bench] 16 cavity2d instances {tag}: graph {sec_g * 1e3:.3f} ms/step, "
-                f"eager {sec_e * 1e3:.3f} ms/step")
+                f"eager {sec_e * 1e3:.3f} ms/step, batch {sec_b * 1e3:.3f} ms/step, "
+                f"batch in a graph {sec_bg * 1e3:.3f} ms/step")
         except Exception as e:
             out[key] = {"error": str(e)}
     return out

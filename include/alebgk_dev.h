@@ -283,6 +283,22 @@ int ab_graph_launch(ab_graph *graph);
 int ab_graph_synchronize(ab_graph *graph);
 int ab_graph_destroy(ab_graph *graph);
 
+/* ---- Several 2-D clouds in one context ----------------------------------
+ * No counterpart in the reference (one cloud per run). ab_batch_create
+ * copies the uploaded 2-D contexts' clouds (any sizes; the same velocity
+ * grid, gas, MLS order and precision, AB_PREC_FP64 or AB_PREC_FP64_TABLE,
+ * the GPU limiter and the continuous collision) into one new context, one
+ * after another, and steps them together: one launch per stage covers every
+ * cloud, while the mass sums, rescales and MLS radii stay per cloud, so each
+ * cloud steps exactly as it would alone. Downloads give the concatenated
+ * arrays; ab_total_mass the sum over the clouds. ab_batch_scatter copies g
+ * and the moments back into the member contexts, which must outlive the
+ * batch. A batch refuses ab_upload, ab_rebuild_neighbours and
+ * ab_cloud_manage (work on the members, then form the batch again);
+ * ab_destroy frees it and leaves the members. */
+int ab_batch_create(ab_ctx *const *ctxs, int n, ab_ctx **out);
+int ab_batch_scatter(ab_ctx *batch);
+
 /* ---- Discrete-Maxwellian collision and its learned warm start ----------
  * No counterpart in the reference. With cfg.collision = AB_COLLISION_DISCRETE
  * every interior particle solves, in FP64, for a (5) such that
