@@ -139,6 +139,8 @@ struct ab_ctx
     int *t2_rep = nullptr;         // representative particle of each class
     int *t2_lat = nullptr;         // (ix, iy, nx, ny) of each particle, 4*N
     int t2_ncls = 0;
+    int *t2_items = nullptr;       // apply work items, 8 ints each: column runs, row runs, corners
+    int t2_nv = 0, t2_nh = 0, t2_nit = 0;   // column runs, row runs, all items
     double t2_dt = -1.0;           // dt the coefficients were built for
 
     // discrete-Maxwellian collision (cfg.collision) and its MLP warm start

@@ -17,6 +17,7 @@
 #include "ab_device.hpp"
 #include "ab_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -194,19 +195,117 @@ __global__ void k_build2(DevCloud c, const int *__restrict__ rep, const int *__r
     else build2<3, false>(c, i, lin, dt, h, lat, out, stride);
 }
 
-// gt = the class stencil applied to both layers; one thread per (particle,
-// node), flat over N * Nv^2. Offsets outside the lattice have coefficient 0
-// and read a clamped (valid) neighbour.
-__global__ __launch_bounds__(256) void k_apply2(int N, int nodes, int gstride,
-                                                const int *__restrict__ lat,
-                                                const int *__restrict__ cls,
-                                                const double *__restrict__ coef,
-                                                const double *__restrict__ g,
-                                                double *__restrict__ gt)
+// Offset index of (ox, oy), -1 outside the stencil, as a table the unrolled
+// loops below fold to constants.
+struct OffTab
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long)N * nodes) return;
-    const int i = (int)(e / nodes), lin = (int)(e - (long)i * nodes);
+    int idx[5][5];
+};
+__host__ __device__ constexpr OffTab make_offtab()
+{
+    OffTab t{};
+    for (int oy = -2; oy <= 2; ++oy)
+        for (int ox = -2; ox <= 2; ++ox) t.idx[oy + 2][ox + 2] = off2_index(ox, oy);
+    return t;
+}
+constexpr OffTab kOffTab = make_offtab();
+
+constexpr int kRun = 8;   // outputs per run item
+
+// A run of up to kRun outputs of one class along a lattice line (HOR: along
+// x, else along y), every output at least two sites from the line's ends,
+// so the inputs along the line stay inside it (an input past the last site is
+// clamped: it feeds only outputs that are not stored). Across the line the
+// five inputs are clamped into the lattice; their coefficients are 0 there.
+// Each input row is loaded once and added into the outputs whose stencil
+// holds it, with the class's 21 coefficients in registers. Column runs sum
+// each output in stencil order (oy outer, ox inner), as the per-particle
+// path does.
+// Item: p0, along stride, count, last along offset, across position, across
+// size, across stride.
+template <bool HOR>
+__device__ __forceinline__ void run2(const int *__restrict__ it, int lin, int nodes, int gstride,
+                                     const int *__restrict__ cls, const double *__restrict__ coef,
+                                     const double *__restrict__ g, double *__restrict__ gt)
+{
+    const int p0 = it[0], as = it[1], cnt = it[2], amax = it[3], pos = it[4], nac = it[5],
+              cs = it[6];
+    const double *cf = coef + (size_t)cls[p0] * kOff2 * nodes + lin;
+    double w[kOff2];
+#pragma unroll
+    for (int o = 0; o < kOff2; ++o) w[o] = cf[(size_t)o * nodes];
+    // g through a buffer resource with 32-bit byte offsets (the setup
+    // checks that g spans less than 2 GiB); the second layer via soffset
+    const __amdgpu_buffer_rsrc_t rg =
+        __builtin_amdgcn_make_buffer_rsrc((void *)g, (short)0, 0x7fffffff, 0x00020000);
+    const int l2 = nodes * 8;
+    int cof[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) cof[c] = (clampi(pos + c - 2, 0, nac - 1) - pos) * cs * gstride * 8;
+    double a1[kRun], a2[kRun];
+#pragma unroll
+    for (int r = 0; r < kRun; ++r) { a1[r] = 0.0; a2[r] = 0.0; }
+    // One input row in flight ahead of the row being added: a row's offset
+    // passes through an empty asm that takes every accumulator, so its loads
+    // issue after the additions of the rows before the current one (without
+    // it the compiler hoists all rows' loads and runs at one wave per SIMD).
+    double x1[5], x2[5], y1[5], y2[5];
+    auto load = [&](int v, double *z1, double *z2) {
+        const int d = v - 2 < amax ? v - 2 : amax;
+        int ro = ((p0 + d * as) * gstride + lin) * 8;
+        static_assert(kRun == 8, "the pin lists eight accumulators per layer");
+        asm volatile("" : "+v"(ro)
+                     : "v"(a1[0]), "v"(a1[1]), "v"(a1[2]), "v"(a1[3]), "v"(a1[4]), "v"(a1[5]),
+                       "v"(a1[6]), "v"(a1[7]), "v"(a2[0]), "v"(a2[1]), "v"(a2[2]), "v"(a2[3]),
+                       "v"(a2[4]), "v"(a2[5]), "v"(a2[6]), "v"(a2[7]));
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+        {
+            z1[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rg, ro + cof[c], 0, 0));
+            z2[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rg, ro + cof[c], l2, 0));
+        }
+    };
+    load(0, x1, x2);
+#pragma unroll
+    for (int v = 0; v < kRun + 4; ++v)
+    {
+        if (v + 1 < kRun + 4) load(v + 1, y1, y2);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int r = 0; r < kRun; ++r)
+        {
+            const int dv = v - 2 - r;
+            if (dv < -2 || dv > 2) continue;
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+            {
+                const int o = HOR ? kOffTab.idx[c][dv + 2] : kOffTab.idx[dv + 2][c];
+                if (o < 0) continue;
+                a1[r] += w[o] * x1[c];
+                a2[r] += w[o] * x2[c];
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) { x1[c] = y1[c]; x2[c] = y2[c]; }
+    }
+#pragma unroll
+    for (int r = 0; r < kRun; ++r)
+        if (r < cnt)
+        {
+            double *out = gt + ((long)p0 + (long)r * as) * gstride + lin;
+            out[0] = a1[r];
+            out[nodes] = a2[r];
+        }
+}
+
+// One particle (the lattice corners): its class stencil with clamped
+// offsets, in stencil order.
+__device__ __forceinline__ void point2(int i, int lin, int nodes, int gstride,
+                                       const int *__restrict__ lat, const int *__restrict__ cls,
+                                       const double *__restrict__ coef,
+                                       const double *__restrict__ g, double *__restrict__ gt)
+{
     const int4 l = reinterpret_cast<const int4 *>(lat)[i];   // ix, iy, nx, ny
     const int ix = l.x, iy = l.y, nx = l.z, ny = l.w;
     const size_t base = (size_t)i - ix - (size_t)nx * iy;
@@ -226,16 +325,37 @@ __global__ __launch_bounds__(256) void k_apply2(int N, int nodes, int gstride,
     gt[(size_t)i * gstride + lin + nodes] = a2;
 }
 
+// gt = the class stencils applied to both layers. One thread per (item,
+// node), S threads per item (the nodes rounded up to whole waves, so each
+// wave takes one kind of item): column runs, then row runs, then corners.
+__global__ __launch_bounds__(256) void k_apply2(int nv, int nh, int nit, int S, int nodes,
+                                                int gstride, const int *__restrict__ items,
+                                                const int *__restrict__ lat,
+                                                const int *__restrict__ cls,
+                                                const double *__restrict__ coef,
+                                                const double *__restrict__ g,
+                                                double *__restrict__ gt)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int item = (int)(e / S), lin = (int)(e - (long)item * S);
+    if (item >= nit || lin >= nodes) return;
+    const int *it = items + 8 * (size_t)item;
+    if (item < nv) run2<false>(it, lin, nodes, gstride, cls, coef, g, gt);
+    else if (item < nv + nh) run2<true>(it, lin, nodes, gstride, cls, coef, g, gt);
+    else point2(it[0], lin, nodes, gstride, lat, cls, coef, g, gt);
+}
+
 }  // namespace
 
 namespace ab {
 
 void table2d_free(ab_ctx *c)
 {
-    void *ptrs[] = {c->t2_coef, c->t2_cls, c->t2_rep, c->t2_lat};
+    void *ptrs[] = {c->t2_coef, c->t2_cls, c->t2_rep, c->t2_lat, c->t2_items};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     c->t2_coef = nullptr; c->t2_cls = nullptr; c->t2_rep = nullptr; c->t2_lat = nullptr;
-    c->t2_ncls = 0;
+    c->t2_items = nullptr;
+    c->t2_ncls = 0; c->t2_nv = c->t2_nh = c->t2_nit = 0;
     c->t2_dt = -1.0;
 }
 
@@ -258,6 +378,8 @@ int table2d_setup(ab_ctx *c)
         { c->err = "the 2-D table transport needs lattice clouds: nx, ny >= 5 and N = nx*ny"; return AB_ERR_UNSUPPORTED; }
     if (cfg.mls_clamp)
     { c->err = "mls_clamp is nonlinear in g; run it with AB_PREC_FP64"; return AB_ERR_UNSUPPORTED; }
+    if ((long long)N * c->gstride * 8 >= (1LL << 31))
+    { c->err = "the 2-D table transport addresses g with 32-bit offsets: N * 2 Nv^2 * 8 bytes must stay below 2 GiB"; return AB_ERR_UNSUPPORTED; }
     std::vector<unsigned char> b(N);
     std::vector<int> nb((size_t)N * AB_MAX_NEIGH), nn(N);
     std::vector<double> w(7 * (size_t)N), x(N), y(N);
@@ -325,15 +447,51 @@ int table2d_setup(ab_ctx *c)
             }
         }
     }
-    if (hipMalloc((void **)&c->t2_coef, sizeof(double) * ncls * kOff2 * (size_t)c->nodes) != hipSuccess ||
+    // apply items: column runs over rows 2 .. ny-3 of every column, row runs
+    // over columns 2 .. nx-3 of rows 0, 1, ny-2, ny-1, and the 16 corners
+    std::vector<int> itv, ith, itc;
+    for (int k = 0; k < ninst; ++k)
+    {
+        const int nx = nxs[k], ny = nys[k], b0 = base[k];
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iy = 2; iy <= ny - 3; iy += kRun)
+            {
+                const int it[8] = {b0 + ix + nx * iy, nx, std::min(kRun, ny - 2 - iy), ny - 1 - iy,
+                                   ix, nx, 1, 0};
+                itv.insert(itv.end(), it, it + 8);
+            }
+        for (int iy : {0, 1, ny - 2, ny - 1})
+        {
+            for (int ix = 2; ix <= nx - 3; ix += kRun)
+            {
+                const int it[8] = {b0 + ix + nx * iy, 1, std::min(kRun, nx - 2 - ix), nx - 1 - ix,
+                                   iy, ny, nx, 0};
+                ith.insert(ith.end(), it, it + 8);
+            }
+            for (int ix : {0, 1, nx - 2, nx - 1})
+            {
+                const int it[8] = {b0 + ix + nx * iy, 0, 0, 0, 0, 0, 0, 0};
+                itc.insert(itc.end(), it, it + 8);
+            }
+        }
+    }
+    std::vector<int> items(itv);
+    items.insert(items.end(), ith.begin(), ith.end());
+    items.insert(items.end(), itc.begin(), itc.end());
+    if (hipMalloc((void **)&c->t2_items, sizeof(int) * items.size()) != hipSuccess ||
+        hipMalloc((void **)&c->t2_coef, sizeof(double) * ncls * kOff2 * (size_t)c->nodes) != hipSuccess ||
         hipMalloc((void **)&c->t2_cls, sizeof(int) * (size_t)N) != hipSuccess ||
         hipMalloc((void **)&c->t2_rep, sizeof(int) * ncls) != hipSuccess ||
         hipMalloc((void **)&c->t2_lat, sizeof(int) * 4 * (size_t)N) != hipSuccess)
     { table2d_free(c); c->err = "2-D table setup: device allocation failed"; return AB_ERR_NOMEM; }
     c->t2_ncls = ncls;
+    c->t2_nv = (int)itv.size() / 8;
+    c->t2_nh = (int)ith.size() / 8;
+    c->t2_nit = (int)items.size() / 8;
     if (hipMemcpyAsync(c->t2_cls, cls.data(), sizeof(int) * N, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(c->t2_rep, rep.data(), sizeof(int) * ncls, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(c->t2_lat, lat.data(), sizeof(int) * 4 * (size_t)N, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(c->t2_items, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
     { table2d_free(c); c->err = "2-D table setup: host-to-device copy failed"; return AB_ERR_HIP; }
     return AB_OK;
@@ -355,9 +513,11 @@ int table2d_transport(ab_ctx *c, double dt)
 {
     int rc;
     if (c->t2_dt != dt && (rc = table2d_prepare(c, dt))) return rc;
-    const long n = (long)c->N * c->nodes;
+    const int S = (c->nodes + 63) / 64 * 64;
+    const long n = (long)c->t2_nit * S;
     k_apply2<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
-        c->N, c->nodes, c->gstride, c->t2_lat, c->t2_cls, c->t2_coef, c->g, c->gt);
+        c->t2_nv, c->t2_nh, c->t2_nit, S, c->nodes, c->gstride, c->t2_items, c->t2_lat, c->t2_cls,
+        c->t2_coef, c->g, c->gt);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 
