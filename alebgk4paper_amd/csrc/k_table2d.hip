@@ -336,7 +336,9 @@ __global__ __launch_bounds__(256) void k_apply2(int nv, int nh, int nit, int S, 
                                                 const double *__restrict__ g,
                                                 double *__restrict__ gt)
 {
-    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // each XCD takes a contiguous range of items, so the runs of
+    // neighbouring columns that share input rows share its L2
+    const long e = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int item = (int)(e / S), lin = (int)(e - (long)item * S);
     if (item >= nit || lin >= nodes) return;
     const int *it = items + 8 * (size_t)item;

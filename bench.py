@@ -288,7 +288,32 @@ def main():
                     help="for rocprofv3 --pmc: the measured step only (warmup, training, "
                          "autotune, --steps steps), then a 256 MiB copy kernel that calibrates "
                          "the byte counters; prints no JSON")
+    ap.add_argument("--batch2d", action="store_true",
+                    help="for rocprofv3: only the 16 cavity2d instances (Nx from {31, 40, 50}, "
+                         "Nv 16, quadratic) as one batch context with the 2-D table transport, "
+                         "5 warmup and --steps steps, then a 256 MiB copy kernel that "
+                         "calibrates the byte counters; prints the step time, no JSON")
     args = ap.parse_args()
+    if args.batch2d:
+        import random
+        import alebgk4paper_amd as A
+        rng = random.Random(20261202)
+        sols = []
+        for nx in [rng.choice([31, 40, 50]) for _ in range(16)]:
+            case = A.configure("cavity2d", nx=nx, nv=16, mls_order=2)
+            cl = A.build_lattice(case, precision=A.PREC_FP64, mls_order=2)
+            sols.append(A.Solver(cl, precision=A.PREC_FP64_TABLE))
+        batch = A.Batch(sols)
+        dt = A.configure("cavity2d", nx=31, nv=16, mls_order=2).dt
+        batch.steps(dt, 5)
+        batch.synchronize()
+        t0 = time.perf_counter()
+        batch.steps(dt, args.steps)
+        batch.synchronize()
+        log(f"[bench] 16 cavity2d instances as one batch (table transport): "
+            f"{(time.perf_counter() - t0) / args.steps * 1e3:.3f} ms/step, N={batch.N}")
+        A.copy_calibration(256 << 20, 2)     # scales a profiler's byte counters
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
