@@ -719,6 +719,9 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
 }
 
 // kScaleG (massOld/massColl) + kMoments3D / kMoments (CudaBackend.cu:1773-1788).
+// One wave per particle, four per workgroup: each lane loads four nodes of
+// both layers before it adds any, and the sums are wave reductions, so a
+// row's loads are in flight together and no workgroup barrier is needed.
 template <int DIM>
 __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict__ g,
                                                   const double *__restrict__ scal, double R,
@@ -727,9 +730,11 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
                                                   double *__restrict__ pmass,
                                                   const int *__restrict__ guard)
 {
-    __shared__ double sh[6 * (kBlock / kWave)];
-    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    constexpr int kPer = kBlock / kWave;
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * kPer + (int)(threadIdx.x / kWave);
+    if (i >= c.N) return;                          // whole wave
     if (guard && !*guard) return;
+    const int lane = threadIdx.x & (kWave - 1);
     if (c.inst) scal += (size_t)c.inst[i] * S_COUNT;
     const double s2 = scal[S_SCALE2];
     const bool doScale = (s2 != 1.0);
@@ -739,11 +744,7 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
     const double dvd = (DIM == 3) ? c.dv * c.dv * c.dv : c.dv * c.dv;
     const bool inner = !c.bnd[i];
     double r[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};   // rho, rux, ruy, ruz, E, mass
-    for (int k = threadIdx.x; k < nodes; k += kBlock)
-    {
-        double f = gp[k];
-        double f2 = 0.0;
-        if constexpr (DIM == 2) f2 = gp[k + nodes];
+    auto node = [&](int k, double f, double f2) {
         if (doScale)
         {
             f *= s2; gp[k] = f;
@@ -766,9 +767,26 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
             else
                 r[4] += (vx * vx + vy * vy) * f * dvd + 2.0 * f2 * dvd;
         }
+    };
+    constexpr int U = 4;
+    int k0 = 0;
+    for (; k0 + U * kWave <= nodes; k0 += U * kWave)
+    {
+        double f[U], f2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            f[u] = gp[k0 + u * kWave + lane];
+            f2[u] = (DIM == 2) ? gp[k0 + u * kWave + lane + nodes] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) node(k0 + u * kWave + lane, f[u], f2[u]);
     }
-    block_sum<6>(r, sh);
-    if (threadIdx.x == 0)
+    for (int k = k0 + lane; k < nodes; k += kWave)
+        node(k, gp[k], (DIM == 2) ? gp[k + nodes] : 0.0);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) r[q] = wave_sum(r[q]);
+    if (lane == 0)
     {
         pmass[i] = r[5];
         if (inner && r[0] > 1e-30)
@@ -988,10 +1006,11 @@ void launch_final(const DevCloud &c, double *g, const double *scal, double R, do
                   double *ux, double *uy, double *uz, double *T, double *pmass, hipStream_t s,
                   const int *guard)
 {
+    const int nb = (c.N + kBlock / kWave - 1) / (kBlock / kWave);
     if (c.dim == 3)
-        k_final<3><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
+        k_final<3><<<nb, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
     else
-        k_final<2><<<c.N, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
+        k_final<2><<<nb, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
 }
 
 void launch_mood_check(const DevCloud &c, double *gt, const double *rho, int *mood,
