@@ -140,6 +140,15 @@ __device__ __forceinline__ void block_sum(T (&v)[NV], T *sh)
     __syncthreads();
 }
 
+// Sums over one wave (every lane gets them): the reductions of kernels that
+// give each particle one wave.
+template <int NV, typename T>
+__device__ __forceinline__ void wave_sums(T (&v)[NV])
+{
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+}
+
 template <typename T>
 __device__ __forceinline__ void block_minmax(T &mn, T &mx, T *sh)
 {

@@ -387,8 +387,9 @@ __global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__r
                                                          double *__restrict__ limflag,
                                                          double *__restrict__ pmass)
 {
-    __shared__ double sh[4 * (kBlock / kWave)];
-    const int i = xcd_remap(blockIdx.x, gridDim.x);
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (int)(threadIdx.x / kWave);
+    if (i >= c.N) return;                              // whole wave: one particle
+    const int lane = threadIdx.x & (kWave - 1);
     const size_t gs = (size_t)c.gstride;
     double *gp = g + (size_t)i * gs;
     const double *tp = gt + (size_t)i * gs;
@@ -401,13 +402,13 @@ __global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__r
     {
         const double dv3 = c.dv * c.dv * c.dv;
         double r[3] = {0.0, 0.0, 0.0};             // mPos, mNeg, mFallback
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double v = tp[k];
             if (v >= 0) r[0] += v;
             else { r[1] += v; r[2] += gp[k]; }
         }
-        block_sum<3>(r, sh);
+        wave_sums<3>(r);
         fires = (r[1] < -1e-30 && (r[0] + r[1]) > 0);
         if (fires)
         {
@@ -418,23 +419,24 @@ __global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__r
         if (!c.bnd[i])
         {
             double rn[1] = {0.0};
-            for (int k = threadIdx.x; k < nodes; k += kBlock)
+            for (int k = lane; k < nodes; k += kWave)
             {
                 double v = tp[k];
                 if (fires) v = ((v < 0.0) ? gp[k] : v) * sc;
                 rn[0] += v;
             }
-            block_sum<1>(rn, sh);
+            wave_sums<1>(rn);
             const double rho_new = rn[0] * dv3;
             const int nn = c.nneigh[i];
             double rmin = 1e300, rmax = -1e300;
-            for (int k = threadIdx.x; k < nn; k += kBlock)
+            for (int k = lane; k < nn; k += kWave)
             {
                 double q = rho[c.neigh[(size_t)i * AB_MAX_NEIGH + k]];
                 rmin = fmin(rmin, q);
                 rmax = fmax(rmax, q);
             }
-            block_minmax(rmin, rmax, sh);
+            rmin = wave_min(rmin);
+            rmax = wave_max(rmax);
             if (!(rmin > rmax))
             {
                 double slack = 0.1 * fmax(rmax - rmin, 1e-12) + 1e-9;
@@ -445,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__r
             }
         }
         double m[1] = {0.0};
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double v = tp[k];
             if (fires) v = ((v < 0.0) ? gp[k] : v) * sc;
@@ -453,23 +455,23 @@ __global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__r
             gp[k] = v;
             m[0] += v * dv3;
         }
-        block_sum<1>(m, sh);
-        if (threadIdx.x == 0) { pmass[i] = m[0]; limflag[i] = flag; }
+        wave_sums<1>(m);
+        if (lane == 0) { pmass[i] = m[0]; limflag[i] = flag; }
     }
     else
     {
         double r[2] = {0.0, 0.0};                  // mPos, mNeg of g1
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double v = tp[k];
             if (v >= 0) r[0] += v; else r[1] += v;
         }
-        block_sum<2>(r, sh);
+        wave_sums<2>(r);
         fires = (r[1] < -1e-30 && r[0] > 1e-30 && (r[0] + r[1]) > 0);
         if (fires) { flag += 1.0; sc = (r[0] + r[1]) / r[0]; }
         const double dv2 = c.dv * c.dv;
         double m[1] = {0.0};
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double v1 = tp[k], v2 = tp[k + nodes];
             if (fires)
@@ -481,8 +483,8 @@ __global__ __launch_bounds__(kBlock) void k_limit_commit(DevCloud c, double *__r
             gp[k + nodes] = v2;
             m[0] += v1 * dv2;
         }
-        block_sum<1>(m, sh);
-        if (threadIdx.x == 0) { pmass[i] = m[0]; limflag[i] = flag; }
+        wave_sums<1>(m);
+        if (lane == 0) { pmass[i] = m[0]; limflag[i] = flag; }
     }
 }
 
@@ -499,9 +501,10 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                                                          double *__restrict__ pmass,
                                                          const int *__restrict__ guard)
 {
-    __shared__ double sh[15 * (kBlock / kWave)];
-    const int i = xcd_remap(blockIdx.x, gridDim.x);
-    if (guard && (!*guard || c.bnd[i])) return;        // whole block: one particle
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (int)(threadIdx.x / kWave);
+    if (i >= c.N) return;                              // whole wave: one particle
+    const int lane = threadIdx.x & (kWave - 1);
+    if (guard && (!*guard || c.bnd[i])) return;        // whole wave: one particle
     if (c.inst) scal += (size_t)c.inst[i] * S_COUNT;
     const double s1 = scal[S_SCALE1];
     const bool doScale = !guard && scal[S_MASS_NEW] > 1e-30;
@@ -519,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
         const double RT2w = 2.0 * R * Tw;
         const double norm = (DIM == 3) ? pow(AB_PI * RT2w, 1.5) : (AB_PI * RT2w);
         double r[2] = {0.0, 0.0};                  // intM, intP
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double vx = vnode(c.vlo, c.dv, k % Nv);
             double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
@@ -537,9 +540,9 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                 r[1] += cn * Mw * dvd;
             }
         }
-        block_sum<2>(r, sh);
+        wave_sums<2>(r);
         const double rho_w = (r[1] > 1e-30) ? (-r[0] / r[1]) : 1.0;
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double vx = vnode(c.vlo, c.dv, k % Nv);
             double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
@@ -561,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
             if constexpr (DIM == 2) gp[k + nodes] = v2;
             m[0] += v * dvd;
         }
-        if (threadIdx.x == 0)
+        if (lane == 0)
         {
             rho[i] = rho_w; ux[i] = Uwx; uy[i] = Uwy; T[i] = Tw;
             if (DIM == 3) uz[i] = Uwz;
@@ -570,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
     else
     {
         double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double vx = vnode(c.vlo, c.dv, k % Nv);
             double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
@@ -593,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                 r[4] += (vx * vx + vy * vy) * f * dvd + 2.0 * f2 * dvd;
             }
         }
-        block_sum<5>(r, sh);
+        wave_sums<5>(r);
         double mrho = rho[i], mux = ux[i], muy = uy[i], muz = (DIM == 3) ? uz[i] : 0.0, mT = T[i];
         if (r[0] > 1e-30)
         {
@@ -634,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                     double S[15];
 #pragma unroll
                     for (int q = 0; q < 15; ++q) S[q] = 0.0;
-                    for (int k = threadIdx.x; k < nodes; k += kBlock)
+                    for (int k = lane; k < nodes; k += kWave)
                     {
                         const double xx = vnode(c.vlo, c.dv, k % Nv) / cs;
                         const double yy = vnode(c.vlo, c.dv, (k / Nv) % Nv) / cs;
@@ -643,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                                              + a[4] * (xx * xx + yy * yy + zz * zz)) * dvd;
                         dm_accum(S, e, xx, yy, zz);
                     }
-                    block_sum<15>(S, sh);
+                    wave_sums<15>(S);
                     double F[5];
                     res = 0.0;
 #pragma unroll
@@ -661,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                 if (!conv)
 #pragma unroll
                     for (int q = 0; q < 5; ++q) a[q] = ac[q];       // continuous fallback
-                if (threadIdx.x == 0)
+                if (lane == 0)
                 {
 #pragma unroll
                     for (int q = 0; q < 5; ++q) dm.alpha[(size_t)i * 5 + q] = a[q];
@@ -679,7 +682,7 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
                 }
             }
         }
-        for (int k = threadIdx.x; k < nodes; k += kBlock)
+        for (int k = lane; k < nodes; k += kWave)
         {
             double vx = vnode(c.vlo, c.dv, k % Nv);
             double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
@@ -708,14 +711,14 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
             gp[k] = f;
             m[0] += f * dvd;
         }
-        if (threadIdx.x == 0)
+        if (lane == 0)
         {
             rho[i] = mrho; ux[i] = mux; uy[i] = muy; T[i] = mT;
             if (DIM == 3) uz[i] = muz;
         }
     }
-    block_sum<1>(m, sh);
-    if (threadIdx.x == 0) pmass[i] = m[0];
+    wave_sums<1>(m);
+    if (lane == 0) pmass[i] = m[0];
 }
 
 // kScaleG (massOld/massColl) + kMoments3D / kMoments (CudaBackend.cu:1773-1788).
@@ -986,20 +989,22 @@ void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int 
 void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
                          double *limflag, double *pmass, hipStream_t s)
 {
-    if (c.dim == 3) k_limit_commit<3><<<c.N, kBlock, 0, s>>>(c, g, gt, rho, limflag, pmass);
-    else k_limit_commit<2><<<c.N, kBlock, 0, s>>>(c, g, gt, rho, limflag, pmass);
+    const int nb = (c.N + kBlock / kWave - 1) / (kBlock / kWave);
+    if (c.dim == 3) k_limit_commit<3><<<nb, kBlock, 0, s>>>(c, g, gt, rho, limflag, pmass);
+    else k_limit_commit<2><<<nb, kBlock, 0, s>>>(c, g, gt, rho, limflag, pmass);
 }
 
 void launch_wall_collide(const DevCloud &c, const DmArgs &dm, double *g, const double *scal,
                          double dt, double tau, double R, double *rho, double *ux, double *uy,
                          double *uz, double *T, double *pmass, hipStream_t s, const int *guard)
 {
+    const int nb = (c.N + kBlock / kWave - 1) / (kBlock / kWave);
     if (c.dim == 3)
-        k_wall_collide<3><<<c.N, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
-                                                 pmass, guard);
+        k_wall_collide<3><<<nb, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
+                                                pmass, guard);
     else
-        k_wall_collide<2><<<c.N, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
-                                                 pmass, guard);
+        k_wall_collide<2><<<nb, kBlock, 0, s>>>(c, dm, g, scal, dt, tau, R, rho, ux, uy, uz, T,
+                                                pmass, guard);
 }
 
 void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
