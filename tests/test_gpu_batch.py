@@ -103,3 +103,32 @@ def test_batch_refusals():
     with pytest.raises(A.ALEBGKError):
         batch.upload(c2)
     batch.close()
+
+
+def test_batch_of_body_clouds_matches_alone():
+    """Clouds around bodies (the reference harness's circle and square
+    clouds, managed around their moved bodies on the device) batched with
+    the direct FP64 transport: each steps as it does alone."""
+    from conftest import load_fixture
+    from test_gpu_cloud import device_body, solver_for
+    _say("[batch] circle and square body clouds, FP64")
+    fixtures = [load_fixture(n) for n in ("cloud_circle_n24_v6", "cloud_square_n24_v6")]
+
+    def managed(fx):
+        m = fx.meta
+        sol = solver_for(fx)
+        dom = (m["xleft"], m["xright"], m["ybottom"], m["ytop"], m["zfront"], m["zback"])
+        sol.cloud_manage([device_body(fx["body"])], fx["bodyid_before"], m["dx"], m["minDist"],
+                         dom)
+        return sol
+
+    members = [managed(fx) for fx in fixtures]
+    alone = [managed(fx) for fx in fixtures]
+    batch = A.Batch(members)
+    batch.steps(1e-12, 4)
+    g, mom = batch.g(), batch.moments()
+    for (a, b), s in zip(batch.ranges, alone):
+        s.steps(1e-12, 4)
+        assert np.array_equal(g[a:b], s.g())
+        assert np.array_equal(mom[a:b], s.moments())
+    batch.close()
