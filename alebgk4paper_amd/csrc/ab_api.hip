@@ -140,13 +140,22 @@ __global__ void k_scatter_wall(int n, const int *__restrict__ idx, const double 
     if (has_mv) { mvx[i] = s[9]; mvy[i] = s[10]; mvz[i] = s[11]; }
 }
 
+// The per-step flags cleared in one launch (a fill kernel, not a memset, so
+// a graph capture holds no memset node).
+__global__ void k_zero_flags(int n, int *mlsflag, double *mlsovr, double *limflag, int *moodfail)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    mlsflag[i] = 0;
+    mlsovr[i] = 0.0;
+    limflag[i] = 0.0;
+    if (i == 0) moodfail[0] = 0;
+}
+
 int zero_step_flags(ab_ctx *c)
 {
     const int nb = (c->N + 255) / 256;
-    k_fill_int<<<nb, 256, 0, c->stream>>>(c->mlsflag, c->N, 0);
-    k_fill_dbl<<<nb, 256, 0, c->stream>>>(c->mlsovr, c->N, 0.0);
-    k_fill_dbl<<<nb, 256, 0, c->stream>>>(c->limflag, c->N, 0.0);
-    k_fill_int<<<1, 256, 0, c->stream>>>(c->moodfail, 1, 0);
+    k_zero_flags<<<nb, 256, 0, c->stream>>>(c->N, c->mlsflag, c->mlsovr, c->limflag, c->moodfail);
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 
