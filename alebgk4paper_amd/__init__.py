@@ -655,6 +655,9 @@ class Batch(Solver):
     def upload(self, cloud):
         raise ALEBGKError("a batch takes its clouds from its members")
 
+    def save_checkpoint(self, path, step, t):
+        raise ALEBGKError("restart files hold one cloud: scatter() the batch and save the members")
+
     def scatter(self):
         self._check(self._lib.ab_batch_scatter(self._ctx), "ab_batch_scatter")
 
