@@ -192,7 +192,9 @@ __global__ __launch_bounds__(kBlock) void k_transport3d(DevCloud c, const double
 // kCopyGt + kTransportNode, 2-D Chu layers (Solver/CudaBackend.cu:199-300):
 // weight through sqrt + mlsWeight, Gauss with partial pivoting on both layers.
 // Row swaps compare unrolled row indices against the pivot so A stays in
-// registers.
+// registers. A is summed in its upper triangle and mirrored: the reference
+// sums every entry, so the lower triangle differs from its by the rounding
+// of w phi_a phi_b against w phi_b phi_a.
 template <int P, bool BND>
 __global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double *__restrict__ g,
                                                         double *__restrict__ gt, double dt,
@@ -246,10 +248,11 @@ __global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double
             const double ssx = dxn / h, ssy = dyn / h;
             basis[0] = 1.0; basis[1] = ssx; basis[2] = ssy;
             if constexpr (P >= 6) { basis[3] = ssx * ssx; basis[4] = ssx * ssy; basis[5] = ssy * ssy; }
+            // A is symmetric: the upper triangle here, mirrored after the sum
 #pragma unroll
             for (int p1 = 0; p1 < P; ++p1)
 #pragma unroll
-                for (int p2 = 0; p2 < P; ++p2) A[p1 * P + p2] += w * basis[p1] * basis[p2];
+                for (int p2 = p1; p2 < P; ++p2) A[p1 * P + p2] += w * basis[p1] * basis[p2];
             const double *gj = g + (size_t)sj[m] * gs;
             const double gn1 = gj[lin], gn2 = gj[lin + Nv2];
 #pragma unroll
@@ -262,6 +265,10 @@ __global__ __launch_bounds__(kBlock) void k_transport2d(DevCloud c, const double
         }
         if (has)
         {
+#pragma unroll
+            for (int p1 = 1; p1 < P; ++p1)
+#pragma unroll
+                for (int p2 = 0; p2 < p1; ++p2) A[p1 * P + p2] = A[p2 * P + p1];
 #pragma unroll
             for (int col = 0; col < P; ++col)
             {
