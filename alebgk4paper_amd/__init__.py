@@ -140,7 +140,7 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy", "ab_autotune",
                "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels",
                "ab_cloud_manage", "ab_cloud_source_map", "ab_download_particles",
-               "ab_batch_create", "ab_batch_scatter"]
+               "ab_batch_create", "ab_batch_scatter", "ab_check_config"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
@@ -526,6 +526,27 @@ class Solver:
                     "ab_cloud_source_map")
         return src, tuple(counts)
 
+    def set_wall_state(self, idx, xyz=None, wn=None, wu=None, mv=None):
+        """Positions, wall normals, wall velocities and mesh velocities of
+        the particles idx (the uploads of cudaSyncBodies,
+        Solver/CudaBackend.cu:1884-1915; ab_set_wall_state). Each array is
+        len(idx) x 3, or None to leave that field."""
+        ii = np.ascontiguousarray(idx, dtype=np.int32)
+        n = len(ii)
+
+        def a3(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(n, 3)
+            return a
+        xyz, wn, wu, mv = a3(xyz), a3(wn), a3(wu), a3(mv)
+        cols = [None, None, None] if xyz is None else [np.ascontiguousarray(xyz[:, k])
+                                                       for k in range(3)]
+        self._check(self._lib.ab_set_wall_state(
+            self._ctx, C.c_int(n), _ptr(ii, _ip), _ptr(cols[0], _dp), _ptr(cols[1], _dp),
+            _ptr(cols[2], _dp), _ptr(wn, _dp), _ptr(wu, _dp), _ptr(mv, _dp)),
+            "ab_set_wall_state")
+
     def body_loads(self, surf_idx, surf_body, centres, areas, wu=None):
         si = np.ascontiguousarray(surf_idx, dtype=np.int32)
         sb = np.ascontiguousarray(surf_body, dtype=np.int32)
@@ -678,8 +699,10 @@ class Graph:
             raise ALEBGKError(f"ab_graph_create: {ERRORS.get(rc, rc)}: {msgs}")
 
     def launch(self):
-        if self._lib.ab_graph_launch(self._g):
-            raise ALEBGKError("ab_graph_launch failed")
+        rc = self._lib.ab_graph_launch(self._g)
+        if rc:
+            msgs = "; ".join(s._lib.ab_last_error(s._ctx).decode() for s in self.solvers)
+            raise ALEBGKError(f"ab_graph_launch: {ERRORS.get(rc, rc)}: {msgs}")
 
     def synchronize(self):
         if self._lib.ab_graph_synchronize(self._g):
@@ -695,6 +718,14 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+def check_config(cfg):
+    """ab_check_config: the checks ab_create makes, on the host only.
+    Returns (status, message); status 0 means ab_create would accept it."""
+    buf = C.create_string_buffer(256)
+    rc = hip_lib().ab_check_config(C.byref(cfg), buf, C.c_int(len(buf)))
+    return int(rc), buf.value.decode()
 
 
 def velocities(cfg):

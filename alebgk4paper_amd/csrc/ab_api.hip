@@ -49,6 +49,7 @@ int dalloc(ab_ctx *ctx, T **p, size_t n)
 
 void free_cloud(ab_ctx *c)
 {
+    c->gen++;
     void *ptrs[] = {c->x, c->y, c->z, c->bnd, c->wnx, c->wny, c->wnz, c->wux, c->wuy, c->wuz,
                     c->wt, c->mvx, c->mvy, c->mvz, c->rho, c->ux, c->uy, c->uz, c->T,
                     c->neigh, c->nneigh, c->ilist, c->blist, c->g, c->gt, c->limflag,
@@ -292,6 +293,18 @@ int graph_prepare(ab_ctx *c, double dt)
     if (!c->uploaded) { c->err = "graph capture before ab_upload"; return AB_ERR_STATE; }
     if (c->prof) { c->err = "graph capture with profiling on"; return AB_ERR_STATE; }
     if (c->dm_rec_left > 0) { c->err = "graph capture while recording training rows"; return AB_ERR_STATE; }
+    // The FP64 steps sum the start-of-step masses from pmass, which every
+    // step leaves current; after an upload it is computed first. Do that
+    // here, eagerly, so the capture (which executes nothing) neither marks
+    // pmass current without computing it nor holds a launch that a replay
+    // after an eager step would not need.
+    if (!table_mode(c) && !c->mass_valid)
+    {
+        launch_mass_partials(c->view(), c->g, c->pmass, c->stream);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->mass_valid = true;
+    }
     if (table_mode(c) && c->coef_dt != dt)
     {
         int rc = table_prepare(c, dt);
@@ -380,6 +393,7 @@ int relayout_cloud(ab_ctx *c, const int *src, int n)
     c->dm_rec = nullptr;
     c->dm_rec_slots = c->dm_rec_next = c->dm_rec_filled = c->dm_rec_left = 0;
     c->mass_valid = false;
+    c->gen++;
     if (rc) { free_cloud(c); c->uploaded = false; c->N = oldN; }
     return rc;
 }
@@ -497,21 +511,54 @@ void prof_end(ab_ctx *c, int stage)
 
 extern "C" {
 
-int ab_create(const ab_config *cfg, ab_ctx **out)
+int ab_check_config(const ab_config *cfg, char *msg, int cap)
 {
-    if (!cfg || !out) return AB_ERR_ARG;
-    *out = nullptr;
+    auto fail = [&](int code, const char *m) {
+        if (msg && cap > 0) std::snprintf(msg, (size_t)cap, "%s", m);
+        return code;
+    };
+    if (msg && cap > 0) msg[0] = 0;
+    if (!cfg) return fail(AB_ERR_ARG, "null configuration");
     if ((cfg->dim != 2 && cfg->dim != 3) || cfg->n_particles <= 0 || cfg->nv < 2 ||
         cfg->nv > ab::kMaxNv || !(cfg->vmax > cfg->vmin) || !(cfg->R > 0) || !(cfg->tau > 0) ||
         !(cfg->radius > 0) || (cfg->mls_order != 1 && cfg->mls_order != 2) ||
         cfg->precision < AB_PREC_FP64 || cfg->precision > AB_PREC_FP64_TABLE ||
         (cfg->collision != AB_COLLISION_CONTINUOUS && cfg->collision != AB_COLLISION_DISCRETE))
-        return AB_ERR_ARG;
-    if (cfg->collision == AB_COLLISION_DISCRETE && cfg->dim != 3) return AB_ERR_UNSUPPORTED;
-    if (cfg->limiter != AB_LIMITER_GPU && cfg->limiter != AB_LIMITER_MOOD) return AB_ERR_ARG;
+        return fail(AB_ERR_ARG, "invalid configuration (dim, N, nv <= 48, velocity grid, gas, "
+                                "radius, MLS order, precision or collision)");
+    if (cfg->collision == AB_COLLISION_DISCRETE && cfg->dim != 3)
+        return fail(AB_ERR_UNSUPPORTED, "the discrete-Maxwellian collision is 3-D only");
+    if (cfg->limiter != AB_LIMITER_GPU && cfg->limiter != AB_LIMITER_MOOD)
+        return fail(AB_ERR_ARG, "unknown limiter");
     if (cfg->limiter == AB_LIMITER_MOOD &&
         (cfg->precision != AB_PREC_FP64 || cfg->collision != AB_COLLISION_CONTINUOUS))
-        return AB_ERR_UNSUPPORTED;
+        return fail(AB_ERR_UNSUPPORTED, "the MOOD limiter runs with AB_PREC_FP64 and the "
+                                        "continuous collision");
+    if (cfg->dim == 3 && cfg->precision != AB_PREC_FP64 && cfg->nx > 0 && cfg->ny > 0)
+    {
+        // one z block of the table transport spans kMaxZBlockPlanes lattice
+        // planes of the stored g, addressed with 32-bit byte offsets
+        const long long nodes = (long long)cfg->nv * cfg->nv * cfg->nv;
+        const long long S = (nodes + ab::kWave - 1) / ab::kWave * ab::kWave;
+        const long long es = cfg->precision == AB_PREC_MIXED ? 4 : 8;
+        const double span = (double)ab::kMaxZBlockPlanes * cfg->nx * cfg->ny * (double)S * es;
+        if (span >= 2147483648.0)
+            return fail(AB_ERR_UNSUPPORTED,
+                        cfg->precision == AB_PREC_MIXED
+                            ? "table transport: one z block of this lattice spans more than "
+                              "2 GiB of g (nx * ny * Nv^3 too large)"
+                            : "table transport: one z block of this lattice spans more than "
+                              "2 GiB of FP64 g; run it with AB_PREC_MIXED or AB_PREC_FP64");
+    }
+    return AB_OK;
+}
+
+int ab_create(const ab_config *cfg, ab_ctx **out)
+{
+    if (!cfg || !out) return AB_ERR_ARG;
+    *out = nullptr;
+    int crc = ab_check_config(cfg, nullptr, 0);
+    if (crc) return crc;
     ab_ctx *c = new (std::nothrow) ab_ctx;
     if (!c) return AB_ERR_NOMEM;
     c->cfg = *cfg;
@@ -602,6 +649,7 @@ int ab_upload(ab_ctx *c, const ab_cloud_view *v)
     }
     c->uploaded = true;
     c->mass_valid = false;
+    c->gen++;
     if (table_mode(c))
     {
         if ((rc = table_setup(c))) { c->uploaded = false; return rc; }
@@ -1144,6 +1192,7 @@ int ab_batch_create(ab_ctx *const *m, int n, ab_ctx **out)
         b->bt_ny.push_back(a->cfg.ny);
         b->bt_h.push_back(a->cfg.radius);
         b->members.push_back(a);
+        b->member_gen.push_back(a->gen);
     }
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(b->ih, b->bt_h.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1164,6 +1213,14 @@ int ab_batch_scatter(ab_ctx *b)
 {
     if (!b) return AB_ERR_ARG;
     if (!b->nbatch) { b->err = "ab_batch_scatter: not a batch"; return AB_ERR_STATE; }
+    for (int k = 0; k < b->nbatch; ++k)
+        if (b->members[k]->gen != b->member_gen[k] ||
+            b->members[k]->N != b->bt_base[k + 1] - b->bt_base[k])
+        {
+            b->err = "ab_batch_scatter: member " + std::to_string(k) +
+                     " was re-uploaded or re-laid out after the batch was formed";
+            return AB_ERR_STATE;
+        }
     HIPCHK(b, hipStreamSynchronize(b->stream));
     for (int k = 0; k < b->nbatch; ++k)
     {
@@ -1176,7 +1233,10 @@ int ab_batch_scatter(ab_ctx *b)
             HIPCHK(b, hipMemcpyAsync(dst[q], src[q] + o, 8 * nk, hipMemcpyDeviceToDevice, b->stream));
         HIPCHK(b, hipMemcpyAsync(a->g, b->g + o * b->gstride, 8 * nk * b->gstride,
                                  hipMemcpyDeviceToDevice, b->stream));
-        a->mass_valid = false;
+        // the member's per-particle masses follow its new g, so its next
+        // step (eager or a graph replay) starts from the right sum
+        launch_mass_partials(a->view(), a->g, a->pmass, b->stream);
+        a->mass_valid = true;
     }
     HIPCHK(b, hipStreamSynchronize(b->stream));
     return AB_OK;

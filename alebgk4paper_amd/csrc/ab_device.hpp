@@ -12,6 +12,10 @@ namespace ab {
 constexpr int kWave = 64;          // CDNA wavefront
 constexpr int kBlock = 256;        // 4 waves per workgroup for the per-particle kernels
 constexpr int kMaxNv = 48;         // velocity nodes per axis (ab_create refuses more)
+// Lattice planes one z block of the 3-D table transport reads (its largest
+// block, 12 outputs, plus the 4-plane stencil). Its buffer descriptors take
+// 32-bit byte offsets, so such a span of the stored g must stay below 2 GiB.
+constexpr int kMaxZBlockPlanes = 12 + 4;
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8). Give every XCD a contiguous range of logical block ids so the

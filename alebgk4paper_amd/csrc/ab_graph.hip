@@ -19,6 +19,7 @@
 struct ab_graph
 {
     std::vector<ab_ctx *> ctxs;
+    std::vector<unsigned long long> gens;   // each context's generation at capture
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -59,6 +60,7 @@ int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph 
     ab_graph *g = new (std::nothrow) ab_graph;
     if (!g) return AB_ERR_NOMEM;
     g->ctxs.assign(ctxs, ctxs + n);
+    for (int k = 0; k < n; ++k) g->gens.push_back(ctxs[k]->gen);
     g->events.assign(n + 1, nullptr);
     bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
     for (auto &e : g->events)
@@ -98,6 +100,16 @@ int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph 
 int ab_graph_launch(ab_graph *g)
 {
     if (!g || !g->exec) return AB_ERR_ARG;
+    // the graph holds the device addresses and sizes of the clouds it was
+    // captured on: refuse a replay once any of them was re-uploaded or
+    // re-laid out
+    for (size_t k = 0; k < g->ctxs.size(); ++k)
+        if (g->ctxs[k]->gen != g->gens[k])
+        {
+            g->ctxs[k]->err = "ab_graph_launch: the cloud changed after the capture (upload or "
+                              "cloud management); create the graph again";
+            return AB_ERR_STATE;
+        }
     return hipGraphLaunch(g->exec, g->stream) == hipSuccess ? AB_OK : AB_ERR_HIP;
 }
 

@@ -106,6 +106,10 @@ struct ab_ctx
     double *pmass = nullptr;       // per-particle partial sums
     double *scal = nullptr;        // ab::ScalarSlot
     bool mass_valid = false;       // pmass holds the current g's masses
+    // Bumped whenever the device arrays are freed, re-allocated or refilled
+    // (upload, re-layout, destroy): graphs and batches record it when they
+    // are made and refuse to run on a cloud that has changed since.
+    unsigned long long gen = 0;
 
     // neighbour rebuild scratch
     int *vox = nullptr, *vcount = nullptr, *vstart = nullptr, *vlist = nullptr;
@@ -173,6 +177,7 @@ struct ab_ctx
     std::vector<int> bt_base, bt_nx, bt_ny;
     std::vector<double> bt_h;
     std::vector<ab_ctx *> members;
+    std::vector<unsigned long long> member_gen;   // members' generations when batched
 
     // cloud management (ab_cloud_manage): the last call's source map
     std::vector<int> cm_src;

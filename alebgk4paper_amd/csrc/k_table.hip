@@ -1683,7 +1683,7 @@ constexpr ApplyShape kShapes32[] = {{9, 3, 1, 1, 0}, {8, 3, 1, 1, 0}, {7, 3, 1, 
                                     {12, 2, 2, 2, 1}, {8, 2, 2, 2, 1}, {6, 2, 2, 2, 1}};
 constexpr ApplyShape kShapes64[] = {{9, 2, 1, 1, 0}, {8, 2, 1, 1, 0}, {7, 2, 1, 1, 0},
                                     {6, 2, 1, 1, 0}};
-constexpr int kMaxBlockPlanes = 12 + 4;
+constexpr int kMaxBlockPlanes = kMaxZBlockPlanes;
 
 template <typename T>
 constexpr int nshapes() { return sizeof(T) == 4 ? 15 : 4; }

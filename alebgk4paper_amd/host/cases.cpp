@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 namespace {
 
@@ -228,31 +229,34 @@ int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double 
                 // applyIC (InitialConditions.cpp:18-81): rest, T = 270 K
                 double *m = mom + 5 * (size_t)n;
                 m[0] = cs->rho0; m[1] = m[2] = m[3] = 0.0; m[4] = 270.0;
-                double *gp = g + (size_t)n * gs;
-                if (ndim == 3)
-                {
-                    for (int kv = 0; kv < Nv; ++kv)
-                        for (int jv = 0; jv < Nv; ++jv)
-                            for (int iv = 0; iv < Nv; ++iv)
-                                gp[iv + Nv * (jv + Nv * kv)] =
-                                    maxwellian3d(vlo + iv * dv, vlo + jv * dv, vlo + kv * dv, 0.0,
-                                                 0.0, 0.0, 270.0, cs->rho0, cs->R);
-                }
-                else
-                {
-                    for (int jv = 0; jv < Nv; ++jv)
-                        for (int iv = 0; iv < Nv; ++iv)
-                        {
-                            const int lin = iv + Nv * jv;
-                            const double Mg1 = maxwellian_g1(vlo + iv * dv, vlo + jv * dv, 0.0,
-                                                             0.0, 270.0, cs->rho0, cs->R);
-                            gp[lin] = Mg1;
-                            gp[lin + Nv * Nv] = 0.5 * cs->R * 270.0 * Mg1;
-                        }
-                }
             }
         }
     }
+    // Every particle starts from the same Maxwellian: one row, evaluated node
+    // by node as applyIC does, copied to all particles.
+    std::vector<double> row(gs);
+    if (ndim == 3)
+    {
+        for (int kv = 0; kv < Nv; ++kv)
+            for (int jv = 0; jv < Nv; ++jv)
+                for (int iv = 0; iv < Nv; ++iv)
+                    row[iv + Nv * (jv + Nv * kv)] =
+                        maxwellian3d(vlo + iv * dv, vlo + jv * dv, vlo + kv * dv, 0.0, 0.0, 0.0,
+                                     270.0, cs->rho0, cs->R);
+    }
+    else
+    {
+        for (int jv = 0; jv < Nv; ++jv)
+            for (int iv = 0; iv < Nv; ++iv)
+            {
+                const int lin = iv + Nv * jv;
+                const double Mg1 =
+                    maxwellian_g1(vlo + iv * dv, vlo + jv * dv, 0.0, 0.0, 270.0, cs->rho0, cs->R);
+                row[lin] = Mg1;
+                row[lin + Nv * Nv] = 0.5 * cs->R * 270.0 * Mg1;
+            }
+    }
+    for (int p = 0; p < n; ++p) std::memcpy(g + (size_t)p * gs, row.data(), gs * sizeof(double));
     return AB_OK;
 }
 

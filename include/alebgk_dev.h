@@ -144,6 +144,14 @@ typedef struct ab_step_stats {
 /* Lifecycle. Replaces the file-static DeviceCloud (CudaBackend.cu:115) with
  * an explicit handle. */
 int ab_create(const ab_config *cfg, ab_ctx **out);
+/* The checks ab_create makes, on the host only (no device is touched):
+ * AB_OK, or the code ab_create would return and a message in msg (cap
+ * bytes, may be NULL). The reference refuses such configurations at upload
+ * with a message (Solver/Solver.cpp:260-267). Besides argument ranges it
+ * refuses a 3-D table pipeline whose z block of stored g would exceed the
+ * 2 GiB its 32-bit buffer offsets reach (for example cavity3d 60^3, Nv=20
+ * with AB_PREC_FP64_TABLE; AB_PREC_MIXED fits). */
+int ab_check_config(const ab_config *cfg, char *msg, int cap);
 int ab_destroy(ab_ctx *ctx);
 const char *ab_last_error(const ab_ctx *ctx);
 
@@ -276,7 +284,10 @@ int ab_copy_calibration(unsigned long long bytes, int reps);
  * gives exactly the eager steps' results. Refused (AB_ERR_STATE) while a
  * context profiles or records training rows; table coefficients are built
  * for dt before the capture. Do not step a captured context eagerly while
- * a replay is in flight. */
+ * a replay is in flight. A graph holds the clouds' device addresses: once
+ * any of its contexts is re-uploaded or re-laid out (ab_upload,
+ * ab_cloud_manage, ab_body_step) ab_graph_launch refuses with AB_ERR_STATE;
+ * create the graph again. */
 typedef struct ab_graph ab_graph;
 int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph **out);
 int ab_graph_launch(ab_graph *graph);
@@ -293,7 +304,9 @@ int ab_graph_destroy(ab_graph *graph);
  * cloud steps exactly as it would alone. Downloads give the concatenated
  * arrays; ab_total_mass the sum over the clouds. ab_batch_scatter copies g
  * and the moments back into the member contexts, which must outlive the
- * batch. A batch refuses ab_upload, ab_rebuild_neighbours and
+ * batch, and keep their clouds: ab_batch_scatter refuses (AB_ERR_STATE) a
+ * member re-uploaded or re-laid out since the batch was formed. A batch
+ * refuses ab_upload, ab_rebuild_neighbours and
  * ab_cloud_manage (work on the members, then form the batch again);
  * ab_destroy frees it and leaves the members. */
 int ab_batch_create(ab_ctx *const *ctxs, int n, ab_ctx **out);

@@ -14,6 +14,7 @@
 //   ref_harness mood    <outdir> <case> <Nx> <Nv> <order> <nsteps> <a> <b>
 //                                                  CPU steps with the MOOD limiter
 //   ref_harness ckptread <file> <outdir> <case> <Nx> <Nv>  restore a file, dump fields
+//   ref_harness pos2d   <outdir> <Nx> <Nv> <a> <T> <cfl>   2-D transport where positivity fires
 //   ref_harness cloud   <outdir> <case> <Nx> <Nv> <scale> <shift> <jitter>
 //                                                  cloud management after a body move
 //
@@ -703,6 +704,52 @@ int modeCloud(const std::string &dir, const std::string &cname, int Nx, int Nv, 
     return 0;
 }
 
+// The 2-D transport with its positivity repair on a field that trips it:
+// opposing streams, u_x = +a left of the middle and -a right of it, at
+// temperature T, so the transported g1 of the particles where the streams
+// meet overshoots below zero. Writes the cloud, g_in, and per MLS order the
+// transported g (gt_o1, gt_o2: the repair already applied, as
+// mlsTransportCompute does) and the particles it repaired (posfired_o*).
+int modePos2d(const std::string &dir, int Nx, int Nv, double a, double T, double cfl)
+{
+    Setup S;
+    build(S, TestCase::Cavity2D, Nx, Nv, 1.0);
+    const double R = S.gc.R, xmid = 0.5 * (S.dom.xleft + S.dom.xright);
+    for (auto &p : S.P)
+    {
+        const double ux = (p.x < xmid) ? a : -a;
+        const int n = p.Nv_local, n2 = n * n;
+        for (int jv = 0; jv < n; ++jv)
+            for (int iv = 0; iv < n; ++iv)
+            {
+                const double g1 = maxwellianG1(p.vlo + iv * p.dv_local, p.vlo + jv * p.dv_local,
+                                               ux, 0.0, T, S.BC.rho, R);
+                p.g[iv + n * jv] = g1;
+                p.g[iv + n * jv + n2] = 0.5 * R * T * g1;
+            }
+    }
+    const double dtT = cfl * std::fabs(S.cp.dx) / S.sp.VMax;
+    writeMeta(dir, S, dtT);
+    writeCloud(dir, S);
+    const size_t N = S.P.size(), gs = S.P[0].g.size();
+    std::vector<double> gin = gatherG(S.P, false);
+    npyF8(dir + "/g_in.npy", gin, {N, gs});
+    int fired[3] = {0, 0, 0};
+    for (int order = 1; order <= 2; ++order)
+    {
+        scatterG(S.P, gin);
+        mlsTransportCompute(S.P, dtT, S.cp, S.dom, S.BC, order);
+        std::string sfx = "_o" + std::to_string(order);
+        npyF8(dir + "/gt" + sfx + ".npy", gatherG(S.P, true), {N, gs});
+        std::vector<uint8_t> m = positivityMask(S.P);
+        for (uint8_t v : m) fired[order] += v;
+        npyU1(dir + "/posfired" + sfx + ".npy", m, {N});
+    }
+    std::printf("wrote %s: N=%zu, positivity fired on %d (linear) / %d (quadratic) particles\n",
+                dir.c_str(), N, fired[1], fired[2]);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 1; }
@@ -725,6 +772,9 @@ int main(int argc, char **argv)
     if (mode == "cloud" && argc == 9)
         return modeCloud(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
                          std::atof(argv[6]), std::atof(argv[7]), std::atof(argv[8]));
+    if (mode == "pos2d" && argc == 8)
+        return modePos2d(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atof(argv[5]),
+                         std::atof(argv[6]), std::atof(argv[7]));
     if (mode == "ckptread" && argc == 7)
         return modeCkptRead(argv[2], argv[3], argv[4], std::atoi(argv[5]), std::atoi(argv[6]));
     std::fprintf(stderr, "bad arguments\n");

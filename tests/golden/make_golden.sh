@@ -20,6 +20,9 @@ H="$ROOT/oracle/_ref/ref_harness"
 # three CPU steps with the MOOD limiter from opposing streams (u_x = +-300 m/s)
 "$H" mood "$HERE/mood_cavity3d_n7_v6" cavity3d 7 6 2 3 300 0
 "$H" mood "$HERE/mood_cavity2d_n12_v8" cavity2d 12 8 2 3 300 0
+# 2-D transport of opposing streams (u_x = +-300 m/s): the quadratic
+# reconstruction goes negative where they meet and the positivity repair fires
+"$H" pos2d "$HERE/pos_cavity2d_n12_v8" 12 8 300 270 0.5
 # cloud management after a body move (remove, merge, refill; 3-D bodies
 # are removed and merged, the 2-D ones also refilled)
 "$H" cloud "$HERE/cloud_sphere_n12_v3" sphere 12 3 2.0 1.3 0.3

@@ -83,3 +83,65 @@ def test_autotune_keeps_results_and_picks_a_shape():
         else:
             assert np.array_equal(g, ref), name
         s.close()
+
+
+def test_eager_step_after_capture_matches_uncaptured():
+    """Capturing records work without executing it. A context captured
+    straight after its upload and then stepped eagerly (before any replay)
+    steps exactly like one that was never captured: the start-of-step mass
+    sums are computed before the capture, not marked current by it."""
+    _say("[graph] eager step after a capture, FP64 3-D and MOOD")
+    case, cl, prec = _instances()[1]
+    dt = 1e-11
+    for limiter in (A.LIMITER_GPU, A.LIMITER_MOOD):
+        plain = A.Solver(cl, precision=prec, limiter=limiter)
+        cap = A.Solver(cl, precision=prec, limiter=limiter)
+        graph = A.Graph([cap], dt, 2)
+        plain.steps(dt, 2)
+        cap.steps(dt, 2)
+        assert np.array_equal(plain.g(), cap.g()), limiter
+        # a replay continues from the eager state exactly as eager steps do
+        graph.launch(); graph.synchronize()
+        plain.steps(dt, 2)
+        assert np.array_equal(plain.g(), cap.g()), limiter
+        graph.close()
+
+
+def test_graph_refuses_replay_after_cloud_changes():
+    """A graph holds its clouds' device addresses: a re-upload or a cloud
+    management pass (which re-lays the cloud out) invalidates it, and the
+    replay is refused instead of running on freed memory."""
+    _say("[graph] refused after upload / cloud management")
+    case, cl, prec = _instances()[1]
+    s = A.Solver(cl, precision=prec)
+    g = A.Graph([s], 1e-11, 1)
+    g.launch(); g.synchronize()
+    s.upload(cl)
+    with pytest.raises(A.ALEBGKError):
+        g.launch()
+    g.close()
+    g = A.Graph([s], 1e-11, 1)
+    g.launch(); g.synchronize()
+    dx = 1e-6 / (case.nx - 1)
+    dom = (case.dom[0], case.dom[1], case.dom[3], case.dom[2], case.dom[4], case.dom[5])
+    s.cloud_manage([], np.full(s.N, -1, np.int32), dx, 0.0, dom)
+    with pytest.raises(A.ALEBGKError):
+        g.launch()
+    g.close()
+    # a graph made afterwards runs
+    g = A.Graph([s], 1e-11, 1)
+    g.launch(); g.synchronize(); g.close()
+
+
+def test_batch_scatter_refuses_changed_member():
+    _say("[batch] scatter refused after a member re-upload")
+    case, cl, prec = _instances()[2]
+    a = A.Solver(cl, precision=prec)
+    b = A.Solver(cl, precision=prec)
+    bt = A.Batch([a, b])
+    bt.step(1e-13)
+    bt.scatter()
+    a.upload(cl)
+    with pytest.raises(A.ALEBGKError):
+        bt.scatter()
+    bt.close()

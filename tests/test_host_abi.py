@@ -90,3 +90,40 @@ def test_solver_fails_loudly_without_gpu():
     cl = A.build_lattice(A.configure("cavity3d", nx=6, nv=4))
     with pytest.raises(A.ALEBGKError):
         A.Solver(cl)
+
+
+def test_check_config_refuses_table_span_beyond_2gib():
+    """The 3-D table transport addresses one z block (16 lattice planes) of
+    stored g with 32-bit byte offsets. cavity3d 60^3, Nv=20: 16 planes of
+    60*60*8000 values are 1.84 GB in FP32 (accepted) and 3.69 GB in FP64
+    (refused before any device work); the FP64 reference formulation has no
+    such block and is accepted."""
+    cl = A.build_lattice(A.configure("cavity3d", nx=6, nv=4), mls_order=2)
+    cfg = cl.cfg
+    cfg.nx = cfg.ny = cfg.nz = 60
+    cfg.n_particles = 60 ** 3
+    cfg.nv = 20
+    for prec, want in ((A.PREC_MIXED, 0), (A.PREC_FP64_TABLE, 4), (A.PREC_FP64, 0)):
+        cfg.precision = prec
+        rc, msg = A.check_config(cfg)
+        assert rc == want, (prec, rc, msg)
+        assert (rc == 0) == (msg == ""), msg
+    cfg.precision = A.PREC_FP64_TABLE
+    assert "2 GiB" in A.check_config(cfg)[1]
+    # the largest FP32 lattice at Nv=20 that fits: 16 * n^2 * 8000 * 4 < 2^31
+    cfg.precision = A.PREC_MIXED
+    cfg.nx = cfg.ny = cfg.nz = 64
+    cfg.n_particles = 64 ** 3
+    assert A.check_config(cfg)[0] == 0
+    cfg.nx = cfg.ny = cfg.nz = 65
+    cfg.n_particles = 65 ** 3
+    assert A.check_config(cfg)[0] == 4
+    # argument ranges
+    cfg.nx = cfg.ny = cfg.nz = 6
+    cfg.n_particles = 216
+    cfg.nv = 49
+    assert A.check_config(cfg)[0] == 1
+    cfg.nv = 8
+    cfg.dim = 2
+    cfg.collision = A.COLLISION_DISCRETE
+    assert A.check_config(cfg)[0] == 4

@@ -192,6 +192,41 @@ def test_transport_matches_reference(name, order):
     assert err.max() < 1e-11, err.max()
 
 
+@pytest.mark.parametrize("order", [1, 2])
+def test_positivity2d_matches_reference(order):
+    """The 2-D positivity repair. The reference CPU 2-D transport ends each
+    particle with the same mass-conserving repair of the g1 layer as the GPU
+    kernel (MLSTransport.cpp:283-307 and kPositivity, CudaBackend.cu:304-326:
+    negatives and their g2 to zero, the rest scaled by (m+ + m-)/m+), so the
+    fixture rows where it fired pin orc_positivity2d: transport, repair, and
+    every row -- repaired or not -- equals the reference's, and the oracle
+    flags exactly the particles the reference repaired."""
+    from conftest import load_fixture
+    fx = load_fixture("pos_cavity2d_n12_v8")
+    cl = cloud_of(fx)
+    m = fx.meta
+    gt = O.transport(cl, fx["g_in"], m["dt_transport"], m["radius"], order)
+    fired = fx["posfired_o%d" % order] != 0
+    # the linear reconstruction of these streams stays positive; the
+    # quadratic one goes negative on 20 particles
+    assert fired.sum() == (0 if order == 1 else 20)
+    lim_gt, lim = O.positivity(cl, fx["g_in"], gt)
+    want = fx["gt_o%d" % order]
+    scale = np.abs(fx["g_in"]).max(axis=1, keepdims=True)
+    err = np.abs(lim_gt - want) / scale
+    assert err.max() < 1e-11, err.max()
+    assert np.array_equal(lim != 0, fired)
+    # the repaired rows really differ from the raw transport, and keep the
+    # g1 mass of the raw row
+    nv2 = cl.Nv * cl.Nv
+    if order == 1:
+        return
+    assert np.all(np.abs(lim_gt[fired] - gt[fired]).max(axis=1) > 0)
+    np.testing.assert_allclose(lim_gt[fired, :nv2].sum(axis=1), gt[fired, :nv2].sum(axis=1),
+                               rtol=1e-13)
+    assert np.all(lim_gt[fired, :nv2] >= 0)
+
+
 def _affine_field(cl, rng, quadratic):
     """g[j, v] = polynomial in (x_j, y_j, z_j) with per-v random coefficients;
     returns g and a function giving the exact value at a point."""
