@@ -60,6 +60,23 @@ def body(shape, centre, size, o=(1.0, 0.0), rm=(1, 0, 0, 0, 1, 0, 0, 0, 1)):
     return b
 
 
+class ABRigidBody(C.Structure):
+    """ab_rigid_body: a rigid body and its dynamic state (RigidBody,
+    RigidBody/Geometry.hpp:19-80)."""
+    _fields_ = [("shape", C.c_int), ("npts", C.c_int), ("rotates", C.c_int),
+                ("size", C.c_double), ("mass", C.c_double), ("T", C.c_double),
+                ("c", C.c_double * 3), ("o", C.c_double * 2), ("rm", C.c_double * 9),
+                ("vel", C.c_double * 3), ("angular_vel", C.c_double), ("om", C.c_double * 3),
+                ("force", C.c_double * 3), ("tq", C.c_double * 3)]
+
+    def state(self):
+        """The dynamic state as a dict of numpy arrays."""
+        return dict(c=np.array(self.c[:]), vel=np.array(self.vel[:]),
+                    om=np.array(self.om[:]), angular_vel=float(self.angular_vel),
+                    o=np.array(self.o[:]), rm=np.array(self.rm[:]),
+                    force=np.array(self.force[:]), tq=np.array(self.tq[:]))
+
+
 class ABConfig(C.Structure):
     _fields_ = [("dim", C.c_int), ("n_particles", C.c_int), ("nv", C.c_int),
                 ("vmin", C.c_double), ("vmax", C.c_double), ("R", C.c_double),
@@ -78,7 +95,7 @@ class ABCloudView(C.Structure):
                 ("wnx", _dp), ("wny", _dp), ("wnz", _dp), ("wux", _dp), ("wuy", _dp),
                 ("wuz", _dp), ("wt", _dp), ("mvx", _dp), ("mvy", _dp), ("mvz", _dp),
                 ("rho", _dp), ("ux", _dp), ("uy", _dp), ("uz", _dp), ("t", _dp),
-                ("g", _dp), ("neigh", _ip), ("nneigh", _ip)]
+                ("g", _dp), ("neigh", _ip), ("nneigh", _ip), ("body_id", _ip)]
 
 
 class ABStepStats(C.Structure):
@@ -140,7 +157,8 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_graph_launch", "ab_graph_synchronize", "ab_graph_destroy", "ab_autotune",
                "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels",
                "ab_cloud_manage", "ab_cloud_source_map", "ab_download_particles",
-               "ab_batch_create", "ab_batch_scatter", "ab_check_config"]
+               "ab_batch_create", "ab_batch_scatter", "ab_check_config", "ab_download_body_ids",
+               "ab_body_step"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
@@ -148,7 +166,8 @@ COLLISION_DISCRETE = 1
 LIMITER_GPU = 0          # the reference's GPU step (cudaStep)
 LIMITER_MOOD = 1         # its CPU step with the MOOD limiter (runSolver)
 HOST_EXPORTS = ["alh_configure", "alh_lattice_size", "alh_build_lattice", "alh_ckpt_info_read",
-                "alh_ckpt_read", "alh_ckpt_write"]
+                "alh_ckpt_read", "alh_ckpt_write", "alh_make_bodies", "alh_body_case_size",
+                "alh_build_body_case"]
 
 
 def hip_lib():
@@ -218,6 +237,9 @@ class Cloud:
     nneigh: np.ndarray | None = None
     mesh: np.ndarray | None = None   # N x 3 ALE mesh velocity
     record: dict | None = None       # restart-file fields not held elsewhere (p, bodyId, flags)
+    body_id: np.ndarray | None = None   # Particle::bodyId (-1: none)
+    bodies: list | None = None       # ABRigidBody of a body case
+    geo: tuple | None = None         # dx, dy, dz, minDist of the lattice
 
     @property
     def N(self):
@@ -249,6 +271,60 @@ def build_lattice(case, precision=PREC_FP64, mls_order=None):
     if rc:
         raise ALEBGKError(f"alh_build_lattice: {ERRORS.get(rc, rc)}")
     return Cloud(cfg, x, y, z, b, wall, mom, g)
+
+
+def make_bodies(case, body_scale=1.0):
+    """makeBodies (TestCases.cpp:251-294): the case's bodies at rest."""
+    arr = (ABRigidBody * 4)()
+    n = C.c_int()
+    rc = host_lib().alh_make_bodies(C.byref(case.raw), C.c_double(body_scale), arr, C.byref(n))
+    if rc:
+        raise ALEBGKError(f"alh_make_bodies: {ERRORS.get(rc, rc)}")
+    return [arr[k] for k in range(n.value)]
+
+
+def build_case(case, precision=PREC_FP64, mls_order=None, body_scale=1.0):
+    """The cloud of any case: build_lattice for the plain cavities, the
+    embedded body case (generateParticles + embedRigidBodies + wall state +
+    parabolic lid + applyIC) for the others. A body cloud carries body_id,
+    bodies and geo (dx, dy, dz, minDist)."""
+    cs = case.raw
+    if mls_order is not None:
+        cs.mls_order = int(mls_order)
+    if not cs.has_body:
+        return build_lattice(case, precision=precision, mls_order=mls_order)
+    n = C.c_int()
+    rc = host_lib().alh_body_case_size(C.byref(cs), C.c_double(body_scale), C.byref(n))
+    if rc:
+        raise ALEBGKError(f"alh_body_case_size: {ERRORS.get(rc, rc)}")
+    N = n.value
+    gs = cs.nv ** 3 if cs.dim == 3 else 2 * cs.nv ** 2
+    x, y, z = np.zeros(N), np.zeros(N), np.zeros(N)
+    b = np.zeros(N, np.uint8)
+    wall = np.zeros((N, 7))
+    mom = np.zeros((N, 5))
+    g = np.zeros((N, gs))
+    bid = np.zeros(N, np.int32)
+    arr = (ABRigidBody * 4)()
+    nb = C.c_int()
+    geo = np.zeros(4)
+    cfg = ABConfig()
+    rc = host_lib().alh_build_body_case(C.byref(cs), C.c_double(body_scale), C.c_int(precision),
+                                        C.byref(cfg), _ptr(x, _dp), _ptr(y, _dp), _ptr(z, _dp),
+                                        _ptr(b, _up), _ptr(wall, _dp), _ptr(mom, _dp),
+                                        _ptr(g, _dp), _ptr(bid, _ip), arr, C.byref(nb),
+                                        _ptr(geo, _dp))
+    if rc:
+        raise ALEBGKError(f"alh_build_body_case: {ERRORS.get(rc, rc)}")
+    return Cloud(cfg, x, y, z, b, wall, mom, g, body_id=bid,
+                 bodies=[arr[k] for k in range(nb.value)], geo=tuple(geo))
+
+
+def box(case):
+    """The case's box as ab_cloud_manage and ab_body_step take it: (xleft,
+    xright, ybottom, ytop, zfront, zback)."""
+    d = case.dom
+    return (d[0], d[1], d[3], d[2], d[4], d[5])
 
 
 class Solver:
@@ -306,6 +382,11 @@ class Solver:
             v.mvx = v.mvy = v.mvz = C.cast(None, _dp)
         v.rho, v.ux, v.uy, v.uz, v.t = [_ptr(a, _dp) for a in keep["m"]]
         v.g = _ptr(keep["g"], _dp)
+        if cloud.body_id is not None:
+            keep["bid"] = np.ascontiguousarray(cloud.body_id, dtype=np.int32)
+            v.body_id = _ptr(keep["bid"], _ip)
+        else:
+            v.body_id = C.cast(None, _ip)
         if cloud.neigh is not None:
             keep["nb"] = np.ascontiguousarray(cloud.neigh, dtype=np.int32)
             keep["nn"] = np.ascontiguousarray(cloud.nneigh, dtype=np.int32)
@@ -525,6 +606,32 @@ class Solver:
         self._check(self._lib.ab_cloud_source_map(self._ctx, _ptr(src, _ip)),
                     "ab_cloud_source_map")
         return src, tuple(counts)
+
+    def body_ids(self):
+        """Particle::bodyId of the device cloud (ab_download_body_ids)."""
+        b = np.zeros(self.N, np.int32)
+        self._check(self._lib.ab_download_body_ids(self._ctx, _ptr(b, _ip)), "ab_download_body_ids")
+        return b
+
+    def body_step(self, dt, bodies, dx, min_dist, dom):
+        """The moving-body block after a step (ab_body_step): Heun-coupled
+        loads, Newton-Euler, contact, surface wall state, cloud management.
+        bodies: ABRigidBody list, updated in place. dom: (xleft, xright,
+        ybottom, ytop, zfront, zback). Returns (loads (nb x 12: L0 force,
+        torque, L1 force, torque), counts (removed, merged, added))."""
+        nb = len(bodies)
+        arr = (ABRigidBody * max(nb, 1))(*bodies)
+        d6 = np.ascontiguousarray(dom, dtype=np.float64)
+        ld = np.zeros(12 * max(nb, 1))
+        counts = (C.c_int * 3)()
+        n_new = C.c_int()
+        self._check(self._lib.ab_body_step(self._ctx, C.c_double(dt), C.c_int(nb), arr,
+                                           C.c_double(dx), C.c_double(min_dist), _ptr(d6, _dp),
+                                           _ptr(ld, _dp), C.byref(n_new), counts), "ab_body_step")
+        for k in range(nb):
+            C.memmove(C.byref(bodies[k]), C.byref(arr[k]), C.sizeof(ABRigidBody))
+        self.N = n_new.value
+        return ld.reshape(-1, 12)[:nb], tuple(counts)
 
     def set_wall_state(self, idx, xyz=None, wn=None, wu=None, mv=None):
         """Positions, wall normals, wall velocities and mesh velocities of

@@ -54,7 +54,8 @@ void free_cloud(ab_ctx *c)
                     c->wt, c->mvx, c->mvy, c->mvz, c->rho, c->ux, c->uy, c->uz, c->T,
                     c->neigh, c->nneigh, c->ilist, c->blist, c->g, c->gt, c->limflag,
                     c->mlsflag, c->mlsovr, c->pmass, c->vox, c->vcount, c->vstart, c->vlist,
-                    c->bmom, c->bidx, c->bwu, c->dm_alpha, c->dm_pass, c->dm_res, c->dm_rec,
+                    c->bmom, c->bidx, c->bwu, c->bbody, c->bcnt, c->bcon, c->bpar, c->bout,
+                    c->bid, c->dm_alpha, c->dm_pass, c->dm_res, c->dm_rec,
                     c->mood, c->moodfail, c->inst, c->ih, c->seg};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     c->inst = c->seg = nullptr; c->ih = nullptr;
@@ -69,7 +70,10 @@ void free_cloud(ab_ctx *c)
     c->g = c->gt = c->limflag = c->mlsovr = c->pmass = nullptr;
     c->mlsflag = nullptr;
     c->vox = c->vcount = c->vstart = c->vlist = nullptr;
-    c->bmom = nullptr; c->bidx = nullptr; c->bwu = nullptr; c->bmom_cap = 0;
+    c->bmom = c->bcon = c->bwu = c->bpar = c->bout = nullptr;
+    c->bidx = c->bbody = c->bcnt = nullptr;
+    c->bmom_cap = 0;
+    c->bid = nullptr;
     table_free(c);
     table2d_free(c);
     c->coef_dt = -1.0;
@@ -86,6 +90,7 @@ int alloc_cloud(ab_ctx *c)
     if ((rc = dalloc(c, &c->bnd, N))) return rc;
     if ((rc = dalloc(c, &c->neigh, N * AB_MAX_NEIGH))) return rc;
     if ((rc = dalloc(c, &c->nneigh, N))) return rc;
+    if ((rc = dalloc(c, &c->bid, N))) return rc;
     if ((rc = dalloc(c, &c->ilist, N))) return rc;
     if ((rc = dalloc(c, &c->blist, N))) return rc;
     if ((rc = dalloc(c, &c->mlsflag, N))) return rc;
@@ -331,6 +336,13 @@ __global__ void k_gather(const T *__restrict__ src, T *__restrict__ dst,
     dst[e] = j >= 0 ? src[(size_t)j * stride + k] : T(0);
 }
 
+__global__ void k_gather_bid(const int *__restrict__ src, int *__restrict__ dst,
+                             const int *__restrict__ map, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = map[i] >= 0 ? src[map[i]] : -1;
+}
+
 // Re-lays the cloud out for n particles: particle i takes the state of old
 // particle src[i] (src: n entries on the device), or zero where src[i] < 0
 // (a particle the caller fills in). The per-step scratch starts at zero;
@@ -349,11 +361,11 @@ int relayout_cloud(ab_ctx *c, const int *src, int n)
     double *og = c->g, *ogt = c->gt, *olim = c->limflag, *oovr = c->mlsovr, *opm = c->pmass;
     double *oalpha = c->dm_alpha, *ores = c->dm_res;
     int *opass = c->dm_pass, *omls = c->mlsflag, *omood = c->mood, *omf = c->moodfail;
-    int *onb = c->neigh, *onn = c->nneigh, *oil = c->ilist, *obl = c->blist;
+    int *onb = c->neigh, *onn = c->nneigh, *oil = c->ilist, *obl = c->blist, *obid = c->bid;
     unsigned char *obnd = c->bnd;
     c->g = c->gt = c->limflag = c->mlsovr = c->pmass = c->dm_alpha = c->dm_res = nullptr;
     c->dm_pass = c->mlsflag = c->mood = c->moodfail = nullptr;
-    c->neigh = c->nneigh = c->ilist = c->blist = nullptr;
+    c->neigh = c->nneigh = c->ilist = c->blist = c->bid = nullptr;
     c->bnd = nullptr;
     const int oldN = c->N;
     c->N = n;
@@ -373,6 +385,8 @@ int relayout_cloud(ab_ctx *c, const int *src, int n)
         gather(ores, c->dm_res, 1);
         gather(opass, c->dm_pass, 1);
         const int nb = (n + 255) / 256;
+        // a particle filled in by the caller belongs to no body
+        if (n) k_gather_bid<<<nb, 256, 0, s>>>(obid, c->bid, src, n);
         if (n)
         {
             k_fill_dbl<<<nb, 256, 0, s>>>(c->limflag, n, 0.0);
@@ -386,7 +400,7 @@ int relayout_cloud(ab_ctx *c, const int *src, int n)
         { c->err = "cloud re-layout: gather failed"; rc = AB_ERR_HIP; }
     }
     void *old[] = {og, ogt, olim, oovr, opm, oalpha, ores, opass, omls, omood, omf,
-                   onb, onn, oil, obl, obnd, c->vox, c->vlist, c->dm_rec};
+                   onb, onn, oil, obl, obnd, obid, c->vox, c->vlist, c->dm_rec};
     for (void *p : old) if (p) (void)hipFree(p);
     for (int k = 0; k < nc; ++k) if (od[k]) (void)hipFree(od[k]);
     c->vox = c->vlist = nullptr;
@@ -624,6 +638,10 @@ int ab_upload(ab_ctx *c, const ab_cloud_view *v)
               up(c->T, v->t);
     if (!ok) { c->err = "ab_upload: host-to-device copy failed"; return AB_ERR_HIP; }
     HIPCHK(c, hipMemcpyAsync(c->bnd, v->boundary, N, hipMemcpyHostToDevice, s));
+    if (v->body_id)
+        HIPCHK(c, hipMemcpyAsync(c->bid, v->body_id, sizeof(int) * (size_t)N, hipMemcpyHostToDevice, s));
+    else
+        k_fill_int<<<(N + 255) / 256, 256, 0, s>>>(c->bid, N, -1);
     // interior / boundary lists
     std::vector<int> il, bl;
     il.reserve(N); bl.reserve(N);
@@ -990,92 +1008,6 @@ int ab_total_mass(ab_ctx *c, double *mass)
     return AB_OK;
 }
 
-int ab_body_loads(ab_ctx *c, int nsurf, const int *surf_idx, const int *surf_body, int nbodies,
-                  const double *body_c, const double *body_area, const double *wu, double *out)
-{
-    if (!c || nsurf < 0 || nbodies <= 0 || !body_c || !body_area || !out ||
-        (nsurf > 0 && (!surf_idx || !surf_body)))
-        return AB_ERR_ARG;
-    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
-    for (int k = 0; k < nsurf; ++k)
-        if (surf_idx[k] < 0 || surf_idx[k] >= c->N || surf_body[k] < 0 || surf_body[k] >= nbodies)
-        { c->err = "ab_body_loads: surface index or body id out of range"; return AB_ERR_ARG; }
-    std::memset(out, 0, sizeof(double) * 6 * (size_t)nbodies);
-    if (nsurf == 0) return AB_OK;
-    if (nsurf > c->bmom_cap)
-    {
-        if (c->bmom) (void)hipFree(c->bmom);
-        if (c->bidx) (void)hipFree(c->bidx);
-        c->bmom = nullptr; c->bidx = nullptr; c->bmom_cap = 0;
-        int rc;
-        if ((rc = dalloc(c, &c->bmom, (size_t)nsurf * 10)) || (rc = dalloc(c, &c->bidx, (size_t)nsurf)))
-            return rc;
-        c->bmom_cap = nsurf;
-    }
-    HIPCHK(c, hipMemcpyAsync(c->bidx, surf_idx, 4 * (size_t)nsurf, hipMemcpyHostToDevice, c->stream));
-    if (table_mode(c))
-        launch_body_moments(c->view(), nullptr, c->gs, c->cfg.precision == AB_PREC_MIXED, c->S,
-                            c->scal + S_LAZY, c->m0, c->bidx, nsurf, c->bmom, c->stream);
-    else
-        launch_body_moments(c->view(), c->g, nullptr, 0, 0, nullptr, nullptr, c->bidx, nsurf,
-                            c->bmom, c->stream);
-    std::vector<double> m((size_t)nsurf * 10);
-    HIPCHK(c, hipMemcpyAsync(m.data(), c->bmom, 8 * m.size(), hipMemcpyDeviceToHost, c->stream));
-    // positions, normals and (if not given) wall velocities of the surface
-    const size_t n8 = 8 * (size_t)c->N;
-    std::vector<double> px(c->N), py(c->N), pz(c->N), nx(c->N), ny(c->N), nz(c->N);
-    std::vector<double> ux, uy, uz;
-    const double *src[] = {c->x, c->y, c->z, c->wnx, c->wny, c->wnz};
-    double *dst[] = {px.data(), py.data(), pz.data(), nx.data(), ny.data(), nz.data()};
-    for (int k = 0; k < 6; ++k) HIPCHK(c, hipMemcpyAsync(dst[k], src[k], n8, hipMemcpyDeviceToHost, c->stream));
-    if (!wu)
-    {
-        ux.resize(c->N); uy.resize(c->N); uz.resize(c->N);
-        HIPCHK(c, hipMemcpyAsync(ux.data(), c->wux, n8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(uy.data(), c->wuy, n8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(uz.data(), c->wuz, n8, hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    // computeBodyLoads accumulation (BodyDynamics.cpp:41-94) from the moments
-    for (int k = 0; k < nsurf; ++k)
-    {
-        const int i = surf_idx[k], b = surf_body[k];
-        const double *q = &m[(size_t)k * 10];
-        const double Ux = wu ? wu[3 * k] : ux[i], Uy = wu ? wu[3 * k + 1] : uy[i];
-        const double Uz = (c->dim == 3) ? (wu ? wu[3 * k + 2] : uz[i]) : 0.0;
-        const double area = body_area[b];
-        const double cdx = px[i] - body_c[3 * b], cdy = py[i] - body_c[3 * b + 1],
-                     cdz = pz[i] - body_c[3 * b + 2];
-        // P_ab = m_ab - U_a m_b - U_b m_a + U_a U_b m0
-        const double pxx = q[4] - 2.0 * Ux * q[1] + Ux * Ux * q[0];
-        const double pyy = q[5] - 2.0 * Uy * q[2] + Uy * Uy * q[0];
-        const double pxy = q[7] - Ux * q[2] - Uy * q[1] + Ux * Uy * q[0];
-        double *o = out + 6 * (size_t)b;
-        if (c->dim == 3)
-        {
-            const double pzz = q[6] - 2.0 * Uz * q[3] + Uz * Uz * q[0];
-            const double pxz = q[8] - Ux * q[3] - Uz * q[1] + Ux * Uz * q[0];
-            const double pyz = q[9] - Uy * q[3] - Uz * q[2] + Uy * Uz * q[0];
-            const double tx = -(nx[i] * pxx + ny[i] * pxy + nz[i] * pxz);
-            const double ty = -(nx[i] * pxy + ny[i] * pyy + nz[i] * pyz);
-            const double tz = -(nx[i] * pxz + ny[i] * pyz + nz[i] * pzz);
-            o[0] += area * tx; o[1] += area * ty; o[2] += area * tz;
-            o[3] += area * (cdy * tz - cdz * ty);
-            o[4] += area * (cdz * tx - cdx * tz);
-            o[5] += area * (cdx * ty - cdy * tx);
-        }
-        else
-        {
-            const double valX = -nx[i] * pxx - ny[i] * pxy;
-            const double valY = -nx[i] * pxy - ny[i] * pyy;
-            o[0] += area * valX;
-            o[1] += area * valY;
-            o[5] += area * (cdx * valY - cdy * valX);
-        }
-    }
-    return AB_OK;
-}
-
 int ab_set_wall_state(ab_ctx *c, int n, const int *idx, const double *x, const double *y,
                       const double *z, const double *wn, const double *wu, const double *mv)
 {
@@ -1181,7 +1113,8 @@ int ab_batch_create(ab_ctx *const *m, int n, ab_ctx **out)
                             hipMemcpyDeviceToDevice, s) == hipSuccess &&
              hipMemcpyAsync(b->neigh + o * AB_MAX_NEIGH, a->neigh, 4 * nk * AB_MAX_NEIGH,
                             hipMemcpyDeviceToDevice, s) == hipSuccess &&
-             hipMemcpyAsync(b->nneigh + o, a->nneigh, 4 * nk, hipMemcpyDeviceToDevice, s) == hipSuccess;
+             hipMemcpyAsync(b->nneigh + o, a->nneigh, 4 * nk, hipMemcpyDeviceToDevice, s) == hipSuccess &&
+             hipMemcpyAsync(b->bid + o, a->bid, 4 * nk, hipMemcpyDeviceToDevice, s) == hipSuccess;
         if (!ok) { b->err = "ab_batch_create: device copy failed"; return fail(AB_ERR_HIP); }
         const size_t ne = nk * AB_MAX_NEIGH;
         k_offset_neigh<<<(unsigned)((ne + 255) / 256), 256, 0, s>>>(b->neigh + o * AB_MAX_NEIGH,

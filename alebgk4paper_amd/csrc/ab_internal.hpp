@@ -11,6 +11,8 @@
 
 namespace ab {
 
+constexpr int kMaxBodies = 16;     // rigid bodies per cloud
+
 // Device view of one cloud: flat SoA arrays, mirroring the fields of the
 // reference's DeviceCloud (Solver/CudaBackend.cu:69-113).
 struct DevCloud
@@ -93,6 +95,7 @@ struct ab_ctx
     double *mvx = nullptr, *mvy = nullptr, *mvz = nullptr;
     double *rho = nullptr, *ux = nullptr, *uy = nullptr, *uz = nullptr, *T = nullptr;
     int *neigh = nullptr, *nneigh = nullptr;
+    int *bid = nullptr;            // Particle::bodyId (-1: none)
     int *ilist = nullptr, *blist = nullptr;   // interior / boundary particle lists
     int n_int = 0, n_bnd = 0;
 
@@ -183,11 +186,14 @@ struct ab_ctx
     std::vector<int> cm_src;
     bool mlp_on = false;           // warm start enabled
 
-    // body loads scratch
-    double *bmom = nullptr;
+    // body loads scratch, grown to the surface-particle count: indices and
+    // body of each surface particle, its raw moments (10), its load
+    // contributions (6), given wall velocities (3); per-body parameters and
+    // results; a counter
     int bmom_cap = 0;
-    int *bidx = nullptr;
-    double *bwu = nullptr;
+    int *bidx = nullptr, *bbody = nullptr, *bcnt = nullptr;
+    double *bmom = nullptr, *bcon = nullptr, *bwu = nullptr;
+    double *bpar = nullptr, *bout = nullptr;
 
     // profiling: event pairs recorded around stages on the context's stream,
     // resolved (synchronised and accumulated) only when the pool fills or the
@@ -291,10 +297,5 @@ void table_free(ab_ctx *ctx);
 // boundary lists from the boundary flags.
 int relayout_cloud(ab_ctx *ctx, const int *src, int n);
 int rebuild_lists(ab_ctx *ctx);
-
-// ---- k_body.hip --------------------------------------------------------------
-void launch_body_moments(const DevCloud &c, const double *g_fp64, const void *g_table,
-                         int table_fp32, int S, const double *lazy, const double *m0,
-                         const int *idx, int n, double *mom, hipStream_t s);
 
 }  // namespace ab

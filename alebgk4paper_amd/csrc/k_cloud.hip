@@ -41,8 +41,6 @@ using namespace ab;
 
 namespace {
 
-constexpr int kMaxBodies = 16;
-
 struct DevBody
 {
     int shape;
@@ -536,7 +534,7 @@ extern "C" int ab_cloud_manage(ab_ctx *c, int nbodies, const ab_body *bodies, co
     { c->err = "ab_cloud_manage: the cloud changes, which the lattice tables cannot follow (AB_PREC_FP64 only)"; return AB_ERR_UNSUPPORTED; }
     if (c->nbatch)
     { c->err = "ab_cloud_manage: manage the members' clouds, then form the batch again"; return AB_ERR_UNSUPPORTED; }
-    if (nbodies < 0 || nbodies > kMaxBodies || (nbodies && !bodies) || !body_id || !dom ||
+    if (nbodies < 0 || nbodies > kMaxBodies || (nbodies && !bodies) || !dom ||
         !(dx != 0.0) || !std::isfinite(dx))
     { c->err = "ab_cloud_manage: invalid argument"; return AB_ERR_ARG; }
     Bodies B{};
@@ -558,7 +556,15 @@ extern "C" int ab_cloud_manage(ab_ctx *c, int nbodies, const ab_body *bodies, co
     if (counts) counts[0] = counts[1] = counts[2] = 0;
     if (n_new) *n_new = N0;
     // the moving-body block runs only with bodies (Solver.cpp:349)
-    if (nbodies == 0) return AB_OK;
+    if (nbodies == 0)
+    {
+        if (body_id && N0)
+        {
+            CMCHK(c, hipMemcpyAsync(c->bid, body_id, sizeof(int) * N0, hipMemcpyHostToDevice, c->stream));
+            CMCHK(c, hipStreamSynchronize(c->stream));
+        }
+        return AB_OK;
+    }
     hipStream_t s = c->stream;
     const bool d3 = c->dim == 3;
     const double dxl = std::fabs(dx);
@@ -567,7 +573,11 @@ extern "C" int ab_cloud_manage(ab_ctx *c, int nbodies, const ab_body *bodies, co
     int *body = nullptr, *surf = nullptr, *nsurf_d = nullptr;
     unsigned char *flag = nullptr;
     if ((rc = sc.get(c, &body, N0)) || (rc = sc.get(c, &nsurf_d, 1))) return rc;
-    CMCHK(c, hipMemcpyAsync(body, body_id, sizeof(int) * N0, hipMemcpyHostToDevice, s));
+    // the body ids given replace the device cloud's; without them the
+    // device's are used (the relayouts below carry c->bid along)
+    if (body_id && N0)
+        CMCHK(c, hipMemcpyAsync(c->bid, body_id, sizeof(int) * N0, hipMemcpyHostToDevice, s));
+    if (N0) CMCHK(c, hipMemcpyAsync(body, c->bid, sizeof(int) * N0, hipMemcpyDeviceToDevice, s));
     auto surface = [&](int &nsurf) -> int {
         const int N = c->N;
         int r;
@@ -707,6 +717,18 @@ extern "C" int ab_cloud_manage(ab_ctx *c, int nbodies, const ab_body *bodies, co
     CMCHK(c, hipStreamSynchronize(s));
     if (n_new) *n_new = c->N;
     if (counts) { counts[0] = removed; counts[1] = merged; counts[2] = added; }
+    return AB_OK;
+}
+
+extern "C" int ab_download_body_ids(ab_ctx *c, int *body_id)
+{
+    if (!c || !body_id) return AB_ERR_ARG;
+    if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
+    if (c->N)
+    {
+        CMCHK(c, hipMemcpyAsync(body_id, c->bid, sizeof(int) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+        CMCHK(c, hipStreamSynchronize(c->stream));
+    }
     return AB_OK;
 }
 

@@ -125,11 +125,19 @@ int alh_lattice_size(const alh_case *cs, int *n)
     return AB_OK;
 }
 
-int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double *x, double *y,
-                      double *z, unsigned char *boundary, double *wall, double *mom, double *g)
+}  // extern "C"
+
+namespace alh {
+
+// generateParticles (ParticleGeneration.cpp:35-114, non-periodic lattice),
+// initWallData (WallState.cpp:16-26: box-wall particles only), the lid
+// (TestCases.cpp:141-144 uniform, :230-249 parabolic) and the macro fields of
+// applyIC (InitialConditions.cpp:18-81) for the lattice of a case. g is
+// left to fill_ic. geo receives dx, dy, dz, minDist.
+int lattice_core(const alh_case *cs, int precision, ab_config *cfg, double *x, double *y,
+                 double *z, unsigned char *boundary, double *wall, double *mom, double geo[4])
 {
-    if (!cs || !cfg || !x || !y || !z || !boundary || !wall || !mom || !g) return AB_ERR_ARG;
-    if (cs->has_body) return AB_ERR_UNSUPPORTED;
+    if (!cs || !cfg || !x || !y || !z || !boundary || !wall || !mom) return AB_ERR_ARG;
     const int ndim = cs->dim;
     const int Nx = cs->nx, Ny = cs->ny, Nz = (ndim == 3) ? cs->nz : 1;
     if (Nx < 2 || Ny < 2 || (ndim == 3 && Nz < 2) || cs->nv < 2) return AB_ERR_ARG;
@@ -142,6 +150,12 @@ int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double 
     const double dz = (ndim == 3 && Nz > 1) ? Lz / (Nz - 1) : 1.0;
     const double radius = 2.5 * std::max({std::fabs(dx), std::fabs(dy),
                                           (ndim == 3 ? std::fabs(dz) : std::fabs(dx))});
+    if (geo)
+    {
+        geo[0] = dx; geo[1] = dy; geo[2] = dz;
+        geo[3] = 0.5 * std::min({std::fabs(dx), std::fabs(dy),
+                                 (ndim == 3 ? std::fabs(dz) : std::fabs(dy))});
+    }
     const double hb = 3.0 * dx, vb = 3.0 * dy;
     const int nbh = (int)std::ceil(Lx / hb) + 1, nbv = (int)std::ceil(Ly / vb) + 1;
     double wb = 1.0;
@@ -169,9 +183,6 @@ int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double 
     if (cs->lid == ALH_LID_UNIFORM_TOP) U[2][0] = cs->lid_speed;
     if (cs->lid == ALH_LID_UNIFORM_FRONT) U[4][0] = cs->lid_speed;
 
-    const int Nv = cs->nv;
-    const double dv = (cs->vmax - cs->vmin) / (Nv - 1), vlo = cs->vmin;
-    const size_t gs = (ndim == 3) ? (size_t)Nv * Nv * Nv : (size_t)2 * Nv * Nv;
     int n = 0;
     for (int k = 0; k < Nz; k++)
     {
@@ -232,8 +243,17 @@ int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double 
             }
         }
     }
-    // Every particle starts from the same Maxwellian: one row, evaluated node
-    // by node as applyIC does, copied to all particles.
+    return AB_OK;
+}
+
+// applyIC's distribution: every particle starts from the same Maxwellian at
+// rest (rho0, 270 K), one row evaluated node by node as applyIC does and
+// copied to all n particles.
+void fill_ic(const alh_case *cs, double *g, size_t n)
+{
+    const int Nv = cs->nv, ndim = cs->dim;
+    const double dv = (cs->vmax - cs->vmin) / (Nv - 1), vlo = cs->vmin;
+    const size_t gs = (ndim == 3) ? (size_t)Nv * Nv * Nv : (size_t)2 * Nv * Nv;
     std::vector<double> row(gs);
     if (ndim == 3)
     {
@@ -256,7 +276,21 @@ int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double 
                 row[lin + Nv * Nv] = 0.5 * cs->R * 270.0 * Mg1;
             }
     }
-    for (int p = 0; p < n; ++p) std::memcpy(g + (size_t)p * gs, row.data(), gs * sizeof(double));
+    for (size_t p = 0; p < n; ++p) std::memcpy(g + p * gs, row.data(), gs * sizeof(double));
+}
+
+}  // namespace alh
+
+extern "C" {
+
+int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double *x, double *y,
+                      double *z, unsigned char *boundary, double *wall, double *mom, double *g)
+{
+    if (!cs || !g) return AB_ERR_ARG;
+    if (cs->has_body) return AB_ERR_UNSUPPORTED;
+    int rc = alh::lattice_core(cs, precision, cfg, x, y, z, boundary, wall, mom, nullptr);
+    if (rc) return rc;
+    alh::fill_ic(cs, g, (size_t)cfg->n_particles);
     return AB_OK;
 }
 

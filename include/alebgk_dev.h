@@ -113,6 +113,7 @@ typedef struct ab_cloud_view {
     const double *g;
     const int *neigh;
     const int *nneigh;
+    const int *body_id;      /* Particle::bodyId (N), or NULL: no bodies */
 } ab_cloud_view;
 
 /* Per-step report (cudaLimiterCounts / cudaMlsCounts / cudaMlsOvershoot /
@@ -201,7 +202,9 @@ int ab_total_mass(ab_ctx *ctx, double *mass);
  * body_c is 3*nbodies centres, body_area the per-particle surface area of
  * each body; wu (3*nsurf) is the wall velocity of each surface particle, or
  * NULL to use the uploaded wall velocity. out = 6*nbodies
- * (Fx, Fy, Fz, Tx, Ty, Tz per body; 2-D: Fx, Fy, 0, 0, 0, Tz). */
+ * (Fx, Fy, Fz, Tx, Ty, Tz per body; 2-D: Fx, Fy, 0, 0, 0, Tz). Moments,
+ * tractions and the per-body sums (a fixed-order reduction) stay on the
+ * device; only the 6*nbodies results come back. */
 int ab_body_loads(ab_ctx *ctx, int nsurf, const int *surf_idx,
                   const int *surf_body, int nbodies, const double *body_c,
                   const double *body_area, const double *wu, double *out);
@@ -227,8 +230,10 @@ int ab_synchronize(ab_ctx *ctx);
  * shape AB_BODY_SQUARE (side size, orientation o), AB_BODY_CIRCLE (radius
  * size), AB_BODY_SPHERE (radius size), AB_BODY_CUBE (side size, body-to-world
  * rotation rm, row-major as Cube::Rm). body_id holds Particle::bodyId of the
- * current cloud (N entries, -1 for none): the body surface particles are the
- * boundary particles with body_id >= 0. dx is CalcParameters::dx, min_dist
+ * current cloud (N entries, -1 for none), or is NULL for the body ids the
+ * device cloud carries (ab_cloud_view.body_id; given ones replace them): the
+ * body surface particles are the boundary particles with body_id >= 0. The
+ * device's body ids follow the new cloud. dx is CalcParameters::dx, min_dist
  * CalcParameters::minDist (<= 0: no merging), dom the box (xleft, xright,
  * ybottom, ytop, zfront, zback); the MLS radius and the velocity grid are the
  * configuration's. On return the cloud holds *n_new particles and counts (may
@@ -252,6 +257,58 @@ int ab_cloud_manage(ab_ctx *ctx, int nbodies, const ab_body *bodies, const int *
                     double dx, double min_dist, const double dom[6], int *n_new,
                     int counts[3]);
 int ab_cloud_source_map(ab_ctx *ctx, int *src);
+/* Particle::bodyId of every particle of the device cloud (uploaded with
+ * ab_cloud_view.body_id, carried through cloud management; -1 for none). */
+int ab_download_body_ids(ab_ctx *ctx, int *body_id);
+
+/* ---- Moving rigid bodies ---------------------------------------------------
+ * A rigid body with its dynamic state (RigidBody and its shapes,
+ * RigidBody/Geometry.hpp:19-362): shape and size as ab_body, npts the surface
+ * particles it was sampled with (the surface area per particle is the
+ * shape's area over npts), mass, surface temperature T, whether it spins
+ * (rotates), centre c, 2-D orientation o, cube rotation rm (body to world,
+ * row-major), velocity vel, 2-D spin angular_vel, 3-D angular velocity om,
+ * and the Heun-averaged force and torque of the last step (2-D: tq[2] is
+ * RigidBody::torque). */
+typedef struct ab_rigid_body {
+    int shape;
+    int npts;
+    int rotates;
+    double size;
+    double mass, T;
+    double c[3];
+    double o[2];
+    double rm[9];
+    double vel[3];
+    double angular_vel;
+    double om[3];
+    double force[3];
+    double tq[3];
+} ab_rigid_body;
+
+/* The moving-body block of one time step, after ab_step: moveRigidBodies
+ * (RigidBody/BodyDynamics.cpp:143-711) and the cloud management that follows
+ * it (manageRemove, mergeCloseParticles, manageRefill with their neighbour
+ * rebuilds, Solver/Solver.cpp:348-444 and :705-714), all on the device's
+ * cloud with no download or upload of g:
+ *   loads L0 from the surface particles' wall velocities; the predicted
+ *   body velocities; loads L1 at those velocities on the same g (one pass of
+ *   raw surface moments serves both); the surface particles moved rigidly
+ *   with the old state; Newton-Euler in FP64 with the averaged loads; the
+ *   box-wall contact (a rigid shift by the surface extent, 0.4 dx clear,
+ *   inelastic); the surface wall state (normal, wall and mesh velocity)
+ *   from the new state; then cloud management around the moved bodies.
+ * bodies (nbodies, in/out) carry the state; the body surface particles are
+ * the boundary particles whose body id (ab_download_body_ids) is >= 0. dx,
+ * min_dist and dom as for ab_cloud_manage. loads (may be NULL) receives
+ * 12 per body: the L0 force and torque, then L1's. n_new and counts as
+ * for ab_cloud_manage. AB_PREC_FP64 only (the cloud leaves the lattice). The
+ * load sums are reduced on the device in a fixed order (the reference sums
+ * particle by particle: equal to rounding); the body arithmetic is the
+ * reference's, operation for operation. */
+int ab_body_step(ab_ctx *ctx, double dt, int nbodies, ab_rigid_body *bodies, double dx,
+                 double min_dist, const double dom[6], double *loads, int *n_new,
+                 int counts[3]);
 
 /* ---- Autotuner of the transport launch shape -----------------------------
  * No counterpart in the reference. Times every launch shape of the table

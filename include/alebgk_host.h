@@ -57,6 +57,25 @@ int alh_lattice_size(const alh_case *cs, int *n);
 int alh_build_lattice(const alh_case *cs, int precision, ab_config *cfg, double *x, double *y,
                       double *z, unsigned char *boundary, double *wall, double *mom, double *g);
 
+/* ---- Body cases -------------------------------------------------------------
+ * makeBodies (TestCases/TestCases.cpp:251-294): the case's body (one for the
+ * body cases, none otherwise) at rest, scaled by body_scale (--body-scale). */
+int alh_make_bodies(const alh_case *cs, double body_scale, ab_rigid_body *bodies,
+                    int *nbodies);
+/* Particles of a body case: the lattice minus the particles inside a body or
+ * within 0.4 dx of its surface, plus the surface particles
+ * (embedRigidBodies, Initialization/BodyEmbedding.cpp:18-104). */
+int alh_body_case_size(const alh_case *cs, double body_scale, int *n);
+/* generateParticles + embedRigidBodies + initWallData + applyLidProfile +
+ * applyIC for a body case (main.cpp:331-345, Solver.cpp applyIC), arrays
+ * sized by alh_body_case_size: as alh_build_lattice, plus body_id (N) and the
+ * bodies (up to 1; *nbodies set). geo (may be NULL) = dx, dy, dz, minDist
+ * (CalcParameters). cfg->nx/ny/nz are 0: the cloud is not a lattice. */
+int alh_build_body_case(const alh_case *cs, double body_scale, int precision, ab_config *cfg,
+                        double *x, double *y, double *z, unsigned char *boundary, double *wall,
+                        double *mom, double *g, int *body_id, ab_rigid_body *bodies,
+                        int *nbodies, double geo[4]);
+
 /* ---- Restart files (Core/Checkpoint.hpp:37-234, format v1) -------------
  * The reference's checkpoint, read and written byte for byte: a 56-byte
  * header, N particle records of 200 bytes, N x gstride FP64 g, nbodies body

@@ -15,6 +15,9 @@
 //                                                  CPU steps with the MOOD limiter
 //   ref_harness ckptread <file> <outdir> <case> <Nx> <Nv>  restore a file, dump fields
 //   ref_harness pos2d   <outdir> <Nx> <Nv> <a> <T> <cfl>   2-D transport where positivity fires
+//   ref_harness bodycase <outdir> <case> <Nx> <Nv> <scale>   a body case's cloud and body
+//   ref_harness bodysteps <outdir> <case> <Nx> <Nv> <scale> <nsteps> <a> <order>
+//                                                  CPU steps with a free body
 //   ref_harness cloud   <outdir> <case> <Nx> <Nv> <scale> <shift> <jitter>
 //                                                  cloud management after a body move
 //
@@ -27,6 +30,7 @@
 //   initWallData             BoundaryConditions/WallState.cpp:16
 //   applyLidProfile          TestCases/TestCases.cpp:230
 //   applyIC                  Initialization/InitialConditions.cpp:18
+//   moveRigidBodies          RigidBody/BodyDynamics.cpp:143
 //   mlsTransportCompute      MLSTransport/MLSTransport.cpp:88
 //   computeMoments           MomentCalculations/Moments.cpp:17
 //   bgkCollision             MomentCalculations/Collision.cpp:70
@@ -750,6 +754,175 @@ int modePos2d(const std::string &dir, int Nx, int Nv, double a, double T, double
     return 0;
 }
 
+// Body record of the harness fixtures: shape (AB_BODY_* numbering), Npts,
+// mass, T, size, centre (3), orientation (2), Rm (9), velocity (3),
+// angularVel, omega (3), force (3), torque vector (3), moment of inertia,
+// surface area per particle: 36 values.
+std::vector<double> bodyRecord(const RigidBody &b)
+{
+    std::vector<double> r(36, 0.0);
+    const double *Rm = b.orientation();
+    if (auto *q = dynamic_cast<const Square *>(&b)) { r[0] = 1; r[4] = q->L; }
+    else if (auto *q = dynamic_cast<const Circle *>(&b)) { r[0] = 2; r[4] = q->R; }
+    else if (auto *q = dynamic_cast<const Sphere *>(&b)) { r[0] = 3; r[4] = q->R; }
+    else if (auto *q = dynamic_cast<const Cube *>(&b)) { r[0] = 4; r[4] = q->L; }
+    r[1] = b.Npts; r[2] = b.mass; r[3] = b.T;
+    r[5] = b.cx; r[6] = b.cy; r[7] = b.cz; r[8] = b.ox; r[9] = b.oy;
+    for (int k = 0; k < 9; ++k) r[10 + k] = Rm ? Rm[k] : (k % 4 == 0 ? 1.0 : 0.0);
+    r[19] = b.velx; r[20] = b.vely; r[21] = b.velz; r[22] = b.angularVel;
+    r[23] = b.omx; r[24] = b.omy; r[25] = b.omz;
+    r[26] = b.forcex; r[27] = b.forcey; r[28] = b.forcez;
+    r[29] = b.tqx; r[30] = b.tqy; r[31] = b.tqz;
+    r[32] = b.getMomentOfInertia(); r[33] = b.getSurfaceArea();
+    return r;
+}
+
+// The cloud of a body case as main() builds it (embedRigidBodies, wall
+// state, parabolic lid, applyIC) and its body.
+int modeBodyCase(const std::string &dir, const std::string &cname, int Nx, int Nv, double scale)
+{
+    Setup S;
+    build(S, caseOf(cname), Nx, Nv, scale);
+    const size_t N = S.P.size(), gs = S.P[0].g.size();
+    writeMeta(dir, S, 0.0);
+    writeCloud(dir, S);
+    // applyIC gives every particle the same row: keep the first lattice
+    // particle's and the last (body-surface) particle's
+    std::vector<double> rows(S.P[0].g);
+    rows.insert(rows.end(), S.P[N - 1].g.begin(), S.P[N - 1].g.end());
+    npyF8(dir + "/g_ic_rows.npy", rows, {2, gs});
+    npyF8(dir + "/mom_ic.npy", moments(S.P), {N, 5});
+    npyF8(dir + "/body.npy", bodyRecord(*S.bodies[0]), {36});
+    std::vector<double> par{scale};
+    npyF8(dir + "/params.npy", par, {1});
+    std::printf("wrote %s: N=%zu\n", dir.c_str(), N);
+    return 0;
+}
+
+// nsteps of the reference's CPU time loop with a free body
+// (Solver/Solver.cpp:632-714: transport, MOOD, commit, mass rescale, wall,
+// moments, collision, moments, post-collision check; moveRigidBodies;
+// manageRemove, mergeCloseParticles, manageRefill with the neighbour
+// rebuilds) from opposing streams u_x = +-a (left / right of the middle)
+// laid over the gas, so the body feels a shear from the first step. Writes
+// the start (g0, mom0, wall0), the body record after every step (steps x 36)
+// with the cloud counts (steps x 4: N, removed, merged, added), and the end
+// cloud (x1, y1, z1, boundary1, bodyid1, wall1, mesh1, g1, mom1).
+int modeBodySteps(const std::string &dir, const std::string &cname, int Nx, int Nv,
+                  double scale, int nsteps, double a, int order)
+{
+    Setup S;
+    build(S, caseOf(cname), Nx, Nv, scale);
+    S.sp.mlsOrder = order;
+    auto &P = S.P;
+    const SimParameters &sp = S.sp;
+    const double dt = sp.dt, R = S.gc.R;
+    const double xmid = 0.5 * (S.dom.xleft + S.dom.xright);
+    for (auto &p : P)
+    {
+        const double ux = (p.x < xmid) ? a : -a, rho = S.BC.rho, T = S.BC.Tw;
+        const int n = p.Nv_local, n2 = n * n;
+        for (int kv = 0; kv < (p.dim == 3 ? n : 1); ++kv)
+            for (int jv = 0; jv < n; ++jv)
+                for (int iv = 0; iv < n; ++iv)
+                {
+                    const double vx = p.vlo + iv * p.dv_local, vy = p.vlo + jv * p.dv_local;
+                    const double vz = p.vlo + kv * p.dv_local;
+                    if (p.dim == 3)
+                        p.g[iv + n * (jv + n * kv)] = maxwellian3D(vx, vy, vz, ux, 0.0, 0.0, T, rho, R);
+                    else
+                    {
+                        const double g1 = maxwellianG1(vx, vy, ux, 0.0, T, rho, R);
+                        p.g[iv + n * jv] = g1;
+                        p.g[iv + n * jv + n2] = 0.5 * R * T * g1;
+                    }
+                }
+    }
+    computeMoments(P, R);
+    auto wallOf = [&]() {
+        std::vector<double> w(P.size() * 7);
+        for (size_t i = 0; i < P.size(); ++i)
+        {
+            const double q[7] = {P[i].wnx, P[i].wny, P[i].wnz, P[i].wUx, P[i].wUy, P[i].wUz, P[i].wT};
+            std::memcpy(&w[7 * i], q, sizeof q);
+        }
+        return w;
+    };
+    writeMeta(dir, S, dt);
+    writeCloud(dir, S);
+    {
+        const size_t N = P.size(), gs = P[0].g.size();
+        npyF8(dir + "/g0.npy", gatherG(P, false), {N, gs});
+        npyF8(dir + "/mom0.npy", moments(P), {N, 5});
+        npyF8(dir + "/wall0.npy", wallOf(), {N, 7});
+        npyF8(dir + "/body0.npy", bodyRecord(*S.bodies[0]), {36});
+    }
+    std::vector<double> hist;
+    std::vector<int32_t> counts;
+    for (int st = 0; st < nsteps; ++st)
+    {
+        double massOld = sumMass(P);
+        mlsTransportCompute(P, dt, S.cp, S.dom, S.BC, sp.mlsOrder);
+        if (sp.limiter == LimiterMode::MOOD)
+        {
+            int nf = checkTransportPAD(P);
+            if (nf > 0) mlsTransportCompute(P, dt, S.cp, S.dom, S.BC, 1, true);
+        }
+        mlsTransportCommit(P);
+        double massNew = sumMass(P);
+        if (massNew > 1.0e-30) scaleG(P, massOld / massNew);
+        applyGravity(P, sp.gravityY, dt);
+        applyBoundaryBC(P, R, S.dom, S.BC);
+        computeMoments(P, R);
+        bgkCollision(P, dt, sp.tao, R, sp.tauVariable, sp.d);
+        computeMoments(P, R);
+        if (sp.limiter == LimiterMode::MOOD && moodCheck(P))
+        {
+            bgkCollision(P, dt, sp.tao, R, sp.tauVariable, sp.d);
+            computeMoments(P, R);
+        }
+        moveRigidBodies(P, S.bodies, dt, S.dom, S.cp);
+        const int removed = manageRemove(P, S.bodies, S.cp);
+        updateVoxelsAndNeighbours(P, S.cp, S.dom);
+        const int merged = mergeCloseParticles(P, S.cp);
+        if (merged > 0) updateVoxelsAndNeighbours(P, S.cp, S.dom);
+        const int added = manageRefill(P, S.bodies, S.sp, S.cp, S.dom, R);
+        if (added > 0) updateVoxelsAndNeighbours(P, S.cp, S.dom);
+        S.cp.N = static_cast<int>(P.size());
+        const std::vector<double> r = bodyRecord(*S.bodies[0]);
+        hist.insert(hist.end(), r.begin(), r.end());
+        counts.insert(counts.end(), {(int32_t)P.size(), removed, merged, added});
+    }
+    npyF8(dir + "/body_hist.npy", hist, {(size_t)nsteps, 36});
+    npyI4(dir + "/counts.npy", counts, {(size_t)nsteps, 4});
+    const size_t N = P.size(), gs = P[0].g.size();
+    std::vector<double> x(N), y(N), z(N), mesh(3 * N);
+    std::vector<uint8_t> bnd(N);
+    std::vector<int32_t> bid(N);
+    for (size_t i = 0; i < N; ++i)
+    {
+        x[i] = P[i].x; y[i] = P[i].y; z[i] = P[i].z;
+        bnd[i] = P[i].boundary ? 1 : 0; bid[i] = P[i].bodyId;
+        mesh[3 * i] = P[i].meshVx; mesh[3 * i + 1] = P[i].meshVy; mesh[3 * i + 2] = P[i].meshVz;
+    }
+    npyF8(dir + "/x1.npy", x, {N});
+    npyF8(dir + "/y1.npy", y, {N});
+    npyF8(dir + "/z1.npy", z, {N});
+    npyU1(dir + "/boundary1.npy", bnd, {N});
+    npyI4(dir + "/bodyid1.npy", bid, {N});
+    npyF8(dir + "/wall1.npy", wallOf(), {N, 7});
+    npyF8(dir + "/mesh1.npy", mesh, {N, 3});
+    npyF8(dir + "/g1.npy", gatherG(P, false), {N, gs});
+    npyF8(dir + "/mom1.npy", moments(P), {N, 5});
+    std::vector<double> par{scale, a, (double)order, (double)nsteps};
+    npyF8(dir + "/params.npy", par, {4});
+    const std::vector<double> r = bodyRecord(*S.bodies[0]);
+    std::printf("wrote %s: N=%zu, body at (%.6e, %.6e, %.6e), v (%.4e, %.4e, %.4e), "
+                "w (%.4e, %.4e, %.4e / %.4e)\n", dir.c_str(), N, r[5], r[6], r[7], r[19], r[20],
+                r[21], r[23], r[24], r[25], r[22]);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 1; }
@@ -772,6 +945,13 @@ int main(int argc, char **argv)
     if (mode == "cloud" && argc == 9)
         return modeCloud(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
                          std::atof(argv[6]), std::atof(argv[7]), std::atof(argv[8]));
+    if (mode == "bodycase" && argc == 7)
+        return modeBodyCase(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
+                            std::atof(argv[6]));
+    if (mode == "bodysteps" && argc == 10)
+        return modeBodySteps(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
+                             std::atof(argv[6]), std::atoi(argv[7]), std::atof(argv[8]),
+                             std::atoi(argv[9]));
     if (mode == "pos2d" && argc == 8)
         return modePos2d(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atof(argv[5]),
                          std::atof(argv[6]), std::atof(argv[7]));

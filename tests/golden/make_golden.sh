@@ -29,6 +29,16 @@ H="$ROOT/oracle/_ref/ref_harness"
 "$H" cloud "$HERE/cloud_cube_n12_v3" cube 12 3 1.5 1.1 0.3
 "$H" cloud "$HERE/cloud_circle_n24_v6" circle 24 6 2.0 1.4 0.3
 "$H" cloud "$HERE/cloud_square_n24_v6" square 24 6 2.0 1.2 0.3
+# the cloud of each body case as main() builds it (embedding, wall state,
+# parabolic lid, initial state) and its body
+"$H" bodycase "$HERE/body_cube_n12_v4" cube 12 4 2.0
+"$H" bodycase "$HERE/body_sphere_n12_v4" sphere 12 4 2.0
+"$H" bodycase "$HERE/body_circle_n24_v6" circle 24 6 2.0
+"$H" bodycase "$HERE/body_square_n24_v6" square 24 6 2.0
+# CPU steps with a free body in opposing streams (u_x = +-300 m/s): the
+# body state after every step and the cloud at the end
+"$H" bodysteps "$HERE/bodysteps_cube_n12_v4" cube 12 4 2.0 40 300 1
+"$H" bodysteps "$HERE/bodysteps_circle_n24_v6" circle 24 6 2.0 40 300 1
 rm -f "$HERE"/ckpt_*/neigh_flat.npy "$HERE"/ckpt_*/nneigh.npy
 ( cd "$HERE" && find . -name '*.npy' -o -name 'meta.json' -o -name '*.ckpt' | sort | xargs sha256sum > MANIFEST.sha256 )
 echo "fixtures written under $HERE"

@@ -124,7 +124,10 @@ def test_graph_refuses_replay_after_cloud_changes():
     g.launch(); g.synchronize()
     dx = 1e-6 / (case.nx - 1)
     dom = (case.dom[0], case.dom[1], case.dom[3], case.dom[2], case.dom[4], case.dom[5])
-    s.cloud_manage([], np.full(s.N, -1, np.int32), dx, 0.0, dom)
+    # a sphere in the middle of the box overruns the particles inside it
+    src, counts = s.cloud_manage([A.body(A.BODY_SPHERE, (5e-7, 5e-7, 5e-7), 2.2 * dx)],
+                                 np.full(s.N, -1, np.int32), dx, 0.0, dom)
+    assert counts[0] > 0
     with pytest.raises(A.ALEBGKError):
         g.launch()
     g.close()

@@ -127,3 +127,59 @@ def test_check_config_refuses_table_span_beyond_2gib():
     cfg.dim = 2
     cfg.collision = A.COLLISION_DISCRETE
     assert A.check_config(cfg)[0] == 4
+
+
+@pytest.mark.parametrize("name,case,nx,nv", [("body_cube_n12_v4", "cavity3d-cube", 12, 4),
+                                             ("body_sphere_n12_v4", "cavity3d-sphere", 12, 4),
+                                             ("body_circle_n24_v6", "cavity2d-circle", 24, 6),
+                                             ("body_square_n24_v6", "cavity2d-square", 24, 6)])
+def test_host_body_case_matches_reference(name, case, nx, nv):
+    """makeBodies + embedRigidBodies + initWallData + applyLidProfile +
+    applyIC for the four body cases at body scale 2: the same particles in
+    the same order, the same flags, body ids, wall state (the parabolic lid
+    and the body normals included), initial state and body, bit for bit."""
+    fx = load_fixture(name)
+    cs = A.configure(case, nx=nx, nv=nv)
+    cl = A.build_case(cs, body_scale=float(fx["params"][0]))
+    for k in ("x", "y", "z"):
+        assert np.array_equal(getattr(cl, k), fx[k]), k
+    assert np.array_equal(cl.boundary, fx["boundary"])
+    assert np.array_equal(cl.body_id, fx["bodyid"])
+    assert np.array_equal(cl.wall, fx["wall"])
+    assert np.array_equal(cl.mom, fx["mom_ic"])
+    rows = fx["g_ic_rows"]
+    assert np.array_equal(cl.g[0], rows[0]) and np.array_equal(cl.g[-1], rows[1])
+    assert np.all(cl.g == cl.g[0])
+    b = cl.bodies[0]
+    r = fx["body"]
+    assert (b.shape, b.npts) == (int(r[0]), int(r[1]))
+    assert (b.mass, b.T, b.size) == (r[2], r[3], r[4])
+    assert list(b.c) == list(r[5:8]) and list(b.o) == list(r[8:10])
+    assert list(b.rm) == list(r[10:19])
+    m = fx.meta
+    assert cl.geo[0] == m["dx"] and cl.geo[3] == m["minDist"]
+    assert cl.cfg.radius == m["radius"] and cl.cfg.n_particles == m["N"]
+    # the lid: a parabolic profile peaking at 10 m/s on y = ytop
+    lid = (cl.boundary == 1) & (cl.body_id < 0) & (np.abs(cl.y - 1e-6) <= 1e-13)
+    assert lid.sum() > 0 and cl.wall[lid, 3].max() <= 10.0
+
+
+def test_lid_parabolic_oracle_matches_reference():
+    """orc_lid_parabolic against applyLidProfile's output in the body-case
+    fixtures: the oracle, given the lattice and the uniform wall state,
+    writes exactly the reference's lid velocities, and touches neither the
+    body surface nor any other wall."""
+    import oracle_ctypes as O
+    for name, nx in (("body_cube_n12_v4", 12), ("body_circle_n24_v6", 24)):
+        fx = load_fixture(name)
+        m = fx.meta
+        dim = m["dim"]
+        geo, *_ = O.lattice(dim, nx, (m["xleft"], m["xright"], m["ytop"], m["ybottom"],
+                                      m["zfront"], m["zback"]), Ny=nx, Nz=nx if dim == 3 else 1)
+        want = fx["wall"][:, 3]
+        wux = np.where((fx["boundary"] == 1) & (fx["bodyid"] < 0) &
+                       (np.abs(fx["y"] - m["ytop"]) <= 1e-13), 0.0, want)
+        got = np.ascontiguousarray(wux.copy())
+        O.lid_parabolic(geo, fx["x"], fx["y"], fx["boundary"], fx["bodyid"], 10.0, got)
+        assert np.array_equal(got, want), name
+        assert np.count_nonzero(got != wux) > 0
