@@ -132,6 +132,10 @@ def test_body_loads_on_device_match_host_sum():
     computeBodyLoads formula, BodyDynamics.cpp:41-94), for the current wall
     velocities and for given ones."""
     fx, cs, cl, nsteps = _start("bodysteps_cube_n12_v4")
+    # the streams load the cube symmetrically (the net force nearly cancels):
+    # a random factor per value gives loads of their own size
+    rng = np.random.default_rng(5)
+    cl.g = np.ascontiguousarray(cl.g * (1.0 + 0.3 * rng.uniform(-1, 1, size=cl.g.shape)))
     sol = A.Solver(cl, precision=A.PREC_FP64)
     surf = np.nonzero((cl.boundary == 1) & (cl.body_id >= 0))[0].astype(np.int32)
     b = cl.bodies[0]
@@ -151,12 +155,13 @@ def test_body_loads_on_device_match_host_sum():
             F[3:] += area * np.cross(r, t)
         return F
 
-    rng = np.random.default_rng(3)
     wu = rng.normal(scale=5.0, size=(len(surf), 3))
     got0 = sol.body_loads(surf, np.zeros(len(surf), np.int32), [b.c[:]], [area])[0]
     got1 = sol.body_loads(surf, np.zeros(len(surf), np.int32), [b.c[:]], [area], wu=wu)[0]
     for got, want in ((got0, host(cl.wall[surf, 3:6])), (got1, host(wu))):
-        assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+        _say(f"  device {got}\n  host   {want}")
+        assert np.abs(got[:3] - want[:3]).max() <= 1e-11 * np.abs(want[:3]).max()
+        assert np.abs(got[3:] - want[3:]).max() <= 1e-11 * np.abs(want[3:]).max()
 
 
 def test_cube_case_gpu_semantics_runs():

@@ -201,10 +201,12 @@ def test_mixed_metric_size_properties():
     assert np.all(np.isfinite(g))
     mom = sol.moments()
     assert np.all(mom[:, 0] > 0) and np.all(np.isfinite(mom))
-    # the lid-edge particles trip the positivity repair every step; FP32
-    # storage may flip a borderline particle, no more
+    # from rest the NAD clamp fires on most particles in the first step and
+    # on the lid's neighbourhood after (54,872 then 15,324, 13,080, 684, 496
+    # measured), the positivity repair on none; FP32 storage may flip a
+    # borderline particle, no more
     for pm, pr, nm, nr in npos:
-        assert pr > 0 and abs(pm - pr) <= 2 and abs(nm - nr) <= 2
+        assert nr > 0 and abs(pm - pr) <= 2 and abs(nm - nr) <= 2
 
 
 # ------------------------------------------------------------- body loads ----
