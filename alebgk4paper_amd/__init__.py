@@ -167,7 +167,8 @@ LIMITER_GPU = 0          # the reference's GPU step (cudaStep)
 LIMITER_MOOD = 1         # its CPU step with the MOOD limiter (runSolver)
 HOST_EXPORTS = ["alh_configure", "alh_lattice_size", "alh_build_lattice", "alh_ckpt_info_read",
                 "alh_ckpt_read", "alh_ckpt_write", "alh_make_bodies", "alh_body_case_size",
-                "alh_build_body_case"]
+                "alh_build_body_case", "alh_diagnostics", "alh_write_vtk", "alh_write_timeseries",
+                "alh_format_step", "alh_write_bodies", "alh_guard_init", "alh_guard_check"]
 
 
 def hip_lib():
@@ -318,6 +319,83 @@ def build_case(case, precision=PREC_FP64, mls_order=None, body_scale=1.0):
         raise ALEBGKError(f"alh_build_body_case: {ERRORS.get(rc, rc)}")
     return Cloud(cfg, x, y, z, b, wall, mom, g, body_id=bid,
                  bodies=[arr[k] for k in range(nb.value)], geo=tuple(geo))
+
+
+class Diag(C.Structure):
+    """alh_diag: computeDiagnostics' means over the interior particles."""
+    _fields_ = [("rho_avg", C.c_double), ("KE", C.c_double), ("T_avg", C.c_double),
+                ("p_avg", C.c_double), ("mood_count", C.c_int), ("n", C.c_int)]
+
+
+def diagnostics(boundary, rho, ux, uy, uz, T, p, mood):
+    """computeDiagnostics (Core/Output.hpp:65-84) through alh_diagnostics."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (rho, ux, uy, uz, T, p)]
+    b = np.ascontiguousarray(boundary, dtype=np.uint8)
+    m = np.ascontiguousarray(mood, dtype=np.uint8)
+    d = Diag()
+    rc = host_lib().alh_diagnostics(C.c_int(len(b)), _ptr(b, _up), *[_ptr(v, _dp) for v in a],
+                                    _ptr(m, _up), C.byref(d))
+    if rc:
+        raise ALEBGKError(f"alh_diagnostics: {ERRORS.get(rc, rc)}")
+    return d
+
+
+def write_vtk(out_dir, step, t, x, y, z, rho, T, p, nv, mood, boundary, ux, uy, uz):
+    """writeVTK (Core/Output.hpp:17-51): out_dir/output_<step>.vtk."""
+    f8 = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z, rho, T, p)]
+    u8 = [np.ascontiguousarray(v, dtype=np.uint8) for v in (mood, boundary)]
+    uu = [np.ascontiguousarray(v, dtype=np.float64) for v in (ux, uy, uz)]
+    rc = host_lib().alh_write_vtk(str(out_dir).encode(), C.c_int(step), C.c_double(t),
+                                  C.c_int(len(f8[0])), *[_ptr(v, _dp) for v in f8], C.c_int(nv),
+                                  *[_ptr(v, _up) for v in u8], *[_ptr(v, _dp) for v in uu])
+    if rc:
+        raise ALEBGKError(f"alh_write_vtk: {ERRORS.get(rc, rc)}")
+
+
+def write_timeseries(path, first, t, d):
+    """writeTimeseries (Core/Output.hpp:89-106): one CSV row."""
+    rc = host_lib().alh_write_timeseries(str(path).encode(), C.c_int(int(first)), C.c_double(t),
+                                         C.byref(d))
+    if rc:
+        raise ALEBGKError(f"alh_write_timeseries: {ERRORS.get(rc, rc)}")
+
+
+def format_step(step, t, dt, d):
+    """printStep's line (Core/Output.hpp:111-115)."""
+    buf = C.create_string_buffer(160)
+    host_lib().alh_format_step(C.c_int(step), C.c_double(t), C.c_double(dt), C.byref(d), buf,
+                               C.c_int(len(buf)))
+    return buf.value.decode()
+
+
+def write_bodies(path, first, t, dim, bodies):
+    """writeBodyLog (Solver/Solver.cpp:96-121): one bodies.csv row per body."""
+    arr = (ABRigidBody * max(len(bodies), 1))(*bodies)
+    rc = host_lib().alh_write_bodies(str(path).encode(), C.c_int(int(first)), C.c_double(t),
+                                     C.c_int(dim), C.c_int(len(bodies)), arr)
+    if rc:
+        raise ALEBGKError(f"alh_write_bodies: {ERRORS.get(rc, rc)}")
+
+
+class Guard(C.Structure):
+    """stability::Guard (Core/Stability.hpp:46-123) through alh_guard_*: the
+    bounds and the ALEBGK_GUARD_* / ALEBGK_NO_GUARD overrides are read when
+    it is made."""
+    _fields_ = [("Tw", C.c_double), ("rho0", C.c_double), ("ke_ref", C.c_double),
+                ("have_ke", C.c_int), ("enabled", C.c_int), ("t_lo", C.c_double),
+                ("t_hi", C.c_double), ("rho_lo", C.c_double), ("rho_hi", C.c_double),
+                ("ke_hi", C.c_double)]
+
+    def __init__(self, wall_temp, rho0):
+        super().__init__()
+        host_lib().alh_guard_init(C.byref(self), C.c_double(wall_temp), C.c_double(rho0))
+
+    def diverged(self, step, t, d):
+        """(True, reason) once the diagnostics leave the bounds."""
+        buf = C.create_string_buffer(256)
+        r = host_lib().alh_guard_check(C.byref(self), C.c_int(step), C.c_double(t), C.byref(d),
+                                       buf, C.c_int(len(buf)))
+        return bool(r), buf.value.decode()
 
 
 def box(case):

@@ -708,9 +708,13 @@ int ab_rebuild_neighbours(ab_ctx *c)
     }
     Timer tm(c);
     tm.tic();
-    if (launch_neighbours(c->view(), f, c->vox, c->vcount, c->vstart, c->vlist, nvox, c->neigh,
-                          c->nneigh, c->stream))
-    { c->err = "neighbour rebuild launch failed"; return AB_ERR_HIP; }
+    if (int e = launch_neighbours(c->view(), f, c->vox, c->vcount, c->vstart, c->vlist, nvox,
+                                  c->neigh, c->nneigh, c->stream))
+    {
+        c->err = std::string("neighbour rebuild launch failed: ") +
+                 hipGetErrorString((hipError_t)e);
+        return AB_ERR_HIP;
+    }
     tm.toc(ST_NEIGH);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // the table transports take their stencils from the lists: check the

@@ -532,32 +532,53 @@ void rotate3(double *Rm, double wx, double wy, double wz, double dt)
 
 using namespace ab;
 
-extern "C" int ab_body_loads(ab_ctx *c, int nsurf, const int *surf_idx, const int *surf_body,
-                             int nbodies, const double *body_c, const double *body_area,
-                             const double *wu, double *out)
+namespace ab {
+namespace {
+
+// Device buffers of one ab_body_loads call.
+struct LoadBufs
+{
+    int *idx, *body;
+    double *wu, *mom, *con, *par, *out;
+};
+
+int check_loads_args(ab_ctx *c, int nsurf, const int *surf_idx, const int *surf_body,
+                     int nbodies, const double *body_c, const double *body_area, double *out,
+                     const char *who)
 {
     if (!c || nsurf < 0 || nbodies <= 0 || nbodies > kMaxBodies || !body_c || !body_area ||
         !out || (nsurf > 0 && (!surf_idx || !surf_body)))
     {
-        if (c) c->err = "ab_body_loads: invalid argument (at most 16 bodies)";
+        if (c) c->err = std::string(who) + ": invalid argument (at most 16 bodies)";
         return AB_ERR_ARG;
     }
     if (!c->uploaded) { c->err = "no cloud uploaded"; return AB_ERR_STATE; }
     for (int k = 0; k < nsurf; ++k)
         if (surf_idx[k] < 0 || surf_idx[k] >= c->N || surf_body[k] < 0 || surf_body[k] >= nbodies)
-        { c->err = "ab_body_loads: surface index or body id out of range"; return AB_ERR_ARG; }
+        {
+            c->err = std::string(who) + ": surface index or body id out of range";
+            return AB_ERR_ARG;
+        }
+    return AB_OK;
+}
+
+// The loads of ab_body_loads on the given buffers: the surface list (and
+// the wall velocities) copied in, moments, contributions, per-body sums.
+// Every buffer is written before it is read.
+int body_loads_on(ab_ctx *c, const LoadBufs &B, int nsurf, const int *surf_idx,
+                  const int *surf_body, int nbodies, const double *body_c,
+                  const double *body_area, const double *wu, double *out)
+{
     std::memset(out, 0, sizeof(double) * 6 * (size_t)nbodies);
-    int rc;
-    if ((rc = body_scratch(c, nsurf))) return rc;
     hipStream_t s = c->stream;
     if (nsurf > 0)
     {
-        BCHK(c, hipMemcpyAsync(c->bidx, surf_idx, sizeof(int) * nsurf, hipMemcpyHostToDevice, s));
-        BCHK(c, hipMemcpyAsync(c->bbody, surf_body, sizeof(int) * nsurf, hipMemcpyHostToDevice, s));
+        BCHK(c, hipMemcpyAsync(B.idx, surf_idx, sizeof(int) * nsurf, hipMemcpyHostToDevice, s));
+        BCHK(c, hipMemcpyAsync(B.body, surf_body, sizeof(int) * nsurf, hipMemcpyHostToDevice, s));
         if (wu)
-            BCHK(c, hipMemcpyAsync(c->bwu, wu, sizeof(double) * 3 * nsurf, hipMemcpyHostToDevice, s));
+            BCHK(c, hipMemcpyAsync(B.wu, wu, sizeof(double) * 3 * nsurf, hipMemcpyHostToDevice, s));
     }
-    launch_moments(c, c->bidx, nsurf, c->bmom);
+    launch_moments(c, B.idx, nsurf, B.mom);
     LoadBody lb[kMaxBodies];
     std::memset(lb, 0, sizeof lb);
     for (int b = 0; b < nbodies; ++b)
@@ -565,8 +586,23 @@ extern "C" int ab_body_loads(ab_ctx *c, int nsurf, const int *surf_idx, const in
         for (int a = 0; a < 3; ++a) lb[b].c[a] = body_c[3 * b + a];
         lb[b].area = body_area[b];
     }
-    return loads(c, c->bidx, c->bbody, nsurf, c->bmom, wu ? c->bwu : nullptr, wu ? 1 : 0, lb,
-                 nbodies, out, c->bcon, c->bpar, c->bout);
+    return loads(c, B.idx, B.body, nsurf, B.mom, wu ? B.wu : nullptr, wu ? 1 : 0, lb, nbodies,
+                 out, B.con, B.par, B.out);
+}
+
+}  // namespace
+}  // namespace ab
+
+extern "C" int ab_body_loads(ab_ctx *c, int nsurf, const int *surf_idx, const int *surf_body,
+                             int nbodies, const double *body_c, const double *body_area,
+                             const double *wu, double *out)
+{
+    int rc = check_loads_args(c, nsurf, surf_idx, surf_body, nbodies, body_c, body_area, out,
+                              "ab_body_loads");
+    if (rc) return rc;
+    if ((rc = body_scratch(c, nsurf))) return rc;
+    const LoadBufs B{c->bidx, c->bbody, c->bwu, c->bmom, c->bcon, c->bpar, c->bout};
+    return body_loads_on(c, B, nsurf, surf_idx, surf_body, nbodies, body_c, body_area, wu, out);
 }
 
 extern "C" int ab_body_step(ab_ctx *c, double dt, int nb, ab_rigid_body *bodies, double dx,

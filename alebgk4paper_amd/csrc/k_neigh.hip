@@ -154,14 +154,15 @@ int launch_neighbours(const DevCloud &c, const ab_config &cfg, int *vox, int *vc
     vg.nbh = cfg.nbhbox; vg.nbv = cfg.nbvbox; vg.nbw = (c.dim == 3) ? cfg.nbwbox : 1;
     vg.radius = cfg.radius;
     const int N = c.N, bs = 256;
-    if (hipMemsetAsync(vcount, 0, sizeof(int) * (size_t)nvox, s) != hipSuccess) return -1;
+    hipError_t e = hipMemsetAsync(vcount, 0, sizeof(int) * (size_t)nvox, s);
+    if (e != hipSuccess) return (int)e;
     k_voxel<<<(N + bs - 1) / bs, bs, 0, s>>>(vg, N, c.x, c.y, c.z, vox, vcount);
     k_scan<<<1, 1024, 0, s>>>(nvox, vcount, vstart);
     k_scatter<<<(N + bs - 1) / bs, bs, 0, s>>>(N, vox, vcount, vstart, vlist);
     k_sort_segments<<<(nvox + bs - 1) / bs, bs, 0, s>>>(nvox, vstart, vlist);
     k_search<<<(N + bs - 1) / bs, bs, 0, s>>>(vg, N, c.x, c.y, c.z, vox, vstart, vlist, neigh,
                                               nneigh);
-    return (hipGetLastError() == hipSuccess) ? 0 : -1;
+    return (int)hipGetLastError();
 }
 
 }  // namespace ab

@@ -60,17 +60,21 @@ def measured_traffic(nx, nv):
     return None, None
 
 
-def cpu_baseline(nx=28, nv=10, order=2):
+def cpu_baseline(nx=40, nv=10, order=2):
     """The reference CPU step (oracle/_ref, built from the reference sources)
-    on a bounded sample: cavity3d at nx^3, Nv=10, quadratic MLS, two steps;
-    the steady second step is timed (step 1 carries the MOOD re-transport at
-    uniform density). Falls back to the oracle port on a 12^3 sample."""
+    on the workload itself: cavity3d at nx^3, Nv=10, quadratic MLS, two
+    steps on every host core this process may use (its CPU affinity, or
+    OMP_NUM_THREADS where the host sets a smaller share); the steady second
+    step is timed (step 1 carries the MOOD re-transport at uniform density).
+    Falls back to the oracle port on a 12^3 sample."""
     threads = os.cpu_count() or 1
     try:
         threads = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
-    threads = min(threads, 16)
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        threads = min(threads, int(share))
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_v3")
     if os.path.exists(exe):
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
@@ -82,9 +86,18 @@ def cpu_baseline(nx=28, nv=10, order=2):
             out = json.loads(r.stdout.strip().splitlines()[-1])
             sec = out["step_seconds"][-1]
             upd = out["N"] * nv ** 3
+            model = ""
+            try:
+                for ln in open("/proc/cpuinfo"):
+                    if ln.startswith("model name"):
+                        model = ln.split(":", 1)[1].strip()
+                        break
+            except OSError:
+                pass
             return {"value": upd / sec, "unit": "phase-space updates/s",
                     "cores": out["threads"], "kind": "reference",
-                    "sample": (f"reference CPU step (OpenMP, x86-64-v3) on cavity3d {nx}^3 "
+                    "sample": (f"reference CPU step (OpenMP, x86-64-v3, {out['threads']} threads"
+                               f"{', ' + model if model else ''}) on cavity3d {nx}^3 "
                                f"(N={out['N']}), Nv={nv}, quadratic MLS; steady step 2 "
                                f"timed: {sec:.2f} s; wall {time.time() - t0:.1f} s")}
         log(f"[bench] reference CPU baseline failed: {r.stderr[-400:]}")
@@ -150,6 +163,83 @@ def cloud_management(A, np):
     return res
 
 
+def production(A, steps=10):
+    """cavity3d at production size: Nx=60 (216,000 particles) x Nv=20 (8,000
+    nodes, 1.73e9 values, 6.9 GB in FP32), quadratic MLS, lid 1 m/s, the
+    mixed step with the reference's continuous collision: step time and the
+    transport contraction's share of the HBM roofline (algorithmic bytes:
+    g read once and gt written once, FP32)."""
+    case = A.configure("cavity3d", nx=60, nv=20, mls_order=2)
+    cl = A.build_lattice(case, precision=A.PREC_MIXED, mls_order=2)
+    N = cl.N
+    s = A.Solver(cl, precision=A.PREC_MIXED)
+    del cl
+    s.steps(case.dt, 3)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.steps(case.dt, steps)
+    s.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    s.profile(True)
+    s.steps(case.dt, 5)
+    prof = s.profile_read()
+    s.profile(False)
+    tr_ms, tr_n = prof["transport"]
+    tr = tr_ms / max(tr_n, 1) * 1e-3
+    upd = N * 20 ** 3
+    alg = upd * 2 * 4
+    mom = s.moments()
+    s.close()
+    res = {"ms_per_step": sec * 1e3, "updates_per_s": upd / sec, "particles": N,
+           "velocity_nodes": 8000, "transport_ms": tr * 1e3,
+           "transport_GBps": alg / tr / 1e9, "transport_hbm_frac": alg / tr / 1e9 / HBM_PEAK_GBS,
+           "finite": bool(__import__("numpy").isfinite(mom).all())}
+    log(f"[bench] cavity3d 60^3 Nv20 mixed: {sec * 1e3:.3f} ms/step, transport "
+        f"{tr * 1e3:.3f} ms ({res['transport_hbm_frac']:.3f} of HBM peak)")
+    return res
+
+
+def cube_case(A, np, steps=10):
+    """cavity3d-cube at Nx=50, Nv=10 (density 4.4, parabolic lid peaking at
+    10 m/s, a free cube, dt 1.5e-11), quadratic MLS: the FP64 step (a body
+    cloud is no lattice) followed by the device moving-body step (Heun loads,
+    Newton-Euler, contact, surface wall state, cloud management) every step."""
+    cs = A.configure("cavity3d-cube", nx=50, nv=10, mls_order=2)
+    cl = A.build_case(cs, mls_order=2)
+    s = A.Solver(cl, precision=A.PREC_FP64)
+    bodies = cl.bodies
+    box = A.box(cs)
+
+    def one():
+        s.step(cs.dt)
+        return s.body_step(cs.dt, bodies, cl.geo[0], cl.geo[3], box)[1]
+
+    for _ in range(2):
+        one()
+    s.synchronize()
+    t0 = time.perf_counter()
+    tot = np.zeros(3, int)
+    for _ in range(steps):
+        tot += np.array(one())
+    s.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step(cs.dt)
+    s.synchronize()
+    fluid = (time.perf_counter() - t0) / steps
+    upd = s.N * 1000
+    st = bodies[0].state()
+    s.close()
+    res = {"ms_per_step": sec * 1e3, "fluid_ms_per_step": fluid * 1e3,
+           "body_block_ms": (sec - fluid) * 1e3, "updates_per_s": upd / sec,
+           "particles": int(s.N), "removed_merged_added": tot.tolist(),
+           "body_omega": st["om"].tolist(), "body_velocity": st["vel"].tolist()}
+    log(f"[bench] cavity3d-cube 50^3 Nv10 fp64 + moving body: {sec * 1e3:.2f} ms/step "
+        f"(fluid {fluid * 1e3:.2f}, body block {(sec - fluid) * 1e3:.2f})")
+    return res
+
+
 def components(A):
     """The other rows of the path, measured beside the headline: the neighbour
     rebuild on the 40^3 cloud, cloud management around a moved sphere in it,
@@ -206,6 +296,14 @@ def components(A):
             log(f"[bench] cavity2d 31^2 Nv16 {tag}: {sec * 1e3:.3f} ms/step")
         except Exception as e:
             out[key] = {"error": str(e)}
+    try:
+        out["production_cavity3d_n60_nv20_mixed"] = production(A)
+    except Exception as e:                          # reported, not hidden
+        out["production_cavity3d_n60_nv20_mixed"] = {"error": str(e)}
+    try:
+        out["cavity3d_cube_n50_nv10_fp64"] = cube_case(A, np)
+    except Exception as e:                          # reported, not hidden
+        out["cavity3d_cube_n50_nv10_fp64"] = {"error": str(e)}
     rng = random.Random(20261202)
     sizes = [rng.choice([31, 40, 50]) for _ in range(16)]
     for tag, prec in (("fp64", A.PREC_FP64), ("fp64_table", A.PREC_FP64_TABLE)):
@@ -395,6 +493,18 @@ def main():
     log(f"[bench] Newton grid passes per solve: {passes}")
     pln.close()
 
+    # ---- the mixed step with the reference's continuous collision -----------
+    mcont = A.Solver(cloud, precision=A.PREC_MIXED)
+    mcont.set_transport_shape(chosen)
+    mcont.steps(dt, args.warmup)
+    mcont.synchronize()
+    t0 = time.perf_counter()
+    mcont.steps(dt, args.steps)
+    mcont.synchronize()
+    sec_mc = (time.perf_counter() - t0) / args.steps
+    log(f"[bench] mixed, continuous collision: {sec_mc * 1e3:.4f} ms/step")
+    mcont.close()
+
     # ---- the reference formulation in FP64 ----------------------------------
     n64 = args.fp64_steps if args.fp64_steps is not None else args.warmup + args.steps
     ref = A.Solver(cloud, precision=A.PREC_FP64)      # the plain FP64 step (continuous BGK)
@@ -407,14 +517,23 @@ def main():
     value64 = updates / sec64
     log(f"[bench] fp64: {sec64 * 1e3:.3f} ms/step, {value64:.4e} updates/s")
     ref.close()
-    err = None
-    if n64 == args.warmup + args.steps:
-        # accuracy within the same collision mode: FP64 with the discrete target
-        ref_dm = A.Solver(cloud, precision=A.PREC_FP64, collision=DISC)
-        ref_dm.steps(dt, n64)
-        err = moment_errors(m_mix, ref_dm.moments(), cloud.boundary == 0)
-        log(f"[bench] mixed+DM+MLP vs FP64+DM moment errors after {n64} steps: {err}")
-        ref_dm.close()
+
+    # ---- FP64 with the same algorithm (tables, discrete collision): the
+    # comparator of the accuracy check, and the speed-up of precision alone --
+    tab = A.Solver(cloud, precision=A.PREC_FP64_TABLE, collision=DISC)
+    tab.steps(dt, args.warmup)
+    tab.synchronize()
+    t0 = time.perf_counter()
+    tab.steps(dt, args.steps)
+    tab.synchronize()
+    sec_t64 = (time.perf_counter() - t0) / args.steps
+    log(f"[bench] fp64 tables + DM: {sec_t64 * 1e3:.4f} ms/step")
+    # the mixed path's moments were taken after warmup + steps steps
+    err = moment_errors(m_mix, tab.moments(), cloud.boundary == 0)
+    within = bool(max(err.values()) <= 1e-5)
+    log(f"[bench] mixed+DM+MLP vs FP64 tables+DM moment errors after "
+        f"{args.warmup + args.steps} steps: {err} (within 1e-5: {within})")
+    tab.close()
 
     comps = components(A)
     traffic, traffic_src = measured_traffic(args.nx, args.nv)
@@ -443,7 +562,12 @@ def main():
         "fp64_value": value64,
         "fp64_ms_per_step": sec64 * 1e3,
         "speedup_vs_fp64": value / value64,
+        "fp64_table_dm_ms_per_step": sec_t64 * 1e3,
+        "speedup_vs_fp64_same_algorithm": sec_t64 / (sec / args.steps),
+        "mixed_continuous_ms_per_step": sec_mc * 1e3,
         "moment_err_vs_fp64": err,
+        "moment_err_reference": "FP64 tables with the discrete collision, same steps",
+        "within_1e-5": within,
         "without_warm_start": {"value": value_plain, "ms_per_step": sec_plain * 1e3 / args.steps},
         "newton_passes_per_solve": passes,
         "mlp": {"train_seconds": train_s, "adam_steps": args.train_iters, "batch": 4096,

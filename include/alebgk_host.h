@@ -76,6 +76,47 @@ int alh_build_body_case(const alh_case *cs, double body_scale, int precision, ab
                         double *mom, double *g, int *body_id, ab_rigid_body *bodies,
                         int *nbodies, double geo[4]);
 
+/* ---- Run outputs (Core/Output.hpp:17-115, Solver/Solver.cpp:96-121) ------
+ * Written with the reference's stream formatting: byte-identical files for
+ * the same values. */
+/* computeDiagnostics: means over the interior (non-boundary) particles. */
+typedef struct alh_diag {
+    double rho_avg, KE, T_avg, p_avg;
+    int mood_count, n;
+} alh_diag;
+int alh_diagnostics(int n, const unsigned char *boundary, const double *rho, const double *ux,
+                    const double *uy, const double *uz, const double *T, const double *p,
+                    const unsigned char *mood, alh_diag *d);
+/* writeVTK: out_dir/output_<step, 6 digits>.vtk, ASCII unstructured grid of
+ * the points with rho, T, p, Nv_local, moodFlag, boundary and velocity. */
+int alh_write_vtk(const char *out_dir, int step, double t, int n, const double *x,
+                  const double *y, const double *z, const double *rho, const double *T,
+                  const double *p, int nv_local, const unsigned char *mood,
+                  const unsigned char *boundary, const double *ux, const double *uy,
+                  const double *uz);
+/* writeTimeseries: one CSV row; first != 0 truncates and writes the header. */
+int alh_write_timeseries(const char *path, int first, double t, const alh_diag *d);
+/* printStep's line (no newline) into out. */
+int alh_format_step(int step, double t, double dt, const alh_diag *d, char *out, int cap);
+/* writeBodyLog: one bodies.csv row per body; first != 0 writes the header. */
+int alh_write_bodies(const char *path, int first, double t, int dim, int nbodies,
+                     const ab_rigid_body *bodies);
+
+/* ---- Stability guard (Core/Stability.hpp:46-123) -------------------------
+ * Stops a run whose diagnostics leave physical bounds: mean T outside
+ * [0.60, 1.30] Tw, mean rho outside [0.20, 5.00] rho0, KE above 100 x its
+ * first positive value, or anything non-finite. ALEBGK_GUARD_TMIN / _TMAX /
+ * _RHOMIN / _RHOMAX / _KEMAX override the factors; ALEBGK_NO_GUARD=1 turns
+ * it off (read at alh_guard_init). */
+typedef struct alh_guard {
+    double Tw, rho0, ke_ref;
+    int have_ke, enabled;
+    double t_lo, t_hi, rho_lo, rho_hi, ke_hi;
+} alh_guard;
+int alh_guard_init(alh_guard *g, double wall_temp, double rho0);
+/* 1: diverged (the reason in why), 0: within bounds. */
+int alh_guard_check(alh_guard *g, int step, double t, const alh_diag *d, char *why, int cap);
+
 /* ---- Restart files (Core/Checkpoint.hpp:37-234, format v1) -------------
  * The reference's checkpoint, read and written byte for byte: a 56-byte
  * header, N particle records of 200 bytes, N x gstride FP64 g, nbodies body

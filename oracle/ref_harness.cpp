@@ -16,6 +16,7 @@
 //   ref_harness ckptread <file> <outdir> <case> <Nx> <Nv>  restore a file, dump fields
 //   ref_harness pos2d   <outdir> <Nx> <Nv> <a> <T> <cfl>   2-D transport where positivity fires
 //   ref_harness bodycase <outdir> <case> <Nx> <Nv> <scale>   a body case's cloud and body
+//   ref_harness writers <outdir> <case> <Nx> <Nv>   the VTK / CSV writers on a cloud
 //   ref_harness bodysteps <outdir> <case> <Nx> <Nv> <scale> <nsteps> <a> <order>
 //                                                  CPU steps with a free body
 //   ref_harness cloud   <outdir> <case> <Nx> <Nv> <scale> <shift> <jitter>
@@ -52,6 +53,7 @@
 #include "Moments.hpp"
 #include "TestCases.hpp"
 #include "Checkpoint.hpp"
+#include "Output.hpp"
 #include "RigidBody.hpp"
 
 #include <chrono>
@@ -923,6 +925,44 @@ int modeBodySteps(const std::string &dir, const std::string &cname, int Nx, int 
     return 0;
 }
 
+// The reference's output writers on a perturbed cloud: its fields (x, y,
+// z, rho, T, p, ux, uy, uz, boundary, moodFlag, Nv_local) and what
+// writeVTK, computeDiagnostics and writeTimeseries (Core/Output.hpp) make of
+// them: output_000007.vtk, timeseries.csv (two rows, t = 1e-10 and 2e-10)
+// and the diagnostics. Some particles carry moodFlag so the count shows.
+int modeWriters(const std::string &dir, const std::string &cname, int Nx, int Nv)
+{
+    Setup S;
+    build(S, caseOf(cname), Nx, Nv, 1.0);
+    perturb(S);
+    computeMoments(S.P, S.gc.R);
+    auto &P = S.P;
+    const size_t N = P.size();
+    for (size_t i = 0; i < N; ++i) P[i].moodFlag = (i % 7 == 3);
+    writeMeta(dir, S, 0.0);
+    std::vector<double> f(N * 9);
+    std::vector<uint8_t> b(N), m(N);
+    for (size_t i = 0; i < N; ++i)
+    {
+        const Particle &q = P[i];
+        const double v[9] = {q.x, q.y, q.z, q.rho, q.T, q.p, q.ux, q.uy, q.uz};
+        std::memcpy(&f[9 * i], v, sizeof v);
+        b[i] = q.boundary ? 1 : 0;
+        m[i] = q.moodFlag ? 1 : 0;
+    }
+    npyF8(dir + "/fields.npy", f, {N, 9});
+    npyU1(dir + "/boundary.npy", b, {N});
+    npyU1(dir + "/mood.npy", m, {N});
+    writeVTK(P, dir, 7, 7e-11);
+    const Diagnostics d = computeDiagnostics(P);
+    std::vector<double> dv{d.rho_avg, d.KE, d.T_avg, d.p_avg, (double)d.moodCount, (double)d.n};
+    npyF8(dir + "/diag.npy", dv, {6});
+    writeTimeseries(dir + "/timeseries.csv", 1e-10, d);
+    writeTimeseries(dir + "/timeseries.csv", 2e-10, d);
+    std::printf("wrote %s: N=%zu\n", dir.c_str(), N);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 1; }
@@ -952,6 +992,8 @@ int main(int argc, char **argv)
         return modeBodySteps(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]),
                              std::atof(argv[6]), std::atoi(argv[7]), std::atof(argv[8]),
                              std::atoi(argv[9]));
+    if (mode == "writers" && argc == 6)
+        return modeWriters(argv[2], argv[3], std::atoi(argv[4]), std::atoi(argv[5]));
     if (mode == "pos2d" && argc == 8)
         return modePos2d(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atof(argv[5]),
                          std::atof(argv[6]), std::atof(argv[7]));

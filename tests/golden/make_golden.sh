@@ -39,6 +39,9 @@ H="$ROOT/oracle/_ref/ref_harness"
 # body state after every step and the cloud at the end
 "$H" bodysteps "$HERE/bodysteps_cube_n12_v4" cube 12 4 2.0 40 300 1
 "$H" bodysteps "$HERE/bodysteps_circle_n24_v6" circle 24 6 2.0 40 300 1
+# the reference's VTK and CSV writers on a perturbed 2-D and 3-D cloud
+"$H" writers "$HERE/writers_cavity3d_n6_v4" cavity3d 6 4
+"$H" writers "$HERE/writers_cavity2d_n8_v4" cavity2d 8 4
 rm -f "$HERE"/ckpt_*/neigh_flat.npy "$HERE"/ckpt_*/nneigh.npy
 ( cd "$HERE" && find . -name '*.npy' -o -name 'meta.json' -o -name '*.ckpt' | sort | xargs sha256sum > MANIFEST.sha256 )
 echo "fixtures written under $HERE"
