@@ -28,6 +28,7 @@ HIP_LIB = os.path.join(PKG, "libalebgk_hip.so")
 HOST_LIB = os.path.join(PKG, "libalebgk_host.so")
 
 MAX_NEIGH = 128
+STATIC_SURF = 8192         # AB_STATIC_SURF: ab_body_loads_static's largest call
 PREC_FP64, PREC_MIXED, PREC_FP64_TABLE = 0, 1, 2
 PREC_NAMES = {PREC_FP64: "fp64", PREC_MIXED: "mixed", PREC_FP64_TABLE: "fp64-table"}
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "call out of order",
@@ -158,7 +159,7 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels",
                "ab_cloud_manage", "ab_cloud_source_map", "ab_download_particles",
                "ab_batch_create", "ab_batch_scatter", "ab_check_config", "ab_download_body_ids",
-               "ab_body_step"]
+               "ab_body_step", "ab_body_loads_static"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0

@@ -582,7 +582,8 @@ int ab_create(const ab_config *cfg, ab_ctx **out)
     c->gstride = (cfg->dim == 3) ? c->nodes : 2 * c->nodes;
     c->vlo = cfg->vmin;
     c->dv = (cfg->vmax - cfg->vmin) / (cfg->nv - 1);
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+    if (hipGetDevice(&c->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipMalloc((void **)&c->scal, sizeof(double) * S_COUNT) != hipSuccess ||
         hipMemset(c->scal, 0, sizeof(double) * S_COUNT) != hipSuccess)

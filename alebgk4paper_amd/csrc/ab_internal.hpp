@@ -84,6 +84,7 @@ struct ab_ctx
     int dim = 3, N = 0, Nv = 0, nodes = 0, gstride = 0;
     double vlo = 0, dv = 0;
     hipStream_t stream = nullptr;
+    int device = 0;                // the device current at ab_create
     std::string err;
     bool uploaded = false;
 

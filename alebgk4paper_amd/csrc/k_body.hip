@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -590,6 +591,60 @@ int body_loads_on(ab_ctx *c, const LoadBufs &B, int nsurf, const int *surf_idx,
                  out, B.con, B.par, B.out);
 }
 
+// ---- scratch of the no-allocation variant ----------------------------------
+// One array per device, for up to kStaticSurf surface particles: indices,
+// body ids, wall velocities (3), moments (10), contributions (6), per-body
+// parameters and results. The library never allocates it; each device
+// holds its own copy, reached through hipGetSymbolAddress.
+constexpr int kStaticSurf = 8192;
+constexpr size_t kOffIdx = 0;
+constexpr size_t kOffBody = kOffIdx + sizeof(int) * kStaticSurf;
+constexpr size_t kOffWu = kOffBody + sizeof(int) * kStaticSurf;
+constexpr size_t kOffMom = kOffWu + sizeof(double) * 3 * kStaticSurf;
+constexpr size_t kOffCon = kOffMom + sizeof(double) * 10 * kStaticSurf;
+constexpr size_t kOffPar = kOffCon + sizeof(double) * 6 * kStaticSurf;
+constexpr size_t kOffOut = kOffPar + sizeof(double) * kParDoubles * kMaxBodies;
+constexpr size_t kScratchBytes = kOffOut + sizeof(double) * 8 * kMaxBodies;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+
+}  // namespace
+}  // namespace ab
+
+static __device__ __align__(256) unsigned char scratch[ab::kScratchBytes];
+
+namespace ab {
+namespace {
+
+// The device address of this device's scratch, looked up once per device
+// with that device current (the array's name on the host is no device
+// address).
+int static_scratch(ab_ctx *c, unsigned char **out)
+{
+    if (c->device < 0 || c->device >= kMaxDevices)
+    { c->err = "static scratch: device index out of range"; return AB_ERR_UNSUPPORTED; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[c->device])
+    {
+        int cur = -1;
+        BCHK(c, hipGetDevice(&cur));
+        if (cur != c->device) BCHK(c, hipSetDevice(c->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != c->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+        {
+            c->err = std::string("static scratch: hipGetSymbolAddress: ") + hipGetErrorString(e);
+            return AB_ERR_HIP;
+        }
+        g_scratch[c->device] = p;
+    }
+    *out = (unsigned char *)g_scratch[c->device];
+    return AB_OK;
+}
+
 }  // namespace
 }  // namespace ab
 
@@ -602,6 +657,28 @@ extern "C" int ab_body_loads(ab_ctx *c, int nsurf, const int *surf_idx, const in
     if (rc) return rc;
     if ((rc = body_scratch(c, nsurf))) return rc;
     const LoadBufs B{c->bidx, c->bbody, c->bwu, c->bmom, c->bcon, c->bpar, c->bout};
+    return body_loads_on(c, B, nsurf, surf_idx, surf_body, nbodies, body_c, body_area, wu, out);
+}
+
+extern "C" int ab_body_loads_static(ab_ctx *c, int nsurf, const int *surf_idx,
+                                    const int *surf_body, int nbodies, const double *body_c,
+                                    const double *body_area, const double *wu, double *out)
+{
+    int rc = check_loads_args(c, nsurf, surf_idx, surf_body, nbodies, body_c, body_area, out,
+                              "ab_body_loads_static");
+    if (rc) return rc;
+    if (nsurf > kStaticSurf)
+    {
+        c->err = "ab_body_loads_static: " + std::to_string(nsurf) +
+                 " surface particles exceed its scratch of " + std::to_string(kStaticSurf) +
+                 " (use ab_body_loads)";
+        return AB_ERR_ARG;
+    }
+    unsigned char *p = nullptr;
+    if ((rc = static_scratch(c, &p))) return rc;
+    const LoadBufs B{(int *)(p + kOffIdx), (int *)(p + kOffBody), (double *)(p + kOffWu),
+                     (double *)(p + kOffMom), (double *)(p + kOffCon), (double *)(p + kOffPar),
+                     (double *)(p + kOffOut)};
     return body_loads_on(c, B, nsurf, surf_idx, surf_body, nbodies, body_c, body_area, wu, out);
 }
 

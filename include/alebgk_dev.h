@@ -208,6 +208,20 @@ int ab_total_mass(ab_ctx *ctx, double *mass);
 int ab_body_loads(ab_ctx *ctx, int nsurf, const int *surf_idx,
                   const int *surf_body, int nbodies, const double *body_c,
                   const double *body_area, const double *wu, double *out);
+/* ab_body_loads without any device allocation. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * scratch lives in a file-scope device array of the library, one per GPU,
+ * for at most AB_STATIC_SURF surface particles (about 1.3 MB per GPU, held
+ * until the process ends); larger calls are refused with AB_ERR_ARG. The
+ * array is found once per device (hipGetSymbolAddress) on the first call;
+ * every call on a GPU uses the same array and overwrites what it reads, so
+ * calls that overlap on one GPU (other threads, other streams) race. Same
+ * arguments and results as ab_body_loads. */
+#define AB_STATIC_SURF 8192
+int ab_body_loads_static(ab_ctx *ctx, int nsurf, const int *surf_idx,
+                         const int *surf_body, int nbodies, const double *body_c,
+                         const double *body_area, const double *wu, double *out);
 
 /* Update wall state and mesh velocity of selected particles after a body
  * moved (the uploads of cudaSyncBodies, CudaBackend.cu:1884-1915). Any array
