@@ -469,7 +469,10 @@ int main(int argc, char **argv)
     alh_diag d{};
     if (step == 0)
     {
+        // the initial state is written from the host cloud, as the
+        // reference does before its first step (Solver.cpp:293): p = rho R T
         snapshot(c, cl.N, snap);
+        for (int i = 0; i < cl.N; ++i) snap.p[i] = snap.rho[i] * cs.R * snap.T[i];
         write_outputs(snap, cl.N, cs.nv, out_dir, 0, 0.0, d);
     }
     int ck_every = 0;
@@ -549,11 +552,13 @@ int main(int argc, char **argv)
         }
         if (diverged) break;
     }
+    // the end of the GPU loop (Solver.cpp:570-577): the moments and the step
+    // line; the files were written on the saveEvery steps
     snapshot(c, cl.N, snap);
-    write_outputs(snap, cl.N, cs.nv, out_dir, (int)step, t, d);
+    alh_diagnostics(cl.N, snap.bnd.data(), snap.rho.data(), snap.ux.data(), snap.uy.data(),
+                    snap.uz.data(), snap.T.data(), snap.p.data(), snap.mood.data(), &d);
     alh_format_step((int)step, t, dt, &d, msg, sizeof msg);
     std::printf("%s\n", msg);
-    alh_write_timeseries(ts_path.c_str(), ts_first, t, &d);
     const double elapsed =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     std::printf("\nDone. Steps=%lld  t=%.4e  Wall=%.2f s\n", step, t, elapsed);
