@@ -159,7 +159,7 @@ HIP_EXPORTS = ["ab_create", "ab_destroy", "ab_last_error", "ab_upload", "ab_rebu
                "ab_set_transport_shape", "ab_copy_calibration", "ab_download_voxels",
                "ab_cloud_manage", "ab_cloud_source_map", "ab_download_particles",
                "ab_batch_create", "ab_batch_scatter", "ab_check_config", "ab_download_body_ids",
-               "ab_body_step", "ab_body_loads_static"]
+               "ab_body_step", "ab_body_loads_static", "ab_body_step_many"]
 MLP_TRAINABLE = 4869
 MLP_PARAMS = 4889
 COLLISION_CONTINUOUS = 0
@@ -397,6 +397,44 @@ class Guard(C.Structure):
         r = host_lib().alh_guard_check(C.byref(self), C.c_int(step), C.c_double(t), C.byref(d),
                                        buf, C.c_int(len(buf)))
         return bool(r), buf.value.decode()
+
+
+class ABBodyTask(C.Structure):
+    _fields_ = [("dt", C.c_double), ("nbodies", C.c_int), ("bodies", C.POINTER(ABRigidBody)),
+                ("dx", C.c_double), ("min_dist", C.c_double), ("dom", C.c_double * 6),
+                ("loads", _dp), ("n_new", _ip), ("counts", _ip)]
+
+
+def body_step_many(solvers, dt, bodies, geos, doms):
+    """ab_body_step_many: the moving-body block of several solvers at once
+    (one host thread per cloud). bodies[k]: solver k's ABRigidBody list,
+    updated in place; geos[k]: its (dx, dy, dz, minDist); doms[k]: its box as
+    box() gives it. Returns the (removed, merged, added) counts per solver."""
+    n = len(solvers)
+    arrs = [(ABRigidBody * max(len(b), 1))(*b) for b in bodies]
+    n_new = (C.c_int * n)()
+    counts = (C.c_int * (3 * n))()
+    tasks = (ABBodyTask * n)()
+    for k in range(n):
+        t = tasks[k]
+        t.dt = float(dt)
+        t.nbodies = len(bodies[k])
+        t.bodies = C.cast(arrs[k], C.POINTER(ABRigidBody))
+        t.dx, t.min_dist = float(geos[k][0]), float(geos[k][3])
+        t.dom[:] = [float(v) for v in doms[k]]
+        t.loads = C.cast(None, _dp)
+        t.n_new = C.cast(C.byref(n_new, 4 * k), _ip)
+        t.counts = C.cast(C.byref(counts, 12 * k), _ip)
+    ctxs = (C.c_void_p * n)(*[s._ctx.value for s in solvers])
+    rc = hip_lib().ab_body_step_many(C.c_int(n), ctxs, tasks)
+    if rc:
+        msgs = "; ".join(s._lib.ab_last_error(s._ctx).decode() for s in solvers)
+        raise ALEBGKError(f"ab_body_step_many: {ERRORS.get(rc, rc)}: {msgs}")
+    for k in range(n):
+        for j in range(len(bodies[k])):
+            C.memmove(C.byref(bodies[k][j]), C.byref(arrs[k][j]), C.sizeof(ABRigidBody))
+        solvers[k].N = n_new[k]
+    return [tuple(counts[3 * k:3 * k + 3]) for k in range(n)]
 
 
 def box(case):

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace ab {
@@ -943,4 +944,35 @@ extern "C" int ab_body_step(ab_ctx *c, double dt, int nb, ab_rigid_body *bodies,
         for (int a = 0; a < 9; ++a) cb[b].rm[a] = B.rm[a];
     }
     return ab_cloud_manage(c, nb, cb, nullptr, dx, min_dist, dom, n_new, counts);
+}
+
+// Several clouds' moving-body blocks at once: one host thread per context,
+// each running ab_body_step on its context's own stream, so the device
+// passes of all instances and their host synchronisations overlap instead
+// of running one instance after another.
+extern "C" int ab_body_step_many(int n, ab_ctx *const *ctxs, const ab_body_task *tasks)
+{
+    if (n <= 0 || n > 64 || !ctxs || !tasks) return AB_ERR_ARG;
+    for (int k = 0; k < n; ++k)
+    {
+        if (!ctxs[k]) return AB_ERR_ARG;
+        for (int q = 0; q < k; ++q)
+            if (ctxs[q] == ctxs[k]) { ctxs[k]->err = "ab_body_step_many: a context appears twice"; return AB_ERR_ARG; }
+    }
+    std::vector<int> rcs(n, AB_OK);
+    std::vector<std::thread> pool;
+    pool.reserve(n);
+    for (int k = 0; k < n; ++k)
+        pool.emplace_back([&, k]() {
+            ab_ctx *c = ctxs[k];
+            const ab_body_task &t = tasks[k];
+            if (hipSetDevice(c->device) != hipSuccess)
+            { c->err = "ab_body_step_many: hipSetDevice failed"; rcs[k] = AB_ERR_HIP; return; }
+            rcs[k] = ab_body_step(c, t.dt, t.nbodies, t.bodies, t.dx, t.min_dist, t.dom, t.loads,
+                                  t.n_new, t.counts);
+        });
+    for (auto &th : pool) th.join();
+    for (int k = 0; k < n; ++k)
+        if (rcs[k]) return rcs[k];
+    return AB_OK;
 }

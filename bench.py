@@ -174,6 +174,8 @@ def production(A, steps=10):
     N = cl.N
     s = A.Solver(cl, precision=A.PREC_MIXED)
     del cl
+    shapes, chosen = s.autotune(case.dt, reps=2)
+    chosen_name = [name for i, name, _ in shapes if i == chosen][0]
     s.steps(case.dt, 3)
     s.synchronize()
     t0 = time.perf_counter()
@@ -193,6 +195,8 @@ def production(A, steps=10):
     res = {"ms_per_step": sec * 1e3, "updates_per_s": upd / sec, "particles": N,
            "velocity_nodes": 8000, "transport_ms": tr * 1e3,
            "transport_GBps": alg / tr / 1e9, "transport_hbm_frac": alg / tr / 1e9 / HBM_PEAK_GBS,
+           "transport_shape": chosen_name,
+           "transport_shapes_ms": {name: ms for _, name, ms in shapes},
            "finite": bool(__import__("numpy").isfinite(mom).all())}
     log(f"[bench] cavity3d 60^3 Nv20 mixed: {sec * 1e3:.3f} ms/step, transport "
         f"{tr * 1e3:.3f} ms ({res['transport_hbm_frac']:.3f} of HBM peak)")
@@ -237,6 +241,63 @@ def cube_case(A, np, steps=10):
            "body_omega": st["om"].tolist(), "body_velocity": st["vel"].tolist()}
     log(f"[bench] cavity3d-cube 50^3 Nv10 fp64 + moving body: {sec * 1e3:.2f} ms/step "
         f"(fluid {fluid * 1e3:.2f}, body block {(sec - fluid) * 1e3:.2f})")
+    return res
+
+
+def batched_circles(A, np, steps=20):
+    """16 independent cavity2d-circle instances (density 11, parabolic lid
+    peaking at 10 m/s, a free circle, dt 1.5e-11), Nx drawn from {31, 40, 50}
+    and the body scale from U[0.15, 0.3] with a fixed seed, Nv 16, quadratic
+    MLS, FP64 (a body cloud is no lattice): every step enqueues the 16 fluid
+    steps on their own streams, then runs the 16 moving-body blocks at once
+    (ab_body_step_many, one host thread per cloud)."""
+    import random
+    rng = random.Random(20261202)
+    specs = [(rng.choice([31, 40, 50]), rng.uniform(0.15, 0.3)) for _ in range(16)]
+    sols, bodies, geos, doms, upd = [], [], [], [], 0
+    for nx, sc in specs:
+        cs = A.configure("cavity2d-circle", nx=nx, nv=16, mls_order=2)
+        cl = A.build_case(cs, body_scale=sc, mls_order=2)
+        sols.append(A.Solver(cl, precision=A.PREC_FP64))
+        bodies.append(cl.bodies)
+        geos.append(cl.geo)
+        doms.append(A.box(cs))
+    dt = cs.dt
+
+    def one():
+        for s in sols:
+            s.step(dt)
+        return A.body_step_many(sols, dt, bodies, geos, doms)
+
+    for _ in range(2):
+        one()
+    for s in sols:
+        s.synchronize()
+    tot = np.zeros(3, int)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in one():
+            tot += np.array(c)
+    for s in sols:
+        s.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    upd = sum(s.N for s in sols) * 256
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for s in sols:
+            s.step(dt)
+    for s in sols:
+        s.synchronize()
+    fluid = (time.perf_counter() - t0) / steps
+    rot = [float(np.arctan2(b[0].o[1], b[0].o[0])) for b in bodies]
+    for s in sols:
+        s.close()
+    res = {"instances": [{"nx": nx, "body_scale": round(sc, 4)} for nx, sc in specs],
+           "nv": 16, "ms_per_step": sec * 1e3, "fluid_ms_per_step": fluid * 1e3,
+           "updates_per_s": upd / sec, "removed_merged_added": tot.tolist(),
+           "body_rotation_rad_after": rot, "steps": steps + 2}
+    log(f"[bench] 16 cavity2d-circle instances, moving bodies: {sec * 1e3:.3f} ms/step "
+        f"(fluid alone {fluid * 1e3:.3f})")
     return res
 
 
@@ -300,6 +361,10 @@ def components(A):
         out["production_cavity3d_n60_nv20_mixed"] = production(A)
     except Exception as e:                          # reported, not hidden
         out["production_cavity3d_n60_nv20_mixed"] = {"error": str(e)}
+    try:
+        out["batched_cavity2d_circle_16_fp64"] = batched_circles(A, np)
+    except Exception as e:                          # reported, not hidden
+        out["batched_cavity2d_circle_16_fp64"] = {"error": str(e)}
     try:
         out["cavity3d_cube_n50_nv10_fp64"] = cube_case(A, np)
     except Exception as e:                          # reported, not hidden

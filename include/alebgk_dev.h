@@ -324,6 +324,24 @@ int ab_body_step(ab_ctx *ctx, double dt, int nbodies, ab_rigid_body *bodies, dou
                  double min_dist, const double dom[6], double *loads, int *n_new,
                  int counts[3]);
 
+/* ab_body_step of several independent clouds at once (no counterpart in the
+ * reference, which has one cloud): task k applies to ctxs[k]; every
+ * context's block runs on its own stream, driven by its own host thread, so
+ * the instances' device passes and synchronisations overlap. Each cloud
+ * ends exactly as ab_body_step alone leaves it. Returns the first failing
+ * task's status (its message in that context's ab_last_error). n <= 64. */
+typedef struct ab_body_task {
+    double dt;
+    int nbodies;
+    ab_rigid_body *bodies;
+    double dx, min_dist;
+    double dom[6];
+    double *loads;           /* may be NULL */
+    int *n_new;              /* may be NULL */
+    int *counts;             /* 3 ints, may be NULL */
+} ab_body_task;
+int ab_body_step_many(int n, ab_ctx *const *ctxs, const ab_body_task *tasks);
+
 /* ---- Autotuner of the transport launch shape -----------------------------
  * No counterpart in the reference. Times every launch shape of the table
  * pipelines' transport contraction on the context's current state, `reps`

@@ -73,6 +73,7 @@ def test_autotune_keeps_results_and_picks_a_shape():
     base.set_transport_shape(0)
     base.steps(dt, 3)
     ref = base.g()
+    blocks = {}
     for i, name, _ in table:
         s = A.Solver(cl, precision=A.PREC_MIXED)
         s.set_transport_shape(i)
@@ -80,9 +81,14 @@ def test_autotune_keeps_results_and_picks_a_shape():
         g = s.g()
         if name.startswith("B"):
             assert np.abs(g - ref).max() <= 1e-6 * np.abs(ref).max(), name
+            blocks[name] = g
         else:
             assert np.array_equal(g, ref), name
         s.close()
+    # the LDS-coefficient kernel does k_apply_mix's products in its order
+    for name, g in blocks.items():
+        if name.endswith("/L"):
+            assert np.array_equal(g, blocks[name[:-2]]), name
 
 
 def test_eager_step_after_capture_matches_uncaptured():
