@@ -704,7 +704,7 @@ int ab_rebuild_neighbours(ab_ctx *c)
     if (!c->vox)
     {
         int rc;
-        if ((rc = dalloc(c, &c->vox, (size_t)c->N)) || (rc = dalloc(c, &c->vlist, (size_t)c->N)))
+        if ((rc = dalloc(c, &c->vox, (size_t)c->N)) || (rc = dalloc(c, &c->vlist, 2 * (size_t)c->N)))
             return rc;
     }
     Timer tm(c);

@@ -115,7 +115,8 @@ struct ab_ctx
     // are made and refuse to run on a cloud that has changed since.
     unsigned long long gen = 0;
 
-    // neighbour rebuild scratch
+    // neighbour rebuild scratch (vlist: 2 N ints, the sorted list and the
+    // scatter before the sort)
     int *vox = nullptr, *vcount = nullptr, *vstart = nullptr, *vlist = nullptr;
     int nvox = 0;
 

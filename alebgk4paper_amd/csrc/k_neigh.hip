@@ -1,12 +1,19 @@
 // k_neigh.hip -- device rebuild of the MLS neighbour lists, identical to
 // updateVoxelsAndNeighbours (Initialization/NeighbourSearch.cpp:17-99).
 //
-// Integer counting sort of particles into the reference's voxels (count,
-// exclusive scan, scatter, then each voxel's short segment sorted by particle
-// index so that the order within a voxel is ascending, as the reference's
-// push_back in index order gives), then one thread per particle scans the 27
-// (9 in 2-D) voxels in the reference order dk, dj, di and keeps j != i with
-// r^2 < radius^2, capped at AB_MAX_NEIGH.
+// Integer counting sort of particles into the reference's voxels: count,
+// exclusive scan, scatter, then each voxel's segment ranked by particle index
+// (one wave per voxel), so that the order within a voxel is ascending, as the
+// reference's push_back in index order gives. The search then runs one
+// workgroup per voxel: every particle of a voxel has the same 27 (9 in 2-D)
+// candidate voxels, and for fixed (dk, dj) the three voxels di = -1, 0, 1 are
+// consecutive in the sorted list, so the candidates form at most 9 contiguous
+// runs in the reference's scan order. The workgroup stages them (ids and
+// positions) in LDS once; each wave then takes one particle of the voxel and
+// tests 64 candidates per pass, and a ballot appends the hits in candidate
+// order, which keeps j != i with r^2 < radius^2 in the reference's order,
+// capped at AB_MAX_NEIGH. A voxel whose candidates exceed the LDS stage reads
+// them from memory in the same order.
 //
 // The FP64 voxel and distance arithmetic must round exactly as the reference
 // host build does (no FMA contraction), so these kernels switch contraction
@@ -74,71 +81,140 @@ __global__ __launch_bounds__(1024) void k_scan(int nvox, int *__restrict__ count
 }
 
 __global__ void k_scatter(int N, const int *__restrict__ vox, int *__restrict__ cursor,
-                          const int *__restrict__ start, int *__restrict__ list)
+                          const int *__restrict__ start, int *__restrict__ tmp)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     int v = vox[i];
     int slot = atomicAdd(&cursor[v], 1);
-    list[start[v] + slot] = i;
+    tmp[start[v] + slot] = i;
 }
 
-__global__ void k_sort_segments(int nvox, const int *__restrict__ start, int *__restrict__ list)
+// One wave per voxel: each id of the scattered segment goes to its rank
+// among the segment's ids (all distinct), which sorts the segment ascending.
+__global__ __launch_bounds__(256) void k_rank_segments(int nvox, const int *__restrict__ start,
+                                                       const int *__restrict__ tmp,
+                                                       int *__restrict__ list)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nvox) return;
-    int lo = start[v], hi = start[v + 1];
-    for (int a = lo + 1; a < hi; ++a)
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (v >= nvox) return;                         // whole wave
+    const int lo = start[v], n = start[v + 1] - lo;
+    if (n <= 64)
     {
-        int key = list[a], b = a - 1;
-        while (b >= lo && list[b] > key) { list[b + 1] = list[b]; --b; }
-        list[b + 1] = key;
+        const int key = (lane < n) ? tmp[lo + lane] : 0x7fffffff;
+        int rank = 0;
+        for (int b = 0; b < n; ++b) rank += (__shfl(key, b) < key);
+        if (lane < n) list[lo + rank] = key;
+        return;
+    }
+    for (int a = lane; a < n; a += 64)
+    {
+        const int key = tmp[lo + a];
+        int rank = 0;
+        for (int b = 0; b < n; ++b) rank += (tmp[lo + b] < key);
+        list[lo + rank] = key;
     }
 }
 
-__global__ void k_search(VoxGrid vg, int N, const double *__restrict__ x,
-                         const double *__restrict__ y, const double *__restrict__ z,
-                         const int *__restrict__ vox, const int *__restrict__ start,
-                         const int *__restrict__ list, int *__restrict__ neigh,
-                         int *__restrict__ nneigh)
+constexpr int kStage = 1536;                       // candidates staged in LDS
+
+// One workgroup (4 waves) per voxel; see the file comment.
+__global__ __launch_bounds__(256) void k_search(VoxGrid vg, const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                const double *__restrict__ z,
+                                                const int *__restrict__ start,
+                                                const int *__restrict__ list,
+                                                int *__restrict__ neigh,
+                                                int *__restrict__ nneigh)
 {
 #pragma clang fp contract(off)
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const double r2max = vg.radius * vg.radius;
-    const int dkmax = (vg.dim == 3) ? 1 : 0;
-    const int v = vox[i];
-    const int vi = v % vg.nbh, tmp = v / vg.nbh;
-    const int vj = tmp % vg.nbv, vk = tmp / vg.nbv;
-    const double xi = x[i], yi = y[i], zi = z[i];
-    int *nb = neigh + (size_t)i * AB_MAX_NEIGH;
-    int nn = 0;
-    bool done = false;
-    for (int dk = -dkmax; dk <= dkmax && !done; dk++)
-        for (int dj = -1; dj <= 1 && !done; dj++)
-            for (int di = -1; di <= 1 && !done; di++)
+    __shared__ double sx[kStage], sy[kStage], sz[kStage];
+    __shared__ int sj[kStage];
+    __shared__ int rlo[9], roff[10], nrow_s;
+    const int v = blockIdx.x;
+    const int plo = start[v], phi = start[v + 1];
+    if (plo == phi) return;                        // whole workgroup
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0)
+    {
+        const int vi = v % vg.nbh, t = v / vg.nbh;
+        const int vj = t % vg.nbv, vk = t / vg.nbv;
+        const int dkmax = (vg.dim == 3) ? 1 : 0;
+        const int ilo = max(vi - 1, 0), ihi = min(vi + 1, vg.nbh - 1);
+        int nr = 0, off = 0;
+        for (int dk = -dkmax; dk <= dkmax; dk++)
+            for (int dj = -1; dj <= 1; dj++)
             {
-                int ni = vi + di, nj = vj + dj, nk = vk + dk;
-                if (ni < 0 || ni >= vg.nbh) continue;
-                if (nj < 0 || nj >= vg.nbv) continue;
-                if (nk < 0 || nk >= vg.nbw) continue;
-                int w = ni + vg.nbh * (nj + vg.nbv * nk);
-                for (int q = start[w]; q < start[w + 1]; ++q)
-                {
-                    int j = list[q];
-                    if (j == i) continue;
-                    double ddx = x[j] - xi;
-                    double ddy = y[j] - yi;
-                    double ddz = (vg.dim == 3) ? z[j] - zi : 0.0;
-                    if (ddx * ddx + ddy * ddy + ddz * ddz < r2max)
-                    {
-                        nb[nn++] = j;
-                        if (nn >= AB_MAX_NEIGH) { done = true; break; }
-                    }
-                }
+                const int nj = vj + dj, nk = vk + dk;
+                if (nj < 0 || nj >= vg.nbv || nk < 0 || nk >= vg.nbw) continue;
+                const int row = vg.nbh * (nj + vg.nbv * nk);
+                rlo[nr] = start[ilo + row];
+                roff[nr] = off;
+                off += start[ihi + row + 1] - start[ilo + row];
+                nr++;
             }
-    nneigh[i] = nn;
-    for (int m = nn; m < AB_MAX_NEIGH; ++m) nb[m] = -1;
+        roff[nr] = off;
+        nrow_s = nr;
+    }
+    __syncthreads();
+    const int nr = nrow_s, total = roff[nr];
+    const bool staged = total <= kStage;
+    // candidate k of the voxel's scan order: its position in the sorted list
+    auto at = [&](int k) {
+        int r = 0;
+        while (r + 1 < nr && roff[r + 1] <= k) ++r;
+        return rlo[r] + (k - roff[r]);
+    };
+    if (staged)
+    {
+        for (int k = tid; k < total; k += 256)
+        {
+            const int j = list[at(k)];
+            sj[k] = j;
+            sx[k] = x[j];
+            sy[k] = y[j];
+            sz[k] = (vg.dim == 3) ? z[j] : 0.0;
+        }
+        __syncthreads();
+    }
+    const double r2max = vg.radius * vg.radius;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int p = plo + wave; p < phi; p += 4)
+    {
+        const int i = list[p];
+        const double xi = x[i], yi = y[i], zi = (vg.dim == 3) ? z[i] : 0.0;
+        int *nb = neigh + (size_t)i * AB_MAX_NEIGH;
+        int nn = 0;
+        for (int k0 = 0; k0 < total && nn < AB_MAX_NEIGH; k0 += 64)
+        {
+            const int k = k0 + lane;
+            bool hit = false;
+            int j = -1;
+            if (k < total)
+            {
+                double xj, yj, zj;
+                if (staged) { j = sj[k]; xj = sx[k]; yj = sy[k]; zj = sz[k]; }
+                else
+                {
+                    j = list[at(k)];
+                    xj = x[j];
+                    yj = y[j];
+                    zj = (vg.dim == 3) ? z[j] : 0.0;
+                }
+                const double ddx = xj - xi;
+                const double ddy = yj - yi;
+                const double ddz = (vg.dim == 3) ? zj - zi : 0.0;
+                hit = (j != i) && (ddx * ddx + ddy * ddy + ddz * ddz < r2max);
+            }
+            const unsigned long long m = __ballot(hit);
+            const int pos = nn + __popcll(m & below);
+            if (hit && pos < AB_MAX_NEIGH) nb[pos] = j;
+            nn = min(nn + __popcll(m), AB_MAX_NEIGH);
+        }
+        if (lane == 0) nneigh[i] = nn;
+        for (int q = nn + lane; q < AB_MAX_NEIGH; q += 64) nb[q] = -1;
+    }
 }
 
 }  // namespace
@@ -158,10 +234,10 @@ int launch_neighbours(const DevCloud &c, const ab_config &cfg, int *vox, int *vc
     if (e != hipSuccess) return (int)e;
     k_voxel<<<(N + bs - 1) / bs, bs, 0, s>>>(vg, N, c.x, c.y, c.z, vox, vcount);
     k_scan<<<1, 1024, 0, s>>>(nvox, vcount, vstart);
-    k_scatter<<<(N + bs - 1) / bs, bs, 0, s>>>(N, vox, vcount, vstart, vlist);
-    k_sort_segments<<<(nvox + bs - 1) / bs, bs, 0, s>>>(nvox, vstart, vlist);
-    k_search<<<(N + bs - 1) / bs, bs, 0, s>>>(vg, N, c.x, c.y, c.z, vox, vstart, vlist, neigh,
-                                              nneigh);
+    int *tmp = vlist + N;                          // vlist holds 2 N ints
+    k_scatter<<<(N + bs - 1) / bs, bs, 0, s>>>(N, vox, vcount, vstart, tmp);
+    k_rank_segments<<<(nvox + 3) / 4, 256, 0, s>>>(nvox, vstart, tmp, vlist);
+    k_search<<<nvox, 256, 0, s>>>(vg, c.x, c.y, c.z, vstart, vlist, neigh, nneigh);
     return (int)hipGetLastError();
 }
 

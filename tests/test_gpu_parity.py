@@ -53,6 +53,31 @@ def test_device_neighbours_match_oracle_metric_size():
     assert nn.max() == 80
 
 
+@pytest.mark.parametrize("case,nx", [("cavity3d", 16), ("cavity2d", 40)])
+def test_device_neighbours_jittered_permuted_cloud(case, nx):
+    """A cloud that is no lattice: positions jittered by up to 0.45 dx and
+    the particles in a random order, so every voxel's scattered segment must
+    be sorted and the candidates cross voxel rows unevenly."""
+    from helpers import geom_of
+    _say(f"[neigh] {case} {nx}, jittered and permuted")
+    cs = A.configure(case, nx=nx, nv=4)
+    cl = A.build_lattice(cs)
+    rng = np.random.default_rng(11)
+    dx = cl.cfg.radius / 2.5                 # the MLS radius is 2.5 dx
+    perm = rng.permutation(cl.N)
+    for k in ("x", "y") + (("z",) if cl.cfg.dim == 3 else ()):
+        a = getattr(cl, k) + 0.45 * dx * rng.uniform(-1, 1, cl.N)
+        setattr(cl, k, np.ascontiguousarray(a[perm]))
+    neigh, nn, _ = O.neighbours(geom_of(cl.cfg, cs), cl.x, cl.y, cl.z)
+    sol = A.Solver(cl)
+    nb, dnn = sol.neighbours()
+    assert np.array_equal(dnn, nn)
+    assert np.array_equal(nb, neigh)
+    sol.rebuild_neighbours()                 # again on the same cloud
+    nb2, dnn2 = sol.neighbours()
+    assert np.array_equal(nb2, nb) and np.array_equal(dnn2, dnn)
+
+
 # -------------------------------------------------------------- transport ----
 
 @pytest.mark.parametrize("nx,nv,order", [(6, 5, 1), (6, 5, 2), (9, 4, 2), (12, 6, 2)])
