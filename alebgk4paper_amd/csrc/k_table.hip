@@ -661,19 +661,15 @@ __device__ __forceinline__ void blk_fma(const CS &cr, const T (&v)[K + 4],
     (one(std::integral_constant<int, TI>{}), ...);
 }
 
-template <typename T, int TX, int TY, int K, bool CLAMP, int U, typename CS>
-__device__ __forceinline__ void blk_step(const BlkAddr<T, K> &a, const CS &cr,
-                                         T (&va)[K + 4], T (&vb)[K + 4], T (&acc)[TX * TY][K])
+// R register buffers of union-column values: the loads run R - 1 columns
+// ahead of the arithmetic (R = 2: one column ahead).
+template <typename T, int TX, int TY, int K, bool CLAMP, int R, int U, typename CS>
+__device__ __forceinline__ void blk_step(const BlkAddr<T, K> &a, const CS &cr, T (&v)[R][K + 4],
+                                         T (&acc)[TX * TY][K])
 {
-    if constexpr (U + 1 < Blk<TX, TY>::NU)
-    {
-        if constexpr ((U & 1) == 0) blk_load<T, TX, TY, K, U + 1, CLAMP>(a, vb);
-        else blk_load<T, TX, TY, K, U + 1, CLAMP>(a, va);
-    }
-    if constexpr ((U & 1) == 0)
-        blk_fma<T, TX, TY, K, U>(cr, va, acc, std::make_integer_sequence<int, TX * TY>{});
-    else
-        blk_fma<T, TX, TY, K, U>(cr, vb, acc, std::make_integer_sequence<int, TX * TY>{});
+    constexpr int ahead = U + R - 1;
+    if constexpr (ahead < Blk<TX, TY>::NU) blk_load<T, TX, TY, K, ahead, CLAMP>(a, v[ahead % R]);
+    blk_fma<T, TX, TY, K, U>(cr, v[U % R], acc, std::make_integer_sequence<int, TX * TY>{});
 #pragma clang loop unroll(full)
     for (int t = 0; t < TX * TY; ++t)
 #pragma clang loop unroll(full)
@@ -681,7 +677,18 @@ __device__ __forceinline__ void blk_step(const BlkAddr<T, K> &a, const CS &cr,
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <typename T, int TX, int TY, int K, bool CLAMP, typename CS, int... U>
+template <typename T, int TX, int TY, int K, bool CLAMP, int R, int... U>
+__device__ __forceinline__ void blk_prime(const BlkAddr<T, K> &a, T (&v)[R][K + 4],
+                                          std::integer_sequence<int, U...>)
+{
+    auto one = [&](auto uic) {
+        constexpr int u = decltype(uic)::value;
+        if constexpr (u < Blk<TX, TY>::NU) blk_load<T, TX, TY, K, u, CLAMP>(a, v[u]);
+    };
+    (one(std::integral_constant<int, U>{}), ...);
+}
+
+template <typename T, int TX, int TY, int K, bool CLAMP, int R, typename CS, int... U>
 __device__ __forceinline__ void blk_columns(const T *__restrict__ g, const CS &cr,
                                             const Lat &L, int bx, int by, int iz0, int lin,
                                             size_t zstride, T (&acc)[TX * TY][K],
@@ -699,71 +706,161 @@ __device__ __forceinline__ void blk_columns(const T *__restrict__ g, const CS &c
     const int pb = (int)(zstride * sizeof(T)), ztop = L.nz - 1 - zlo;
 #pragma clang loop unroll(full)
     for (int zz = 0; zz < K + 4; ++zz) a.zof[zz] = (zz < ztop ? zz : ztop) * pb;
-    T va[K + 4], vb[K + 4];
-    blk_load<T, TX, TY, K, 0, CLAMP>(a, va);
-    (blk_step<T, TX, TY, K, CLAMP, U>(a, cr, va, vb, acc), ...);
+    T v[R][K + 4];
+    blk_prime<T, TX, TY, K, CLAMP, R>(a, v, std::make_integer_sequence<int, R - 1>{});
+    (blk_step<T, TX, TY, K, CLAMP, R, U>(a, cr, v, acc), ...);
 }
 
-// End-plane pair of a block: per stencil column, the two classes'
-// coefficients once for the block, then every output's column values.
-template <typename T, int TX, int TY, bool TOP, bool CLAMP, int C, int... TI>
-__device__ __forceinline__ void blk_pair_step(const T *__restrict__ g, __amdgpu_buffer_rsrc_t ra,
-                                              __amdgpu_buffer_rsrc_t rb, const Lat &L, int bx,
-                                              int by, int zlo, int lin, T (&a0)[TX * TY],
-                                              T (&a1)[TX * TY], std::integer_sequence<int, TI...>)
+// End planes of a block: planes 0 and 1 (TOP: nz-1 and nz-2) have classes
+// of their own. The coefficients of their taps that stay inside the lattice
+// (the table holds zeros for the others) are loaded into registers in one
+// batch, 51 for the plane on the wall (A) and 72 for the next plane (B);
+// then the union columns are walked as in the z-march, each column's four
+// end planes loaded one column ahead and added into every output whose
+// stencil holds the column.
+template <bool TOP>
+struct EndTaps
 {
-    constexpr Col k = col_at(C);
-    constexpr int lo = TOP ? 2 - k.d : 0, hi = TOP ? 3 : 1 + k.d;
-    T ca[5], cb[5], gv[TX * TY][4];
-#pragma clang loop unroll(full)
-    for (int j = 0; j <= 2 * k.d; ++j)
+    static constexpr int lo_a(int d) { return TOP ? -d : 0; }
+    static constexpr int hi_a(int d) { return TOP ? 0 : d; }
+    static constexpr int lo_b(int d) { return TOP ? -d : -1; }
+    static constexpr int hi_b(int d) { return TOP ? 1 : d; }
+    // window index (the end's four planes, ascending) of tap dz
+    static constexpr int va(int dz) { return TOP ? 3 + dz : dz; }
+    static constexpr int vb(int dz) { return TOP ? 2 + dz : 1 + dz; }
+    static constexpr int base_a(int c)
     {
-        ca[j] = bload_so<T>(ra, lin, (k.o0 + j) * kWave * (int)sizeof(T));
-        cb[j] = bload_so<T>(rb, lin, (k.o0 + j) * kWave * (int)sizeof(T));
+        int n = 0;
+        for (int k = 0; k < c; ++k) n += hi_a(col_at(k).d) - lo_a(col_at(k).d) + 1;
+        return n;
     }
-    const int pb = L.nx * L.ny * L.S * (int)sizeof(T);
-    auto load = [&](auto tic) {
-        constexpr int t = decltype(tic)::value;
-        int xx = bx + t % TX + k.ox, yy = by + t / TX + k.oy;
-        if constexpr (CLAMP) { xx = clampi(xx, 0, L.nx - 1); yy = clampi(yy, 0, L.ny - 1); }
-        const __amdgpu_buffer_rsrc_t rg =
-            brsrc(g + ((size_t)xx + (size_t)L.nx * (yy + (size_t)L.ny * zlo)) * L.S);
+    static constexpr int base_b(int c)
+    {
+        int n = 0;
+        for (int k = 0; k < c; ++k) n += hi_b(col_at(k).d) - lo_b(col_at(k).d) + 1;
+        return n;
+    }
+    static constexpr int NA = base_a(kCols), NB = base_b(kCols);
+};
+
+template <typename T, bool TOP, int... C>
+__device__ __forceinline__ void end_taps(__amdgpu_buffer_rsrc_t rw, __amdgpu_buffer_rsrc_t rn,
+                                         int lin, T (&ca)[EndTaps<TOP>::NA],
+                                         T (&cb)[EndTaps<TOP>::NB], std::integer_sequence<int, C...>)
+{
+    using E = EndTaps<TOP>;
+    auto one = [&](auto cic) {
+        constexpr int c = decltype(cic)::value;
+        constexpr Col k = col_at(c);
 #pragma clang loop unroll(full)
-        for (int j = lo; j <= hi; ++j) gv[t][j] = bload_so<T>(rg, lin, j * pb);
+        for (int dz = E::lo_a(k.d); dz <= E::hi_a(k.d); ++dz)
+            ca[E::base_a(c) + dz - E::lo_a(k.d)] =
+                bload_so<T>(rw, lin, (k.o0 + dz + k.d) * kWave * (int)sizeof(T));
+#pragma clang loop unroll(full)
+        for (int dz = E::lo_b(k.d); dz <= E::hi_b(k.d); ++dz)
+            cb[E::base_b(c) + dz - E::lo_b(k.d)] =
+                bload_so<T>(rn, lin, (k.o0 + dz + k.d) * kWave * (int)sizeof(T));
     };
-    (load(std::integral_constant<int, TI>{}), ...);
+    (one(std::integral_constant<int, C>{}), ...);
+}
+
+// The end planes of union column U that any output's taps reach.
+template <typename T, int TX, int TY, bool TOP, bool CLAMP, int U>
+__device__ __forceinline__ void end_load(const BlkAddr<T, 1> &a, int pb, T (&v)[4])
+{
+    constexpr UCol uc = Blk<TX, TY>::ucol(U);
+    constexpr int lo = TOP ? 2 - uc.d : 0, hi = TOP ? 3 : 1 + uc.d;
+    const int so = a.template cofs<U, TX, TY, CLAMP>();
 #pragma clang loop unroll(full)
-    for (int t = 0; t < TX * TY; ++t)
-#pragma clang loop unroll(full)
-        for (int j = 0; j <= 2 * k.d; ++j)
+    for (int k = lo; k <= hi; ++k) v[k] = bload_so<T>(a.r, a.lin, so + k * pb);
+}
+
+template <typename T, int TX, int TY, bool TOP, int U, int... TI>
+__device__ __forceinline__ void end_fma(const T (&ca)[EndTaps<TOP>::NA],
+                                        const T (&cb)[EndTaps<TOP>::NB], const T (&v)[4],
+                                        T (&aw)[TX * TY], T (&an)[TX * TY],
+                                        std::integer_sequence<int, TI...>)
+{
+    using E = EndTaps<TOP>;
+    constexpr UCol uc = Blk<TX, TY>::ucol(U);
+    auto one = [&](auto tic) {
+        constexpr int t = decltype(tic)::value;
+        constexpr int c = col_of(uc.ox - t % TX, uc.oy - t / TX);
+        if constexpr (c >= 0)
         {
-            a0[t] += ca[j] * gv[t][pair_idx<TOP>(0, j - k.d)];
-            a1[t] += cb[j] * gv[t][pair_idx<TOP>(1, j - k.d)];
-        }
+            constexpr int d = col_at(c).d;
 #pragma clang loop unroll(full)
-    for (int t = 0; t < TX * TY; ++t) asm volatile("" : "+v"(a0[t]), "+v"(a1[t])::"memory");
+            for (int dz = E::lo_a(d); dz <= E::hi_a(d); ++dz)
+                aw[t] += ca[E::base_a(c) + dz - E::lo_a(d)] * v[E::va(dz)];
+#pragma clang loop unroll(full)
+            for (int dz = E::lo_b(d); dz <= E::hi_b(d); ++dz)
+                an[t] += cb[E::base_b(c) + dz - E::lo_b(d)] * v[E::vb(dz)];
+        }
+    };
+    (one(std::integral_constant<int, TI>{}), ...);
+}
+
+// Ring of kEndRing column buffers: the loads run kEndRing - 1 columns ahead
+// of the arithmetic, which is only a few FMAs per column at the ends.
+constexpr int kEndRing = 7;
+
+template <typename T, int TX, int TY, bool TOP, bool CLAMP, int U>
+__device__ __forceinline__ void end_step(const BlkAddr<T, 1> &a, int pb,
+                                         const T (&ca)[EndTaps<TOP>::NA],
+                                         const T (&cb)[EndTaps<TOP>::NB], T (&v)[kEndRing][4],
+                                         T (&aw)[TX * TY], T (&an)[TX * TY])
+{
+    constexpr int ahead = U + kEndRing - 1;
+    if constexpr (ahead < Blk<TX, TY>::NU)
+        end_load<T, TX, TY, TOP, CLAMP, ahead>(a, pb, v[ahead % kEndRing]);
+    end_fma<T, TX, TY, TOP, U>(ca, cb, v[U % kEndRing], aw, an,
+                               std::make_integer_sequence<int, TX * TY>{});
+#pragma clang loop unroll(full)
+    for (int t = 0; t < TX * TY; ++t) asm volatile("" : "+v"(aw[t]), "+v"(an[t])::"memory");
     __builtin_amdgcn_sched_barrier(0);
 }
 
-template <typename T, int TX, int TY, bool TOP, bool CLAMP, int... C>
-__device__ __forceinline__ void blk_pair(const T *__restrict__ g, const T *__restrict__ cfa,
-                                         const T *__restrict__ cfb, const Lat &L, int bx, int by,
-                                         int lin, T (&a0)[TX * TY], T (&a1)[TX * TY],
-                                         std::integer_sequence<int, C...>)
+template <typename T, int TX, int TY, bool TOP, bool CLAMP, int... U>
+__device__ __forceinline__ void end_prime(const BlkAddr<T, 1> &a, int pb, T (&v)[kEndRing][4],
+                                          std::integer_sequence<int, U...>)
 {
+    auto one = [&](auto uic) {
+        constexpr int u = decltype(uic)::value;
+        if constexpr (u < Blk<TX, TY>::NU) end_load<T, TX, TY, TOP, CLAMP, u>(a, pb, v[u]);
+    };
+    (one(std::integral_constant<int, U>{}), ...);
+}
+
+// aw: the plane on the wall (0, or nz-1 for TOP), an: the next one.
+template <typename T, int TX, int TY, bool TOP, bool CLAMP, int... U>
+__device__ __forceinline__ void blk_ends(const T *__restrict__ g, const T *__restrict__ cfw,
+                                         const T *__restrict__ cfn, const Lat &L, int bx, int by,
+                                         int lin, T (&aw)[TX * TY], T (&an)[TX * TY],
+                                         std::integer_sequence<int, U...>)
+{
+    T ca[EndTaps<TOP>::NA], cb[EndTaps<TOP>::NB];
+    end_taps<T, TOP>(brsrc(cfw), brsrc(cfn), lin, ca, cb, std::make_integer_sequence<int, kCols>{});
     const int zlo = TOP ? L.nz - 4 : 0;
-    const __amdgpu_buffer_rsrc_t ra = brsrc(cfa), rb = brsrc(cfb);
+    const int pb = L.nx * L.ny * L.S * (int)sizeof(T);
+    BlkAddr<T, 1> a;
+    a.r = brsrc(g + ((ptrdiff_t)(bx - 2) + (ptrdiff_t)L.nx * (by - 2)) * L.S +
+                (size_t)zlo * L.nx * L.ny * L.S);
+    a.lin = lin;
+    a.col = L.S * (int)sizeof(T);
+    a.row = L.nx * L.S * (int)sizeof(T);
+    a.bx = bx; a.by = by; a.nx = L.nx; a.ny = L.ny;
 #pragma clang loop unroll(full)
-    for (int t = 0; t < TX * TY; ++t) { a0[t] = T(0); a1[t] = T(0); }
-    (blk_pair_step<T, TX, TY, TOP, CLAMP, C>(g, ra, rb, L, bx, by, zlo, lin, a0, a1,
-                                             std::make_integer_sequence<int, TX * TY>{}), ...);
+    for (int t = 0; t < TX * TY; ++t) { aw[t] = T(0); an[t] = T(0); }
+    T v[kEndRing][4];
+    end_prime<T, TX, TY, TOP, CLAMP>(a, pb, v, std::make_integer_sequence<int, kEndRing - 1>{});
+    (end_step<T, TX, TY, TOP, CLAMP, U>(a, pb, ca, cb, v, aw, an), ...);
 }
 
 // One wave = a TX x TY block of columns with origin (bx, by), all of the
 // x/y class (cx, cy), x one 64-node chunk. Interior blocks (cx = cy = 2)
 // lie two columns inside the lattice; edge blocks (CLAMP) run along an edge
 // of the lattice, their stencil columns clamped to it.
-template <typename T, int TX, int TY, int K, bool CLAMP, int CM = 0>
+template <typename T, int TX, int TY, int K, bool CLAMP, int CM = 0, int R = 2>
 __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
                                          const T *__restrict__ coef, int chunk, int bx, int by,
                                          int cx, int cy, int slab, int voff,
@@ -781,13 +878,13 @@ __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, 
     };
     auto end_pair = [&](auto top) {
         constexpr bool TOP = decltype(top)::value;
-        const int iz = TOP ? L.nz - 2 : 0;
-        T a0[TX * TY], a1[TX * TY];
-        blk_pair<T, TX, TY, TOP, CLAMP>(gw, cls_rows(category(iz, L.nz)),
-                                        cls_rows(category(iz + 1, L.nz)), L, bx, by, voff, a0, a1,
-                                        std::make_integer_sequence<int, kCols>{});
+        const int iw = TOP ? L.nz - 1 : 0, in = TOP ? L.nz - 2 : 1;
+        T aw[TX * TY], an[TX * TY];
+        blk_ends<T, TX, TY, TOP, CLAMP>(gw, cls_rows(category(iw, L.nz)),
+                                        cls_rows(category(in, L.nz)), L, bx, by, voff, aw, an,
+                                        std::make_integer_sequence<int, Blk<TX, TY>::NU>{});
 #pragma unroll
-        for (int t = 0; t < TX * TY; ++t) { store(t, iz, a0[t]); store(t, iz + 1, a1[t]); }
+        for (int t = 0; t < TX * TY; ++t) { store(t, iw, aw[t]); store(t, in, an[t]); }
         __builtin_amdgcn_sched_barrier(0);
     };
     if (slab < 0)
@@ -806,7 +903,7 @@ __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, 
             for (int t = 0; t < TX * TY; ++t)
 #pragma unroll
                 for (int k = 0; k < K; ++k) acc[t][k] = T(0);
-            blk_columns<T, TX, TY, K, CLAMP>(gw, cr, L, bx, by, iz0, voff, plane * L.S, acc,
+            blk_columns<T, TX, TY, K, CLAMP, R>(gw, cr, L, bx, by, iz0, voff, plane * L.S, acc,
                                              std::make_integer_sequence<int, Blk<TX, TY>::NU>{});
 #pragma unroll
             for (int t = 0; t < TX * TY; ++t)
@@ -831,13 +928,13 @@ __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, 
 
 // Interior block blk (row-major over the interior). Requires (nx - 4) % TX
 // == 0 and (ny - 4) % TY == 0.
-template <typename T, int TX, int TY, int K>
+template <typename T, int TX, int TY, int K, int R = 2>
 __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
                                            const T *__restrict__ coef, int chunk, int blk, int slab,
                                            int voff)
 {
     const int nbx = (L.nx - 4) / TX;
-    block_at<T, TX, TY, K, false>(L, g, gt, coef, chunk, 2 + (blk % nbx) * TX,
+    block_at<T, TX, TY, K, false, 0, R>(L, g, gt, coef, chunk, 2 + (blk % nbx) * TX,
                                   2 + (blk / nbx) * TY, 2, 2, slab, voff);
 }
 
@@ -869,7 +966,8 @@ __host__ __device__ inline int edge_items(int nx, int ny)
     return edge_blocks(nx, ny) ? (ny - 4) + (nx - 4) + 16 : nx * ny - (nx - 4) * (ny - 4);
 }
 
-template <typename T, int TX, int TY, int K, int KE, bool SLAB>
+// R: the register buffers of the z-march (loads R - 1 union columns ahead).
+template <typename T, int TX, int TY, int K, int KE, bool SLAB, int R = 2>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
 {
@@ -883,8 +981,8 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
     const int chunk = wv / per, r = wv % per;               // chunk-major
     if (r < nitem)
     {
-        if constexpr (SLAB) block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r % nblk, r / nblk - 1, voff);
-        else block_wave<T, TX, TY, K>(L, g, gt, coef, chunk, r, -2, voff);
+        if constexpr (SLAB) block_wave<T, TX, TY, K, R>(L, g, gt, coef, chunk, r % nblk, r / nblk - 1, voff);
+        else block_wave<T, TX, TY, K, R>(L, g, gt, coef, chunk, r, -2, voff);
         return;
     }
     int e = r - nitem;
@@ -897,16 +995,16 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
         if (e < 4 * qy)                                      // x edges: 1 x 4 along y
         {
             const int side = e / qy, ix = side < 2 ? side : L.nx - 4 + side;
-            block_at<T, 1, 4, KB, true>(L, g, gt, coef, chunk, ix, 2 + 4 * (e % qy),
-                                        category(ix, L.nx), 2, -2, voff);
+            block_at<T, 1, 4, KB, true, 0, R>(L, g, gt, coef, chunk, ix, 2 + 4 * (e % qy),
+                                              category(ix, L.nx), 2, -2, voff);
             return;
         }
         e -= 4 * qy;
         if (e < 4 * qx)                                      // y edges: 4 x 1 along x
         {
             const int side = e / qx, iy = side < 2 ? side : L.ny - 4 + side;
-            block_at<T, 4, 1, KB, true>(L, g, gt, coef, chunk, 2 + 4 * (e % qx), iy, 2,
-                                        category(iy, L.ny), -2, voff);
+            block_at<T, 4, 1, KB, true, 0, R>(L, g, gt, coef, chunk, 2 + 4 * (e % qx), iy, 2,
+                                              category(iy, L.ny), -2, voff);
             return;
         }
         e -= 4 * qx;                                         // the corners
