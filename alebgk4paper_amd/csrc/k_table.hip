@@ -938,8 +938,8 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
                                   2 + (blk / nbx) * TY, 2, 2, slab, voff);
 }
 
-// The whole transport in one launch: per chunk, first the interior blocks
-// (TX x TY columns per wave), then the edge columns (z blocks of KE). One
+// The whole transport in one launch: per chunk, the edge columns (z blocks
+// of KE), then the interior blocks (TX x TY columns per wave). One
 // launch keeps the edge waves from running as a second, short and
 // latency-bound grid after the blocks. Where the interior is a multiple of 4
 // columns wide, the edge columns along an edge share a class and run as
@@ -961,9 +961,20 @@ __host__ __device__ inline bool edge_blocks(int nx, int ny)
 
 // edge waves per chunk: blocks along the four x edges and four y edges plus
 // the corners, or one wave per edge column
-__host__ __device__ inline int edge_items(int nx, int ny)
+__host__ __device__ inline int edge_items(int nx, int ny, int slabs = 1)
 {
-    return edge_blocks(nx, ny) ? (ny - 4) + (nx - 4) + 16 : nx * ny - (nx - 4) * (ny - 4);
+    return edge_blocks(nx, ny) ? ((ny - 4) + (nx - 4)) * slabs + 16 : nx * ny - (nx - 4) * (ny - 4);
+}
+
+// Edge blocks march in z blocks of at most kEdgeK (12 would raise the
+// kernel's register count above three waves per SIMD).
+constexpr int kEdgeK = 8;
+
+// Items per edge block: the whole column, or in the SLAB order its end
+// planes and each of its z blocks (of kEdgeK planes) as items of their own.
+__host__ __device__ inline int edge_slabs(bool slab, int nz)
+{
+    return slab ? zblocks<kEdgeK>(nz) + 1 : 1;
 }
 
 // R: the register buffers of the z-march (loads R - 1 union columns ahead).
@@ -973,12 +984,17 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
 {
     const int nblk = ((L.nx - 4) / TX) * ((L.ny - 4) / TY);
     const int nitem = SLAB ? nblk * (zblocks<K>(L.nz) + 1) : nblk;
-    const int per = nitem + edge_items(L.nx, L.ny);
+    const int eslabs = edge_slabs(SLAB, L.nz);
+    const int per = nitem + edge_items(L.nx, L.ny, eslabs);
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
     const int voff = (threadIdx.x & (kWave - 1)) * (int)sizeof(T);
     if (wv >= per * L.nchunk) return;
-    const int chunk = wv / per, r = wv % per;               // chunk-major
+    const int chunk = wv / per;                             // chunk-major
+    // within a chunk the edge items first: they march whole columns and are
+    // the longest, so started last they would set the launch's tail
+    const int ne = per - nitem, r0 = wv % per;
+    const int r = r0 < ne ? nitem + r0 : r0 - ne;
     if (r < nitem)
     {
         if constexpr (SLAB) block_wave<T, TX, TY, K, R>(L, g, gt, coef, chunk, r % nblk, r / nblk - 1, voff);
@@ -986,28 +1002,31 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
         return;
     }
     int e = r - nitem;
-    // edge blocks march in z blocks of at most 8 (12 would raise the
-    // kernel's register count above three waves per SIMD)
-    constexpr int KB = KE > 8 ? 8 : KE;
+    constexpr int KB = KE > kEdgeK ? kEdgeK : KE;
+    static_assert(!SLAB || KB == kEdgeK, "edge slabs are counted in z blocks of kEdgeK");
     if (edge_blocks(L.nx, L.ny))
     {
         const int qy = (L.ny - 4) / 4, qx = (L.nx - 4) / 4;
-        if (e < 4 * qy)                                      // x edges: 1 x 4 along y
+        const int nblke = 4 * (qx + qy);
+        if (e < nblke * eslabs)
         {
-            const int side = e / qy, ix = side < 2 ? side : L.nx - 4 + side;
-            block_at<T, 1, 4, KB, true, 0, R>(L, g, gt, coef, chunk, ix, 2 + 4 * (e % qy),
-                                              category(ix, L.nx), 2, -2, voff);
+            // edge block eb, slab sl (-2: the whole column; -1: the end
+            // planes; k >= 0: z block k)
+            const int eb = e / eslabs, sl = SLAB ? e % eslabs - 1 : -2;
+            if (eb < 4 * qy)                                 // x edges: 1 x 4 along y
+            {
+                const int side = eb / qy, ix = side < 2 ? side : L.nx - 4 + side;
+                block_at<T, 1, 4, KB, true, 0, R>(L, g, gt, coef, chunk, ix, 2 + 4 * (eb % qy),
+                                                  category(ix, L.nx), 2, sl, voff);
+                return;
+            }
+            const int f = eb - 4 * qy;                       // y edges: 4 x 1 along x
+            const int side = f / qx, iy = side < 2 ? side : L.ny - 4 + side;
+            block_at<T, 4, 1, KB, true, 0, R>(L, g, gt, coef, chunk, 2 + 4 * (f % qx), iy, 2,
+                                              category(iy, L.ny), sl, voff);
             return;
         }
-        e -= 4 * qy;
-        if (e < 4 * qx)                                      // y edges: 4 x 1 along x
-        {
-            const int side = e / qx, iy = side < 2 ? side : L.ny - 4 + side;
-            block_at<T, 4, 1, KB, true, 0, R>(L, g, gt, coef, chunk, 2 + 4 * (e % qx), iy, 2,
-                                              category(iy, L.ny), -2, voff);
-            return;
-        }
-        e -= 4 * qx;                                         // the corners
+        e -= nblke * eslabs;                                 // the corners
         const int a = e & 3, c = e >> 2;
         const int ix = a < 2 ? a : L.nx - 4 + a, iy = c < 2 ? c : L.ny - 4 + c;
         column_wave<T, KE>(L, g, gt, coef, chunk, ix, iy, voff);
@@ -1557,6 +1576,15 @@ __device__ __forceinline__ int list_particle(const int *__restrict__ list, int n
     return w < n ? __builtin_amdgcn_readfirstlane(list[w]) : -1;
 }
 
+// The same list from its end: a pass that follows another over the same
+// rows starts with the rows the other read last, which the memory-side cache
+// still holds.
+__device__ __forceinline__ int list_particle_rev(const int *__restrict__ list, int n)
+{
+    const int w = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) / kWave);
+    return w < n ? __builtin_amdgcn_readfirstlane(list[n - 1 - w]) : -1;
+}
+
 // One wave per boundary particle (the boundary list): positivity and the
 // diffuse wall. Partials: [0] committed mass, [1] mass after the wall, [2]
 // the wall density, [6] limiter bits.
@@ -1728,7 +1756,8 @@ __global__ __launch_bounds__(256) void k_dm_solve(int N, const unsigned char *__
 }
 
 // The collision of the interior particles towards the target k_dm_solve left
-// (one wave per particle); partials [1..5].
+// (one wave per particle, the list in reverse: k_post read it forwards);
+// partials [1..5].
 template <typename T>
 __global__ __launch_bounds__(256) void k_collide(DevCloud c, Lat L, T *__restrict__ g,
                                                  const T *__restrict__ gt, Ref ref,
@@ -1737,7 +1766,7 @@ __global__ __launch_bounds__(256) void k_collide(DevCloud c, Lat L, T *__restric
                                                  double *__restrict__ part,
                                                  const int *__restrict__ ilist, int ni)
 {
-    const int i = list_particle(ilist, ni);
+    const int i = list_particle_rev(ilist, ni);
     const int lane = threadIdx.x & (kWave - 1);
     if (i < 0) return;
     const double Lo = scal[S_LAZY];
@@ -1928,7 +1957,7 @@ void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *sc
         {
             const int nblk = inner / (sh.TX * sh.TY);
             const int nw = (sh.S ? nblk * ((L.nz - 4 + sh.K - 1) / sh.K + 1) : nblk) +
-                           edge_items(L.nx, L.ny);
+                           edge_items(L.nx, L.ny, edge_slabs(sh.S != 0, L.nz));
             switch (i)
             {
             case 8: k_apply_mix<T, 2, 2, 8, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
@@ -1976,13 +2005,14 @@ bool shape_fits(int i, int nx, int ny)
 }
 
 // Default shape before tuning: in FP32 the 2 x 2 column blocks with K = 12
-// where they tile the interior (the fastest shape measured at 40^3, Nv 10,
-// by a factor 1.7 over single columns); otherwise, and in FP64, the
-// single-column z block that divides the interior run.
+// in the slab order where they tile the interior (the fastest shape measured
+// at 40^3, Nv 10, by a factor 1.9 over single columns, and as fast as the
+// whole-column order at 60^3); otherwise, and in FP64, the single-column z
+// block that divides the interior run.
 template <typename T>
 int default_shape(int nx, int ny, int nz)
 {
-    if (sizeof(T) == 4 && shape_fits<T>(11, nx, ny)) return 11;
+    if (sizeof(T) == 4 && shape_fits<T>(12, nx, ny)) return 12;
     const int run = nz - 4;
     return run % 9 == 0 ? 0 : (run % 7 == 0 ? 2 : 1);
 }

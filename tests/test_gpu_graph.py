@@ -57,13 +57,15 @@ def test_graph_refuses_profiling_context():
     g.launch(); g.synchronize(); g.close()
 
 
-def test_autotune_keeps_results_and_picks_a_shape():
+@pytest.mark.parametrize("nx,nv", [(12, 8), (16, 6)])
+def test_autotune_keeps_results_and_picks_a_shape(nx, nv):
     """Single-column transport shapes give bit-identical steps (the same
     products in the same order per output); block shapes visit an output's
     stencil columns in another order, so they agree to FP32 rounding. The
-    autotuner times every shape that fits and keeps one."""
-    _say("[autotune] cavity3d 12^3 nv8 mixed")
-    case, cl, ocl, neigh, nn = setup("cavity3d", 12, 8, order=2, precision=A.PREC_MIXED)
+    autotuner times every shape that fits and keeps one. At 16^3 the slab
+    shapes' last z block of the edge blocks is a partial one."""
+    _say(f"[autotune] cavity3d {nx}^3 nv{nv} mixed")
+    case, cl, ocl, neigh, nn = setup("cavity3d", nx, nv, order=2, precision=A.PREC_MIXED)
     dt = 1e-11
     base = A.Solver(cl, precision=A.PREC_MIXED)
     table, chosen = base.autotune(dt, reps=2)
