@@ -52,18 +52,51 @@ constexpr int kNcls = 125;    // 5 categories per axis
 
 struct Col { int ox, oy, d, o0; };
 
-// Columns in canonical order; o0 is the index of the column's oz = -d offset.
+// Columns in canonical order; o0 is the index of the column's oz = -d offset
+// (the running sum of 2 d + 1). Tables, not loops: the kernels' templates
+// evaluate these per (union column, output) at compile time.
 __host__ __device__ constexpr Col col_at(int c)
 {
-    constexpr int t[kCols][3] = {
-        {-1, -1, 2}, {0, -1, 2}, {1, -1, 2}, {-1, 0, 2}, {0, 0, 2}, {1, 0, 2},
-        {-1, 1, 2}, {0, 1, 2}, {1, 1, 2},
-        {-2, -1, 1}, {-2, 0, 1}, {-2, 1, 1}, {2, -1, 1}, {2, 0, 1}, {2, 1, 1},
-        {-1, -2, 1}, {0, -2, 1}, {1, -2, 1}, {-1, 2, 1}, {0, 2, 1}, {1, 2, 1}};
-    int o0 = 0;
-    for (int k = 0; k < c; ++k) o0 += 2 * t[k][2] + 1;
-    return Col{t[c][0], t[c][1], t[c][2], o0};
+    constexpr int t[kCols][4] = {
+        {-1, -1, 2, 0}, {0, -1, 2, 5}, {1, -1, 2, 10}, {-1, 0, 2, 15}, {0, 0, 2, 20},
+        {1, 0, 2, 25}, {-1, 1, 2, 30}, {0, 1, 2, 35}, {1, 1, 2, 40},
+        {-2, -1, 1, 45}, {-2, 0, 1, 48}, {-2, 1, 1, 51}, {2, -1, 1, 54}, {2, 0, 1, 57},
+        {2, 1, 1, 60}, {-1, -2, 1, 63}, {0, -2, 1, 66}, {1, -2, 1, 69}, {-1, 2, 1, 72},
+        {0, 2, 1, 75}, {1, 2, 1, 78}};
+    return Col{t[c][0], t[c][1], t[c][2], t[c][3]};
 }
+
+// Stencil column of offset (ox, oy), or -1.
+__host__ __device__ constexpr int col_of(int ox, int oy)
+{
+    constexpr int t[5][5] = {{-1, 9, 10, 11, -1}, {15, 0, 3, 6, 18}, {16, 1, 4, 7, 19},
+                             {17, 2, 5, 8, 20}, {-1, 12, 13, 14, -1}};
+    return (ox < -2 || ox > 2 || oy < -2 || oy > 2) ? -1 : t[ox + 2][oy + 2];
+}
+
+// The tables agree with each other and with the stencil |o|^2 < 6.25.
+constexpr bool col_tables_agree()
+{
+    int o0 = 0, n = 0;
+    for (int c = 0; c < kCols; ++c)
+    {
+        const Col k = col_at(c);
+        if (k.o0 != o0 || col_of(k.ox, k.oy) != c) return false;
+        o0 += 2 * k.d + 1;
+    }
+    for (int ox = -2; ox <= 2; ++ox)
+        for (int oy = -2; oy <= 2; ++oy)
+        {
+            int d = -1;
+            for (int oz = 0; oz <= 2; ++oz)
+                if (4 * (ox * ox + oy * oy + oz * oz) < 25) d = oz;
+            const int c = col_of(ox, oy);
+            if ((c < 0) != (d < 0) || (c >= 0 && col_at(c).d != d)) return false;
+            n += (c >= 0);
+        }
+    return o0 == kOff && n == kCols;
+}
+static_assert(col_tables_agree(), "stencil column tables");
 
 // Canonical index of lattice offset (ox, oy, oz), or -1 outside the stencil.
 __host__ __device__ inline int canon_index(int ox, int oy, int oz)
@@ -377,29 +410,6 @@ __device__ __forceinline__ void apply_pair(const T *__restrict__ g, const T *__r
     (pair_step<T, TOP, CC>(g, ra, rb, L, ix, iy, zlo, lin, g0, a0, b0, g1, a1, b1, acc0, acc1), ...);
 }
 
-// Sources of a class's 81 coefficient rows for the z march. The original
-// holds them in 81 VGPRs per lane (T cr[kOff]); these read them where they
-// are used instead: from a workgroup's LDS copy of the chunk's interior
-// class (loaded once per workgroup, one ds_read per coefficient use, ~1 per
-// 12 FMAs), or straight from the table in memory (a buffer load per use).
-// Either frees the 81 registers for occupancy; the products and their order
-// are unchanged, so the results are bit-identical.
-template <typename T>
-struct LdsCoef
-{
-    const T *p;     // the LDS rows plus the lane: row o at p[o * kWave]
-    __device__ __forceinline__ T operator[](int o) const { return p[o * kWave]; }
-};
-
-template <typename T>
-struct GlobCoef
-{
-    __amdgpu_buffer_rsrc_t r;   // the class's rows for this chunk
-    int voff;                   // the lane's byte offset
-    __device__ __forceinline__ T operator[](int o) const
-    { return bload_so<T>(r, voff, o * kWave * (int)sizeof(T)); }
-};
-
 // Interior: the values of column CC for outputs iz0 .. iz0+K-1.
 template <typename T, int K, int CC>
 __device__ __forceinline__ void col_load(const T *__restrict__ g, const Lat &L, int ix, int iy,
@@ -475,7 +485,7 @@ __device__ __forceinline__ void column_of(const Lat &L, int e, bool edges, int &
 
 // One wave = lattice column (ix, iy) x one 64-node chunk: the end-plane
 // pairs, then the interior march in z blocks of K outputs.
-template <typename T, int K, bool GLOB = false>
+template <typename T, int K>
 __device__ __forceinline__ void column_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
                                             const T *__restrict__ coef, int chunk, int ix, int iy,
                                             int voff)
@@ -506,7 +516,7 @@ __device__ __forceinline__ void column_wave(const Lat &L, const T *__restrict__ 
     end_pair(std::true_type{});
 
     // Interior planes iz = 2 .. nz-3 share one class: coefficients in
-    // registers (or read per use, GLOB)
+    // registers
     const T *cf = coef + ((size_t)(cx + 5 * (cy + 5 * 2)) * L.nchunk + chunk) * kOff * kWave;
     auto march = [&](const auto &cr) {
         for (int iz0 = 2; iz0 <= L.nz - 3; iz0 += K)
@@ -522,16 +532,11 @@ __device__ __forceinline__ void column_wave(const Lat &L, const T *__restrict__ 
                     bstore(gtw + ((size_t)col + plane * (iz0 + k)) * L.S, voff, acc[k]);
         }
     };
-    if constexpr (GLOB)
-        march(GlobCoef<T>{brsrc(cf), voff});
-    else
-    {
-        T cr[kOff];
-        const __amdgpu_buffer_rsrc_t rc = brsrc(cf);
+    T cr[kOff];
+    const __amdgpu_buffer_rsrc_t rc = brsrc(cf);
 #pragma unroll
-        for (int o = 0; o < kOff; ++o) cr[o] = bload_so<T>(rc, voff, o * kWave * (int)sizeof(T));
-        march(cr);
-    }
+    for (int o = 0; o < kOff; ++o) cr[o] = bload_so<T>(rc, voff, o * kWave * (int)sizeof(T));
+    march(cr);
 }
 
 template <typename T, int K, int W>
@@ -562,13 +567,6 @@ void k_apply(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__rest
 // every output whose stencil contains it: for 2 x 2 outputs, 32 union
 // columns instead of 4 x 21.
 
-// Stencil column of offset (ox, oy), or -1.
-__host__ __device__ constexpr int col_of(int ox, int oy)
-{
-    for (int c = 0; c < kCols; ++c)
-        if (col_at(c).ox == ox && col_at(c).oy == oy) return c;
-    return -1;
-}
 
 struct UCol { int ox, oy, d; };
 
@@ -578,8 +576,10 @@ struct Blk
     static constexpr int NT = TX * TY;
     // union columns relative to output (0, 0), row-major; d: the deepest
     // z reach any output needs from it
-    static constexpr UCol find(int want, int *count)
+    struct Tab { UCol u[(TX + 4) * (TY + 4)]; int n; };
+    static constexpr Tab make()
     {
+        Tab r{};
         int n = 0;
         for (int oy = -2; oy <= TY + 1; ++oy)
             for (int ox = -2; ox <= TX + 1; ++ox)
@@ -591,16 +591,14 @@ struct Blk
                         const int c = col_of(ox - tx, oy - ty);
                         if (c >= 0 && col_at(c).d > d) d = col_at(c).d;
                     }
-                if (!d) continue;
-                if (n == want) { if (count) *count = n + 1; return UCol{ox, oy, d}; }
-                ++n;
+                if (d) r.u[n++] = UCol{ox, oy, d};
             }
-        if (count) *count = n;
-        return UCol{0, 0, 0};
+        r.n = n;
+        return r;
     }
-    static constexpr int count() { int n = 0; find(-1, &n); return n; }
-    static constexpr int NU = count();
-    static constexpr UCol ucol(int u) { return find(u, nullptr); }
+    static constexpr Tab tab = make();
+    static constexpr int NU = tab.n;
+    static constexpr UCol ucol(int u) { return tab.u[u]; }
 };
 
 // Addressing of a block's z block: one buffer descriptor at the lowest union
@@ -800,20 +798,23 @@ __device__ __forceinline__ void end_fma(const T (&ca)[EndTaps<TOP>::NA],
     (one(std::integral_constant<int, TI>{}), ...);
 }
 
-// Ring of kEndRing column buffers: the loads run kEndRing - 1 columns ahead
-// of the arithmetic, which is only a few FMAs per column at the ends.
-constexpr int kEndRing = 7;
+// Ring of column buffers: the loads run ring - 1 columns ahead of the
+// arithmetic, which is only a few FMAs per column at the ends (blocks of six
+// or more outputs do more per column and have fewer registers to spare).
+template <int NT>
+constexpr int end_ring() { return NT >= 6 ? 4 : 7; }
 
 template <typename T, int TX, int TY, bool TOP, bool CLAMP, int U>
 __device__ __forceinline__ void end_step(const BlkAddr<T, 1> &a, int pb,
                                          const T (&ca)[EndTaps<TOP>::NA],
-                                         const T (&cb)[EndTaps<TOP>::NB], T (&v)[kEndRing][4],
-                                         T (&aw)[TX * TY], T (&an)[TX * TY])
+                                         const T (&cb)[EndTaps<TOP>::NB],
+                                         T (&v)[end_ring<TX * TY>()][4], T (&aw)[TX * TY],
+                                         T (&an)[TX * TY])
 {
-    constexpr int ahead = U + kEndRing - 1;
+    constexpr int RG = end_ring<TX * TY>(), ahead = U + RG - 1;
     if constexpr (ahead < Blk<TX, TY>::NU)
-        end_load<T, TX, TY, TOP, CLAMP, ahead>(a, pb, v[ahead % kEndRing]);
-    end_fma<T, TX, TY, TOP, U>(ca, cb, v[U % kEndRing], aw, an,
+        end_load<T, TX, TY, TOP, CLAMP, ahead>(a, pb, v[ahead % RG]);
+    end_fma<T, TX, TY, TOP, U>(ca, cb, v[U % RG], aw, an,
                                std::make_integer_sequence<int, TX * TY>{});
 #pragma clang loop unroll(full)
     for (int t = 0; t < TX * TY; ++t) asm volatile("" : "+v"(aw[t]), "+v"(an[t])::"memory");
@@ -821,7 +822,8 @@ __device__ __forceinline__ void end_step(const BlkAddr<T, 1> &a, int pb,
 }
 
 template <typename T, int TX, int TY, bool TOP, bool CLAMP, int... U>
-__device__ __forceinline__ void end_prime(const BlkAddr<T, 1> &a, int pb, T (&v)[kEndRing][4],
+__device__ __forceinline__ void end_prime(const BlkAddr<T, 1> &a, int pb,
+                                          T (&v)[end_ring<TX * TY>()][4],
                                           std::integer_sequence<int, U...>)
 {
     auto one = [&](auto uic) {
@@ -851,8 +853,9 @@ __device__ __forceinline__ void blk_ends(const T *__restrict__ g, const T *__res
     a.bx = bx; a.by = by; a.nx = L.nx; a.ny = L.ny;
 #pragma clang loop unroll(full)
     for (int t = 0; t < TX * TY; ++t) { aw[t] = T(0); an[t] = T(0); }
-    T v[kEndRing][4];
-    end_prime<T, TX, TY, TOP, CLAMP>(a, pb, v, std::make_integer_sequence<int, kEndRing - 1>{});
+    constexpr int RG = end_ring<TX * TY>();
+    T v[RG][4];
+    end_prime<T, TX, TY, TOP, CLAMP>(a, pb, v, std::make_integer_sequence<int, RG - 1>{});
     (end_step<T, TX, TY, TOP, CLAMP, U>(a, pb, ca, cb, v, aw, an), ...);
 }
 
@@ -860,11 +863,10 @@ __device__ __forceinline__ void blk_ends(const T *__restrict__ g, const T *__res
 // x/y class (cx, cy), x one 64-node chunk. Interior blocks (cx = cy = 2)
 // lie two columns inside the lattice; edge blocks (CLAMP) run along an edge
 // of the lattice, their stencil columns clamped to it.
-template <typename T, int TX, int TY, int K, bool CLAMP, int CM = 0, int R = 2>
+template <typename T, int TX, int TY, int K, bool CLAMP, int R = 2>
 __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
                                          const T *__restrict__ coef, int chunk, int bx, int by,
-                                         int cx, int cy, int slab, int voff,
-                                         const T *lds = nullptr)
+                                         int cx, int cy, int slab, int voff)
 {
     const size_t plane = (size_t)L.nx * L.ny;
     const size_t c0 = (size_t)chunk * kWave;
@@ -912,18 +914,11 @@ __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, 
                     if (iz0 + k <= L.nz - 3) store(t, iz0 + k, acc[t][k]);
         }
     };
-    if constexpr (CM == 1)
-        march(LdsCoef<T>{lds + (voff / (int)sizeof(T))});
-    else if constexpr (CM == 2)
-        march(GlobCoef<T>{brsrc(cls_rows(2)), voff});
-    else
-    {
-        T cr[kOff];
-        const __amdgpu_buffer_rsrc_t rc = brsrc(cls_rows(2));
+    T cr[kOff];
+    const __amdgpu_buffer_rsrc_t rc = brsrc(cls_rows(2));
 #pragma unroll
-        for (int o = 0; o < kOff; ++o) cr[o] = bload_so<T>(rc, voff, o * kWave * (int)sizeof(T));
-        march(cr);
-    }
+    for (int o = 0; o < kOff; ++o) cr[o] = bload_so<T>(rc, voff, o * kWave * (int)sizeof(T));
+    march(cr);
 }
 
 // Interior block blk (row-major over the interior). Requires (nx - 4) % TX
@@ -934,7 +929,7 @@ __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g
                                            int voff)
 {
     const int nbx = (L.nx - 4) / TX;
-    block_at<T, TX, TY, K, false, 0, R>(L, g, gt, coef, chunk, 2 + (blk % nbx) * TX,
+    block_at<T, TX, TY, K, false, R>(L, g, gt, coef, chunk, 2 + (blk % nbx) * TX,
                                   2 + (blk / nbx) * TY, 2, 2, slab, voff);
 }
 
@@ -1016,13 +1011,13 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
             if (eb < 4 * qy)                                 // x edges: 1 x 4 along y
             {
                 const int side = eb / qy, ix = side < 2 ? side : L.nx - 4 + side;
-                block_at<T, 1, 4, KB, true, 0, R>(L, g, gt, coef, chunk, ix, 2 + 4 * (eb % qy),
+                block_at<T, 1, 4, KB, true, R>(L, g, gt, coef, chunk, ix, 2 + 4 * (eb % qy),
                                                   category(ix, L.nx), 2, sl, voff);
                 return;
             }
             const int f = eb - 4 * qy;                       // y edges: 4 x 1 along x
             const int side = f / qx, iy = side < 2 ? side : L.ny - 4 + side;
-            block_at<T, 4, 1, KB, true, 0, R>(L, g, gt, coef, chunk, 2 + 4 * (f % qx), iy, 2,
+            block_at<T, 4, 1, KB, true, R>(L, g, gt, coef, chunk, 2 + 4 * (f % qx), iy, 2,
                                               category(iy, L.ny), sl, voff);
             return;
         }
@@ -1035,82 +1030,6 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
     int ix, iy;
     column_of(L, e, true, ix, iy);
     column_wave<T, KE>(L, g, gt, coef, chunk, ix, iy, voff);
-}
-
-// The transport with the interior class's coefficients shared through LDS.
-// A workgroup is 4 waves of one chunk: either 4 interior 2 x 2 blocks, which
-// first copy the chunk's 81 interior coefficient rows (81 x 64 values) into
-// LDS together and then march z reading each coefficient from LDS where it
-// is used; or 4 edge items (1 x 4 / 4 x 1 edge blocks, corner columns), which
-// read their classes' coefficients from memory per use. No wave holds the 81
-// rows in registers, so the kernel runs at twice the waves per SIMD of
-// k_apply_mix. The products per output and their order are k_apply_mix's:
-// the results are bit-identical to its 2 x 2 shape with the same K.
-template <typename T, int K, int KE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-void k_apply_lds(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
-{
-    __shared__ T rows[kOff * kWave];
-    const int nblk = ((L.nx - 4) / 2) * ((L.ny - 4) / 2);
-    const int nedge = edge_items(L.nx, L.ny);
-    const int wint = (nblk + 3) / 4, wedge = (nedge + 3) / 4, per = wint + wedge;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    if (b >= per * L.nchunk) return;                         // the whole workgroup
-    const int chunk = b / per, w = b % per;                  // chunk-major
-    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
-    const int voff = (threadIdx.x & (kWave - 1)) * (int)sizeof(T);
-    if (w < wint)
-    {
-        // the chunk's interior class (2, 2, 2) rows, copied by all 256 lanes
-        const T *src = coef + ((size_t)(2 + 5 * (2 + 5 * 2)) * L.nchunk + chunk) * kOff * kWave;
-        for (int k = (int)threadIdx.x; k < kOff * kWave; k += 256) rows[k] = src[k];
-        __syncthreads();
-        const int item = 4 * w + wave;
-        if (item < nblk)
-        {
-            const int nbx = (L.nx - 4) / 2;
-            block_at<T, 2, 2, K, false, 1>(L, g, gt, coef, chunk, 2 + (item % nbx) * 2,
-                                           2 + (item / nbx) * 2, 2, 2, -2, voff, rows);
-        }
-        return;
-    }
-    int e = 4 * (w - wint) + wave;
-    if (e >= nedge) return;
-    constexpr int KB = KE > 8 ? 8 : KE;
-    if (edge_blocks(L.nx, L.ny))
-    {
-        const int qy = (L.ny - 4) / 4, qx = (L.nx - 4) / 4;
-        if (e < 4 * qy)                                      // x edges: 1 x 4 along y
-        {
-            const int side = e / qy, ix = side < 2 ? side : L.nx - 4 + side;
-            block_at<T, 1, 4, KB, true, 2>(L, g, gt, coef, chunk, ix, 2 + 4 * (e % qy),
-                                           category(ix, L.nx), 2, -2, voff);
-            return;
-        }
-        e -= 4 * qy;
-        if (e < 4 * qx)                                      // y edges: 4 x 1 along x
-        {
-            const int side = e / qx, iy = side < 2 ? side : L.ny - 4 + side;
-            block_at<T, 4, 1, KB, true, 2>(L, g, gt, coef, chunk, 2 + 4 * (e % qx), iy, 2,
-                                           category(iy, L.ny), -2, voff);
-            return;
-        }
-        e -= 4 * qx;                                         // the corners
-        const int a = e & 3, c = e >> 2;
-        const int ix = a < 2 ? a : L.nx - 4 + a, iy = c < 2 ? c : L.ny - 4 + c;
-        column_wave<T, KE, true>(L, g, gt, coef, chunk, ix, iy, voff);
-        return;
-    }
-    int ix, iy;
-    column_of(L, e, true, ix, iy);
-    column_wave<T, KE, true>(L, g, gt, coef, chunk, ix, iy, voff);
-}
-
-// workgroups of a k_apply_lds launch
-inline int lds_groups(const Lat &L)
-{
-    const int nblk = ((L.nx - 4) / 2) * ((L.ny - 4) / 2);
-    return ((nblk + 3) / 4 + (edge_items(L.nx, L.ny) + 3) / 4) * L.nchunk;
 }
 
 // ---- per-particle limiters, wall, moments, collision -------------------------
@@ -1918,18 +1837,14 @@ bool is_fp32(const ab_ctx *ctx) { return ctx->cfg.precision == AB_PREC_MIXED; }
 // in the same order per output, so the results do not depend on the choice.
 // TX x TY > 1: interior blocks + edge columns; S: z-slab order of the blocks
 struct ApplyShape { int K, W, TX, TY, S; };
-constexpr ApplyShape kShapes32[] = {{9, 3, 1, 1, 0}, {8, 3, 1, 1, 0}, {7, 3, 1, 1, 0},
-                                    {6, 3, 1, 1, 0}, {12, 3, 1, 1, 0}, {10, 3, 1, 1, 0},
-                                    {5, 3, 1, 1, 0}, {4, 3, 1, 1, 0}, {8, 2, 2, 2, 0},
-                                    {6, 2, 2, 2, 0}, {4, 2, 2, 2, 0}, {12, 2, 2, 2, 0},
-                                    {12, 2, 2, 2, 1}, {8, 2, 2, 2, 1}, {6, 2, 2, 2, 1},
-                                    {12, 4, 2, 2, 2}, {8, 4, 2, 2, 2}};
+constexpr ApplyShape kShapes32[] = {{12, 3, 1, 1, 0}, {9, 3, 1, 1, 0}, {12, 2, 2, 2, 0},
+                                    {12, 2, 2, 2, 1}, {8, 2, 2, 3, 1}, {6, 2, 2, 4, 1}};
 constexpr ApplyShape kShapes64[] = {{9, 2, 1, 1, 0}, {8, 2, 1, 1, 0}, {7, 2, 1, 1, 0},
                                     {6, 2, 1, 1, 0}};
 constexpr int kMaxBlockPlanes = kMaxZBlockPlanes;
 
 template <typename T>
-constexpr int nshapes() { return sizeof(T) == 4 ? 17 : 4; }
+constexpr int nshapes() { return sizeof(T) == 4 ? 6 : 4; }
 
 template <typename T>
 ApplyShape shape_at(int i) { return sizeof(T) == 4 ? kShapes32[i] : kShapes64[i]; }
@@ -1945,14 +1860,6 @@ void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *sc
     if constexpr (sizeof(T) == 4)
     {
         const ApplyShape sh = kShapes32[i];
-        if (sh.S == 2)
-        {
-            // interior coefficients through LDS (k_apply_lds)
-            const int ng = lds_groups(L);
-            if (sh.K == 12) k_apply_lds<T, 12, 12><<<ng, 256, 0, s>>>(L, g, gt, cf);
-            else k_apply_lds<T, 8, 12><<<ng, 256, 0, s>>>(L, g, gt, cf);
-            return;
-        }
         if (sh.TX > 1)
         {
             const int nblk = inner / (sh.TX * sh.TY);
@@ -1960,28 +1867,16 @@ void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *sc
                            edge_items(L.nx, L.ny, edge_slabs(sh.S != 0, L.nz));
             switch (i)
             {
-            case 8: k_apply_mix<T, 2, 2, 8, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 9: k_apply_mix<T, 2, 2, 6, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 10: k_apply_mix<T, 2, 2, 4, 8, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 12: k_apply_mix<T, 2, 2, 12, 12, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 13: k_apply_mix<T, 2, 2, 8, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 14: k_apply_mix<T, 2, 2, 6, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 3: k_apply_mix<T, 2, 2, 12, 12, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 4: k_apply_mix<T, 2, 3, 8, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 5: k_apply_mix<T, 2, 4, 6, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
             default: k_apply_mix<T, 2, 2, 12, 12, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
             }
             return;
         }
         const int nb = waves(all);
-        switch (i)
-        {
-        case 0: k_apply<T, 9, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        case 1: k_apply<T, 8, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        case 2: k_apply<T, 7, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        case 3: k_apply<T, 6, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        case 4: k_apply<T, 12, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        case 5: k_apply<T, 10, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        case 6: k_apply<T, 5, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        default: k_apply<T, 4, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal); break;
-        }
+        if (i == 1) k_apply<T, 9, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal);
+        else k_apply<T, 12, 3><<<nb, 256, 0, s>>>(L, g, gt, cf, scal);
     }
     else
     {
@@ -2004,15 +1899,16 @@ bool shape_fits(int i, int nx, int ny)
     return sh.TX == 1 || ((nx - 4) % sh.TX == 0 && (ny - 4) % sh.TY == 0 && nx > 4 && ny > 4);
 }
 
-// Default shape before tuning: in FP32 the 2 x 2 column blocks with K = 12
-// in the slab order where they tile the interior (the fastest shape measured
-// at 40^3, Nv 10, by a factor 1.9 over single columns, and as fast as the
-// whole-column order at 60^3); otherwise, and in FP64, the single-column z
-// block that divides the interior run.
+// Default shape before tuning: in FP32 the 2 x 4 column blocks with K = 6
+// in the slab order where they tile the interior (the fastest shape measured,
+// 267 us at 40^3 / Nv 10 and 860 us at 60^3 / Nv 10, against 612 and 1934
+// us for single columns), else the 2 x 2 blocks, else single columns; in
+// FP64 the single-column z block that divides the interior run.
 template <typename T>
 int default_shape(int nx, int ny, int nz)
 {
-    if (sizeof(T) == 4 && shape_fits<T>(12, nx, ny)) return 12;
+    if (sizeof(T) == 4)
+        return shape_fits<T>(5, nx, ny) ? 5 : (shape_fits<T>(3, nx, ny) ? 3 : 0);
     const int run = nz - 4;
     return run % 9 == 0 ? 0 : (run % 7 == 0 ? 2 : 1);
 }

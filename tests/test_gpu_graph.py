@@ -70,7 +70,7 @@ def test_autotune_keeps_results_and_picks_a_shape(nx, nv):
     base = A.Solver(cl, precision=A.PREC_MIXED)
     table, chosen = base.autotune(dt, reps=2)
     _say(f"  {table} -> {chosen}")
-    assert chosen in [i for i, _, _ in table] and len(table) >= 8
+    assert chosen in [i for i, _, _ in table] and len(table) >= 4
     assert all(ms > 0 for _, _, ms in table)
     base.set_transport_shape(0)
     base.steps(dt, 3)
@@ -87,10 +87,9 @@ def test_autotune_keeps_results_and_picks_a_shape(nx, nv):
         else:
             assert np.array_equal(g, ref), name
         s.close()
-    # the LDS-coefficient kernel does k_apply_mix's products in its order
-    for name, g in blocks.items():
-        if name.endswith("/L"):
-            assert np.array_equal(g, blocks[name[:-2]]), name
+    # the slab order does the whole-column order's products in its order
+    if "B2x2/K12" in blocks and "B2x2/K12/S" in blocks:
+        assert np.array_equal(blocks["B2x2/K12"], blocks["B2x2/K12/S"])
 
 
 def test_eager_step_after_capture_matches_uncaptured():
