@@ -241,7 +241,54 @@ def cube_case(A, np, steps=10):
            "body_omega": st["om"].tolist(), "body_velocity": st["vel"].tolist()}
     log(f"[bench] cavity3d-cube 50^3 Nv10 fp64 + moving body: {sec * 1e3:.2f} ms/step "
         f"(fluid {fluid * 1e3:.2f}, body block {(sec - fluid) * 1e3:.2f})")
+    res["learned_warm_start"] = cube_case_dm(A, np)
     return res
+
+
+def cube_case_dm(A, np, record=4, steps=5):
+    """The same case with the discrete-Maxwellian collision (FP64: a body
+    cloud is no lattice) and the learned warm start trained on pairs from
+    this run's own plain steps: `record` steps recorded (before the first
+    body move: a cloud re-layout drops recorded rows), the MLP trained on
+    the GPU, then `steps` steps with the moving body, with and without the
+    warm start. Reports the step times and the Newton passes per solve."""
+    cs = A.configure("cavity3d-cube", nx=50, nv=10, mls_order=2)
+    cl = A.build_case(cs, mls_order=2)
+    s = A.Solver(cl, precision=A.PREC_FP64, collision=A.COLLISION_DISCRETE)
+    bodies = cl.bodies
+    box = A.box(cs)
+    s.dm_record(record)
+    for _ in range(record):
+        s.step(cs.dt)
+    st_p = s.step(cs.dt, stats=True)
+    s.mlp_init(seed=1)
+    loss0, loss1 = s.mlp_train(400, batch=4096, lr=2e-3, seed=5)
+
+    def run(on):
+        s.mlp_enable(on)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step(cs.dt)
+            s.body_step(cs.dt, bodies, cl.geo[0], cl.geo[3], box)
+        s.synchronize()
+        st = s.step(cs.dt, stats=True)
+        s.body_step(cs.dt, bodies, cl.geo[0], cl.geo[3], box)
+        return (time.perf_counter() - t0) / (steps + 1), st
+
+    sec_l, st_l = run(True)
+    sec_p, st_q = run(False)
+    passes = {"plain": st_p["dm_passes"] / max(st_p["dm_particles"], 1),
+              "plain_moving_body": st_q["dm_passes"] / max(st_q["dm_particles"], 1),
+              "learned_moving_body": st_l["dm_passes"] / max(st_l["dm_particles"], 1)}
+    om = bodies[0].state()["om"].tolist()
+    s.close()
+    log(f"[bench] cavity3d-cube 50^3 Nv10 fp64 + discrete collision + moving body: "
+        f"{sec_l * 1e3:.2f} ms/step with the learned warm start, {sec_p * 1e3:.2f} without; "
+        f"Newton passes per solve {passes}")
+    return {"ms_per_step_learned": sec_l * 1e3, "ms_per_step_plain": sec_p * 1e3,
+            "newton_passes_per_solve": passes, "mlp_loss": [loss0, loss1],
+            "recorded_steps": record, "body_omega": om}
 
 
 def batched_circles(A, np, steps=20):
