@@ -297,3 +297,29 @@ def test_set_wall_state_matches_upload(prec):
     q = f.particles()
     assert np.array_equal(q["x"][sub], xyz[:, 0] + 1e-9)
     assert np.array_equal(np.delete(q["x"], sub), np.delete(cl.x, sub))
+
+
+def test_body_loads_on_table_storage():
+    """ab_body_loads reads the table pipelines' storage g = L (M0 + d): after
+    one step (L != 1) the loads on the lid particles of a perturbed cavity
+    agree with the FP64 formulation's, to 1e-9 for FP64 tables and 1e-5
+    for the FP32 deviations of the mixed pipeline."""
+    _say("[body] loads on table storage, cavity3d 12^3 nv6")
+    case, cl, ocl, neigh, nn = setup("cavity3d", 12, 6, order=2)
+    top = cl.z.max()
+    surf = np.nonzero((cl.boundary == 1) & (cl.z == top))[0].astype(np.int32)
+    centre = np.array([[0.3e-6, 0.45e-6, 0.5e-6]])
+    area = [1e-15]
+    out = {}
+    for prec in (A.PREC_FP64, A.PREC_FP64_TABLE, A.PREC_MIXED):
+        s = A.Solver(cl, precision=prec)
+        s.step(case.dt)
+        out[prec] = s.body_loads(surf, np.zeros(len(surf), np.int32), centre, area)[0]
+        s.close()
+    ref = out[A.PREC_FP64]
+    _say(f"  FP64 {ref}\n  tab  {out[A.PREC_FP64_TABLE]}\n  mix  {out[A.PREC_MIXED]}")
+    for prec, tol in ((A.PREC_FP64_TABLE, 1e-9), (A.PREC_MIXED, 1e-5)):
+        for sl in (slice(0, 3), slice(3, 6)):
+            den = np.abs(ref[sl]).max()
+            assert den > 0
+            assert np.abs(out[prec][sl] - ref[sl]).max() <= tol * den, (prec, sl)
