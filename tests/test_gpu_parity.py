@@ -78,6 +78,30 @@ def test_device_neighbours_jittered_permuted_cloud(case, nx):
     assert np.array_equal(nb2, nb) and np.array_equal(dnn2, dnn)
 
 
+def test_device_neighbours_dense_cloud_caps_and_unstaged_voxels():
+    """A 16^3 cloud squeezed into a third of the box along each axis (same
+    voxel grid and radius): about 27 times the lattice density, so every
+    particle has more than 128 candidates in range (the list is capped at
+    the first 128 in scan order) and voxels see more candidates than the
+    search stages in LDS (those read them from memory in the same order)."""
+    from helpers import geom_of
+    _say("[neigh] dense cloud: caps and unstaged voxels")
+    cs = A.configure("cavity3d", nx=16, nv=4)
+    cl = A.build_lattice(cs)
+    rng = np.random.default_rng(3)
+    dx = cl.cfg.radius / 2.5
+    for k in ("x", "y", "z"):
+        a = getattr(cl, k)
+        lo = a.min()
+        setattr(cl, k, np.ascontiguousarray(lo + (a - lo) / 3.0 + 0.01 * dx * rng.uniform(-1, 1, cl.N)))
+    neigh, nn, _ = O.neighbours(geom_of(cl.cfg, cs), cl.x, cl.y, cl.z)
+    assert nn.max() == 128 and (nn == 128).mean() > 0.5
+    sol = A.Solver(cl)
+    nb, dnn = sol.neighbours()
+    assert np.array_equal(dnn, nn)
+    assert np.array_equal(nb, neigh)
+
+
 # -------------------------------------------------------------- transport ----
 
 @pytest.mark.parametrize("nx,nv,order", [(6, 5, 1), (6, 5, 2), (9, 4, 2), (12, 6, 2)])
