@@ -863,7 +863,7 @@ __device__ __forceinline__ void blk_ends(const T *__restrict__ g, const T *__res
 // x/y class (cx, cy), x one 64-node chunk. Interior blocks (cx = cy = 2)
 // lie two columns inside the lattice; edge blocks (CLAMP) run along an edge
 // of the lattice, their stencil columns clamped to it.
-template <typename T, int TX, int TY, int K, bool CLAMP, int R = 2>
+template <typename T, int TX, int TY, int K, bool CLAMP, int R = 2, int ZB = 1>
 __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
                                          const T *__restrict__ coef, int chunk, int bx, int by,
                                          int cx, int cy, int slab, int voff)
@@ -896,7 +896,9 @@ __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, 
         if (slab == -1) return;
     }
 
-    const int zb = slab < 0 ? 2 : 2 + slab * K, ze = slab < 0 ? L.nz - 3 : min(zb + K - 1, L.nz - 3);
+    // slab k >= 0: the ZB z blocks from plane 2 + k ZB K
+    const int zb = slab < 0 ? 2 : 2 + slab * ZB * K;
+    const int ze = slab < 0 ? L.nz - 3 : min(zb + ZB * K - 1, L.nz - 3);
     auto march = [&](const auto &cr) {
         for (int iz0 = zb; iz0 <= ze; iz0 += K)
         {
@@ -923,13 +925,13 @@ __device__ __forceinline__ void block_at(const Lat &L, const T *__restrict__ g, 
 
 // Interior block blk (row-major over the interior). Requires (nx - 4) % TX
 // == 0 and (ny - 4) % TY == 0.
-template <typename T, int TX, int TY, int K, int R = 2>
+template <typename T, int TX, int TY, int K, int R = 2, int ZB = 1>
 __device__ __forceinline__ void block_wave(const Lat &L, const T *__restrict__ g, T *__restrict__ gt,
                                            const T *__restrict__ coef, int chunk, int blk, int slab,
                                            int voff)
 {
     const int nbx = (L.nx - 4) / TX;
-    block_at<T, TX, TY, K, false, R>(L, g, gt, coef, chunk, 2 + (blk % nbx) * TX,
+    block_at<T, TX, TY, K, false, R, ZB>(L, g, gt, coef, chunk, 2 + (blk % nbx) * TX,
                                   2 + (blk / nbx) * TY, 2, 2, slab, voff);
 }
 
@@ -973,13 +975,14 @@ __host__ __device__ inline int edge_slabs(bool slab, int nz)
 }
 
 // R: the register buffers of the z-march (loads R - 1 union columns ahead).
-template <typename T, int TX, int TY, int K, int KE, bool SLAB, int R = 2>
+// SLAB: 0 whole columns; n > 0 items of n z blocks of the interior blocks.
+template <typename T, int TX, int TY, int K, int KE, int SLAB, int R = 2>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__restrict__ coef)
 {
     const int nblk = ((L.nx - 4) / TX) * ((L.ny - 4) / TY);
-    const int nitem = SLAB ? nblk * (zblocks<K>(L.nz) + 1) : nblk;
-    const int eslabs = edge_slabs(SLAB, L.nz);
+    const int nitem = SLAB ? nblk * (zblocks<SLAB * K>(L.nz) + 1) : nblk;
+    const int eslabs = edge_slabs(SLAB != 0, L.nz);
     const int per = nitem + edge_items(L.nx, L.ny, eslabs);
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     const int wv = __builtin_amdgcn_readfirstlane(b * (int)(blockDim.x / kWave) + (int)threadIdx.x / kWave);
@@ -992,7 +995,7 @@ void k_apply_mix(Lat L, const T *__restrict__ g, T *__restrict__ gt, const T *__
     const int r = r0 < ne ? nitem + r0 : r0 - ne;
     if (r < nitem)
     {
-        if constexpr (SLAB) block_wave<T, TX, TY, K, R>(L, g, gt, coef, chunk, r % nblk, r / nblk - 1, voff);
+        if constexpr (SLAB) block_wave<T, TX, TY, K, R, SLAB>(L, g, gt, coef, chunk, r % nblk, r / nblk - 1, voff);
         else block_wave<T, TX, TY, K, R>(L, g, gt, coef, chunk, r, -2, voff);
         return;
     }
@@ -1835,16 +1838,17 @@ bool is_fp32(const ab_ctx *ctx) { return ctx->cfg.precision == AB_PREC_MIXED; }
 // K outputs per z block (values reused across K + 4 planes), W the waves per
 // SIMD the register allocation targets. Every shape does the same products
 // in the same order per output, so the results do not depend on the choice.
-// TX x TY > 1: interior blocks + edge columns; S: z-slab order of the blocks
+// TX x TY > 1: interior blocks + edge columns; S > 0: the slab order, items of S z blocks
 struct ApplyShape { int K, W, TX, TY, S; };
 constexpr ApplyShape kShapes32[] = {{12, 3, 1, 1, 0}, {9, 3, 1, 1, 0}, {12, 2, 2, 2, 0},
-                                    {12, 2, 2, 2, 1}, {8, 2, 2, 3, 1}, {6, 2, 2, 4, 1}};
+                                    {12, 2, 2, 2, 1}, {8, 2, 2, 3, 1}, {6, 2, 2, 4, 1},
+                                    {6, 2, 2, 4, 2}};
 constexpr ApplyShape kShapes64[] = {{9, 2, 1, 1, 0}, {8, 2, 1, 1, 0}, {7, 2, 1, 1, 0},
                                     {6, 2, 1, 1, 0}};
 constexpr int kMaxBlockPlanes = kMaxZBlockPlanes;
 
 template <typename T>
-constexpr int nshapes() { return sizeof(T) == 4 ? 6 : 4; }
+constexpr int nshapes() { return sizeof(T) == 4 ? 7 : 4; }
 
 template <typename T>
 ApplyShape shape_at(int i) { return sizeof(T) == 4 ? kShapes32[i] : kShapes64[i]; }
@@ -1863,13 +1867,14 @@ void launch_shape(int i, Lat L, const T *g, T *gt, const T *cf, const double *sc
         if (sh.TX > 1)
         {
             const int nblk = inner / (sh.TX * sh.TY);
-            const int nw = (sh.S ? nblk * ((L.nz - 4 + sh.K - 1) / sh.K + 1) : nblk) +
+            const int nw = (sh.S ? nblk * ((L.nz - 4 + sh.S * sh.K - 1) / (sh.S * sh.K) + 1) : nblk) +
                            edge_items(L.nx, L.ny, edge_slabs(sh.S != 0, L.nz));
             switch (i)
             {
             case 3: k_apply_mix<T, 2, 2, 12, 12, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
             case 4: k_apply_mix<T, 2, 3, 8, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
-            case 5: k_apply_mix<T, 2, 4, 6, 8, true><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 5: k_apply_mix<T, 2, 4, 6, 8, 1><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
+            case 6: k_apply_mix<T, 2, 4, 6, 8, 2><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
             default: k_apply_mix<T, 2, 2, 12, 12, false><<<waves(nw), 256, 0, s>>>(L, g, gt, cf); break;
             }
             return;
@@ -2311,7 +2316,7 @@ int table_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32],
             const ApplyShape sh = f32 ? shape_at<float>(i) : shape_at<double>(i);
             if (sh.TX > 1)
                 snprintf(names[i], 32, "B%dx%d/K%d%s", sh.TX, sh.TY, sh.K,
-                         sh.S == 2 ? "/L" : (sh.S ? "/S" : ""));
+                         sh.S == 2 ? "/S2" : (sh.S ? "/S" : ""));
             else snprintf(names[i], 32, "K%d/W%d", sh.K, sh.W);
         }
         if (best < 0 || tmin < tbest) { best = i; tbest = tmin; }

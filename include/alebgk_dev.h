@@ -350,7 +350,8 @@ int ab_body_step_many(int n, ab_ctx *const *ctxs, const ab_body_task *tasks);
  * following steps. Shapes: one lattice column per wave with z blocks of K
  * outputs ("K9/W3": bit-identical results for every K), or TX x TY blocks of
  * interior columns per wave ("B2x4/K6/S": the same sum in another order,
- * equal to FP32 rounding; "/S": items of one z block, edge columns first).
+ * equal to FP32 rounding; "/S": items of one z block, "/S2" of two, edge
+ * columns first).
  * names/ms (cap entries each, may be NULL) receive every
  * candidate's name and best time in ms, "n/a" and -1 for a shape that does
  * not tile the lattice; *n the number of candidates, *chosen the index kept.
