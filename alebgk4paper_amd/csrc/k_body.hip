@@ -960,17 +960,32 @@ extern "C" int ab_body_step_many(int n, ab_ctx *const *ctxs, const ab_body_task 
             if (ctxs[q] == ctxs[k]) { ctxs[k]->err = "ab_body_step_many: a context appears twice"; return AB_ERR_ARG; }
     }
     std::vector<int> rcs(n, AB_OK);
+    auto run = [&](int k) {
+        ab_ctx *c = ctxs[k];
+        const ab_body_task &t = tasks[k];
+        if (hipSetDevice(c->device) != hipSuccess)
+        { c->err = "ab_body_step_many: hipSetDevice failed"; rcs[k] = AB_ERR_HIP; return; }
+        rcs[k] = ab_body_step(c, t.dt, t.nbodies, t.bodies, t.dx, t.min_dist, t.dom, t.loads,
+                              t.n_new, t.counts);
+    };
     std::vector<std::thread> pool;
-    pool.reserve(n);
-    for (int k = 0; k < n; ++k)
-        pool.emplace_back([&, k]() {
-            ab_ctx *c = ctxs[k];
-            const ab_body_task &t = tasks[k];
-            if (hipSetDevice(c->device) != hipSuccess)
-            { c->err = "ab_body_step_many: hipSetDevice failed"; rcs[k] = AB_ERR_HIP; return; }
-            rcs[k] = ab_body_step(c, t.dt, t.nbodies, t.bodies, t.dx, t.min_dist, t.dom, t.loads,
-                                  t.n_new, t.counts);
-        });
+    int started = 0;
+    try
+    {
+        pool.reserve(n);
+        for (; started < n; ++started) pool.emplace_back(run, started);
+    }
+    catch (...)
+    {
+        // no more threads: the remaining tasks run here, one after another
+    }
+    if (started < n)
+    {
+        int cur = 0;
+        const bool have = hipGetDevice(&cur) == hipSuccess;
+        for (int k = started; k < n; ++k) run(k);
+        if (have) (void)hipSetDevice(cur);           // the caller's device as it was
+    }
     for (auto &th : pool) th.join();
     for (int k = 0; k < n; ++k)
         if (rcs[k]) return rcs[k];
