@@ -23,7 +23,8 @@ struct ab_graph
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<hipEvent_t> events;
+    std::vector<hipEvent_t> events;         // the capture's fork and join
+    std::vector<hipEvent_t> order;          // a launch's ordering against the contexts' streams
 };
 
 namespace ab {
@@ -38,6 +39,7 @@ void graph_free(ab_graph *g)
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     for (auto e : g->events) if (e) (void)hipEventDestroy(e);
+    for (auto e : g->order) if (e) (void)hipEventDestroy(e);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
@@ -62,8 +64,11 @@ int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph 
     g->ctxs.assign(ctxs, ctxs + n);
     for (int k = 0; k < n; ++k) g->gens.push_back(ctxs[k]->gen);
     g->events.assign(n + 1, nullptr);
+    g->order.assign(n + 1, nullptr);
     bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
     for (auto &e : g->events)
+        ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    for (auto &e : g->order)
         ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
     // everything enqueued so far completes before the capture starts
     for (int k = 0; k < n && ok; ++k) ok = hipStreamSynchronize(ctxs[k]->stream) == hipSuccess;
@@ -110,7 +115,20 @@ int ab_graph_launch(ab_graph *g)
                               "cloud management); create the graph again";
             return AB_ERR_STATE;
         }
-    return hipGraphLaunch(g->exec, g->stream) == hipSuccess ? AB_OK : AB_ERR_HIP;
+    // the replay runs after the work already enqueued on every context's
+    // stream, and what is enqueued on them afterwards runs after the replay,
+    // as if the steps had been enqueued eagerly
+    const int n = (int)g->ctxs.size();
+    for (int k = 0; k < n; ++k)
+        if (hipEventRecord(g->order[k], g->ctxs[k]->stream) != hipSuccess ||
+            hipStreamWaitEvent(g->stream, g->order[k], 0) != hipSuccess)
+            return AB_ERR_HIP;
+    if (hipGraphLaunch(g->exec, g->stream) != hipSuccess ||
+        hipEventRecord(g->order[n], g->stream) != hipSuccess)
+        return AB_ERR_HIP;
+    for (int k = 0; k < n; ++k)
+        if (hipStreamWaitEvent(g->ctxs[k]->stream, g->order[n], 0) != hipSuccess) return AB_ERR_HIP;
+    return AB_OK;
 }
 
 int ab_graph_synchronize(ab_graph *g)

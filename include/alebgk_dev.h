@@ -371,11 +371,13 @@ int ab_copy_calibration(unsigned long long bytes, int reps);
  * the host). ab_graph_create captures nsteps steps of each of the n
  * contexts (uploaded, distinct) into one graph in which every context is an
  * independent branch; ab_graph_launch enqueues one replay (asynchronous, on
- * the graph's own stream) and ab_graph_synchronize waits for it. A replay
- * gives exactly the eager steps' results. Refused (AB_ERR_STATE) while a
- * context profiles or records training rows; table coefficients are built
- * for dt before the capture. Do not step a captured context eagerly while
- * a replay is in flight. A graph holds the clouds' device addresses: once
+ * the graph's own stream, ordered after the work already enqueued on every
+ * member context's stream; work enqueued on those streams afterwards waits
+ * for the replay, as after eager steps) and ab_graph_synchronize waits for
+ * it. A replay gives exactly the eager steps' results. Refused
+ * (AB_ERR_STATE) while a context profiles or records training rows; table
+ * coefficients are built for dt before the capture. A graph holds the
+ * clouds' device addresses: once
  * any of its contexts is re-uploaded or re-laid out (ab_upload,
  * ab_cloud_manage, ab_body_step) ab_graph_launch refuses with AB_ERR_STATE;
  * create the graph again. */

@@ -155,3 +155,24 @@ def test_batch_scatter_refuses_changed_member():
     with pytest.raises(A.ALEBGKError):
         bt.scatter()
     bt.close()
+
+
+def test_graph_launch_is_ordered_with_the_contexts_streams():
+    """A replay runs after the work already enqueued on its contexts'
+    streams and before what is enqueued on them afterwards: eager steps,
+    replays and a download interleaved with no synchronisation between them
+    give exactly the eager steps' results."""
+    _say("[graph] ordering of replays against eager work, cavity3d 16^3 nv6 FP64")
+    case, cl, ocl, neigh, nn = setup("cavity3d", 16, 6, order=2)
+    dt = case.dt
+    plain = A.Solver(cl, precision=A.PREC_FP64)
+    mixed = A.Solver(cl, precision=A.PREC_FP64)
+    graph = A.Graph([mixed], dt, 2)
+    plain.steps(dt, 6)
+    mixed.steps(dt, 1)          # eager, then a replay, then eager, then a replay
+    graph.launch()
+    mixed.steps(dt, 1)
+    graph.launch()
+    g = mixed.g()               # downloads on the context's stream: after the replay
+    assert np.array_equal(g, plain.g())
+    graph.close()
