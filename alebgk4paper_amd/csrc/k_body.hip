@@ -27,7 +27,6 @@
 #include <cstring>
 #include <mutex>
 #include <string>
-#include <thread>
 #include <vector>
 
 namespace ab {
@@ -946,10 +945,10 @@ extern "C" int ab_body_step(ab_ctx *c, double dt, int nb, ab_rigid_body *bodies,
     return ab_cloud_manage(c, nb, cb, nullptr, dx, min_dist, dom, n_new, counts);
 }
 
-// Several clouds' moving-body blocks at once: one host thread per context,
-// each running ab_body_step on its context's own stream, so the device
-// passes of all instances and their host synchronisations overlap instead
-// of running one instance after another.
+// Several clouds' moving-body blocks in one call, one context after another
+// on the calling thread. (One host thread per context overlapped the
+// instances' synchronisations, but crashed inside the runtime's kernel
+// launch when a profiler traced the 16 threads; it is not used.)
 extern "C" int ab_body_step_many(int n, ab_ctx *const *ctxs, const ab_body_task *tasks)
 {
     if (n <= 0 || n > 64 || !ctxs || !tasks) return AB_ERR_ARG;
@@ -960,33 +959,18 @@ extern "C" int ab_body_step_many(int n, ab_ctx *const *ctxs, const ab_body_task 
             if (ctxs[q] == ctxs[k]) { ctxs[k]->err = "ab_body_step_many: a context appears twice"; return AB_ERR_ARG; }
     }
     std::vector<int> rcs(n, AB_OK);
-    auto run = [&](int k) {
+    int cur = 0;
+    const bool have = hipGetDevice(&cur) == hipSuccess;
+    for (int k = 0; k < n; ++k)
+    {
         ab_ctx *c = ctxs[k];
         const ab_body_task &t = tasks[k];
         if (hipSetDevice(c->device) != hipSuccess)
-        { c->err = "ab_body_step_many: hipSetDevice failed"; rcs[k] = AB_ERR_HIP; return; }
+        { c->err = "ab_body_step_many: hipSetDevice failed"; rcs[k] = AB_ERR_HIP; continue; }
         rcs[k] = ab_body_step(c, t.dt, t.nbodies, t.bodies, t.dx, t.min_dist, t.dom, t.loads,
                               t.n_new, t.counts);
-    };
-    std::vector<std::thread> pool;
-    int started = 0;
-    try
-    {
-        pool.reserve(n);
-        for (; started < n; ++started) pool.emplace_back(run, started);
     }
-    catch (...)
-    {
-        // no more threads: the remaining tasks run here, one after another
-    }
-    if (started < n)
-    {
-        int cur = 0;
-        const bool have = hipGetDevice(&cur) == hipSuccess;
-        for (int k = started; k < n; ++k) run(k);
-        if (have) (void)hipSetDevice(cur);           // the caller's device as it was
-    }
-    for (auto &th : pool) th.join();
+    if (have) (void)hipSetDevice(cur);               // the caller's device as it was
     for (int k = 0; k < n; ++k)
         if (rcs[k]) return rcs[k];
     return AB_OK;

@@ -296,8 +296,8 @@ def batched_circles(A, np, steps=20):
     peaking at 10 m/s, a free circle, dt 1.5e-11), Nx drawn from {31, 40, 50}
     and the body scale from U[0.15, 0.3] with a fixed seed, Nv 16, quadratic
     MLS, FP64 (a body cloud is no lattice): every step enqueues the 16 fluid
-    steps on their own streams, then runs the 16 moving-body blocks at once
-    (ab_body_step_many, one host thread per cloud)."""
+    steps on their own streams, then runs the 16 moving-body blocks
+    (ab_body_step_many, one cloud after another)."""
     import random
     rng = random.Random(20261202)
     specs = [(rng.choice([31, 40, 50]), rng.uniform(0.15, 0.3)) for _ in range(16)]

@@ -324,12 +324,12 @@ int ab_body_step(ab_ctx *ctx, double dt, int nbodies, ab_rigid_body *bodies, dou
                  double min_dist, const double dom[6], double *loads, int *n_new,
                  int counts[3]);
 
-/* ab_body_step of several independent clouds at once (no counterpart in the
- * reference, which has one cloud): task k applies to ctxs[k]; every
- * context's block runs on its own stream, driven by its own host thread, so
- * the instances' device passes and synchronisations overlap. Each cloud
- * ends exactly as ab_body_step alone leaves it. Returns the first failing
- * task's status (its message in that context's ab_last_error). n <= 64. */
+/* ab_body_step of several independent clouds in one call (no counterpart in
+ * the reference, which has one cloud): task k applies to ctxs[k], the
+ * contexts one after another on the calling thread (whose current device
+ * is restored). Each cloud ends exactly as ab_body_step alone leaves it.
+ * Returns the first failing task's status (its message in that context's
+ * ab_last_error). n <= 64. */
 typedef struct ab_body_task {
     double dt;
     int nbodies;
