@@ -602,6 +602,7 @@ int ab_destroy(ab_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_cloud(c);
     mlp_free(c);
+    for (void *p : c->cm_pool) if (p) (void)hipFree(p);
     if (c->scal) (void)hipFree(c->scal);
     for (auto e : c->evpool) if (e) (void)hipEventDestroy(e);
     if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -184,8 +184,12 @@ struct ab_ctx
     std::vector<ab_ctx *> members;
     std::vector<unsigned long long> member_gen;   // members' generations when batched
 
-    // cloud management (ab_cloud_manage): the last call's source map
+    // cloud management (ab_cloud_manage): the last call's source map, and
+    // its device scratch kept between calls (slot k: the k-th buffer a call
+    // asks for, grown when a call needs more)
     std::vector<int> cm_src;
+    std::vector<void *> cm_pool;
+    std::vector<size_t> cm_pool_bytes;
     bool mlp_on = false;           // warm start enabled
 
     // body loads scratch, grown to the surface-particle count: indices and

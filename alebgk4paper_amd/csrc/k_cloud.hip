@@ -454,29 +454,44 @@ __global__ void k_fill_rows(int n0, int na, int nc, const double *__restrict__ a
     }
 }
 
-template <typename T>
-int dmalloc(ab_ctx *c, T **p, size_t n)
-{
-    if (hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
-    {
-        c->err = "cloud management: device allocation failed";
-        *p = nullptr;
-        return AB_ERR_NOMEM;
-    }
-    return AB_OK;
-}
-
-// Device scratch of one call, freed on every return path.
+// Device scratch of one call: the k-th request takes the context's pool
+// slot k, allocated (or grown) only when it is smaller than asked and kept
+// for the next call, so steady calls allocate and free nothing (a free
+// synchronises the whole device). The requests of one call never share a
+// slot, and a call's stream is synchronised before it returns.
 struct Scratch
 {
-    std::vector<void *> ptrs;
-    ~Scratch() { for (void *p : ptrs) if (p) (void)hipFree(p); }
+    size_t next = 0;
+    ab_ctx *owner = nullptr;
+    // an early return may leave kernels using the slots in flight
+    ~Scratch() { if (owner) (void)hipStreamSynchronize(owner->stream); }
     template <typename T>
     int get(ab_ctx *c, T **p, size_t n)
     {
-        int rc = dmalloc(c, p, n);
-        if (!rc) ptrs.push_back(*p);
-        return rc;
+        owner = c;
+        const size_t bytes = std::max<size_t>(n, 1) * sizeof(T), k = next++;
+        if (k >= c->cm_pool.size())
+        {
+            c->cm_pool.push_back(nullptr);
+            c->cm_pool_bytes.push_back(0);
+        }
+        if (c->cm_pool_bytes[k] < bytes)
+        {
+            if (c->cm_pool[k]) (void)hipFree(c->cm_pool[k]);
+            c->cm_pool[k] = nullptr;
+            c->cm_pool_bytes[k] = 0;
+            void *q = nullptr;
+            if (hipMalloc(&q, bytes) != hipSuccess)
+            {
+                c->err = "cloud management: device allocation failed";
+                *p = nullptr;
+                return AB_ERR_NOMEM;
+            }
+            c->cm_pool[k] = q;
+            c->cm_pool_bytes[k] = bytes;
+        }
+        *p = static_cast<T *>(c->cm_pool[k]);
+        return AB_OK;
     }
 };
 
