@@ -718,17 +718,22 @@ int ab_rebuild_neighbours(ab_ctx *c)
         return AB_ERR_HIP;
     }
     tm.toc(ST_NEIGH);
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     // the table transports take their stencils from the lists: check the
-    // classes again and rebuild the coefficients at the next step
+    // classes again (on the host) and rebuild the coefficients at the next
+    // step; other clouds need no host synchronisation (later work on the
+    // stream, downloads included, runs after the rebuild)
     int rc = AB_OK;
     if (table_mode(c) && c->gs)
     {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         rc = table_recheck(c);
         c->coef_dt = -1.0;
     }
     else if (table2d(c) && c->t2_coef)
+    {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
         rc = table2d_setup(c);
+    }
     if (rc) c->uploaded = false;
     return rc;
 }

@@ -412,7 +412,7 @@ def components(A):
         s.synchronize()
         ms = (time.perf_counter() - t0) * 1e3 / n
         out["neighbour_rebuild_40cube"] = {"ms": ms, "particles_per_s": cl.N / ms * 1e3,
-                                           "bound": "latency (integer counting sort, LDS-staged search; 6 launches and a host sync)"}
+                                           "bound": "latency (integer counting sort, LDS-staged search; 6 launches, back to back)"}
         s.close()
         log(f"[bench] neighbour rebuild 40^3: {ms:.3f} ms")
     except Exception as e:                          # reported, not hidden

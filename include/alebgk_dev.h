@@ -163,7 +163,9 @@ int ab_upload(ab_ctx *ctx, const ab_cloud_view *view);
 
 /* updateVoxelsAndNeighbours (Initialization/NeighbourSearch.cpp:17) on the
  * device: integer counting sort into voxels, then the 27-voxel scan in the
- * reference order; lists are identical to the reference's. */
+ * reference order; lists are identical to the reference's. Asynchronous on
+ * the context's stream, except in the table pipelines, which wait for the
+ * lists to check the lattice classes again. */
 int ab_rebuild_neighbours(ab_ctx *ctx);
 int ab_download_neighbours(ab_ctx *ctx, int *neigh, int *nneigh);
 /* The voxel of every particle from the last device rebuild (Particle::voxel,
