@@ -291,6 +291,8 @@ int do_step(ab_ctx *c, double dt)
 
 namespace ab {
 
+int graph_coefficients(ab_ctx *c, double dt);
+
 // The step without host work, for graph capture; graph_prepare refuses what
 // a capture cannot hold and builds the table coefficients for dt eagerly.
 int graph_prepare(ab_ctx *c, double dt)
@@ -310,6 +312,15 @@ int graph_prepare(ab_ctx *c, double dt)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         c->mass_valid = true;
     }
+    return graph_coefficients(c, dt);
+}
+
+// The table pipelines' coefficients for dt, built on the context's stream
+// when they are not current (a graph replays the apply without building
+// them: an eager step at another dt, or a wall-state change, leaves them
+// stale).
+int graph_coefficients(ab_ctx *c, double dt)
+{
     if (table_mode(c) && c->coef_dt != dt)
     {
         int rc = table_prepare(c, dt);
@@ -852,6 +863,7 @@ int ab_autotune(ab_ctx *c, double dt, int reps, int cap, char (*names)[32], floa
 {
     if (!c || reps < 1 || cap < 0 || !(dt > 0.0)) return AB_ERR_ARG;
     if (!c->uploaded) { c->err = "ab_autotune before ab_upload"; return AB_ERR_STATE; }
+    if (table2d(c)) return table2d_autotune(c, dt, reps, cap, names, ms, n, chosen);
     if (!table_mode(c)) { c->err = "ab_autotune tunes the table pipelines' transport"; return AB_ERR_UNSUPPORTED; }
     return table_autotune(c, dt, reps, cap, names, ms, n, chosen);
 }
@@ -859,6 +871,7 @@ int ab_autotune(ab_ctx *c, double dt, int reps, int cap, char (*names)[32], floa
 int ab_set_transport_shape(ab_ctx *c, int shape)
 {
     if (!c) return AB_ERR_ARG;
+    if (table2d(c)) return table2d_set_run(c, shape);
     if (!table_mode(c)) { c->err = "transport shapes exist in the table pipelines"; return AB_ERR_UNSUPPORTED; }
     return table_set_shape(c, shape);
 }

@@ -20,6 +20,8 @@ struct ab_graph
 {
     std::vector<ab_ctx *> ctxs;
     std::vector<unsigned long long> gens;   // each context's generation at capture
+    std::vector<unsigned long long> tgens;  // and its 2-D table arrays' generation
+    double dt = 0.0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -30,6 +32,7 @@ struct ab_graph
 namespace ab {
 int graph_step(ab_ctx *c, double dt);   // ab_api.hip: the eager step, no host work
 int graph_prepare(ab_ctx *c, double dt);
+int graph_coefficients(ab_ctx *c, double dt);
 }  // namespace ab
 
 namespace {
@@ -62,7 +65,12 @@ int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph 
     ab_graph *g = new (std::nothrow) ab_graph;
     if (!g) return AB_ERR_NOMEM;
     g->ctxs.assign(ctxs, ctxs + n);
-    for (int k = 0; k < n; ++k) g->gens.push_back(ctxs[k]->gen);
+    for (int k = 0; k < n; ++k)
+    {
+        g->gens.push_back(ctxs[k]->gen);
+        g->tgens.push_back(ctxs[k]->tgen);
+    }
+    g->dt = dt;
     g->events.assign(n + 1, nullptr);
     g->order.assign(n + 1, nullptr);
     bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
@@ -109,12 +117,30 @@ int ab_graph_launch(ab_graph *g)
     // captured on: refuse a replay once any of them was re-uploaded or
     // re-laid out
     for (size_t k = 0; k < g->ctxs.size(); ++k)
+    {
         if (g->ctxs[k]->gen != g->gens[k])
         {
             g->ctxs[k]->err = "ab_graph_launch: the cloud changed after the capture (upload or "
                               "cloud management); create the graph again";
             return AB_ERR_STATE;
         }
+        if (g->ctxs[k]->tgen != g->tgens[k])
+        {
+            g->ctxs[k]->err = "ab_graph_launch: the 2-D table transport was set up again after the "
+                              "capture (neighbour rebuild, wall state or run length); create the "
+                              "graph again";
+            return AB_ERR_STATE;
+        }
+    }
+    // the captured steps apply the table coefficients without building them:
+    // build them for the graph's dt where an eager step at another dt or a
+    // wall-state change left them stale (on the context's stream, so before
+    // the replay)
+    for (size_t k = 0; k < g->ctxs.size(); ++k)
+    {
+        const int rc = ab::graph_coefficients(g->ctxs[k], g->dt);
+        if (rc) return rc;
+    }
     // the replay runs after the work already enqueued on every context's
     // stream, and what is enqueued on them afterwards runs after the replay,
     // as if the steps had been enqueued eagerly

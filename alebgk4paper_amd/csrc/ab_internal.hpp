@@ -114,6 +114,10 @@ struct ab_ctx
     // (upload, re-layout, destroy): graphs and batches record it when they
     // are made and refuse to run on a cloud that has changed since.
     unsigned long long gen = 0;
+    // Bumped whenever the 2-D table transport's device arrays are allocated
+    // again (its setup after a neighbour rebuild or a wall-state change, a
+    // new run length): graphs record it and refuse to replay stale pointers.
+    unsigned long long tgen = 0;
 
     // neighbour rebuild scratch (vlist: 2 N ints, the sorted list and the
     // scatter before the sort)
@@ -150,6 +154,7 @@ struct ab_ctx
     int t2_ncls = 0;
     int *t2_items = nullptr;       // apply work items, 8 ints each: column runs, row runs, corners
     int t2_nv = 0, t2_nh = 0, t2_nit = 0;   // column runs, row runs, all items
+    int t2_run = -1;               // run length index of the apply (-1: the default, 8 outputs)
     double t2_dt = -1.0;           // dt the coefficients were built for
 
     // discrete-Maxwellian collision (cfg.collision) and its MLP warm start
@@ -290,6 +295,12 @@ int table2d_setup(ab_ctx *ctx);
 int table2d_transport(ab_ctx *ctx, double dt);
 int table2d_prepare(ab_ctx *ctx, double dt);
 void table2d_free(ab_ctx *ctx);
+// The apply's run lengths (outputs per work item along a lattice line):
+// timed, or one forced (-1: the default); every run length gives
+// bit-identical results.
+int table2d_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32], float *ms,
+                     int *n, int *chosen);
+int table2d_set_run(ab_ctx *ctx, int run);
 
 // Discrete-Maxwellian settings of the next step (records training rows when
 // asked); defined in ab_api.hip.

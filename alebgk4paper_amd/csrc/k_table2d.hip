@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -210,9 +211,42 @@ __host__ __device__ constexpr OffTab make_offtab()
 }
 constexpr OffTab kOffTab = make_offtab();
 
-constexpr int kRun = 8;   // outputs per run item
+// Run lengths of the apply (outputs per work item along a lattice line):
+// longer runs load fewer input rows per output ((RUN + 4) / RUN), shorter
+// ones hold fewer accumulators and give more items. Each output's sum takes
+// the same terms in the same order for every run length.
+constexpr int kRuns[] = {4, 8, 12};
+constexpr int kNRuns = 3;
+constexpr int kRunDefault = 1;   // 8 outputs
 
-// A run of up to kRun outputs of one class along a lattice line (HOR: along
+// The row offset `ro` passes through an empty asm that takes every
+// accumulator of the run, so the loads of a row issue after the additions
+// of the rows before it.
+template <int RUN>
+__device__ __forceinline__ void pin_after(int &ro, const double *a1, const double *a2)
+{
+    if constexpr (RUN == 4)
+        asm volatile("" : "+v"(ro)
+                     : "v"(a1[0]), "v"(a1[1]), "v"(a1[2]), "v"(a1[3]), "v"(a2[0]), "v"(a2[1]),
+                       "v"(a2[2]), "v"(a2[3]));
+    else if constexpr (RUN == 8)
+        asm volatile("" : "+v"(ro)
+                     : "v"(a1[0]), "v"(a1[1]), "v"(a1[2]), "v"(a1[3]), "v"(a1[4]), "v"(a1[5]),
+                       "v"(a1[6]), "v"(a1[7]), "v"(a2[0]), "v"(a2[1]), "v"(a2[2]), "v"(a2[3]),
+                       "v"(a2[4]), "v"(a2[5]), "v"(a2[6]), "v"(a2[7]));
+    else
+    {
+        static_assert(RUN == 12, "pins exist for runs of 4, 8 and 12 outputs");
+        asm volatile("" : "+v"(ro)
+                     : "v"(a1[0]), "v"(a1[1]), "v"(a1[2]), "v"(a1[3]), "v"(a1[4]), "v"(a1[5]),
+                       "v"(a1[6]), "v"(a1[7]), "v"(a1[8]), "v"(a1[9]), "v"(a1[10]), "v"(a1[11]));
+        asm volatile("" : "+v"(ro)
+                     : "v"(a2[0]), "v"(a2[1]), "v"(a2[2]), "v"(a2[3]), "v"(a2[4]), "v"(a2[5]),
+                       "v"(a2[6]), "v"(a2[7]), "v"(a2[8]), "v"(a2[9]), "v"(a2[10]), "v"(a2[11]));
+    }
+}
+
+// A run of up to RUN outputs of one class along a lattice line (HOR: along
 // x, else along y), every output at least two sites from the line's ends,
 // so the inputs along the line stay inside it (an input past the last site is
 // clamped: it feeds only outputs that are not stored). Across the line the
@@ -223,7 +257,7 @@ constexpr int kRun = 8;   // outputs per run item
 // path does.
 // Item: p0, along stride, count, last along offset, across position, across
 // size, across stride.
-template <bool HOR>
+template <bool HOR, int RUN>
 __device__ __forceinline__ void run2(const int *__restrict__ it, int lin, int nodes, int gstride,
                                      const int *__restrict__ cls, const double *__restrict__ coef,
                                      const double *__restrict__ g, double *__restrict__ gt)
@@ -242,22 +276,18 @@ __device__ __forceinline__ void run2(const int *__restrict__ it, int lin, int no
     int cof[5];
 #pragma unroll
     for (int c = 0; c < 5; ++c) cof[c] = (clampi(pos + c - 2, 0, nac - 1) - pos) * cs * gstride * 8;
-    double a1[kRun], a2[kRun];
+    double a1[RUN], a2[RUN];
 #pragma unroll
-    for (int r = 0; r < kRun; ++r) { a1[r] = 0.0; a2[r] = 0.0; }
+    for (int r = 0; r < RUN; ++r) { a1[r] = 0.0; a2[r] = 0.0; }
     // One input row in flight ahead of the row being added: a row's offset
-    // passes through an empty asm that takes every accumulator, so its loads
-    // issue after the additions of the rows before the current one (without
-    // it the compiler hoists all rows' loads and runs at one wave per SIMD).
+    // is pinned after the additions of the rows before the current one
+    // (without it the compiler hoists all rows' loads and runs at one wave
+    // per SIMD).
     double x1[5], x2[5], y1[5], y2[5];
     auto load = [&](int v, double *z1, double *z2) {
         const int d = v - 2 < amax ? v - 2 : amax;
         int ro = ((p0 + d * as) * gstride + lin) * 8;
-        static_assert(kRun == 8, "the pin lists eight accumulators per layer");
-        asm volatile("" : "+v"(ro)
-                     : "v"(a1[0]), "v"(a1[1]), "v"(a1[2]), "v"(a1[3]), "v"(a1[4]), "v"(a1[5]),
-                       "v"(a1[6]), "v"(a1[7]), "v"(a2[0]), "v"(a2[1]), "v"(a2[2]), "v"(a2[3]),
-                       "v"(a2[4]), "v"(a2[5]), "v"(a2[6]), "v"(a2[7]));
+        pin_after<RUN>(ro, a1, a2);
 #pragma unroll
         for (int c = 0; c < 5; ++c)
         {
@@ -267,12 +297,12 @@ __device__ __forceinline__ void run2(const int *__restrict__ it, int lin, int no
     };
     load(0, x1, x2);
 #pragma unroll
-    for (int v = 0; v < kRun + 4; ++v)
+    for (int v = 0; v < RUN + 4; ++v)
     {
-        if (v + 1 < kRun + 4) load(v + 1, y1, y2);
+        if (v + 1 < RUN + 4) load(v + 1, y1, y2);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int r = 0; r < kRun; ++r)
+        for (int r = 0; r < RUN; ++r)
         {
             const int dv = v - 2 - r;
             if (dv < -2 || dv > 2) continue;
@@ -290,7 +320,7 @@ __device__ __forceinline__ void run2(const int *__restrict__ it, int lin, int no
         for (int c = 0; c < 5; ++c) { x1[c] = y1[c]; x2[c] = y2[c]; }
     }
 #pragma unroll
-    for (int r = 0; r < kRun; ++r)
+    for (int r = 0; r < RUN; ++r)
         if (r < cnt)
         {
             double *out = gt + ((long)p0 + (long)r * as) * gstride + lin;
@@ -328,6 +358,7 @@ __device__ __forceinline__ void point2(int i, int lin, int nodes, int gstride,
 // gt = the class stencils applied to both layers. One thread per (item,
 // node), S threads per item (the nodes rounded up to whole waves, so each
 // wave takes one kind of item): column runs, then row runs, then corners.
+template <int RUN>
 __global__ __launch_bounds__(256) void k_apply2(int nv, int nh, int nit, int S, int nodes,
                                                 int gstride, const int *__restrict__ items,
                                                 const int *__restrict__ lat,
@@ -342,9 +373,111 @@ __global__ __launch_bounds__(256) void k_apply2(int nv, int nh, int nit, int S, 
     const int item = (int)(e / S), lin = (int)(e - (long)item * S);
     if (item >= nit || lin >= nodes) return;
     const int *it = items + 8 * (size_t)item;
-    if (item < nv) run2<false>(it, lin, nodes, gstride, cls, coef, g, gt);
-    else if (item < nv + nh) run2<true>(it, lin, nodes, gstride, cls, coef, g, gt);
+    if (item < nv) run2<false, RUN>(it, lin, nodes, gstride, cls, coef, g, gt);
+    else if (item < nv + nh) run2<true, RUN>(it, lin, nodes, gstride, cls, coef, g, gt);
     else point2(it[0], lin, nodes, gstride, lat, cls, coef, g, gt);
+}
+
+// The lattice instances of a 2-D context: particle ranges and sizes.
+struct Inst2
+{
+    std::vector<int> base, nx, ny;
+    std::vector<double> h;
+};
+
+Inst2 instances2(const ab_ctx *c)
+{
+    if (c->nbatch > 0) return Inst2{c->bt_base, c->bt_nx, c->bt_ny, c->bt_h};
+    return Inst2{{0, c->N}, {c->cfg.nx}, {c->cfg.ny}, {c->cfg.radius}};
+}
+
+// The apply's work items for runs of `run` outputs: column runs over rows
+// 2 .. ny-3 of every column, row runs over columns 2 .. nx-3 of rows 0, 1,
+// ny-2, ny-1, and the 16 corners of each instance.
+std::vector<int> apply_items2(const Inst2 &L, int run, int &nv, int &nh)
+{
+    std::vector<int> itv, ith, itc;
+    for (size_t k = 0; k < L.nx.size(); ++k)
+    {
+        const int nx = L.nx[k], ny = L.ny[k], b0 = L.base[k];
+        for (int ix = 0; ix < nx; ++ix)
+            for (int iy = 2; iy <= ny - 3; iy += run)
+            {
+                const int it[8] = {b0 + ix + nx * iy, nx, std::min(run, ny - 2 - iy), ny - 1 - iy,
+                                   ix, nx, 1, 0};
+                itv.insert(itv.end(), it, it + 8);
+            }
+        for (int iy : {0, 1, ny - 2, ny - 1})
+        {
+            for (int ix = 2; ix <= nx - 3; ix += run)
+            {
+                const int it[8] = {b0 + ix + nx * iy, 1, std::min(run, nx - 2 - ix), nx - 1 - ix,
+                                   iy, ny, nx, 0};
+                ith.insert(ith.end(), it, it + 8);
+            }
+            for (int ix : {0, 1, nx - 2, nx - 1})
+            {
+                const int it[8] = {b0 + ix + nx * iy, 0, 0, 0, 0, 0, 0, 0};
+                itc.insert(itc.end(), it, it + 8);
+            }
+        }
+    }
+    nv = (int)itv.size() / 8;
+    nh = (int)ith.size() / 8;
+    std::vector<int> items(itv);
+    items.insert(items.end(), ith.begin(), ith.end());
+    items.insert(items.end(), itc.begin(), itc.end());
+    return items;
+}
+
+// Builds and uploads the items of run length index `run` (-1: the default)
+// into a fresh device array (the old one is freed after the stream drains).
+int upload_items2(ab_ctx *c, int run)
+{
+    int nv = 0, nh = 0;
+    const std::vector<int> items =
+        apply_items2(instances2(c), kRuns[run < 0 ? kRunDefault : run], nv, nh);
+    int *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(int) * items.size()) != hipSuccess)
+    { c->err = "2-D table setup: device allocation failed"; return AB_ERR_NOMEM; }
+    if (hipMemcpyAsync(d, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+    {
+        (void)hipFree(d);
+        c->err = "2-D table setup: host-to-device copy failed";
+        return AB_ERR_HIP;
+    }
+    if (c->t2_items) (void)hipFree(c->t2_items);
+    c->t2_items = d;
+    c->t2_nv = nv;
+    c->t2_nh = nh;
+    c->t2_nit = (int)items.size() / 8;
+    c->t2_run = run;
+    c->tgen++;
+    return AB_OK;
+}
+
+template <int RUN>
+int apply2_launch(ab_ctx *c)
+{
+    const int S = (c->nodes + 63) / 64 * 64;
+    const long n = (long)c->t2_nit * S;
+    k_apply2<RUN><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
+        c->t2_nv, c->t2_nh, c->t2_nit, S, c->nodes, c->gstride, c->t2_items, c->t2_lat, c->t2_cls,
+        c->t2_coef, c->g, c->gt);
+    if (hipGetLastError() != hipSuccess) { c->err = "2-D table apply launch failed"; return AB_ERR_HIP; }
+    return AB_OK;
+}
+
+int apply2(ab_ctx *c)
+{
+    switch (c->t2_run < 0 ? kRunDefault : c->t2_run)
+    {
+    case 0: return apply2_launch<kRuns[0]>(c);
+    case 2: return apply2_launch<kRuns[2]>(c);
+    default: return apply2_launch<kRuns[1]>(c);
+    }
 }
 
 }  // namespace
@@ -353,6 +486,7 @@ namespace ab {
 
 void table2d_free(ab_ctx *c)
 {
+    c->tgen++;
     void *ptrs[] = {c->t2_coef, c->t2_cls, c->t2_rep, c->t2_lat, c->t2_items};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     c->t2_coef = nullptr; c->t2_cls = nullptr; c->t2_rep = nullptr; c->t2_lat = nullptr;
@@ -370,9 +504,9 @@ int table2d_setup(ab_ctx *c)
 {
     const ab_config &cfg = c->cfg;
     const int N = c->N;
-    std::vector<int> base{0, N}, nxs{cfg.nx}, nys{cfg.ny};
-    std::vector<double> hs{cfg.radius};
-    if (c->nbatch > 0) { base = c->bt_base; nxs = c->bt_nx; nys = c->bt_ny; hs = c->bt_h; }
+    const Inst2 inst = instances2(c);
+    const std::vector<int> &base = inst.base, &nxs = inst.nx, &nys = inst.ny;
+    const std::vector<double> &hs = inst.h;
     const int ninst = (int)nxs.size();
     table2d_free(c);
     for (int k = 0; k < ninst; ++k)
@@ -449,54 +583,20 @@ int table2d_setup(ab_ctx *c)
             }
         }
     }
-    // apply items: column runs over rows 2 .. ny-3 of every column, row runs
-    // over columns 2 .. nx-3 of rows 0, 1, ny-2, ny-1, and the 16 corners
-    std::vector<int> itv, ith, itc;
-    for (int k = 0; k < ninst; ++k)
-    {
-        const int nx = nxs[k], ny = nys[k], b0 = base[k];
-        for (int ix = 0; ix < nx; ++ix)
-            for (int iy = 2; iy <= ny - 3; iy += kRun)
-            {
-                const int it[8] = {b0 + ix + nx * iy, nx, std::min(kRun, ny - 2 - iy), ny - 1 - iy,
-                                   ix, nx, 1, 0};
-                itv.insert(itv.end(), it, it + 8);
-            }
-        for (int iy : {0, 1, ny - 2, ny - 1})
-        {
-            for (int ix = 2; ix <= nx - 3; ix += kRun)
-            {
-                const int it[8] = {b0 + ix + nx * iy, 1, std::min(kRun, nx - 2 - ix), nx - 1 - ix,
-                                   iy, ny, nx, 0};
-                ith.insert(ith.end(), it, it + 8);
-            }
-            for (int ix : {0, 1, nx - 2, nx - 1})
-            {
-                const int it[8] = {b0 + ix + nx * iy, 0, 0, 0, 0, 0, 0, 0};
-                itc.insert(itc.end(), it, it + 8);
-            }
-        }
-    }
-    std::vector<int> items(itv);
-    items.insert(items.end(), ith.begin(), ith.end());
-    items.insert(items.end(), itc.begin(), itc.end());
-    if (hipMalloc((void **)&c->t2_items, sizeof(int) * items.size()) != hipSuccess ||
-        hipMalloc((void **)&c->t2_coef, sizeof(double) * ncls * kOff2 * (size_t)c->nodes) != hipSuccess ||
+    if (hipMalloc((void **)&c->t2_coef, sizeof(double) * ncls * kOff2 * (size_t)c->nodes) != hipSuccess ||
         hipMalloc((void **)&c->t2_cls, sizeof(int) * (size_t)N) != hipSuccess ||
         hipMalloc((void **)&c->t2_rep, sizeof(int) * ncls) != hipSuccess ||
         hipMalloc((void **)&c->t2_lat, sizeof(int) * 4 * (size_t)N) != hipSuccess)
     { table2d_free(c); c->err = "2-D table setup: device allocation failed"; return AB_ERR_NOMEM; }
     c->t2_ncls = ncls;
-    c->t2_nv = (int)itv.size() / 8;
-    c->t2_nh = (int)ith.size() / 8;
-    c->t2_nit = (int)items.size() / 8;
     if (hipMemcpyAsync(c->t2_cls, cls.data(), sizeof(int) * N, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(c->t2_rep, rep.data(), sizeof(int) * ncls, hipMemcpyHostToDevice, s) != hipSuccess ||
         hipMemcpyAsync(c->t2_lat, lat.data(), sizeof(int) * 4 * (size_t)N, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(c->t2_items, items.data(), sizeof(int) * items.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
     { table2d_free(c); c->err = "2-D table setup: host-to-device copy failed"; return AB_ERR_HIP; }
-    return AB_OK;
+    const int rc = upload_items2(c, c->t2_run);
+    if (rc) table2d_free(c);
+    return rc;
 }
 
 int table2d_prepare(ab_ctx *c, double dt)
@@ -515,12 +615,63 @@ int table2d_transport(ab_ctx *c, double dt)
 {
     int rc;
     if (c->t2_dt != dt && (rc = table2d_prepare(c, dt))) return rc;
-    const int S = (c->nodes + 63) / 64 * 64;
-    const long n = (long)c->t2_nit * S;
-    k_apply2<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(
-        c->t2_nv, c->t2_nh, c->t2_nit, S, c->nodes, c->gstride, c->t2_items, c->t2_lat, c->t2_cls,
-        c->t2_coef, c->g, c->gt);
-    return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;
+    return apply2(c);
+}
+
+// Times the apply at every run length (`reps` launches after a warm-up; the
+// apply writes only gt) and keeps the fastest. New items invalidate the
+// graphs captured on the context before.
+int table2d_autotune(ab_ctx *c, double dt, int reps, int cap, char (*names)[32], float *ms,
+                     int *n, int *chosen)
+{
+    int rc;
+    if (!c->t2_coef) { c->err = "2-D table transport before its setup"; return AB_ERR_STATE; }
+    if (c->t2_dt != dt && (rc = table2d_prepare(c, dt))) return rc;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) { c->err = "autotune: event creation failed"; return AB_ERR_HIP; }
+    if (hipEventCreate(&e1) != hipSuccess)
+    {
+        (void)hipEventDestroy(e0);
+        c->err = "autotune: event creation failed";
+        return AB_ERR_HIP;
+    }
+    int best = -1;
+    float tbest = 0.0f;
+    rc = AB_OK;
+    for (int i = 0; i < kNRuns && !rc; ++i)
+    {
+        rc = upload_items2(c, i);
+        float tmin = 0.0f;
+        for (int r = 0; r <= reps && !rc; ++r)           // r == 0 warms up
+        {
+            (void)hipEventRecord(e0, c->stream);
+            rc = apply2(c);
+            (void)hipEventRecord(e1, c->stream);
+            float t = 0.0f;
+            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+            { c->err = "autotune: timing failed"; rc = AB_ERR_HIP; }
+            if (r == 1 || (r > 1 && t < tmin)) tmin = t;
+        }
+        if (rc) break;
+        if (ms && i < cap) ms[i] = tmin;
+        if (names && i < cap) snprintf(names[i], 32, "R%d", kRuns[i]);
+        if (best < 0 || tmin < tbest) { best = i; tbest = tmin; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rc) rc = upload_items2(c, best);
+    if (rc) return rc;
+    if (n) *n = kNRuns;
+    if (chosen) *chosen = best;
+    return AB_OK;
+}
+
+int table2d_set_run(ab_ctx *c, int run)
+{
+    if (run < -1 || run >= kNRuns) { c->err = "transport shape index out of range"; return AB_ERR_ARG; }
+    if (!c->t2_coef) { c->t2_run = run; return AB_OK; }   // taken at the setup
+    if (run == c->t2_run) return AB_OK;
+    return upload_items2(c, run);
 }
 
 }  // namespace ab

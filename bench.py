@@ -481,6 +481,11 @@ def components(A):
             sec_e = (time.perf_counter() - t0) / (reps * per)
             # the same instances as one batch context: one launch per stage
             batch = A.Batch(sols)
+            runs = None
+            if prec == A.PREC_FP64_TABLE:          # the apply's run length, timed
+                table, chosen = batch.autotune(dt, reps=3)
+                runs = {"apply_ms": {name: ms for _, name, ms in table},
+                        "chosen": table[chosen][1]}
             batch.steps(dt, per)
             batch.synchronize()
             t0 = time.perf_counter()
@@ -504,6 +509,8 @@ def components(A):
                         "batch_ms_per_step": sec_b * 1e3,
                         "batch_graph_ms_per_step": sec_bg * 1e3,
                         "batch_graph_updates_per_s": upd / sec_bg}
+            if runs:
+                out[key]["batch_run_lengths"] = runs
             for sv in sols:
                 sv.close()
             log(f"[bench] 16 cavity2d instances {tag}: graph {sec_g * 1e3:.3f} ms/step, "

@@ -357,7 +357,10 @@ int ab_body_step_many(int n, ab_ctx *const *ctxs, const ab_body_task *tasks);
  * names/ms (cap entries each, may be NULL) receive every
  * candidate's name and best time in ms, "n/a" and -1 for a shape that does
  * not tile the lattice; *n the number of candidates, *chosen the index kept.
- * ab_set_transport_shape forces an index (-1: the default for the lattice). */
+ * ab_set_transport_shape forces an index (-1: the default for the lattice).
+ * On a 2-D table context the candidates are the apply's run lengths ("R4",
+ * "R8", "R12": outputs per work item along a lattice line, bit-identical
+ * results; -1 gives R8). */
 int ab_autotune(ab_ctx *ctx, double dt, int reps, int cap, char (*names)[32], float *ms,
                 int *n, int *chosen);
 int ab_set_transport_shape(ab_ctx *ctx, int shape);
@@ -378,11 +381,14 @@ int ab_copy_calibration(unsigned long long bytes, int reps);
  * for the replay, as after eager steps) and ab_graph_synchronize waits for
  * it. A replay gives exactly the eager steps' results. Refused
  * (AB_ERR_STATE) while a context profiles or records training rows; table
- * coefficients are built for dt before the capture. A graph holds the
- * clouds' device addresses: once
- * any of its contexts is re-uploaded or re-laid out (ab_upload,
- * ab_cloud_manage, ab_body_step) ab_graph_launch refuses with AB_ERR_STATE;
- * create the graph again. */
+ * coefficients are built for dt before the capture, and again before a
+ * replay where an eager step at another dt or a wall-state change left them
+ * stale (on the context's stream; the launch stays asynchronous). A graph
+ * holds the clouds' device addresses: once any of its contexts is
+ * re-uploaded or re-laid out (ab_upload, ab_cloud_manage, ab_body_step), or
+ * a 2-D table context's transport is set up again (ab_rebuild_neighbours,
+ * ab_set_wall_state, ab_autotune, ab_set_transport_shape), ab_graph_launch
+ * refuses with AB_ERR_STATE; create the graph again. */
 typedef struct ab_graph ab_graph;
 int ab_graph_create(ab_ctx *const *ctxs, int n, double dt, int nsteps, ab_graph **out);
 int ab_graph_launch(ab_graph *graph);

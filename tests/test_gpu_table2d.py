@@ -91,3 +91,61 @@ def test_table2d_mixed_stays_refused():
     case, cl, ocl, neigh, nn = setup("cavity2d", 10, 6, order=2)
     with pytest.raises(A.ALEBGKError):
         A.Solver(cl, precision=A.PREC_MIXED)
+
+
+@pytest.mark.parametrize("nx", [13, 31])
+def test_table2d_run_lengths_bit_identical(nx):
+    """The apply's run lengths (4, 8, 12 outputs per item) sum every output
+    in the same order: forced or autotuned, gt is bit-identical. nx=13
+    leaves ragged runs (9 interior rows per column)."""
+    _say(f"[table2d run lengths] cavity2d nx={nx} nv=16")
+    case, cl, ocl, neigh, nn = setup("cavity2d", nx, 16, order=2)
+    dt = _dt(nx)
+    sol = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    sol.transport_only(dt)
+    want = sol.gt()
+    for shape in (0, 1, 2, -1):
+        sol.set_transport_shape(shape)
+        sol.transport_only(dt)
+        assert np.array_equal(sol.gt(), want), shape
+    table, chosen = sol.autotune(dt, reps=2)
+    assert [name for _, name, _ in table] == ["R4", "R8", "R12"]
+    assert 0 <= chosen < 3
+    sol.transport_only(dt)
+    assert np.array_equal(sol.gt(), want)
+    with pytest.raises(A.ALEBGKError):
+        sol.set_transport_shape(3)
+
+
+def test_table2d_graph_refuses_new_run_length():
+    """A graph holds the apply's work items: a new run length after the
+    capture makes the replay refuse instead of reading freed items."""
+    _say("[table2d graph] cavity2d 12^2 nv8, run length changed after capture")
+    case, cl, ocl, neigh, nn = setup("cavity2d", 12, 8, order=2)
+    sol = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    g = A.Graph([sol], _dt(12), 1)
+    g.launch()
+    g.synchronize()
+    sol.set_transport_shape(0)
+    with pytest.raises(A.ALEBGKError, match="set up again"):
+        g.launch()
+    g.close()
+
+
+def test_table2d_graph_rebuilds_coefficients_for_its_dt():
+    """An eager step at another dt rebuilds the coefficients in place; the
+    next replay builds them for the graph's dt again, so eager and graphed
+    sequences agree bit for bit."""
+    _say("[table2d graph] cavity2d 12^2 nv8, eager step at another dt between replays")
+    case, cl, ocl, neigh, nn = setup("cavity2d", 12, 8, order=2)
+    dt = _dt(12)
+    a = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    b = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    g = A.Graph([a], dt, 1)
+    a.step(0.5 * dt)
+    g.launch()
+    g.synchronize()
+    b.step(0.5 * dt)
+    b.step(dt)
+    assert np.array_equal(a.g(), b.g())
+    g.close()
