@@ -202,22 +202,41 @@ int fp64_step(ab_ctx *c, double dt)
     tm.tic();
     if ((rc = fp64_transport(c, dt))) return rc;
     tm.toc(ST_TRANSPORT);
+    const bool fused = table2d(c);
+    if (fused)
+    {
+        // the 2-D table formulation: positivity, commit, wall and collision
+        // in one pass, both rescales in the end-of-step one
+        tm.tic();
+        launch_commit_collide2(v, c->g, c->gt, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux, c->uy,
+                               c->T, c->limflag, c->t2_pnew, c->pmass, s);
+        tm.toc(ST_WALL_COLLIDE);
+        tm.tic();
+        launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_FUSED, c->cfg.no_mass_fix, s, c->seg,
+                           c->nbatch, c->t2_pnew);
+        tm.toc(ST_REDUCE);
+    }
+    else
+    {
+        tm.tic();
+        launch_limit_commit(v, c->g, c->gt, c->rho, c->limflag, c->pmass, s);
+        tm.toc(ST_LIMIT_COMMIT);
+        tm.tic();
+        launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s, c->seg, c->nbatch);
+        tm.toc(ST_REDUCE);
+        tm.tic();
+        launch_wall_collide(v, dm_args(c), c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux,
+                            c->uy, c->uz, c->T, c->pmass, s);
+        tm.toc(ST_WALL_COLLIDE);
+        dm_step_done(c);
+        tm.tic();
+        launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, c->cfg.no_mass_fix, s, c->seg,
+                           c->nbatch);
+        tm.toc(ST_REDUCE);
+    }
     tm.tic();
-    launch_limit_commit(v, c->g, c->gt, c->rho, c->limflag, c->pmass, s);
-    tm.toc(ST_LIMIT_COMMIT);
-    tm.tic();
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_NEW, 0, s, c->seg, c->nbatch);
-    tm.toc(ST_REDUCE);
-    tm.tic();
-    launch_wall_collide(v, dm_args(c), c->g, c->scal, dt, c->cfg.tau, c->cfg.R, c->rho, c->ux,
-                        c->uy, c->uz, c->T, c->pmass, s);
-    tm.toc(ST_WALL_COLLIDE);
-    dm_step_done(c);
-    tm.tic();
-    launch_reduce_mass(c->N, c->pmass, c->scal, R_MASS_COLL, c->cfg.no_mass_fix, s, c->seg, c->nbatch);
-    tm.toc(ST_REDUCE);
-    tm.tic();
-    launch_final(v, c->g, c->scal, c->cfg.R, c->rho, c->ux, c->uy, c->uz, c->T, c->pmass, s);
+    launch_final(v, c->g, c->scal, c->cfg.R, c->rho, c->ux, c->uy, c->uz, c->T, c->pmass, s,
+                 nullptr, fused);
     tm.toc(ST_FINAL);
     c->mass_valid = true;
     return hipGetLastError() == hipSuccess ? AB_OK : AB_ERR_HIP;

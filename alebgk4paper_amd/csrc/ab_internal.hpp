@@ -62,7 +62,10 @@ enum ScalarSlot
     S_COUNT = 8
 };
 
-enum ReduceMode { R_MASS_OLD = 0, R_MASS_NEW = 1, R_MASS_COLL = 2 };
+// R_MASS_FUSED: the committed masses (second array) and the collided
+// masses of the unscaled rows (first array) of the 2-D table step's fused
+// commit and collision, giving both scales at once.
+enum ReduceMode { R_MASS_OLD = 0, R_MASS_NEW = 1, R_MASS_COLL = 2, R_MASS_FUSED = 3 };
 
 // Stages timed by ab_profile, in report order.
 enum Stage
@@ -156,6 +159,7 @@ struct ab_ctx
     int t2_nv = 0, t2_nh = 0, t2_nit = 0;   // column runs, row runs, all items
     int t2_run = -1;               // run length index of the apply (-1: the default, 8 outputs)
     double t2_dt = -1.0;           // dt the coefficients were built for
+    double *t2_pnew = nullptr;     // committed mass per particle of the fused commit and collision
 
     // discrete-Maxwellian collision (cfg.collision) and its MLP warm start
     double *dm_alpha = nullptr;    // N x 5 solution of the last step
@@ -256,7 +260,8 @@ void launch_mass_partials(const DevCloud &c, const double *g, double *pmass, hip
 // One sum per instance when seg is given (nseg instances, first particles
 // seg[0..nseg], scalars S_COUNT apart); else one over all N.
 void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int nofix,
-                        hipStream_t s, const int *seg = nullptr, int nseg = 1);
+                        hipStream_t s, const int *seg = nullptr, int nseg = 1,
+                        const double *pnew = nullptr);
 void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
                          double *limflag, double *pmass, hipStream_t s);
 // guard (may be null): when given, the launch acts only if *guard is set,
@@ -266,9 +271,18 @@ void launch_wall_collide(const DevCloud &c, const DmArgs &dm, double *g, const d
                          double tau, double R, double *rho, double *ux, double *uy,
                          double *uz, double *T, double *pmass, hipStream_t s,
                          const int *guard = nullptr);
+// fold_scale1: after launch_commit_collide2, whose wall densities still
+// lack the first rescale, the boundary particles' rho is multiplied by it.
 void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
                   double *ux, double *uy, double *uz, double *T, double *pmass,
-                  hipStream_t s, const int *guard = nullptr);
+                  hipStream_t s, const int *guard = nullptr, bool fold_scale1 = false);
+// The 2-D step's positivity, commit, wall and collision in one pass (the
+// first rescale folded into the end-of-step one): g from gt, the committed
+// masses into pnew, the collided masses into pmass.
+void launch_commit_collide2(const DevCloud &c, double *g, const double *gt, double dt,
+                            double tau, double R, double *rho, double *ux, double *uy,
+                            double *T, double *limflag, double *pnew, double *pmass,
+                            hipStream_t s);
 
 // ---- k_neigh.hip -------------------------------------------------------------
 int launch_neighbours(const DevCloud &c, const ab_config &cfg, int *vox, int *vcount,

@@ -350,9 +350,10 @@ __global__ __launch_bounds__(kBlock) void k_mass_partials(DevCloud c, const doub
 // its scalars (S_COUNT apart) in the same order as a cloud of its own.
 __global__ __launch_bounds__(1024) void k_reduce_mass(int N, const double *__restrict__ pmass,
                                                       double *scal, int mode, int nofix,
-                                                      const int *__restrict__ seg)
+                                                      const int *__restrict__ seg,
+                                                      const double *__restrict__ pnew)
 {
-    __shared__ double sh[1024 / kWave];
+    __shared__ double sh[1024 / kWave], shn[1024 / kWave];
     int lo = 0, hi = N;
     if (seg)
     {
@@ -360,16 +361,32 @@ __global__ __launch_bounds__(1024) void k_reduce_mass(int N, const double *__res
         hi = seg[blockIdx.x + 1];
         scal += (size_t)blockIdx.x * S_COUNT;
     }
-    double s = 0.0;
-    for (int i = lo + threadIdx.x; i < hi; i += 1024) s += pmass[i];
+    double s = 0.0, sn = 0.0;
+    for (int i = lo + threadIdx.x; i < hi; i += 1024)
+    {
+        s += pmass[i];
+        if (pnew) sn += pnew[i];
+    }
     s = wave_sum(s);
-    if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = s;
+    sn = wave_sum(sn);
+    if ((threadIdx.x & (kWave - 1)) == 0) { sh[threadIdx.x / kWave] = s; shn[threadIdx.x / kWave] = sn; }
     __syncthreads();
     if (threadIdx.x == 0)
     {
-        double t = 0.0;
-        for (int q = 0; q < 1024 / kWave; ++q) t += sh[q];
-        if (mode == R_MASS_OLD)
+        double t = 0.0, tn = 0.0;
+        for (int q = 0; q < 1024 / kWave; ++q) { t += sh[q]; tn += shn[q]; }
+        if (mode == R_MASS_FUSED)
+        {
+            // massOld / massNew, the collided mass after that rescale, and the
+            // one end-of-step factor that the unscaled collided rows need:
+            // massOld / massColl * massOld / massNew = massOld / t
+            const double s1 = (tn > 1e-30) ? scal[S_MASS_OLD] / tn : 1.0;
+            scal[S_MASS_NEW] = tn;
+            scal[S_SCALE1] = s1;
+            scal[S_MASS_COLL] = s1 * t;
+            scal[S_SCALE2] = (!nofix && s1 * t > 1e-30) ? scal[S_MASS_OLD] / t : s1;
+        }
+        else if (mode == R_MASS_OLD)
             scal[S_MASS_OLD] = t;
         else if (mode == R_MASS_NEW)
         {
@@ -728,6 +745,152 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
     if (lane == 0) pmass[i] = m[0];
 }
 
+// kPositivity + kCommit + kScaleG + kWallBC / kMoments + kCollisionNode of
+// the 2-D step (CudaBackend.cu:304-336, 478-485, 338-389, 124-158,
+// 1165-1179) in one pass per particle for the 2-D table formulation: gt is
+// read (the row stays in cache between the passes over it) and the collided
+// g written once, where k_limit_commit<2> and k_wall_collide<2> read and
+// write g twice. The rescale between commit and collision is left out: the
+// wall density, the density moment and the Maxwellians scale with g, u and
+// T do not, so the collided row of the unscaled commit is the collided row
+// of the rescaled one divided by massOld / massNew. The reduction then takes
+// the committed masses (pnew) and the collided masses (pmass) and gives
+// k_final the one factor that the rows need; the wall densities get the
+// first factor there. Results agree with the separate kernels to rounding.
+__global__ __launch_bounds__(kBlock) void k_commit_collide2(DevCloud c, double *__restrict__ g,
+                                                            const double *__restrict__ gt,
+                                                            double dt, double tau, double R,
+                                                            double *rho, double *ux, double *uy,
+                                                            double *T, double *__restrict__ limflag,
+                                                            double *__restrict__ pnew,
+                                                            double *__restrict__ pmass)
+{
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / kWave) + (int)(threadIdx.x / kWave);
+    if (i >= c.N) return;                              // whole wave: one particle
+    const int lane = threadIdx.x & (kWave - 1);
+    const size_t gs = (size_t)c.gstride;
+    double *gp = g + (size_t)i * gs;
+    const double *tp = gt + (size_t)i * gs;
+    const int nodes = c.nodes, Nv = c.Nv;
+    const double dvd = c.dv * c.dv;
+
+    double rp[2] = {0.0, 0.0};                     // mPos, mNeg of g1
+    for (int k = lane; k < nodes; k += kWave)
+    {
+        const double v = tp[k];
+        if (v >= 0) rp[0] += v; else rp[1] += v;
+    }
+    wave_sums<2>(rp);
+    const bool fires = (rp[1] < -1e-30 && rp[0] > 1e-30 && (rp[0] + rp[1]) > 0);
+    const double sc = fires ? (rp[0] + rp[1]) / rp[0] : 1.0;
+    auto committed = [&](int k, double &v1, double &v2) {
+        v1 = tp[k];
+        v2 = tp[k + nodes];
+        if (fires)
+        {
+            if (v1 < 0) { v1 = 0; v2 = 0; }
+            else { v1 *= sc; v2 *= sc; }
+        }
+    };
+    double m[1] = {0.0};
+    double mnew = 0.0;
+
+    if (c.bnd[i])
+    {
+        const double nx = c.wnx[i], ny = c.wny[i];
+        const double Uwx = c.wux[i], Uwy = c.wuy[i];
+        const double Tw = c.wt[i];
+        const double RT2w = 2.0 * R * Tw;
+        const double norm = AB_PI * RT2w;
+        double r[3] = {0.0, 0.0, 0.0};             // intM, intP, committed mass
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            const double vx = vnode(c.vlo, c.dv, k % Nv);
+            const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
+            double v, v2;
+            committed(k, v, v2);
+            const double cn = wall_cn2(vx, vy, Uwx, Uwy, nx, ny);
+            if (cn < 0.0) r[0] += cn * v * dvd;
+            else if (cn > 0.0)
+            {
+                const double e2 = (vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy);
+                const double Mw = exp(-(e2) / RT2w) / norm;
+                r[1] += cn * Mw * dvd;
+            }
+            r[2] += v * dvd;
+        }
+        wave_sums<3>(r);
+        const double rho_w = (r[1] > 1e-30) ? (-r[0] / r[1]) : 1.0;
+        mnew = r[2];
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            const double vx = vnode(c.vlo, c.dv, k % Nv);
+            const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
+            double v, v2;
+            committed(k, v, v2);
+            const double cn = wall_cn2(vx, vy, Uwx, Uwy, nx, ny);
+            if (cn > 0.0)
+            {
+                const double e2 = (vx - Uwx) * (vx - Uwx) + (vy - Uwy) * (vy - Uwy);
+                v = rho_w * exp(-(e2) / RT2w) / norm;
+                v2 = 0.5 * R * Tw * v;
+            }
+            gp[k] = v;
+            gp[k + nodes] = v2;
+            m[0] += v * dvd;
+        }
+        if (lane == 0) { rho[i] = rho_w; ux[i] = Uwx; uy[i] = Uwy; T[i] = Tw; }
+    }
+    else
+    {
+        double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            const double vx = vnode(c.vlo, c.dv, k % Nv);
+            const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
+            double f, f2;
+            committed(k, f, f2);
+            r[0] += f * dvd;
+            r[1] += vx * f * dvd;
+            r[2] += vy * f * dvd;
+            r[4] += (vx * vx + vy * vy) * f * dvd + 2.0 * f2 * dvd;
+        }
+        wave_sums<5>(r);
+        mnew = r[0];
+        double mrho = rho[i], mux = ux[i], muy = uy[i], mT = T[i];
+        if (r[0] > 1e-30)
+        {
+            mrho = r[0];
+            mux = r[1] / r[0];
+            muy = r[2] / r[0];
+            const double t1 = (r[4] / r[0] - (mux * mux + muy * muy)) / (3.0 * R);
+            mT = t1 > 1e-6 ? t1 : 1e-6;
+        }
+        const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
+        for (int k = lane; k < nodes; k += kWave)
+        {
+            const double vx = vnode(c.vlo, c.dv, k % Nv);
+            const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
+            double f, f2;
+            committed(k, f, f2);
+            const double Mg1 = maxwellian_g1(vx, vy, mux, muy, mT, mrho, R);
+            f = coeff * (f + lam * Mg1);
+            f2 = coeff * (f2 + lam * 0.5 * R * mT * Mg1);
+            gp[k] = f;
+            gp[k + nodes] = f2;
+            m[0] += f * dvd;
+        }
+        if (lane == 0) { rho[i] = mrho; ux[i] = mux; uy[i] = muy; T[i] = mT; }
+    }
+    wave_sums<1>(m);
+    if (lane == 0)
+    {
+        pmass[i] = m[0];
+        pnew[i] = mnew;
+        limflag[i] = fires ? 1.0 : 0.0;
+    }
+}
+
 // kScaleG (massOld/massColl) + kMoments3D / kMoments (CudaBackend.cu:1773-1788).
 // One wave per particle, four per workgroup: each lane loads four nodes of
 // both layers before it adds any, and the sums are wave reductions, so a
@@ -738,7 +901,8 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
                                                   double *rho, double *ux, double *uy,
                                                   double *uz, double *T,
                                                   double *__restrict__ pmass,
-                                                  const int *__restrict__ guard)
+                                                  const int *__restrict__ guard,
+                                                  bool fold_scale1)
 {
     constexpr int kPer = kBlock / kWave;
     const int i = xcd_remap(blockIdx.x, gridDim.x) * kPer + (int)(threadIdx.x / kWave);
@@ -747,6 +911,7 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
     const int lane = threadIdx.x & (kWave - 1);
     if (c.inst) scal += (size_t)c.inst[i] * S_COUNT;
     const double s2 = scal[S_SCALE2];
+    const double s1 = scal[S_SCALE1];
     const bool doScale = (s2 != 1.0);
     const size_t gs = (size_t)c.gstride;
     double *gp = g + (size_t)i * gs;
@@ -799,6 +964,7 @@ __global__ __launch_bounds__(kBlock) void k_final(DevCloud c, double *__restrict
     if (lane == 0)
     {
         pmass[i] = r[5];
+        if (!inner && fold_scale1) rho[i] *= s1;
         if (inner && r[0] > 1e-30)
         {
             double mux = r[1] / r[0], muy = r[2] / r[0], muz = 0.0, u2;
@@ -988,9 +1154,9 @@ void launch_mass_partials(const DevCloud &c, const double *g, double *pmass, hip
 }
 
 void launch_reduce_mass(int N, const double *pmass, double *scal, int mode, int nofix,
-                        hipStream_t s, const int *seg, int nseg)
+                        hipStream_t s, const int *seg, int nseg, const double *pnew)
 {
-    k_reduce_mass<<<seg ? nseg : 1, 1024, 0, s>>>(N, pmass, scal, mode, nofix, seg);
+    k_reduce_mass<<<seg ? nseg : 1, 1024, 0, s>>>(N, pmass, scal, mode, nofix, seg, pnew);
 }
 
 void launch_limit_commit(const DevCloud &c, double *g, const double *gt, const double *rho,
@@ -1016,13 +1182,25 @@ void launch_wall_collide(const DevCloud &c, const DmArgs &dm, double *g, const d
 
 void launch_final(const DevCloud &c, double *g, const double *scal, double R, double *rho,
                   double *ux, double *uy, double *uz, double *T, double *pmass, hipStream_t s,
-                  const int *guard)
+                  const int *guard, bool fold_scale1)
 {
     const int nb = (c.N + kBlock / kWave - 1) / (kBlock / kWave);
     if (c.dim == 3)
-        k_final<3><<<nb, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
+        k_final<3><<<nb, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard,
+                                         fold_scale1);
     else
-        k_final<2><<<nb, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard);
+        k_final<2><<<nb, kBlock, 0, s>>>(c, g, scal, R, rho, ux, uy, uz, T, pmass, guard,
+                                         fold_scale1);
+}
+
+void launch_commit_collide2(const DevCloud &c, double *g, const double *gt, double dt,
+                            double tau, double R, double *rho, double *ux, double *uy,
+                            double *T, double *limflag, double *pnew, double *pmass,
+                            hipStream_t s)
+{
+    const int nb = (c.N + kBlock / kWave - 1) / (kBlock / kWave);
+    k_commit_collide2<<<nb, kBlock, 0, s>>>(c, g, gt, dt, tau, R, rho, ux, uy, T, limflag, pnew,
+                                            pmass);
 }
 
 void launch_mood_check(const DevCloud &c, double *gt, const double *rho, int *mood,

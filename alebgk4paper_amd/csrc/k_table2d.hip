@@ -487,10 +487,10 @@ namespace ab {
 void table2d_free(ab_ctx *c)
 {
     c->tgen++;
-    void *ptrs[] = {c->t2_coef, c->t2_cls, c->t2_rep, c->t2_lat, c->t2_items};
+    void *ptrs[] = {c->t2_coef, c->t2_cls, c->t2_rep, c->t2_lat, c->t2_items, c->t2_pnew};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     c->t2_coef = nullptr; c->t2_cls = nullptr; c->t2_rep = nullptr; c->t2_lat = nullptr;
-    c->t2_items = nullptr;
+    c->t2_items = nullptr; c->t2_pnew = nullptr;
     c->t2_ncls = 0; c->t2_nv = c->t2_nh = c->t2_nit = 0;
     c->t2_dt = -1.0;
 }
@@ -586,7 +586,8 @@ int table2d_setup(ab_ctx *c)
     if (hipMalloc((void **)&c->t2_coef, sizeof(double) * ncls * kOff2 * (size_t)c->nodes) != hipSuccess ||
         hipMalloc((void **)&c->t2_cls, sizeof(int) * (size_t)N) != hipSuccess ||
         hipMalloc((void **)&c->t2_rep, sizeof(int) * ncls) != hipSuccess ||
-        hipMalloc((void **)&c->t2_lat, sizeof(int) * 4 * (size_t)N) != hipSuccess)
+        hipMalloc((void **)&c->t2_lat, sizeof(int) * 4 * (size_t)N) != hipSuccess ||
+        hipMalloc((void **)&c->t2_pnew, sizeof(double) * (size_t)N) != hipSuccess)
     { table2d_free(c); c->err = "2-D table setup: device allocation failed"; return AB_ERR_NOMEM; }
     c->t2_ncls = ncls;
     if (hipMemcpyAsync(c->t2_cls, cls.data(), sizeof(int) * N, hipMemcpyHostToDevice, s) != hipSuccess ||

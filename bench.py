@@ -541,6 +541,7 @@ def main():
     ap.add_argument("--batch2d", action="store_true",
                     help="for rocprofv3: only the 16 cavity2d instances (Nx from {31, 40, 50}, "
                          "Nv 16, quadratic) as one batch context with the 2-D table transport, "
+                         "its run length autotuned, "
                          "5 warmup and --steps steps, then a 256 MiB copy kernel that "
                          "calibrates the byte counters; prints the step time, no JSON")
     args = ap.parse_args()
@@ -555,6 +556,10 @@ def main():
             sols.append(A.Solver(cl, precision=A.PREC_FP64_TABLE))
         batch = A.Batch(sols)
         dt = A.configure("cavity2d", nx=31, nv=16, mls_order=2).dt
+        table, chosen = batch.autotune(dt, reps=3)
+        log(f"[bench] 2-D apply run lengths: " +
+            ", ".join(f"{name} {ms * 1e3:.1f} us" for _, name, ms in table) +
+            f"; kept {table[chosen][1]}")
         batch.steps(dt, 5)
         batch.synchronize()
         t0 = time.perf_counter()
