@@ -149,3 +149,60 @@ def test_table2d_graph_rebuilds_coefficients_for_its_dt():
     b.step(dt)
     assert np.array_equal(a.g(), b.g())
     g.close()
+
+
+def test_table2d_step_with_positivity_matches_oracle():
+    """Negative values planted in g1 of a few particles make the positivity
+    limiter fire after the transport. The table step's fused commit, wall
+    and collision (with the first mass rescale folded into the last) flags
+    the same particles as the oracle, and g, the moments and the step's
+    masses agree with the oracle and with the FP64 path."""
+    _say("[table2d positivity] cavity2d 16^2 nv8, negatives planted in 24 rows")
+    case, cl, ocl, neigh, nn = setup("cavity2d", 16, 8, order=2)
+    nodes = 64
+    g = cl.g.copy()
+    rng = np.random.default_rng(11)
+    rows = rng.choice(cl.N, 24, replace=False)
+    g[np.ix_(rows, np.arange(0, nodes, 5))] *= -0.3
+    cl.g = g
+    dt = _dt(16)
+    tab = A.Solver(cl, precision=A.PREC_FP64_TABLE)
+    ref = A.Solver(cl, precision=A.PREC_FP64)
+    st = O.State(ocl, cl.g, cl.mom)
+    fired = 0
+    for k in range(3):
+        stats = tab.step(dt, stats=True)
+        rstats = ref.step(dt, stats=True)
+        m_old = O.step(ocl, st, dt, case.tau, case.R, cl.cfg.radius, 2)
+        assert abs(stats["mass_old"] - m_old) <= 1e-12 * m_old, k
+        for key in ("mass_after_transport", "mass_after_collision"):
+            assert abs(stats[key] - rstats[key]) <= 1e-12 * rstats[key], (k, key)
+        fl = st.limflag.astype(np.int64)
+        assert stats["n_positivity"] == int(np.count_nonzero(fl & 1)), k
+        fired += stats["n_positivity"]
+    assert fired > 0
+    mom, lim = tab.moments(with_limflag=True)
+    assert rel_err(tab.g(), st.g) < 1e-10
+    for c in (0, 1, 2, 4):
+        assert rel_err(mom[:, c], st.mom[:, c], floor=1e-300) < 1e-10, c
+    assert np.array_equal(lim, st.limflag)
+
+
+def test_table2d_step_without_mass_fix_matches_oracle():
+    """With the end-of-step rescale switched off, the table step still applies
+    the first rescale (massOld / massNew) that its fused commit and collision
+    leave out: g, the moments (the wall densities too) and the flags agree
+    with the oracle's step without the fix."""
+    _say("[table2d no mass fix] cavity2d 12^2 nv8, 3 steps")
+    case, cl, ocl, neigh, nn = setup("cavity2d", 12, 8, order=2)
+    dt = _dt(12)
+    tab = A.Solver(cl, precision=A.PREC_FP64_TABLE, no_mass_fix=True)
+    st = O.State(ocl, cl.g, cl.mom)
+    for k in range(3):
+        tab.step(dt)
+        O.step(ocl, st, dt, case.tau, case.R, cl.cfg.radius, 2, nofix=1)
+    mom, lim = tab.moments(with_limflag=True)
+    assert rel_err(tab.g(), st.g) < 1e-10
+    for c in (0, 1, 2, 4):
+        assert rel_err(mom[:, c], st.mom[:, c], floor=1e-300) < 1e-10, c
+    assert np.array_equal(lim, st.limflag)
