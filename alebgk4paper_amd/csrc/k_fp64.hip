@@ -757,6 +757,11 @@ __global__ __launch_bounds__(kBlock) void k_wall_collide(DevCloud c, DmArgs dm, 
 // the committed masses (pnew) and the collided masses (pmass) and gives
 // k_final the one factor that the rows need; the wall densities get the
 // first factor there. Results agree with the separate kernels to rounding.
+// U > 0: rows of at most U * 64 nodes (Nv <= 16 for U = 4), which each lane
+// loads into registers once (U values of each layer); U == 0: any row, read
+// again from memory in every pass. Both sum in the same order, so they give
+// identical results.
+template <int U>
 __global__ __launch_bounds__(kBlock) void k_commit_collide2(DevCloud c, double *__restrict__ g,
                                                             const double *__restrict__ gt,
                                                             double dt, double tau, double R,
@@ -773,19 +778,37 @@ __global__ __launch_bounds__(kBlock) void k_commit_collide2(DevCloud c, double *
     const double *tp = gt + (size_t)i * gs;
     const int nodes = c.nodes, Nv = c.Nv;
     const double dvd = c.dv * c.dv;
+    const int nu = U > 0 ? U : (nodes + kWave - 1) / kWave;   // passes of a wave over the row
+
+    double t1[U > 0 ? U : 1], t2[U > 0 ? U : 1];
+    if constexpr (U > 0)
+    {
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+        {
+            const int k = lane + u * kWave;
+            t1[u] = k < nodes ? tp[k] : 0.0;
+            t2[u] = k < nodes ? tp[k + nodes] : 0.0;
+        }
+    }
+    auto raw1 = [&](int u, int k) { if constexpr (U > 0) return t1[u]; else return tp[k]; };
+    auto raw2 = [&](int u, int k) { if constexpr (U > 0) return t2[u]; else return tp[k + nodes]; };
 
     double rp[2] = {0.0, 0.0};                     // mPos, mNeg of g1
-    for (int k = lane; k < nodes; k += kWave)
+#pragma unroll
+    for (int u = 0; u < nu; ++u)
     {
-        const double v = tp[k];
+        const int k = lane + u * kWave;
+        if (k >= nodes) continue;
+        const double v = raw1(u, k);
         if (v >= 0) rp[0] += v; else rp[1] += v;
     }
     wave_sums<2>(rp);
     const bool fires = (rp[1] < -1e-30 && rp[0] > 1e-30 && (rp[0] + rp[1]) > 0);
     const double sc = fires ? (rp[0] + rp[1]) / rp[0] : 1.0;
-    auto committed = [&](int k, double &v1, double &v2) {
-        v1 = tp[k];
-        v2 = tp[k + nodes];
+    auto committed = [&](int u, int k, double &v1, double &v2) {
+        v1 = raw1(u, k);
+        v2 = raw2(u, k);
         if (fires)
         {
             if (v1 < 0) { v1 = 0; v2 = 0; }
@@ -803,12 +826,15 @@ __global__ __launch_bounds__(kBlock) void k_commit_collide2(DevCloud c, double *
         const double RT2w = 2.0 * R * Tw;
         const double norm = AB_PI * RT2w;
         double r[3] = {0.0, 0.0, 0.0};             // intM, intP, committed mass
-        for (int k = lane; k < nodes; k += kWave)
+#pragma unroll
+        for (int u = 0; u < nu; ++u)
         {
+            const int k = lane + u * kWave;
+            if (k >= nodes) continue;
             const double vx = vnode(c.vlo, c.dv, k % Nv);
             const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
             double v, v2;
-            committed(k, v, v2);
+            committed(u, k, v, v2);
             const double cn = wall_cn2(vx, vy, Uwx, Uwy, nx, ny);
             if (cn < 0.0) r[0] += cn * v * dvd;
             else if (cn > 0.0)
@@ -822,12 +848,15 @@ __global__ __launch_bounds__(kBlock) void k_commit_collide2(DevCloud c, double *
         wave_sums<3>(r);
         const double rho_w = (r[1] > 1e-30) ? (-r[0] / r[1]) : 1.0;
         mnew = r[2];
-        for (int k = lane; k < nodes; k += kWave)
+#pragma unroll
+        for (int u = 0; u < nu; ++u)
         {
+            const int k = lane + u * kWave;
+            if (k >= nodes) continue;
             const double vx = vnode(c.vlo, c.dv, k % Nv);
             const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
             double v, v2;
-            committed(k, v, v2);
+            committed(u, k, v, v2);
             const double cn = wall_cn2(vx, vy, Uwx, Uwy, nx, ny);
             if (cn > 0.0)
             {
@@ -844,12 +873,15 @@ __global__ __launch_bounds__(kBlock) void k_commit_collide2(DevCloud c, double *
     else
     {
         double r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int k = lane; k < nodes; k += kWave)
+#pragma unroll
+        for (int u = 0; u < nu; ++u)
         {
+            const int k = lane + u * kWave;
+            if (k >= nodes) continue;
             const double vx = vnode(c.vlo, c.dv, k % Nv);
             const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
             double f, f2;
-            committed(k, f, f2);
+            committed(u, k, f, f2);
             r[0] += f * dvd;
             r[1] += vx * f * dvd;
             r[2] += vy * f * dvd;
@@ -863,16 +895,19 @@ __global__ __launch_bounds__(kBlock) void k_commit_collide2(DevCloud c, double *
             mrho = r[0];
             mux = r[1] / r[0];
             muy = r[2] / r[0];
-            const double t1 = (r[4] / r[0] - (mux * mux + muy * muy)) / (3.0 * R);
-            mT = t1 > 1e-6 ? t1 : 1e-6;
+            const double t1m = (r[4] / r[0] - (mux * mux + muy * muy)) / (3.0 * R);
+            mT = t1m > 1e-6 ? t1m : 1e-6;
         }
         const double lam = dt / tau, coeff = 1.0 / (1.0 + lam);
-        for (int k = lane; k < nodes; k += kWave)
+#pragma unroll
+        for (int u = 0; u < nu; ++u)
         {
+            const int k = lane + u * kWave;
+            if (k >= nodes) continue;
             const double vx = vnode(c.vlo, c.dv, k % Nv);
             const double vy = vnode(c.vlo, c.dv, (k / Nv) % Nv);
             double f, f2;
-            committed(k, f, f2);
+            committed(u, k, f, f2);
             const double Mg1 = maxwellian_g1(vx, vy, mux, muy, mT, mrho, R);
             f = coeff * (f + lam * Mg1);
             f2 = coeff * (f2 + lam * 0.5 * R * mT * Mg1);
@@ -1199,8 +1234,20 @@ void launch_commit_collide2(const DevCloud &c, double *g, const double *gt, doub
                             hipStream_t s)
 {
     const int nb = (c.N + kBlock / kWave - 1) / (kBlock / kWave);
-    k_commit_collide2<<<nb, kBlock, 0, s>>>(c, g, gt, dt, tau, R, rho, ux, uy, T, limflag, pnew,
-                                            pmass);
+    // rows of up to 256 nodes (Nv <= 16) stay in registers
+    const int u = (c.nodes + kWave - 1) / kWave;
+    if (u == 1)
+        k_commit_collide2<1><<<nb, kBlock, 0, s>>>(c, g, gt, dt, tau, R, rho, ux, uy, T, limflag,
+                                                   pnew, pmass);
+    else if (u == 2)
+        k_commit_collide2<2><<<nb, kBlock, 0, s>>>(c, g, gt, dt, tau, R, rho, ux, uy, T, limflag,
+                                                   pnew, pmass);
+    else if (u <= 4)
+        k_commit_collide2<4><<<nb, kBlock, 0, s>>>(c, g, gt, dt, tau, R, rho, ux, uy, T, limflag,
+                                                   pnew, pmass);
+    else
+        k_commit_collide2<0><<<nb, kBlock, 0, s>>>(c, g, gt, dt, tau, R, rho, ux, uy, T, limflag,
+                                                   pnew, pmass);
 }
 
 void launch_mood_check(const DevCloud &c, double *gt, const double *rho, int *mood,
