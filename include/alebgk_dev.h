@@ -46,7 +46,11 @@ extern "C" {
 #define AB_PREC_FP64_TABLE 2  /* 3-D: the MIXED pipeline with FP64 storage
                                  and arithmetic. 2-D: the FP64 step with
                                  the MLS transport as per-class coefficient
-                                 tables (lattice clouds; MIXED is 3-D only) */
+                                 tables, and positivity, commit, wall and
+                                 collision in one pass with the two mass
+                                 rescales applied as one (equal to the
+                                 FP64 step to rounding; lattice clouds;
+                                 MIXED is 3-D only) */
 
 typedef struct ab_ctx ab_ctx;
 
